@@ -1,0 +1,246 @@
+"""ctypes binding of libifem_gpu.so (the C ABI in include/ifem_gpu.h).
+
+The library is loaded from the package directory only; if it is missing or a
+call fails, an exception is raised -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libifem_gpu.so")
+
+# every symbol declared in include/ifem_gpu.h
+EXPORTS = [
+    "ifem_gpu_last_error", "ifem_gpu_abi_version", "ifem_gpu_ctx_create", "ifem_gpu_ctx_destroy",
+    "ifem_gpu_ctx_stream", "ifem_gpu_synchronize", "ifem_gpu_patch_create", "ifem_gpu_sam_bind",
+    "ifem_gpu_init_assembly", "ifem_gpu_assemble", "ifem_gpu_assemble_neumann",
+    "ifem_gpu_solver_defaults", "ifem_gpu_solve", "ifem_gpu_system_info", "ifem_gpu_get_system",
+    "ifem_gpu_get_values_fp32", "ifem_gpu_get_solution", "ifem_gpu_expand_solution",
+    "ifem_gpu_spmv", "ifem_gpu_set_rhs", "ifem_gpu_device_pointers", "ifem_gpu_last_timings",
+]
+
+FP64, FP32 = 64, 32
+
+
+class GpuError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ifem_gpu error {code}: {msg}")
+        self.code = code
+
+
+class Function(C.Structure):
+    _fields_ = [("type", C.c_int), ("nmodes", C.c_int), ("scale", C.c_double),
+                ("mean", C.c_double), ("modes", C.POINTER(C.c_double))]
+
+
+class Integrand(C.Structure):
+    _fields_ = [("kind", C.c_int), ("E", C.c_double), ("nu", C.c_double),
+                ("kappa", Function), ("source", Function), ("body", C.c_double * 3)]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("dtol", C.c_double),
+                ("maxits", C.c_int), ("precision", C.c_int), ("inner_rtol", C.c_double),
+                ("check_every", C.c_int), ("verbosity", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("outer_iterations", C.c_int), ("converged", C.c_int),
+                ("rel_residual", C.c_double), ("solve_ms", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libifem_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "(python -m ifem_amd.build); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    vp, ip, dp = C.c_void_p, C.c_int, C.c_double
+    L.ifem_gpu_last_error.restype = C.c_char_p
+    L.ifem_gpu_ctx_create.argtypes = [ip, vp, C.POINTER(vp)]
+    L.ifem_gpu_ctx_destroy.argtypes = [vp]
+    L.ifem_gpu_ctx_stream.argtypes = [vp]
+    L.ifem_gpu_ctx_stream.restype = vp
+    L.ifem_gpu_synchronize.argtypes = [vp]
+    L.ifem_gpu_patch_create.argtypes = [vp, C.POINTER(ip), C.POINTER(ip),
+                                        C.POINTER(C.POINTER(dp)), C.POINTER(dp), ip,
+                                        C.POINTER(ip)]
+    L.ifem_gpu_sam_bind.argtypes = [vp, ip, ip, ip, C.POINTER(ip), ip]
+    L.ifem_gpu_init_assembly.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.ifem_gpu_assemble.argtypes = [vp, C.POINTER(Integrand), C.POINTER(ip), ip]
+    L.ifem_gpu_assemble_neumann.argtypes = [vp, ip, C.POINTER(dp), ip]
+    L.ifem_gpu_solver_defaults.argtypes = [C.POINTER(SolverParams)]
+    L.ifem_gpu_solver_defaults.restype = None
+    L.ifem_gpu_solve.argtypes = [vp, C.POINTER(SolverParams), C.POINTER(Stats)]
+    L.ifem_gpu_system_info.argtypes = [vp, C.POINTER(ip), C.POINTER(C.c_int64), C.POINTER(ip)]
+    L.ifem_gpu_get_system.argtypes = [vp, vp, vp, vp, vp]
+    L.ifem_gpu_get_values_fp32.argtypes = [vp, vp]
+    L.ifem_gpu_get_solution.argtypes = [vp, vp]
+    L.ifem_gpu_expand_solution.argtypes = [vp, vp]
+    L.ifem_gpu_spmv.argtypes = [vp, vp, vp]
+    L.ifem_gpu_set_rhs.argtypes = [vp, vp]
+    L.ifem_gpu_device_pointers.argtypes = [vp] + [C.POINTER(vp)] * 5
+    L.ifem_gpu_last_timings.argtypes = [vp, C.POINTER(dp), C.POINTER(dp)]
+    _lib = L
+    return L
+
+
+def check(rc: int):
+    if rc != 0:
+        raise GpuError(rc, lib().ifem_gpu_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Session:
+    """One context holding one patch and its assembled system (AlgEqSystem analogue)."""
+
+    def __init__(self, device: int = 0, stream=None):
+        L = lib()
+        self._keep = []
+        h = C.c_void_p()
+        check(L.ifem_gpu_ctx_create(device, C.c_void_p(stream) if stream else None, C.byref(h)))
+        self.h = h
+        self.neq = 0
+        self.ncomp = 0
+        self.nnz = 0
+
+    def close(self):
+        if self.h:
+            lib().ifem_gpu_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        return lib().ifem_gpu_ctx_stream(self.h) or 0
+
+    def synchronize(self):
+        check(lib().ifem_gpu_synchronize(self.h))
+
+    def add_patch(self, patch) -> int:
+        order = (C.c_int * 3)(*[q + 1 for q in patch.p])
+        ncoef = (C.c_int * 3)(*patch.n)
+        ks = [np.ascontiguousarray(k, dtype=np.float64) for k in patch.knots]
+        kp = (C.POINTER(C.c_double) * 3)(*[k.ctypes.data_as(C.POINTER(C.c_double)) for k in ks])
+        co = np.ascontiguousarray(patch.coefs.reshape(-1), dtype=np.float64)
+        pid = C.c_int(-1)
+        check(lib().ifem_gpu_patch_create(self.h, order, ncoef, kp,
+                                          co.ctypes.data_as(C.POINTER(C.c_double)), 0,
+                                          C.byref(pid)))
+        return pid.value
+
+    def bind(self, patch_id: int, ncomp: int, meqn: np.ndarray, neq: int):
+        m = np.ascontiguousarray(meqn, dtype=np.int32)
+        check(lib().ifem_gpu_sam_bind(self.h, patch_id, ncomp, len(m),
+                                      m.ctypes.data_as(C.POINTER(C.c_int)), neq))
+        self.neq = neq
+        self.ncomp = ncomp
+
+    def init_assembly(self) -> int:
+        nnz = C.c_int64(0)
+        check(lib().ifem_gpu_init_assembly(self.h, C.byref(nnz)))
+        self.nnz = nnz.value
+        return self.nnz
+
+    @staticmethod
+    def _function(f, keep) -> Function:
+        s = Function()
+        s.type = int(f.type)
+        s.scale = float(f.scale)
+        s.mean = float(f.mean)
+        s.nmodes = f.nmodes
+        if f.nmodes:
+            m = np.ascontiguousarray(f.modes.reshape(-1), dtype=np.float64)
+            keep.append(m)
+            s.modes = m.ctypes.data_as(C.POINTER(C.c_double))
+        return s
+
+    def assemble(self, itg, ng, precision: int = FP64):
+        keep = []
+        s = Integrand()
+        s.kind = int(itg.kind)
+        s.E = float(itg.E)
+        s.nu = float(itg.nu)
+        s.kappa = self._function(itg.kappa, keep)
+        s.source = self._function(itg.source, keep)
+        for i in range(3):
+            s.body[i] = float(itg.body[i])
+        ngv = (C.c_int * 3)(*ng)
+        check(lib().ifem_gpu_assemble(self.h, C.byref(s), ngv, precision))
+
+    def add_traction(self, face: int, t, ng: int):
+        """face: 0..5 (u0,u1,v0,v1,w0,w1) -> IFEM face index face+1."""
+        tv = (C.c_double * 3)(*[float(x) for x in t] + [0.0] * (3 - len(t)))
+        check(lib().ifem_gpu_assemble_neumann(self.h, face + 1, tv, ng))
+
+    def solve(self, rtol=1e-8, atol=1e-300, maxits=100000, precision=FP64, inner_rtol=1e-4,
+              check_every=32, dtol=1e30):
+        p = SolverParams()
+        lib().ifem_gpu_solver_defaults(C.byref(p))
+        p.rtol, p.atol, p.dtol, p.maxits = rtol, atol, dtol, maxits
+        p.precision, p.inner_rtol, p.check_every = precision, inner_rtol, check_every
+        st = Stats()
+        rc = lib().ifem_gpu_solve(self.h, C.byref(p), C.byref(st))
+        if rc not in (0, 4):
+            check(rc)
+        return {"iterations": st.iterations, "outer": st.outer_iterations,
+                "converged": bool(st.converged), "rel_residual": st.rel_residual,
+                "solve_ms": st.solve_ms, "rc": rc}
+
+    def system(self, values=True):
+        rowptr = np.zeros(self.neq + 1, dtype=np.int64)
+        colind = np.zeros(max(self.nnz, 1), dtype=np.int32)
+        vals = np.zeros(max(self.nnz, 1), dtype=np.float64) if values else None
+        rhs = np.zeros(max(self.neq, 1), dtype=np.float64)
+        check(lib().ifem_gpu_get_system(self.h, _ptr(rowptr), _ptr(colind),
+                                        _ptr(vals) if values else None, _ptr(rhs)))
+        return rowptr, colind[:self.nnz], (vals[:self.nnz] if values else None), rhs[:self.neq]
+
+    def values_fp32(self):
+        v = np.zeros(max(self.nnz, 1), dtype=np.float32)
+        check(lib().ifem_gpu_get_values_fp32(self.h, _ptr(v)))
+        return v[:self.nnz]
+
+    def solution(self):
+        x = np.zeros(max(self.neq, 1))
+        check(lib().ifem_gpu_get_solution(self.h, _ptr(x)))
+        return x[:self.neq]
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.neq)
+        check(lib().ifem_gpu_spmv(self.h, _ptr(x), _ptr(y)))
+        return y
+
+    def set_rhs(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        check(lib().ifem_gpu_set_rhs(self.h, _ptr(b)))
+
+    def timings(self):
+        a = C.c_double(0.0); s = C.c_double(0.0)
+        check(lib().ifem_gpu_last_timings(self.h, C.byref(a), C.byref(s)))
+        return a.value, s.value

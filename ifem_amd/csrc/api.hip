@@ -1,0 +1,378 @@
+// api.hip -- C ABI entry points of libifem_gpu.so: contexts, patch upload,
+// SAM binding and data access.  Kernels live in assemble.hip / solver.hip.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "internal.hpp"
+
+namespace ifem {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg)
+{
+  g_last_error = msg;
+  return code;
+}
+
+static void free_system(Context& c)
+{
+  solver_free(c);
+  hipFree(c.d_eq);
+  hipFree(c.d_rowptr);
+  hipFree(c.d_colind);
+  hipFree(c.d_val);
+  hipFree(c.d_val32);
+  hipFree(c.d_rhs);
+  c.d_eq = nullptr;
+  c.d_rowptr = nullptr;
+  c.d_colind = nullptr;
+  c.d_val = nullptr;
+  c.d_val32 = nullptr;
+  c.d_rhs = nullptr;
+  c.pattern_ready = c.fp64_ready = c.fp32_ready = false;
+  c.nnz = 0;
+}
+
+} // namespace ifem
+
+using namespace ifem;
+
+#define CHECK_CTX(ctx)                                                       \
+  do {                                                                       \
+    if (!(ctx)) return fail(IFEM_GPU_EINVAL, "null context");                \
+    hipError_t e__ = hipSetDevice((ctx)->c.device);                          \
+    if (e__ != hipSuccess)                                                   \
+      return fail(IFEM_GPU_EDEVICE, std::string("hipSetDevice: ") +          \
+                                        hipGetErrorString(e__));             \
+  } while (0)
+
+extern "C" {
+
+const char* ifem_gpu_last_error(void) { return g_last_error.c_str(); }
+
+int ifem_gpu_abi_version(void) { return IFEM_GPU_ABI_VERSION; }
+
+int ifem_gpu_ctx_create(int device, void* stream, ifem_gpu_ctx** out)
+{
+  if (!out) return fail(IFEM_GPU_EINVAL, "null output pointer");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return fail(IFEM_GPU_EDEVICE, "no HIP device available");
+  if (device < 0 || device >= ndev)
+    return fail(IFEM_GPU_EINVAL, "device index out of range");
+  IFEM_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  IFEM_HIP(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(IFEM_GPU_EUNSUPPORTED,
+                std::string("libifem_gpu is built for gfx950, device is ") + prop.gcnArchName);
+  ifem_gpu_ctx* ctx = new (std::nothrow) ifem_gpu_ctx();
+  if (!ctx) return fail(IFEM_GPU_ENOMEM, "out of host memory");
+  ctx->c.device = device;
+  if (stream) {
+    ctx->c.stream = (hipStream_t)stream;
+  } else {
+    if (hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
+      delete ctx;
+      return fail(IFEM_GPU_EDEVICE, "hipStreamCreate failed");
+    }
+    ctx->c.own_stream = true;
+  }
+  if (hipEventCreate(&ctx->c.ev0) != hipSuccess || hipEventCreate(&ctx->c.ev1) != hipSuccess) {
+    delete ctx;
+    return fail(IFEM_GPU_EDEVICE, "hipEventCreate failed");
+  }
+  *out = ctx;
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_ctx_destroy(ifem_gpu_ctx* ctx)
+{
+  if (!ctx) return IFEM_GPU_OK;
+  hipSetDevice(ctx->c.device);
+  hipStreamSynchronize(ctx->c.stream);
+  free_system(ctx->c);
+  for (auto& p : ctx->c.patches) {
+    for (int d = 0; d < 3; d++) hipFree(p.d_knots[d]);
+    hipFree(p.d_coefs);
+  }
+  for (void* p : ctx->c.scratch_tables) hipFree(p);
+  hipEventDestroy(ctx->c.ev0);
+  hipEventDestroy(ctx->c.ev1);
+  if (ctx->c.own_stream) hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+  return IFEM_GPU_OK;
+}
+
+void* ifem_gpu_ctx_stream(ifem_gpu_ctx* ctx) { return ctx ? (void*)ctx->c.stream : nullptr; }
+
+int ifem_gpu_synchronize(ifem_gpu_ctx* ctx)
+{
+  CHECK_CTX(ctx);
+  IFEM_HIP(hipStreamSynchronize(ctx->c.stream));
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_patch_create(ifem_gpu_ctx* ctx, const int order[3], const int ncoef[3],
+                          const double* const knots[3], const double* coefs, int rational,
+                          int* patch_id)
+{
+  CHECK_CTX(ctx);
+  if (!order || !ncoef || !knots || !coefs || !patch_id)
+    return fail(IFEM_GPU_EINVAL, "patch_create: null argument");
+  if (rational) return fail(IFEM_GPU_EUNSUPPORTED, "patch_create: rational patches are not supported");
+  PatchRec P;
+  for (int d = 0; d < 3; d++) {
+    const int p = order[d] - 1, n = ncoef[d];
+    if (p < 1 || p > kMaxP)
+      return fail(IFEM_GPU_EUNSUPPORTED, "patch_create: degree must be 1.." + std::to_string(kMaxP));
+    if (n <= p) return fail(IFEM_GPU_EINVAL, "patch_create: ncoef must exceed the degree");
+    if (!knots[d]) return fail(IFEM_GPU_EINVAL, "patch_create: null knot vector");
+    const double* t = knots[d];
+    for (int i = 0; i + 1 < n + p + 1; i++)
+      if (t[i + 1] < t[i]) return fail(IFEM_GPU_EINVAL, "patch_create: knots must be non-decreasing");
+    for (int i = 0; i < p; i++)
+      if (t[i] != t[p] || t[n + i + 1] != t[n])
+        return fail(IFEM_GPU_EUNSUPPORTED, "patch_create: knot vectors must be open");
+    for (int s = p; s < n; s++)
+      if (!(t[s + 1] > t[s]))
+        return fail(IFEM_GPU_EUNSUPPORTED, "patch_create: repeated interior knots are not supported");
+    P.p[d] = p;
+    P.n[d] = n;
+    P.nel[d] = n - p;
+    P.knots[d].assign(t, t + n + p + 1);
+  }
+  const size_t nnod = (size_t)P.n[0] * P.n[1] * P.n[2];
+  for (int d = 0; d < 3; d++) {
+    IFEM_HIP(hipMalloc(&P.d_knots[d], sizeof(double) * P.knots[d].size()));
+    IFEM_HIP(hipMemcpy(P.d_knots[d], P.knots[d].data(), sizeof(double) * P.knots[d].size(),
+                       hipMemcpyHostToDevice));
+  }
+  IFEM_HIP(hipMalloc(&P.d_coefs, sizeof(double) * 3 * nnod));
+  IFEM_HIP(hipMemcpy(P.d_coefs, coefs, sizeof(double) * 3 * nnod, hipMemcpyHostToDevice));
+  ctx->c.patches.push_back(P);
+  *patch_id = (int)ctx->c.patches.size() - 1;
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_sam_bind(ifem_gpu_ctx* ctx, int patch_id, int ncomp, int ndof, const int* meqn,
+                      int neq)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (patch_id < 0 || patch_id >= (int)c.patches.size())
+    return fail(IFEM_GPU_EINVAL, "sam_bind: unknown patch");
+  if (ncomp != 1 && ncomp != 3) return fail(IFEM_GPU_EUNSUPPORTED, "sam_bind: ncomp must be 1 or 3");
+  const PatchRec& P = c.patches[patch_id];
+  const int64_t nnod = (int64_t)P.n[0] * P.n[1] * P.n[2];
+  if (!meqn || ndof != nnod * ncomp) return fail(IFEM_GPU_EINVAL, "sam_bind: ndof != nnod*ncomp");
+  std::vector<int> eq(ndof);
+  int last = 0;
+  for (int i = 0; i < ndof; i++) {
+    if (meqn[i] < 0)
+      return fail(IFEM_GPU_EUNSUPPORTED, "sam_bind: constrained (negative) equations are not supported");
+    if (meqn[i] > 0) {
+      if (meqn[i] != last + 1)
+        return fail(IFEM_GPU_EUNSUPPORTED,
+                    "sam_bind: free DOFs must be numbered 1..neq in increasing DOF order");
+      last = meqn[i];
+    }
+    eq[i] = meqn[i] - 1;
+  }
+  if (last != neq) return fail(IFEM_GPU_EINVAL, "sam_bind: neq does not match meqn");
+  free_system(c);
+  c.patch = patch_id;
+  c.ncomp = ncomp;
+  c.ndof = ndof;
+  c.neq = neq;
+  c.h_eq = eq;
+  IFEM_HIP(hipMalloc(&c.d_eq, sizeof(int) * ndof));
+  IFEM_HIP(hipMemcpy(c.d_eq, eq.data(), sizeof(int) * ndof, hipMemcpyHostToDevice));
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_init_assembly(ifem_gpu_ctx* ctx, int64_t* nnz)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (c.patch < 0) return fail(IFEM_GPU_EINVAL, "init_assembly: no SAM bound");
+  int rc = build_pattern(c);
+  if (rc) return rc;
+  if (nnz) *nnz = c.nnz;
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_assemble(ifem_gpu_ctx* ctx, const ifem_gpu_integrand* itg, const int ng[3],
+                      int precision)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (!itg || !ng) return fail(IFEM_GPU_EINVAL, "assemble: null argument");
+  if (!c.pattern_ready) return fail(IFEM_GPU_EINVAL, "assemble: call ifem_gpu_init_assembly first");
+  if (precision != IFEM_GPU_FP64 && precision != IFEM_GPU_FP32)
+    return fail(IFEM_GPU_EINVAL, "assemble: precision must be 64 or 32");
+  const int want = itg->kind == IFEM_GPU_ELASTICITY ? 3 : (itg->kind == IFEM_GPU_POISSON ? 1 : 0);
+  if (!want) return fail(IFEM_GPU_EUNSUPPORTED, "assemble: unknown integrand kind");
+  if (want != c.ncomp)
+    return fail(IFEM_GPU_EINVAL, "assemble: integrand component count differs from the SAM's");
+  return assemble_volume(c, *itg, ng, precision);
+}
+
+int ifem_gpu_assemble_neumann(ifem_gpu_ctx* ctx, int face, const double* traction, int ng)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (!traction) return fail(IFEM_GPU_EINVAL, "assemble_neumann: null traction");
+  if (face < 1 || face > 6) return fail(IFEM_GPU_EINVAL, "assemble_neumann: face must be 1..6");
+  if (ng < 1 || ng > kMaxNG) return fail(IFEM_GPU_EUNSUPPORTED, "assemble_neumann: ng out of range");
+  if (!c.d_rhs) return fail(IFEM_GPU_EINVAL, "assemble_neumann: assemble the volume terms first");
+  return assemble_traction(c, face - 1, traction, ng);
+}
+
+void ifem_gpu_solver_defaults(ifem_gpu_solver_params* p)
+{
+  if (!p) return;
+  p->rtol = 1e-6;   // LinSolParams defaults (src/LinAlg/LinSolParams.C:70-76)
+  p->atol = 1e-20;
+  p->dtol = 1e6;
+  p->maxits = 1000;
+  p->precision = IFEM_GPU_FP64;
+  p->inner_rtol = 1e-4;
+  p->check_every = 32;
+  p->verbosity = 0;
+}
+
+int ifem_gpu_solve(ifem_gpu_ctx* ctx, const ifem_gpu_solver_params* params, ifem_gpu_stats* stats)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (!params) return fail(IFEM_GPU_EINVAL, "solve: null params");
+  if (!c.fp64_ready) return fail(IFEM_GPU_EINVAL, "solve: no FP64 system assembled");
+  if (params->precision == IFEM_GPU_FP32 && !c.fp32_ready)
+    return fail(IFEM_GPU_EINVAL, "solve: FP32 solve needs an IFEM_GPU_FP32 assembly");
+  return ifem::solve(c, *params, stats);
+}
+
+int ifem_gpu_system_info(ifem_gpu_ctx* ctx, int* neq, int64_t* nnz, int* ncomp)
+{
+  if (!ctx) return fail(IFEM_GPU_EINVAL, "null context");
+  if (neq) *neq = ctx->c.neq;
+  if (nnz) *nnz = ctx->c.nnz;
+  if (ncomp) *ncomp = ctx->c.ncomp;
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_get_system(ifem_gpu_ctx* ctx, int64_t* rowptr, int32_t* colind, double* values,
+                        double* rhs)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (!c.pattern_ready) return fail(IFEM_GPU_EINVAL, "get_system: no pattern");
+  IFEM_HIP(hipStreamSynchronize(c.stream));
+  if (rowptr)
+    IFEM_HIP(hipMemcpy(rowptr, c.d_rowptr, sizeof(int64_t) * (c.neq + 1), hipMemcpyDeviceToHost));
+  if (colind)
+    IFEM_HIP(hipMemcpy(colind, c.d_colind, sizeof(int32_t) * c.nnz, hipMemcpyDeviceToHost));
+  if (values) {
+    if (!c.fp64_ready) return fail(IFEM_GPU_EINVAL, "get_system: no FP64 values");
+    IFEM_HIP(hipMemcpy(values, c.d_val, sizeof(double) * c.nnz, hipMemcpyDeviceToHost));
+  }
+  if (rhs) {
+    if (!c.d_rhs) return fail(IFEM_GPU_EINVAL, "get_system: no RHS");
+    IFEM_HIP(hipMemcpy(rhs, c.d_rhs, sizeof(double) * c.neq, hipMemcpyDeviceToHost));
+  }
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_get_values_fp32(ifem_gpu_ctx* ctx, float* values)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (!c.fp32_ready || !values) return fail(IFEM_GPU_EINVAL, "get_values_fp32: no FP32 values");
+  IFEM_HIP(hipStreamSynchronize(c.stream));
+  IFEM_HIP(hipMemcpy(values, c.d_val32, sizeof(float) * c.nnz, hipMemcpyDeviceToHost));
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_get_solution(ifem_gpu_ctx* ctx, double* x)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (!c.w.x || !x) return fail(IFEM_GPU_EINVAL, "get_solution: no solution");
+  IFEM_HIP(hipStreamSynchronize(c.stream));
+  IFEM_HIP(hipMemcpy(x, c.w.x, sizeof(double) * c.neq, hipMemcpyDeviceToHost));
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_expand_solution(ifem_gpu_ctx* ctx, double* xdof)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (!c.w.x || !xdof) return fail(IFEM_GPU_EINVAL, "expand_solution: no solution");
+  std::vector<double> x(c.neq);
+  IFEM_HIP(hipStreamSynchronize(c.stream));
+  IFEM_HIP(hipMemcpy(x.data(), c.w.x, sizeof(double) * c.neq, hipMemcpyDeviceToHost));
+  for (int i = 0; i < c.ndof; i++) xdof[i] = c.h_eq[i] >= 0 ? x[c.h_eq[i]] : 0.0;
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_spmv(ifem_gpu_ctx* ctx, const double* x, double* y)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (!c.fp64_ready || !x || !y) return fail(IFEM_GPU_EINVAL, "spmv: no FP64 system or null vector");
+  double *dx = nullptr, *dy = nullptr;
+  IFEM_HIP(hipMalloc(&dx, sizeof(double) * c.neq));
+  IFEM_HIP(hipMalloc(&dy, sizeof(double) * c.neq));
+  int rc = IFEM_GPU_OK;
+  if (hipMemcpyAsync(dx, x, sizeof(double) * c.neq, hipMemcpyHostToDevice, c.stream) != hipSuccess)
+    rc = fail(IFEM_GPU_EDEVICE, "spmv: copy in failed");
+  if (!rc) rc = spmv_device(c, dx, dy);
+  if (!rc && hipMemcpyAsync(y, dy, sizeof(double) * c.neq, hipMemcpyDeviceToHost, c.stream) != hipSuccess)
+    rc = fail(IFEM_GPU_EDEVICE, "spmv: copy out failed");
+  if (!rc && hipStreamSynchronize(c.stream) != hipSuccess) rc = fail(IFEM_GPU_EDEVICE, "spmv: sync failed");
+  hipFree(dx);
+  hipFree(dy);
+  return rc;
+}
+
+int ifem_gpu_set_rhs(ifem_gpu_ctx* ctx, const double* b)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (!c.d_rhs || !b) return fail(IFEM_GPU_EINVAL, "set_rhs: no system or null vector");
+  IFEM_HIP(hipMemcpyAsync(c.d_rhs, b, sizeof(double) * c.neq, hipMemcpyHostToDevice, c.stream));
+  IFEM_HIP(hipStreamSynchronize(c.stream));
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_device_pointers(ifem_gpu_ctx* ctx, void** rowptr, void** colind, void** values,
+                             void** rhs, void** solution)
+{
+  if (!ctx) return fail(IFEM_GPU_EINVAL, "null context");
+  Context& c = ctx->c;
+  if (rowptr) *rowptr = c.d_rowptr;
+  if (colind) *colind = c.d_colind;
+  if (values) *values = c.d_val;
+  if (rhs) *rhs = c.d_rhs;
+  if (solution) *solution = c.w.x;
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_last_timings(ifem_gpu_ctx* ctx, double* assemble_ms, double* solve_ms)
+{
+  if (!ctx) return fail(IFEM_GPU_EINVAL, "null context");
+  if (assemble_ms) *assemble_ms = ctx->c.last_assemble_ms;
+  if (solve_ms) *solve_ms = ctx->c.last_solve_ms;
+  return IFEM_GPU_OK;
+}
+
+} // extern "C"

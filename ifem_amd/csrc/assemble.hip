@@ -1,0 +1,968 @@
+// assemble.hip -- symbolic pattern, numeric volume assembly and Neumann
+// traction kernels for one tensor-product B-spline patch on gfx950.
+//
+// Numeric assembly is row-owned ("owner computes"): one workgroup owns the
+// line of control points (i = 0..n1-1, j, k), walks the element layers
+// e1 = 0..nel1-1 and, for each layer, integrates the (p+1)^2 elements of
+// the layer that touch the line.  Contributions go into an LDS window that
+// holds p+1 complete rows in dense neighbour layout (2p+1)^3 x ncomp^2; after
+// layer e1 the row of node e1 can get no more contributions and is written
+// to the CSR arrays once, compacted over eliminated / absent neighbours.
+// Every K entry is written exactly once, coalesced, without atomics, and
+// the summation order is fixed, so results are bitwise reproducible.
+//
+// This replaces the reference's element loop with colour groups and the
+// per-entry std::find scatter (src/ASM/ASMs3D.C:2071-2335,
+// src/LinAlg/SparseMatrix.C:614-688); the per-GP arithmetic follows
+// utl::Jacobian (src/Utility/CoordinateMapping.C:24-63) and the integrands
+// of Integrand::evalInt.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace ifem {
+
+// ---------------------------------------------------------------------------
+// Gauss-Legendre rule by Newton iteration on the Legendre polynomial in long
+// double (same nodes/weights as the reference tables G1..G10,
+// src/Utility/GaussQuadrature.C:17-60, to the last bit or one ulp).
+static void gauss_rule(int n, double* xi, double* w)
+{
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int i = 0; i < n; i++) {
+    long double x = cosl(pi * (i + 0.75L) / (n + 0.5L));
+    long double dp = 0.0L;
+    for (int it = 0; it < 100; it++) {
+      long double p0 = 1.0L, p1 = x;
+      for (int k = 1; k < n; k++) {
+        long double p2 = ((2 * k + 1) * x * p1 - k * p0) / (k + 1);
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (x * p1 - p0) / (x * x - 1.0L);
+      long double dx = p1 / dp;
+      x -= dx;
+      if (fabsl(dx) < 1e-30L) break;
+    }
+    if (2 * i + 1 == n) x = 0.0L;
+    long double p0 = 1.0L, p1 = x;
+    for (int k = 1; k < n; k++) {
+      long double p2 = ((2 * k + 1) * x * p1 - k * p0) / (k + 1);
+      p0 = p1;
+      p1 = p2;
+    }
+    dp = n * (x * p1 - p0) / (x * x - 1.0L);
+    xi[n - 1 - i] = (double)x;
+    w[n - 1 - i] = (double)(2.0L / ((1.0L - x * x) * dp * dp));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 1D B-spline values and first derivatives on span s (Cox-de Boor,
+// Piegl & Tiller A2.2/A2.3), the per-direction factor of GoTools'
+// SplineVolume::computeBasisGrid (src/ASM/ASMs3D.C:4114-4143).
+template <int P>
+__device__ inline void bspline_1d(const double* __restrict__ t, int s, double u, double* N,
+                                  double* dN)
+{
+  double left[P + 1], right[P + 1], ndu[P + 1][P + 1];
+  ndu[0][0] = 1.0;
+#pragma unroll
+  for (int j = 1; j <= P; j++) {
+    left[j] = u - t[s + 1 - j];
+    right[j] = t[s + j] - u;
+    double saved = 0.0;
+#pragma unroll
+    for (int r = 0; r < j; r++) {
+      ndu[j][r] = right[r + 1] + left[j - r];
+      double temp = ndu[r][j - 1] / ndu[j][r];
+      ndu[r][j] = saved + right[r + 1] * temp;
+      saved = left[j - r] * temp;
+    }
+    ndu[j][j] = saved;
+  }
+#pragma unroll
+  for (int r = 0; r <= P; r++) {
+    N[r] = ndu[r][P];
+    double d = 0.0;
+    if (r >= 1) d += ndu[r - 1][P - 1] / ndu[P][r - 1];
+    if (r <= P - 1) d -= ndu[r][P - 1] / ndu[P][r];
+    dN[r] = P * d;
+  }
+}
+
+struct GaussArgs {
+  double xi[kMaxNG];
+  double w[kMaxNG];
+};
+
+// Tables per direction: for element e and Gauss point g, N and dN/du of the
+// p+1 nonzero functions, plus the span length.
+template <int P>
+__global__ void k_basis_tables(const double* __restrict__ t, int nel, int ng, GaussArgs ga,
+                               double* __restrict__ tabN, double* __restrict__ tabD,
+                               double* __restrict__ span)
+{
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nel * ng) return;
+  const int e = id / ng, g = id % ng;
+  const int s = e + P;
+  const double u0 = t[s], u1 = t[s + 1];
+  const double u = 0.5 * ((u1 - u0) * ga.xi[g] + u1 + u0); // SplineUtils.C:217
+  double N[P + 1], dN[P + 1];
+  bspline_1d<P>(t, s, u, N, dN);
+#pragma unroll
+  for (int a = 0; a <= P; a++) {
+    tabN[id * (P + 1) + a] = N[a];
+    tabD[id * (P + 1) + a] = dN[a];
+  }
+  if (g == 0) span[e] = u1 - u0;
+}
+
+// ---------------------------------------------------------------------------
+// Scalar fields evaluated at Gauss points (same definitions as the oracle).
+__device__ inline double eval_field(const DevFunction& f, double x, double y, double z)
+{
+  switch (f.type) {
+  case IFEM_GPU_FN_CONST:
+    return f.scale;
+  case IFEM_GPU_FN_SINPROD:
+    return f.scale * sin(M_PI * x) * sin(M_PI * y) * sin(M_PI * z);
+  case IFEM_GPU_FN_FOURIER:
+  case IFEM_GPU_FN_EXP_FOURIER: {
+    double s = 0.0;
+    for (int m = 0; m < f.nmodes; m++) {
+      const double* md = f.modes + 5 * m;
+      s += md[4] * cos(md[0] * x + md[1] * y + md[2] * z + md[3]);
+    }
+    const double g = f.mean + f.scale * s;
+    return f.type == IFEM_GPU_FN_EXP_FOURIER ? exp(g) : g;
+  }
+  default:
+    return 0.0;
+  }
+}
+
+struct AsmArgs {
+  int n1, n2, n3;
+  int nel1, nel2, nel3;
+  const double* coefs;                 // nnod*3
+  const double *tabN1, *tabD1, *span1; // direction tables
+  const double *tabN2, *tabD2, *span2;
+  const double *tabN3, *tabD3, *span3;
+  double gw[kMaxNG];                   // Gauss weights
+  const int* eq;                       // per DOF: 0-based eq or -1
+  const int64_t* rowptr;
+  double* val64;
+  float* val32;
+  double* rhs;                         // FP64 RHS (nullptr: not written)
+  DevFunction kappa, source;
+  double body[3];
+  double lam, mu;
+};
+
+// Chunk of layer elements processed together so that CH*NG^3 ~ one block.
+template <int P, int NG>
+struct Chunk {
+  static constexpr int P1 = P + 1;
+  static constexpr int NB = P1 * P1;
+  static constexpr int GP3 = NG * NG * NG;
+  static constexpr int v = (256 / GP3) < 1 ? 1 : ((256 / GP3) > NB ? NB : (256 / GP3));
+};
+
+constexpr int kBlock = 256;
+
+// T: arithmetic type of the element integrals (double: FP64 path, float:
+// FP32 path).  NC: components (1 Poisson, 3 elasticity).  SPLIT: number of
+// blocks per line sharing the row components (NC/SPLIT rows each).
+template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS>
+__global__ __launch_bounds__(kBlock) void k_assemble_lines(AsmArgs a)
+{
+  constexpr int P1 = P + 1;
+  constexpr int NEN = P1 * P1 * P1;
+  constexpr int GP3 = NG * NG * NG;
+  constexpr int NB = P1 * P1;
+  constexpr int W1 = 2 * P + 1;
+  constexpr int WN = W1 * W1 * W1;
+  constexpr int NCR = NC / SPLIT;
+  constexpr bool SACC = (NC == 3 && SPLIT == 1); // accumulate S_ij, apply lam/mu later
+  constexpr int NS = NC == 1 ? 1 : (SACC ? 9 : 3);
+  constexpr int CH = Chunk<P, NG>::v;
+  constexpr int GEO = (NC == 3 ? 9 : 0) + 3 * P1;
+  constexpr int NRHS = HAS_RHS ? P1 * NC : 1;
+
+  __shared__ T win[P1][NCR][WN][NC];
+  __shared__ T geo[CH * GP3][GEO];
+  __shared__ T stage[CH][NEN][P1][NS];
+  __shared__ T rstage[HAS_RHS ? CH * GP3 : 1][NRHS];
+  __shared__ T rwin[P1][NC];
+  __shared__ double cp[P1][W1][W1][3];
+  __shared__ T tN1[NG][P1], tD1[NG][P1];
+  __shared__ T tN2[P1][NG][P1], tD2[P1][NG][P1];
+  __shared__ T tN3[P1][NG][P1], tD3[P1][NG][P1];
+  __shared__ double sp2[P1], sp3[P1], sp1;
+  __shared__ int be2[NB], be3[NB];
+  __shared__ int nbundle;
+
+  const int tid = threadIdx.x;
+  const int line = blockIdx.x;
+  const int jc = line % a.n2, kc = line / a.n2;
+  const int split = blockIdx.y; // row components [split*NCR, split*NCR+NCR)
+
+  // bundle of elements touching the line, fixed order (e3 outer, e2 inner)
+  if (tid == 0) {
+    int nb = 0;
+    for (int m3 = 0; m3 < P1; m3++)
+      for (int m2 = 0; m2 < P1; m2++) {
+        const int e2 = jc - P + m2, e3 = kc - P + m3;
+        if (e2 >= 0 && e2 < a.nel2 && e3 >= 0 && e3 < a.nel3) {
+          be2[nb] = e2;
+          be3[nb] = e3;
+          nb++;
+        }
+      }
+    nbundle = nb;
+  }
+  // direction 2/3 tables for elements jc-P..jc and kc-P..kc
+  for (int t = tid; t < P1 * NG * P1; t += kBlock) {
+    const int m = t / (NG * P1), r = t % (NG * P1);
+    const int g = r / P1, q = r % P1;
+    const int e2 = jc - P + m, e3 = kc - P + m;
+    T n2 = 0, d2 = 0, n3 = 0, d3 = 0;
+    if (e2 >= 0 && e2 < a.nel2) {
+      n2 = (T)a.tabN2[(e2 * NG + g) * P1 + q];
+      d2 = (T)a.tabD2[(e2 * NG + g) * P1 + q];
+    }
+    if (e3 >= 0 && e3 < a.nel3) {
+      n3 = (T)a.tabN3[(e3 * NG + g) * P1 + q];
+      d3 = (T)a.tabD3[(e3 * NG + g) * P1 + q];
+    }
+    tN2[m][g][q] = n2;
+    tD2[m][g][q] = d2;
+    tN3[m][g][q] = n3;
+    tD3[m][g][q] = d3;
+  }
+  if (tid < P1) {
+    const int e2 = jc - P + tid, e3 = kc - P + tid;
+    sp2[tid] = (e2 >= 0 && e2 < a.nel2) ? a.span2[e2] : 0.0;
+    sp3[tid] = (e3 >= 0 && e3 < a.nel3) ? a.span3[e3] : 0.0;
+  }
+  for (int t = tid; t < P1 * NCR * WN * NC; t += kBlock) (&win[0][0][0][0])[t] = T(0);
+  for (int t = tid; t < P1 * NC; t += kBlock) (&rwin[0][0])[t] = T(0);
+  __syncthreads();
+  const int nb = nbundle;
+
+  const T lam = (T)a.lam, mu = (T)a.mu;
+
+  for (int e1 = 0; e1 < a.nel1; e1++) {
+    // ---- stage control points and direction-1 tables of this layer -------
+    for (int t = tid; t < P1 * W1 * W1; t += kBlock) {
+      const int b1 = t % P1, r = t / P1;
+      const int jj = r % W1, kk = r / W1;
+      const int i = e1 + b1, j = jc - P + jj, k = kc - P + kk;
+      double x = 0, y = 0, z = 0;
+      if (j >= 0 && j < a.n2 && k >= 0 && k < a.n3) {
+        const int64_t node = (int64_t)i + (int64_t)a.n1 * (j + (int64_t)a.n2 * k);
+        x = a.coefs[3 * node];
+        y = a.coefs[3 * node + 1];
+        z = a.coefs[3 * node + 2];
+      }
+      cp[b1][jj][kk][0] = x;
+      cp[b1][jj][kk][1] = y;
+      cp[b1][jj][kk][2] = z;
+    }
+    for (int t = tid; t < NG * P1; t += kBlock) {
+      (&tN1[0][0])[t] = (T)a.tabN1[e1 * NG * P1 + t];
+      (&tD1[0][0])[t] = (T)a.tabD1[e1 * NG * P1 + t];
+    }
+    if (tid == 0) sp1 = a.span1[e1];
+    __syncthreads();
+
+    for (int c0 = 0; c0 < nb; c0 += CH) {
+      const int nch = (nb - c0) < CH ? (nb - c0) : CH;
+
+      // ---- phase A: geometry per (element, Gauss point) -------------------
+      for (int t = tid; t < nch * GP3; t += kBlock) {
+        const int le = t / GP3, q = t % GP3;
+        const int e2 = be2[c0 + le], e3 = be3[c0 + le];
+        const int m2 = e2 - jc + P, m3 = e3 - kc + P; // table slots
+        const int q1 = q % NG, q2 = (q / NG) % NG, q3 = q / (NG * NG);
+        T J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        T X[3] = {0, 0, 0};
+#pragma unroll
+        for (int b3 = 0; b3 < P1; b3++)
+#pragma unroll
+          for (int b2 = 0; b2 < P1; b2++)
+#pragma unroll
+            for (int b1 = 0; b1 < P1; b1++) {
+              const T n1 = tN1[q1][b1], d1 = tD1[q1][b1];
+              const T n2 = tN2[m2][q2][b2], d2 = tD2[m2][q2][b2];
+              const T n3 = tN3[m3][q3][b3], d3 = tD3[m3][q3][b3];
+              const T g0 = d1 * n2 * n3, g1 = n1 * d2 * n3, g2 = n1 * n2 * d3;
+              const T nv = n1 * n2 * n3;
+              const double* xb = cp[b1][e2 + b2 - jc + P][e3 + b3 - kc + P];
+#pragma unroll
+              for (int i = 0; i < 3; i++) {
+                const T xi = (T)xb[i];
+                J[3 * i + 0] += xi * g0;
+                J[3 * i + 1] += xi * g1;
+                J[3 * i + 2] += xi * g2;
+                X[i] += xi * nv;
+              }
+            }
+        // matrix::det / matrix::inverse (src/LinAlg/matrix.h:791-862)
+        const T det = J[0] * (J[4] * J[8] - J[7] * J[5]) - J[1] * (J[3] * J[8] - J[6] * J[5]) +
+                      J[2] * (J[3] * J[7] - J[6] * J[4]);
+        T Ji[9];
+        T wdet;
+        if (det <= T(1e-12) && det >= T(-1e-12)) {
+#pragma unroll
+          for (int m = 0; m < 9; m++) Ji[m] = 0;
+          wdet = 0;
+        } else {
+          Ji[0] = (J[4] * J[8] - J[7] * J[5]) / det;
+          Ji[3] = -(J[3] * J[8] - J[6] * J[5]) / det;
+          Ji[6] = (J[3] * J[7] - J[6] * J[4]) / det;
+          Ji[1] = -(J[1] * J[8] - J[7] * J[2]) / det;
+          Ji[4] = (J[0] * J[8] - J[6] * J[2]) / det;
+          Ji[7] = -(J[0] * J[7] - J[6] * J[1]) / det;
+          Ji[2] = (J[1] * J[5] - J[4] * J[2]) / det;
+          Ji[5] = -(J[0] * J[5] - J[3] * J[2]) / det;
+          Ji[8] = (J[0] * J[4] - J[3] * J[1]) / det;
+          const double dV = 0.125 * (sp1 * sp2[m2] * sp3[m3]);
+          wdet = det * (T)(dV * a.gw[q1] * a.gw[q2] * a.gw[q3]);
+        }
+        T coef = T(1);
+        if (NC == 1 && a.kappa.type != IFEM_GPU_FN_ZERO)
+          coef = (T)eval_field(a.kappa, (double)X[0], (double)X[1], (double)X[2]);
+        if (NC == 3) {
+#pragma unroll
+          for (int m = 0; m < 9; m++) geo[t][m] = Ji[m];
+        }
+        const int a2r = jc - e2, a3r = kc - e3; // row node's local (a2, a3)
+        const T n2a = tN2[m2][q2][a2r], d2a = tD2[m2][q2][a2r];
+        const T n3a = tN3[m3][q3][a3r], d3a = tD3[m3][q3][a3r];
+        T fval = T(0);
+        if (HAS_RHS && NC == 1) fval = (T)eval_field(a.source, (double)X[0], (double)X[1], (double)X[2]);
+#pragma unroll
+        for (int a1 = 0; a1 < P1; a1++) {
+          const T n1 = tN1[q1][a1], d1 = tD1[q1][a1];
+          const T r0 = d1 * n2a * n3a, r1 = n1 * d2a * n3a, r2 = n1 * n2a * d3a;
+          // g_a = J^-T grad_ref N_a  (dNdX = dNdu * J^-1)
+          const T ga0 = r0 * Ji[0] + r1 * Ji[3] + r2 * Ji[6];
+          const T ga1 = r0 * Ji[1] + r1 * Ji[4] + r2 * Ji[7];
+          const T ga2 = r0 * Ji[2] + r1 * Ji[5] + r2 * Ji[8];
+          const T s = wdet * coef;
+          if (NC == 1) {
+            // h_a = J^-1 (w kappa g_a): S_ab = grad_ref N_b . h_a
+            const T w0 = s * ga0, w1 = s * ga1, w2 = s * ga2;
+            geo[t][3 * a1 + 0] = Ji[0] * w0 + Ji[1] * w1 + Ji[2] * w2;
+            geo[t][3 * a1 + 1] = Ji[3] * w0 + Ji[4] * w1 + Ji[5] * w2;
+            geo[t][3 * a1 + 2] = Ji[6] * w0 + Ji[7] * w1 + Ji[8] * w2;
+          } else {
+            geo[t][9 + 3 * a1 + 0] = s * ga0;
+            geo[t][9 + 3 * a1 + 1] = s * ga1;
+            geo[t][9 + 3 * a1 + 2] = s * ga2;
+          }
+          if (HAS_RHS) {
+            const T Na = n1 * n2a * n3a;
+            if (NC == 1) {
+              rstage[t][a1] = (fval * wdet) * Na;
+            } else {
+#pragma unroll
+              for (int c = 0; c < NC; c++) rstage[t][a1 * NC + c] = wdet * (T)a.body[c] * Na;
+            }
+          }
+        }
+      }
+      __syncthreads();
+
+      // ---- RHS: fixed-order reduction over (element, GP) -------------------
+      if (HAS_RHS) {
+        const int v = tid >> 4, l = tid & 15;
+        T s = T(0);
+        if (v < P1 * NC)
+          for (int pr = l; pr < nch * GP3; pr += 16) s += rstage[pr][v];
+#pragma unroll
+        for (int off = 8; off >= 1; off >>= 1) s += __shfl_xor(s, off, 16);
+        if (v < P1 * NC && l == 0) {
+          const int a1 = v / NC, c = v % NC;
+          rwin[(e1 + a1) % P1][c] += s;
+        }
+      }
+
+      // ---- phase B: element-row x column-node products ----------------------
+      for (int t = tid; t < nch * NEN; t += kBlock) {
+        const int le = t / NEN, b = t % NEN;
+        const int b1 = b % P1, b2 = (b / P1) % P1, b3 = b / (P1 * P1);
+        const int e2 = be2[c0 + le], e3 = be3[c0 + le];
+        const int m2 = e2 - jc + P, m3 = e3 - kc + P;
+        T acc[P1][NS];
+#pragma unroll
+        for (int a1 = 0; a1 < P1; a1++)
+#pragma unroll
+          for (int s = 0; s < NS; s++) acc[a1][s] = 0;
+        for (int q3 = 0; q3 < NG; q3++) {
+          const T n3 = tN3[m3][q3][b3], d3 = tD3[m3][q3][b3];
+          for (int q2 = 0; q2 < NG; q2++) {
+            const T n2 = tN2[m2][q2][b2], d2 = tD2[m2][q2][b2];
+            const T n23 = n2 * n3, d23 = d2 * n3, n2d3 = n2 * d3;
+#pragma unroll
+            for (int q1 = 0; q1 < NG; q1++) {
+              const int pr = le * GP3 + (q3 * NG + q2) * NG + q1;
+              const T n1 = tN1[q1][b1], d1 = tD1[q1][b1];
+              const T r0 = d1 * n23, r1 = n1 * d23, r2 = n1 * n2d3;
+              const T* G = geo[pr];
+              if (NC == 1) {
+#pragma unroll
+                for (int a1 = 0; a1 < P1; a1++)
+                  acc[a1][0] += r0 * G[3 * a1] + r1 * G[3 * a1 + 1] + r2 * G[3 * a1 + 2];
+              } else {
+                const T gb0 = r0 * G[0] + r1 * G[3] + r2 * G[6];
+                const T gb1 = r0 * G[1] + r1 * G[4] + r2 * G[7];
+                const T gb2 = r0 * G[2] + r1 * G[5] + r2 * G[8];
+#pragma unroll
+                for (int a1 = 0; a1 < P1; a1++) {
+                  const T wa0 = G[9 + 3 * a1], wa1 = G[10 + 3 * a1], wa2 = G[11 + 3 * a1];
+                  if (SACC) {
+                    acc[a1][0] += wa0 * gb0; acc[a1][1] += wa0 * gb1; acc[a1][2] += wa0 * gb2;
+                    acc[a1][3] += wa1 * gb0; acc[a1][4] += wa1 * gb1; acc[a1][5] += wa1 * gb2;
+                    acc[a1][6] += wa2 * gb0; acc[a1][7] += wa2 * gb1; acc[a1][8] += wa2 * gb2;
+                  } else {
+                    const T wa[3] = {wa0, wa1, wa2}, gb[3] = {gb0, gb1, gb2};
+                    const int ic = split * NCR;
+                    const T dot = wa0 * gb0 + wa1 * gb1 + wa2 * gb2;
+#pragma unroll
+                    for (int jj = 0; jj < 3; jj++)
+                      acc[a1][jj] += lam * wa[ic] * gb[jj] + mu * wa[jj] * gb[ic] +
+                                     (jj == ic ? mu * dot : T(0));
+                  }
+                }
+              }
+            }
+          }
+        }
+#pragma unroll
+        for (int a1 = 0; a1 < P1; a1++)
+#pragma unroll
+          for (int s = 0; s < NS; s++) stage[le][b][a1][s] = acc[a1][s];
+      }
+      __syncthreads();
+
+      // ---- phase C: ordered reduction of the chunk into the row window ------
+      for (int t = tid; t < P1 * P1 * W1 * W1; t += kBlock) {
+        const int a1 = t % P1, r = t / P1;
+        const int b1 = r % P1, r2 = r / P1;
+        const int cjr = r2 % W1, ckr = r2 / W1;
+        const int cj = jc - P + cjr, ck = kc - P + ckr;
+        T s[NS];
+#pragma unroll
+        for (int m = 0; m < NS; m++) s[m] = 0;
+        bool any = false;
+        for (int le = 0; le < nch; le++) {
+          const int b2 = cj - be2[c0 + le], b3 = ck - be3[c0 + le];
+          if (b2 < 0 || b2 > P || b3 < 0 || b3 > P) continue;
+          any = true;
+          const int b = b1 + P1 * (b2 + P1 * b3);
+#pragma unroll
+          for (int m = 0; m < NS; m++) s[m] += stage[le][b][a1][m];
+        }
+        if (!any) continue;
+        const int slot = (e1 + a1) % P1;
+        const int di = b1 - a1, dj = cj - jc, dk = ck - kc;
+        const int nbr = ((dk + P) * W1 + (dj + P)) * W1 + (di + P);
+        if (NC == 1) {
+          win[slot][0][nbr][0] += s[0];
+        } else if (SACC) {
+          const T tr = s[0] + s[4] + s[8];
+#pragma unroll
+          for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+              win[slot][i][nbr][j] += lam * s[3 * i + j] + mu * s[3 * j + i] + (i == j ? mu * tr : T(0));
+        } else {
+#pragma unroll
+          for (int j = 0; j < 3; j++) win[slot][0][nbr][j] += s[j];
+        }
+      }
+      __syncthreads();
+    }
+
+    // ---- phase D: rows that are complete after layer e1 ---------------------
+    const int ilast = (e1 == a.nel1 - 1) ? e1 + P : e1;
+    for (int i = e1; i <= ilast; i++) {
+      const int slot = i % P1;
+      const int64_t node = (int64_t)i + (int64_t)a.n1 * (jc + (int64_t)a.n2 * kc);
+      const int wave = tid >> 6, lane = tid & 63;
+      if (wave < NCR) {
+        const int ic = split * NCR + wave;
+        const int req = a.eq[node * NC + ic];
+        if (req >= 0) {
+          const int64_t base = a.rowptr[req];
+          int64_t pos = 0;
+          for (int t0 = 0; t0 < WN * NC; t0 += 64) {
+            const int t = t0 + lane;
+            bool valid = false;
+            if (t < WN * NC) {
+              const int nbr = t / NC, cc = t % NC;
+              const int di = nbr % W1 - P, dj = (nbr / W1) % W1 - P, dk = nbr / (W1 * W1) - P;
+              const int ii = i + di, jj = jc + dj, kk = kc + dk;
+              if (ii >= 0 && ii < a.n1 && jj >= 0 && jj < a.n2 && kk >= 0 && kk < a.n3) {
+                const int64_t nn = (int64_t)ii + (int64_t)a.n1 * (jj + (int64_t)a.n2 * kk);
+                valid = a.eq[nn * NC + cc] >= 0;
+              }
+            }
+            const unsigned long long m = __ballot(valid);
+            if (valid) {
+              const int64_t at = base + pos + __popcll(m & ((1ull << lane) - 1ull));
+              const T v = win[slot][wave][t / NC][t % NC];
+              if (a.val32) a.val32[at] = (float)v;
+              else a.val64[at] = (double)v;
+            }
+            pos += __popcll(m);
+          }
+          if (HAS_RHS && a.rhs && lane == 0) a.rhs[req] = (double)rwin[slot][ic];
+        }
+      }
+      __syncthreads();
+      for (int t = tid; t < NCR * WN * NC; t += kBlock) (&win[slot][0][0][0])[t] = T(0);
+      if (tid < NC) rwin[slot][tid] = T(0);
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Symbolic pattern: row lengths, scan, columns.  Two nodes couple when their
+// index boxes are within p in every direction (open knots, simple interior
+// knots); eliminated DOFs are dropped.  Column order = DOF order =
+// equation order (the SAM numbering is monotone), i.e. sorted.
+__global__ void k_row_lengths(int n1, int n2, int n3, int p, int nc, const int* __restrict__ eq,
+                              int64_t* __restrict__ rowlen, int ndof)
+{
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndof) return;
+  const int r = eq[d];
+  if (r < 0) return;
+  const int node = d / nc;
+  const int i = node % n1, j = (node / n1) % n2, k = node / (n1 * n2);
+  int64_t cnt = 0;
+  for (int kk = max(0, k - p); kk <= min(n3 - 1, k + p); kk++)
+    for (int jj = max(0, j - p); jj <= min(n2 - 1, j + p); jj++)
+      for (int ii = max(0, i - p); ii <= min(n1 - 1, i + p); ii++) {
+        const int64_t nn = (int64_t)ii + (int64_t)n1 * (jj + (int64_t)n2 * kk);
+        for (int c = 0; c < nc; c++) cnt += eq[nn * nc + c] >= 0;
+      }
+  rowlen[r] = cnt;
+}
+
+// One wave per row DOF; lanes write consecutive columns (ballot compaction).
+__global__ void k_fill_columns(int n1, int n2, int n3, int p, int nc, const int* __restrict__ eq,
+                               const int64_t* __restrict__ rowptr, int32_t* __restrict__ colind,
+                               int ndof)
+{
+  const int d = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (d >= ndof) return;
+  const int r = eq[d];
+  if (r < 0) return;
+  const int node = d / nc;
+  const int i = node % n1, j = (node / n1) % n2, k = node / (n1 * n2);
+  const int W1 = 2 * p + 1;
+  const int total = W1 * W1 * W1 * nc;
+  int64_t pos = rowptr[r];
+  for (int t0 = 0; t0 < total; t0 += 64) {
+    const int t = t0 + lane;
+    bool valid = false;
+    int col = 0;
+    if (t < total) {
+      const int nbr = t / nc, cc = t % nc;
+      const int ii = i + nbr % W1 - p, jj = j + (nbr / W1) % W1 - p, kk = k + nbr / (W1 * W1) - p;
+      if (ii >= 0 && ii < n1 && jj >= 0 && jj < n2 && kk >= 0 && kk < n3) {
+        const int64_t nn = (int64_t)ii + (int64_t)n1 * (jj + (int64_t)n2 * kk);
+        col = eq[nn * nc + cc];
+        valid = col >= 0;
+      }
+    }
+    const unsigned long long m = __ballot(valid);
+    if (valid) colind[pos + __popcll(m & ((1ull << lane) - 1ull))] = col;
+    pos += __popcll(m);
+  }
+}
+
+// Exclusive scan of int64 (block-local scan, scan of block sums, add).
+constexpr int kScanBlock = 1024;
+
+__global__ void k_scan_local(const int64_t* __restrict__ in, int64_t* __restrict__ out,
+                             int64_t* __restrict__ bsum, int n)
+{
+  __shared__ int64_t s[kScanBlock];
+  const int i = blockIdx.x * kScanBlock + threadIdx.x;
+  s[threadIdx.x] = i < n ? in[i] : 0;
+  __syncthreads();
+  for (int off = 1; off < kScanBlock; off <<= 1) {
+    int64_t v = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += v;
+    __syncthreads();
+  }
+  if (i < n) out[i + 1] = s[threadIdx.x]; // inclusive into out[1..]
+  if (threadIdx.x == kScanBlock - 1) bsum[blockIdx.x] = s[threadIdx.x];
+}
+
+__global__ void k_scan_sums(int64_t* __restrict__ bsum, int nb)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    int64_t run = 0;
+    for (int b = 0; b < nb; b++) {
+      const int64_t v = bsum[b];
+      bsum[b] = run;
+      run += v;
+    }
+  }
+}
+
+__global__ void k_scan_add(int64_t* __restrict__ out, const int64_t* __restrict__ bsum, int n)
+{
+  const int i = blockIdx.x * kScanBlock + threadIdx.x;
+  if (i < n) out[i + 1] += bsum[blockIdx.x];
+  if (i == 0) out[0] = 0;
+}
+
+int build_pattern(Context& c)
+{
+  const PatchRec& P = c.patches[c.patch];
+  if (P.p[0] != P.p[1] || P.p[0] != P.p[2])
+    return fail(IFEM_GPU_EUNSUPPORTED, "init_assembly: the GPU path needs equal degrees in all directions");
+  const int p = P.p[0];
+  hipFree(c.d_rowptr);
+  hipFree(c.d_colind);
+  hipFree(c.d_val);
+  hipFree(c.d_val32);
+  hipFree(c.d_rhs);
+  c.d_rowptr = nullptr;
+  c.d_colind = nullptr;
+  c.d_val = nullptr;
+  c.d_val32 = nullptr;
+  c.d_rhs = nullptr;
+  c.fp64_ready = c.fp32_ready = false;
+  const int neq = c.neq;
+  int64_t* rowlen = nullptr;
+  int64_t* bsum = nullptr;
+  const int nb = (neq + kScanBlock - 1) / kScanBlock;
+  IFEM_HIP(hipMalloc(&c.d_rowptr, sizeof(int64_t) * (neq + 1)));
+  IFEM_HIP(hipMalloc(&rowlen, sizeof(int64_t) * (neq > 0 ? neq : 1)));
+  IFEM_HIP(hipMalloc(&bsum, sizeof(int64_t) * (nb > 0 ? nb : 1)));
+  IFEM_HIP(hipMemsetAsync(c.d_rowptr, 0, sizeof(int64_t) * (neq + 1), c.stream));
+  if (neq > 0) {
+    k_row_lengths<<<(c.ndof + 255) / 256, 256, 0, c.stream>>>(P.n[0], P.n[1], P.n[2], p, c.ncomp,
+                                                              c.d_eq, rowlen, c.ndof);
+    k_scan_local<<<nb, kScanBlock, 0, c.stream>>>(rowlen, c.d_rowptr, bsum, neq);
+    k_scan_sums<<<1, 64, 0, c.stream>>>(bsum, nb);
+    k_scan_add<<<nb, kScanBlock, 0, c.stream>>>(c.d_rowptr, bsum, neq);
+  }
+  IFEM_HIP(hipGetLastError());
+  int64_t nnz = 0;
+  IFEM_HIP(hipMemcpyAsync(&nnz, c.d_rowptr + neq, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+  IFEM_HIP(hipStreamSynchronize(c.stream));
+  hipFree(rowlen);
+  hipFree(bsum);
+  c.nnz = nnz;
+  IFEM_HIP(hipMalloc(&c.d_colind, sizeof(int32_t) * (nnz > 0 ? nnz : 1)));
+  if (neq > 0) {
+    const int waves = 4;
+    k_fill_columns<<<(c.ndof + waves - 1) / waves, 64 * waves, 0, c.stream>>>(
+        P.n[0], P.n[1], P.n[2], p, c.ncomp, c.d_eq, c.d_rowptr, c.d_colind, c.ndof);
+  }
+  IFEM_HIP(hipGetLastError());
+  IFEM_HIP(hipMalloc(&c.d_val, sizeof(double) * (nnz > 0 ? nnz : 1)));
+  IFEM_HIP(hipMalloc(&c.d_rhs, sizeof(double) * (neq > 0 ? neq : 1)));
+  IFEM_HIP(hipMemsetAsync(c.d_rhs, 0, sizeof(double) * (neq > 0 ? neq : 1), c.stream));
+  IFEM_HIP(hipStreamSynchronize(c.stream));
+  c.pattern_ready = true;
+  return solver_setup(c);
+}
+
+// ---------------------------------------------------------------------------
+// Host dispatch of the numeric assembly.
+struct Tables {
+  double *tabN[3] = {nullptr, nullptr, nullptr}, *tabD[3] = {nullptr, nullptr, nullptr};
+  double* span[3] = {nullptr, nullptr, nullptr};
+  ~Tables()
+  {
+    for (int d = 0; d < 3; d++) {
+      hipFree(tabN[d]);
+      hipFree(tabD[d]);
+      hipFree(span[d]);
+    }
+  }
+};
+
+template <int P>
+static int make_tables(Context& c, const PatchRec& Pt, int ng, Tables& tb, GaussArgs& ga)
+{
+  for (int d = 0; d < 3; d++) {
+    const size_t cnt = (size_t)Pt.nel[d] * ng * (P + 1);
+    IFEM_HIP(hipMalloc(&tb.tabN[d], sizeof(double) * cnt));
+    IFEM_HIP(hipMalloc(&tb.tabD[d], sizeof(double) * cnt));
+    IFEM_HIP(hipMalloc(&tb.span[d], sizeof(double) * Pt.nel[d]));
+    const int tot = Pt.nel[d] * ng;
+    k_basis_tables<P><<<(tot + 127) / 128, 128, 0, c.stream>>>(Pt.d_knots[d], Pt.nel[d], ng, ga,
+                                                              tb.tabN[d], tb.tabD[d], tb.span[d]);
+  }
+  IFEM_HIP(hipGetLastError());
+  return IFEM_GPU_OK;
+}
+
+static int upload_function(Context& c, const ifem_gpu_function& f, DevFunction& df)
+{
+  df.type = f.type;
+  df.nmodes = f.nmodes;
+  df.scale = f.scale;
+  df.mean = f.mean;
+  df.modes = nullptr;
+  if (f.type < 0 || f.type > IFEM_GPU_FN_EXP_FOURIER)
+    return fail(IFEM_GPU_EINVAL, "assemble: unknown field type");
+  if ((f.type == IFEM_GPU_FN_FOURIER || f.type == IFEM_GPU_FN_EXP_FOURIER) && f.nmodes > 0) {
+    if (!f.modes) return fail(IFEM_GPU_EINVAL, "assemble: null mode table");
+    double* d = nullptr;
+    IFEM_HIP(hipMalloc(&d, sizeof(double) * 5 * f.nmodes));
+    IFEM_HIP(hipMemcpyAsync(d, f.modes, sizeof(double) * 5 * f.nmodes, hipMemcpyHostToDevice, c.stream));
+    c.scratch_tables.push_back(d);
+    df.modes = d;
+  } else {
+    df.nmodes = 0;
+  }
+  return IFEM_GPU_OK;
+}
+
+template <int P, int NG, int NC, typename T>
+static int launch_lines(Context& c, const AsmArgs& args, bool has_rhs)
+{
+  constexpr int SPLIT = (NC == 3 && P >= 3) ? 3 : 1;
+  dim3 grid(args.n2 * args.n3, SPLIT);
+  if (has_rhs)
+    k_assemble_lines<P, NG, NC, T, SPLIT, true><<<grid, kBlock, 0, c.stream>>>(args);
+  else
+    k_assemble_lines<P, NG, NC, T, SPLIT, false><<<grid, kBlock, 0, c.stream>>>(args);
+  IFEM_HIP(hipGetLastError());
+  return IFEM_GPU_OK;
+}
+
+template <int P, int NG>
+static int dispatch_nc(Context& c, const AsmArgs& args, int nc, bool fp32, bool has_rhs)
+{
+  if (nc == 1) return fp32 ? launch_lines<P, NG, 1, float>(c, args, has_rhs)
+                           : launch_lines<P, NG, 1, double>(c, args, has_rhs);
+  return fp32 ? launch_lines<P, NG, 3, float>(c, args, has_rhs)
+              : launch_lines<P, NG, 3, double>(c, args, has_rhs);
+}
+
+template <int P>
+static int dispatch_ng(Context& c, const AsmArgs& args, int ng, int nc, bool fp32, bool has_rhs)
+{
+  if (ng == P + 1) return dispatch_nc<P, P + 1>(c, args, nc, fp32, has_rhs);
+  if (ng == P + 2) return dispatch_nc<P, P + 2>(c, args, nc, fp32, has_rhs);
+  return fail(IFEM_GPU_EUNSUPPORTED, "assemble: the GPU path supports ng = p+1 or p+2");
+}
+
+template <int P>
+static int assemble_p(Context& c, const ifem_gpu_integrand& itg, int ng, int precision)
+{
+  const PatchRec& Pt = c.patches[c.patch];
+  GaussArgs ga{};
+  gauss_rule(ng, ga.xi, ga.w);
+  Tables tb;
+  int rc = make_tables<P>(c, Pt, ng, tb, ga);
+  if (rc) return rc;
+  AsmArgs a{};
+  a.n1 = Pt.n[0];
+  a.n2 = Pt.n[1];
+  a.n3 = Pt.n[2];
+  a.nel1 = Pt.nel[0];
+  a.nel2 = Pt.nel[1];
+  a.nel3 = Pt.nel[2];
+  a.coefs = Pt.d_coefs;
+  a.tabN1 = tb.tabN[0]; a.tabD1 = tb.tabD[0]; a.span1 = tb.span[0];
+  a.tabN2 = tb.tabN[1]; a.tabD2 = tb.tabD[1]; a.span2 = tb.span[1];
+  a.tabN3 = tb.tabN[2]; a.tabD3 = tb.tabD[2]; a.span3 = tb.span[2];
+  for (int g = 0; g < kMaxNG; g++) a.gw[g] = ga.w[g];
+  a.eq = c.d_eq;
+  a.rowptr = c.d_rowptr;
+  const bool fp32 = precision == IFEM_GPU_FP32;
+  if (fp32) {
+    if (!c.d_val32) IFEM_HIP(hipMalloc(&c.d_val32, sizeof(float) * (c.nnz > 0 ? c.nnz : 1)));
+    a.val32 = c.d_val32;
+    a.val64 = nullptr;
+    a.rhs = nullptr;
+  } else {
+    a.val64 = c.d_val;
+    a.val32 = nullptr;
+    a.rhs = c.d_rhs;
+  }
+  for (void* p : c.scratch_tables) hipFree(p);
+  c.scratch_tables.clear();
+  rc = upload_function(c, itg.kappa, a.kappa);
+  if (!rc) rc = upload_function(c, itg.source, a.source);
+  if (rc) return rc;
+  for (int i = 0; i < 3; i++) a.body[i] = itg.body[i];
+  if (itg.kind == IFEM_GPU_ELASTICITY) {
+    a.lam = itg.E * itg.nu / ((1.0 + itg.nu) * (1.0 - 2.0 * itg.nu));
+    a.mu = itg.E / (2.0 * (1.0 + itg.nu));
+  }
+  bool has_rhs = false;
+  if (!fp32) {
+    if (c.ncomp == 1) has_rhs = itg.source.type != IFEM_GPU_FN_ZERO;
+    else has_rhs = itg.body[0] != 0.0 || itg.body[1] != 0.0 || itg.body[2] != 0.0;
+    IFEM_HIP(hipMemsetAsync(c.d_rhs, 0, sizeof(double) * (c.neq > 0 ? c.neq : 1), c.stream));
+  }
+  IFEM_HIP(hipEventRecord(c.ev0, c.stream));
+  rc = dispatch_ng<P>(c, a, ng, c.ncomp, fp32, has_rhs);
+  if (rc) return rc;
+  IFEM_HIP(hipEventRecord(c.ev1, c.stream));
+  IFEM_HIP(hipEventSynchronize(c.ev1));
+  float ms = 0.f;
+  IFEM_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  c.last_assemble_ms = ms;
+  if (fp32) c.fp32_ready = true;
+  else c.fp64_ready = true;
+  return IFEM_GPU_OK;
+}
+
+int assemble_volume(Context& c, const ifem_gpu_integrand& itg, const int ng[3], int precision)
+{
+  const PatchRec& Pt = c.patches[c.patch];
+  if (ng[0] != ng[1] || ng[0] != ng[2])
+    return fail(IFEM_GPU_EUNSUPPORTED, "assemble: the GPU path needs equal Gauss counts per direction");
+  if (ng[0] < 1 || ng[0] > kMaxNG) return fail(IFEM_GPU_EUNSUPPORTED, "assemble: ng out of range");
+  switch (Pt.p[0]) {
+  case 1: return assemble_p<1>(c, itg, ng[0], precision);
+  case 2: return assemble_p<2>(c, itg, ng[0], precision);
+  case 3: return assemble_p<3>(c, itg, ng[0], precision);
+  default: return fail(IFEM_GPU_EUNSUPPORTED, "assemble: degree out of range");
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Neumann traction on one face: one thread per (face node, component); sums
+// over the face elements holding the node and their ng x ng Gauss points in
+// fixed order (src/ASM/ASMs3D.C:2547-2812; face utl::Jacobian,
+// src/Utility/CoordinateMapping.C:90-133: dS = |J_t1 x J_t2|, zero if J is
+// singular; detJxW = dS * dA * w_i * w_j with dA = 0.25 * area).
+template <int P>
+__global__ void k_traction(int n1, int n2, int n3, int nel1, int nel2, int nel3,
+                           const double* __restrict__ coefs, const double* k1,
+                           const double* k2, const double* k3, int face, int ng, GaussArgs ga,
+                           double t0, double t1, double t2, int nc, const int* __restrict__ eq,
+                           double* __restrict__ rhs)
+{
+  constexpr int P1 = P + 1;
+  const int dn = face / 2, side = face % 2;
+  const int n[3] = {n1, n2, n3}, nel[3] = {nel1, nel2, nel3};
+  const double* kn[3] = {k1, k2, k3};
+  const int ta = dn == 0 ? 1 : 0, tb = dn == 2 ? 1 : 2; // tangent directions, ascending
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nf = n[ta] * n[tb];
+  if (id >= nf * nc) return;
+  const int comp = id % nc, fnode = id / nc;
+  const int ia = fnode % n[ta], ib = fnode / n[ta];
+  int idx[3];
+  idx[dn] = side ? n[dn] - 1 : 0;
+  idx[ta] = ia;
+  idx[tb] = ib;
+  const int64_t node = (int64_t)idx[0] + (int64_t)n1 * (idx[1] + (int64_t)n2 * idx[2]);
+  const int r = eq[node * nc + comp];
+  if (r < 0) return;
+  const double tv = comp == 0 ? t0 : (comp == 1 ? t1 : t2);
+  // faceDir = +-(dn+1): t1 = 1 + |faceDir| % 3, t2 = 1 + t1 % 3 (1-based)
+  const int T1 = (dn + 1) % 3, T2 = (T1 + 1) % 3;
+  const int en = side ? nel[dn] - 1 : 0;
+  double sum = 0.0;
+  for (int eb = max(0, ib - P); eb <= min(nel[tb] - 1, ib); eb++)
+    for (int ea = max(0, ia - P); ea <= min(nel[ta] - 1, ia); ea++) {
+      int e[3];
+      e[dn] = en;
+      e[ta] = ea;
+      e[tb] = eb;
+      double spn[3];
+      for (int d = 0; d < 3; d++) spn[d] = kn[d][e[d] + P + 1] - kn[d][e[d] + P];
+      const double dA = 0.25 * (spn[ta] * spn[tb]);
+      int loc[3];
+      for (int d = 0; d < 3; d++) loc[d] = idx[d] - e[d];
+      for (int gb = 0; gb < ng; gb++)
+        for (int gA = 0; gA < ng; gA++) {
+          double Nd[3][P1], dNd[3][P1];
+          for (int d = 0; d < 3; d++) {
+            const int s = e[d] + P;
+            const double* kt = kn[d];
+            double u;
+            if (d == dn) u = side ? kt[n[d]] : kt[P];
+            else {
+              const double xi = ga.xi[d == ta ? gA : gb];
+              u = 0.5 * ((kt[s + 1] - kt[s]) * xi + kt[s + 1] + kt[s]);
+            }
+            bspline_1d<P>(kt, s, u, Nd[d], dNd[d]);
+          }
+          double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+          for (int b3 = 0; b3 < P1; b3++)
+            for (int b2 = 0; b2 < P1; b2++)
+              for (int b1 = 0; b1 < P1; b1++) {
+                const int64_t nb = (int64_t)(e[0] + b1) + (int64_t)n1 * ((e[1] + b2) + (int64_t)n2 * (e[2] + b3));
+                const double g0 = dNd[0][b1] * Nd[1][b2] * Nd[2][b3];
+                const double g1 = Nd[0][b1] * dNd[1][b2] * Nd[2][b3];
+                const double g2 = Nd[0][b1] * Nd[1][b2] * dNd[2][b3];
+                for (int i = 0; i < 3; i++) {
+                  const double x = coefs[3 * nb + i];
+                  J[3 * i + 0] += x * g0;
+                  J[3 * i + 1] += x * g1;
+                  J[3 * i + 2] += x * g2;
+                }
+              }
+          const double v1[3] = {J[T1], J[3 + T1], J[6 + T1]};
+          const double v2[3] = {J[T2], J[3 + T2], J[6 + T2]};
+          const double nx = v1[1] * v2[2] - v1[2] * v2[1];
+          const double ny = v1[2] * v2[0] - v1[0] * v2[2];
+          const double nz = v1[0] * v2[1] - v1[1] * v2[0];
+          double dS = sqrt(nx * nx + ny * ny + nz * nz);
+          const double det = J[0] * (J[4] * J[8] - J[7] * J[5]) - J[1] * (J[3] * J[8] - J[6] * J[5]) +
+                             J[2] * (J[3] * J[7] - J[6] * J[4]);
+          if (det <= 1e-12 && det >= -1e-12) dS = 0.0;
+          if (dS == 0.0) continue;
+          const double w = dS * (dA * ga.w[gA] * ga.w[gb]);
+          const double Na = Nd[0][loc[0]] * Nd[1][loc[1]] * Nd[2][loc[2]];
+          sum += w * tv * Na;
+        }
+    }
+  rhs[r] += sum;
+}
+
+int assemble_traction(Context& c, int face0, const double* t, int ng)
+{
+  const PatchRec& Pt = c.patches[c.patch];
+  GaussArgs ga{};
+  gauss_rule(ng, ga.xi, ga.w);
+  const int dn = face0 / 2;
+  const int ta = dn == 0 ? 1 : 0, tb = dn == 2 ? 1 : 2;
+  const int tot = Pt.n[ta] * Pt.n[tb] * c.ncomp;
+  const double t1 = c.ncomp > 1 ? t[1] : 0.0, t2 = c.ncomp > 2 ? t[2] : 0.0;
+#define LAUNCH_TR(PP)                                                                          \
+  k_traction<PP><<<(tot + 127) / 128, 128, 0, c.stream>>>(                                      \
+      Pt.n[0], Pt.n[1], Pt.n[2], Pt.nel[0], Pt.nel[1], Pt.nel[2], Pt.d_coefs, Pt.d_knots[0],    \
+      Pt.d_knots[1], Pt.d_knots[2], face0, ng, ga, t[0], t1, t2, c.ncomp, c.d_eq, c.d_rhs)
+  switch (Pt.p[0]) {
+  case 1: LAUNCH_TR(1); break;
+  case 2: LAUNCH_TR(2); break;
+  case 3: LAUNCH_TR(3); break;
+  default: return fail(IFEM_GPU_EUNSUPPORTED, "traction: degree out of range");
+  }
+#undef LAUNCH_TR
+  IFEM_HIP(hipGetLastError());
+  IFEM_HIP(hipStreamSynchronize(c.stream));
+  return IFEM_GPU_OK;
+}
+
+} // namespace ifem

@@ -1,0 +1,104 @@
+// internal.hpp -- shared state and helpers of libifem_gpu.so (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ifem_gpu.h"
+
+namespace ifem {
+
+constexpr int kMaxP = 3;
+constexpr int kMaxNG = 5;
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define IFEM_HIP(call)                                                        \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      return ::ifem::fail(IFEM_GPU_EDEVICE, std::string(#call) + ": " +       \
+                                                hipGetErrorString(e_));       \
+  } while (0)
+
+// Device-side description of one scalar field (matches ifem_gpu_function).
+struct DevFunction {
+  int type = 0;
+  int nmodes = 0;
+  double scale = 0.0;
+  double mean = 0.0;
+  const double* modes = nullptr; // device, nmodes*5
+};
+
+struct PatchRec {
+  int p[3] = {0, 0, 0};
+  int n[3] = {0, 0, 0};
+  int nel[3] = {0, 0, 0};
+  double* d_knots[3] = {nullptr, nullptr, nullptr};
+  double* d_coefs = nullptr;
+  std::vector<double> knots[3]; // host copies
+};
+
+// Solver work vectors (equation order).
+struct SolverWork {
+  int n = 0;
+  double *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  double* dinv = nullptr;
+  float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
+  float* dinv32 = nullptr;
+  double* partial = nullptr; // per-block partial sums
+  double* scal = nullptr;    // device scalars
+  unsigned* counter = nullptr;
+  double* h_scal = nullptr;  // pinned host mirror
+  int nblocks = 0;
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::vector<PatchRec> patches;
+
+  // bound system
+  int patch = -1;
+  int ncomp = 0, ndof = 0, neq = 0;
+  int* d_eq = nullptr;          // per DOF: 0-based equation or -1
+  std::vector<int> h_eq;
+  int64_t nnz = 0;
+  int64_t* d_rowptr = nullptr;
+  int32_t* d_colind = nullptr;
+  double* d_val = nullptr;
+  float* d_val32 = nullptr;
+  double* d_rhs = nullptr;
+  bool pattern_ready = false;
+  bool fp64_ready = false, fp32_ready = false;
+
+  SolverWork w;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_assemble_ms = 0.0, last_solve_ms = 0.0;
+
+  // owned device buffers for integrand mode tables
+  std::vector<void*> scratch_tables;
+};
+
+// assemble.hip
+int build_pattern(Context& c);
+int assemble_volume(Context& c, const ifem_gpu_integrand& itg, const int ng[3],
+                    int precision);
+int assemble_traction(Context& c, int face0, const double* t, int ng);
+
+// solver.hip
+int solver_setup(Context& c);
+int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st);
+int spmv_device(Context& c, const double* d_x, double* d_y);
+void solver_free(Context& c);
+
+} // namespace ifem
+
+struct ifem_gpu_ctx {
+  ifem::Context c;
+};

@@ -1,0 +1,513 @@
+// solver.hip -- Jacobi-preconditioned CG on the device, FP64 and FP32-inner
+// with FP64 iterative refinement.
+//
+// Stands in for SystemMatrix::solve with PETSc KSPCG + PCJACOBI or dune
+// CGSolver + SeqJac (src/LinAlg/PETScMatrix.C:778-832, 980-990;
+// src/LinAlg/ISTLMatrix.C:169-196), with the LinSolParams convergence
+// contract (rtol, atol, maxits; src/LinAlg/LinSolParams.C:68-77).
+//
+// One iteration is three launches on a fixed grid:
+//   k_spmv_dot : q = A p, partial <p,q>;  last block: alpha = rz / <p,q>
+//   k_update   : x += alpha p, r -= alpha q, z = D^-1 r, partial <r,z>, <r,r>;
+//                last block: beta, convergence flag, iteration count
+//   k_pupdate  : p = z + beta p
+// Scalars stay on the device; the host reads them every check_every
+// iterations.  Reductions are two-level with fixed shapes (per-block partials
+// summed in index order by the last block to finish), so results are
+// bitwise reproducible run to run.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "internal.hpp"
+
+namespace ifem {
+
+constexpr int kThreads = 256;
+constexpr int kGrid = 2048; // fixed grid: partial sums have a fixed layout
+
+enum Scal {
+  S_RZ = 0, S_ALPHA, S_BETA, S_RR, S_DONE, S_TOL2, S_ITER, S_PQ, S_RZN, S_MAXIT, S_BB, S_NSCAL
+};
+
+template <typename A>
+__device__ inline A block_sum(A v, A* sh)
+{
+  // fixed-shape reduction: wave butterflies, then wave totals in order
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  A s = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kThreads / 64; w++) s += sh[w];
+  __syncthreads();
+  return s; // valid in thread 0
+}
+
+__device__ inline double load_coherent(const double* p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum of nparts partials in index order by one block (deterministic).
+__device__ inline double final_sum(const double* part, int nparts, double* sh)
+{
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += kThreads) s += load_coherent(part + i);
+  return block_sum<double>(s, sh);
+}
+
+__device__ inline bool last_block(unsigned* counter)
+{
+  __shared__ bool am_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned t = atomicAdd(counter, 1u);
+    am_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  return am_last;
+}
+
+// ---------------------------------------------------------------------------
+// SpMV q = A p (CSR), LANES lanes per row, plus partial <p,q>.
+template <typename V, typename ACC, int LANES, bool DOT>
+__global__ __launch_bounds__(kThreads) void k_spmv_dot(
+    int n, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
+    const V* __restrict__ val, const V* __restrict__ p, V* __restrict__ q,
+    double* __restrict__ part, unsigned* counter, double* __restrict__ scal)
+{
+  if (DOT && scal[S_DONE] != 0.0) return;
+  __shared__ double sh[kThreads / 64];
+  const int groups_per_block = kThreads / LANES;
+  const int g = threadIdx.x / LANES, l = threadIdx.x % LANES;
+  double pq = 0.0;
+  for (int row = blockIdx.x * groups_per_block + g; row < n; row += gridDim.x * groups_per_block) {
+    const int64_t b = rowptr[row], e = rowptr[row + 1];
+    ACC s = 0;
+    for (int64_t k = b + l; k < e; k += LANES) s += (ACC)val[k] * (ACC)p[colind[k]];
+#pragma unroll
+    for (int off = LANES / 2; off >= 1; off >>= 1) s += __shfl_xor(s, off, LANES);
+    if (l == 0) {
+      q[row] = (V)s;
+      if (DOT) pq += (double)((ACC)p[row] * s);
+    }
+  }
+  if (!DOT) return;
+  const double bs = block_sum<double>(pq, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+  if (last_block(counter)) {
+    const double tot = final_sum(part, gridDim.x, sh);
+    if (threadIdx.x == 0) {
+      scal[S_PQ] = tot;
+      if (tot != 0.0 && __builtin_isfinite(tot)) {
+        scal[S_ALPHA] = scal[S_RZ] / tot;
+      } else {
+        scal[S_ALPHA] = 0.0;
+        scal[S_DONE] = 2.0; // breakdown
+      }
+      *counter = 0u;
+    }
+  }
+}
+
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_update(int n, V* __restrict__ x, V* __restrict__ r,
+                                                     V* __restrict__ z, const V* __restrict__ p,
+                                                     const V* __restrict__ q,
+                                                     const V* __restrict__ dinv,
+                                                     double* __restrict__ part, unsigned* counter,
+                                                     double* __restrict__ scal)
+{
+  if (scal[S_DONE] != 0.0) return;
+  __shared__ double sh[kThreads / 64];
+  const V alpha = (V)scal[S_ALPHA];
+  double rz = 0.0, rr = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    const V xi = x[i] + alpha * p[i];
+    const V ri = r[i] - alpha * q[i];
+    const V zi = dinv[i] * ri;
+    x[i] = xi;
+    r[i] = ri;
+    z[i] = zi;
+    rz += (double)(ri * zi);
+    rr += (double)(ri * ri);
+  }
+  const double brz = block_sum<double>(rz, sh);
+  const double brr = block_sum<double>(rr, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = brz;
+    part[gridDim.x + blockIdx.x] = brr;
+  }
+  if (last_block(counter)) {
+    const double trz = final_sum(part, gridDim.x, sh);
+    const double trr = final_sum(part + gridDim.x, gridDim.x, sh);
+    if (threadIdx.x == 0) {
+      scal[S_ITER] += 1.0;
+      scal[S_RR] = trr;
+      scal[S_BETA] = scal[S_RZ] != 0.0 ? trz / scal[S_RZ] : 0.0;
+      scal[S_RZ] = trz;
+      if (!(trr > scal[S_TOL2])) scal[S_DONE] = 1.0;       // converged (or NaN)
+      else if (scal[S_ITER] >= scal[S_MAXIT]) scal[S_DONE] = 3.0;
+      *counter = 0u;
+    }
+  }
+}
+
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_pupdate(int n, V* __restrict__ p,
+                                                      const V* __restrict__ z,
+                                                      const double* __restrict__ scal)
+{
+  if (scal[S_DONE] != 0.0) return;
+  const V beta = (V)scal[S_BETA];
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
+    p[i] = z[i] + beta * p[i];
+}
+
+// CG start from residual r (already in place): z = D^-1 r, p = z, rz, rr.
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_start(int n, const V* __restrict__ r,
+                                                    V* __restrict__ z, V* __restrict__ p,
+                                                    const V* __restrict__ dinv,
+                                                    double* __restrict__ part,
+                                                    unsigned* counter, double* __restrict__ scal)
+{
+  __shared__ double sh[kThreads / 64];
+  double rz = 0.0, rr = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    const V ri = r[i];
+    const V zi = dinv[i] * ri;
+    z[i] = zi;
+    p[i] = zi;
+    rz += (double)(ri * zi);
+    rr += (double)(ri * ri);
+  }
+  const double brz = block_sum<double>(rz, sh);
+  const double brr = block_sum<double>(rr, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = brz;
+    part[gridDim.x + blockIdx.x] = brr;
+  }
+  if (last_block(counter)) {
+    const double trz = final_sum(part, gridDim.x, sh);
+    const double trr = final_sum(part + gridDim.x, gridDim.x, sh);
+    if (threadIdx.x == 0) {
+      scal[S_RZ] = trz;
+      scal[S_RR] = trr;
+      scal[S_DONE] = (trr > scal[S_TOL2]) ? 0.0 : 1.0;
+      if (scal[S_ITER] >= scal[S_MAXIT]) scal[S_DONE] = 3.0;
+      *counter = 0u;
+    }
+  }
+}
+
+// r = b - y (FP64) and ||r||^2 into scal[slot].
+__global__ __launch_bounds__(kThreads) void k_residual(int n, const double* __restrict__ b,
+                                                       const double* __restrict__ y,
+                                                       double* __restrict__ r,
+                                                       double* __restrict__ part,
+                                                       unsigned* counter, double* __restrict__ scal,
+                                                       int slot)
+{
+  __shared__ double sh[kThreads / 64];
+  double rr = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    const double ri = b[i] - y[i];
+    r[i] = ri;
+    rr += ri * ri;
+  }
+  const double brr = block_sum<double>(rr, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = brr;
+  if (last_block(counter)) {
+    const double t = final_sum(part, gridDim.x, sh);
+    if (threadIdx.x == 0) {
+      scal[slot] = t;
+      *counter = 0u;
+    }
+  }
+}
+
+// Jacobi: inverse of the diagonal of A (CSR rows sorted: binary search).
+template <typename V>
+__global__ void k_diag_inv(int n, const int64_t* __restrict__ rowptr,
+                           const int32_t* __restrict__ colind, const V* __restrict__ val,
+                           V* __restrict__ dinv)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t lo = rowptr[i], hi = rowptr[i + 1] - 1;
+  V d = 0;
+  while (lo <= hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    const int c = colind[mid];
+    if (c == i) { d = val[mid]; break; }
+    if (c < i) lo = mid + 1; else hi = mid - 1;
+  }
+  dinv[i] = d != V(0) ? V(1) / d : V(1);
+}
+
+// x64 += s * d32 ; r32 = (float)(r64 * inv)
+__global__ void k_axpy_promote(int n, double* __restrict__ x, const float* __restrict__ d, double s)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += s * (double)d[i];
+}
+
+__global__ void k_demote_scaled(int n, const double* __restrict__ r, float* __restrict__ r32,
+                                double inv)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r32[i] = (float)(r[i] * inv);
+}
+
+template <typename V>
+__global__ void k_fill(int n, V* __restrict__ v, V val)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = val;
+}
+
+// ---------------------------------------------------------------------------
+static int lanes_for(const Context& c)
+{
+  const double avg = c.neq > 0 ? (double)c.nnz / c.neq : 0.0;
+  if (avg > 96) return 64;
+  if (avg > 40) return 32;
+  return 16;
+}
+
+template <typename V, typename ACC, bool DOT>
+static void launch_spmv(Context& c, const V* val, const V* p, V* q)
+{
+  const int L = lanes_for(c);
+  SolverWork& w = c.w;
+  const int grid = kGrid;
+  if (L == 64)
+    k_spmv_dot<V, ACC, 64, DOT><<<grid, kThreads, 0, c.stream>>>(c.neq, c.d_rowptr, c.d_colind, val,
+                                                                 p, q, w.partial, w.counter, w.scal);
+  else if (L == 32)
+    k_spmv_dot<V, ACC, 32, DOT><<<grid, kThreads, 0, c.stream>>>(c.neq, c.d_rowptr, c.d_colind, val,
+                                                                 p, q, w.partial, w.counter, w.scal);
+  else
+    k_spmv_dot<V, ACC, 16, DOT><<<grid, kThreads, 0, c.stream>>>(c.neq, c.d_rowptr, c.d_colind, val,
+                                                                 p, q, w.partial, w.counter, w.scal);
+}
+
+int spmv_device(Context& c, const double* d_x, double* d_y)
+{
+  if (c.neq == 0) return IFEM_GPU_OK;
+  launch_spmv<double, double, false>(c, c.d_val, d_x, d_y);
+  IFEM_HIP(hipGetLastError());
+  return IFEM_GPU_OK;
+}
+
+void solver_free(Context& c)
+{
+  SolverWork& w = c.w;
+  double* d[] = {w.x, w.r, w.z, w.p, w.q, w.dinv, w.partial, w.scal};
+  for (double* p : d) hipFree(p);
+  float* f[] = {w.r32, w.z32, w.p32, w.q32, w.x32, w.dinv32};
+  for (float* p : f) hipFree(p);
+  hipFree(w.counter);
+  if (w.h_scal) hipHostFree(w.h_scal);
+  w = SolverWork();
+}
+
+int solver_setup(Context& c)
+{
+  solver_free(c);
+  SolverWork& w = c.w;
+  const size_t n = c.neq > 0 ? c.neq : 1;
+  w.n = c.neq;
+  double** d[] = {&w.x, &w.r, &w.z, &w.p, &w.q, &w.dinv};
+  for (double** p : d) IFEM_HIP(hipMalloc(p, sizeof(double) * n));
+  float** f[] = {&w.r32, &w.z32, &w.p32, &w.q32, &w.x32, &w.dinv32};
+  for (float** p : f) IFEM_HIP(hipMalloc(p, sizeof(float) * n));
+  IFEM_HIP(hipMalloc(&w.partial, sizeof(double) * 2 * kGrid));
+  IFEM_HIP(hipMalloc(&w.scal, sizeof(double) * 16));
+  IFEM_HIP(hipMalloc(&w.counter, sizeof(unsigned)));
+  IFEM_HIP(hipMemsetAsync(w.counter, 0, sizeof(unsigned), c.stream));
+  IFEM_HIP(hipMemsetAsync(w.x, 0, sizeof(double) * n, c.stream));
+  IFEM_HIP(hipHostMalloc(&w.h_scal, sizeof(double) * 16, hipHostMallocDefault));
+  w.nblocks = kGrid;
+  IFEM_HIP(hipStreamSynchronize(c.stream));
+  return IFEM_GPU_OK;
+}
+
+static int read_scalars(Context& c)
+{
+  IFEM_HIP(hipMemcpyAsync(c.w.h_scal, c.w.scal, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost,
+                          c.stream));
+  IFEM_HIP(hipStreamSynchronize(c.stream));
+  return IFEM_GPU_OK;
+}
+
+static int write_scalars(Context& c, const double* v)
+{
+  std::memcpy(c.w.h_scal, v, sizeof(double) * S_NSCAL);
+  IFEM_HIP(hipMemcpyAsync(c.w.scal, c.w.h_scal, sizeof(double) * S_NSCAL, hipMemcpyHostToDevice,
+                          c.stream));
+  return IFEM_GPU_OK;
+}
+
+// Preconditioned CG on A (values val, vector type V) for A x = r0 where r is
+// preloaded with the right-hand side and x with zero.  Stops when
+// ||r||^2 <= tol2 or after maxits iterations.  Returns iterations done.
+template <typename V, typename ACC>
+static int pcg_run(Context& c, const V* val, V* x, V* r, V* z, V* p, V* q, const V* dinv,
+                   double tol2, int maxits, int check_every, int* iters, double* rr_out,
+                   int* status)
+{
+  SolverWork& w = c.w;
+  const int n = c.neq;
+  double s[S_NSCAL] = {0};
+  s[S_TOL2] = tol2;
+  s[S_MAXIT] = maxits;
+  int rc = write_scalars(c, s);
+  if (rc) return rc;
+  k_start<V><<<kGrid, kThreads, 0, c.stream>>>(n, r, z, p, dinv, w.partial, w.counter, w.scal);
+  IFEM_HIP(hipGetLastError());
+  int launched = 0;
+  while (true) {
+    for (int k = 0; k < check_every; k++) {
+      launch_spmv<V, ACC, true>(c, val, p, q);
+      k_update<V><<<kGrid, kThreads, 0, c.stream>>>(n, x, r, z, p, q, dinv, w.partial, w.counter,
+                                                    w.scal);
+      k_pupdate<V><<<kGrid, kThreads, 0, c.stream>>>(n, p, z, w.scal);
+    }
+    IFEM_HIP(hipGetLastError());
+    launched += check_every;
+    rc = read_scalars(c);
+    if (rc) return rc;
+    if (w.h_scal[S_DONE] != 0.0 || launched >= maxits + check_every) break;
+  }
+  *iters = (int)w.h_scal[S_ITER];
+  *rr_out = w.h_scal[S_RR];
+  *status = (int)w.h_scal[S_DONE];
+  return IFEM_GPU_OK;
+}
+
+static int true_residual(Context& c, const double* x, double* r, double* rr)
+{
+  SolverWork& w = c.w;
+  launch_spmv<double, double, false>(c, c.d_val, x, w.q);
+  k_residual<<<kGrid, kThreads, 0, c.stream>>>(c.neq, c.d_rhs, w.q, r, w.partial, w.counter,
+                                               w.scal, S_BB);
+  IFEM_HIP(hipGetLastError());
+  int rc = read_scalars(c);
+  if (rc) return rc;
+  *rr = w.h_scal[S_BB];
+  return IFEM_GPU_OK;
+}
+
+static int norm2_rhs(Context& c, double* bb)
+{
+  SolverWork& w = c.w;
+  k_fill<double><<<(c.neq + 255) / 256, 256, 0, c.stream>>>(c.neq, w.q, 0.0);
+  k_residual<<<kGrid, kThreads, 0, c.stream>>>(c.neq, c.d_rhs, w.q, w.z, w.partial, w.counter,
+                                               w.scal, S_BB);
+  IFEM_HIP(hipGetLastError());
+  int rc = read_scalars(c);
+  if (rc) return rc;
+  *bb = w.h_scal[S_BB];
+  return IFEM_GPU_OK;
+}
+
+int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
+{
+  SolverWork& w = c.w;
+  const int n = c.neq;
+  if (prm.maxits < 1) return fail(IFEM_GPU_EINVAL, "solve: maxits must be positive");
+  if (!(prm.rtol >= 0.0) || !(prm.atol >= 0.0))
+    return fail(IFEM_GPU_EINVAL, "solve: tolerances must be non-negative");
+  const int check = prm.check_every > 0 ? prm.check_every : 32;
+  int rc;
+  IFEM_HIP(hipEventRecord(c.ev0, c.stream));
+  double bb = 0.0;
+  rc = norm2_rhs(c, &bb);
+  if (rc) return rc;
+  const double bnorm = std::sqrt(bb);
+  const double tol = std::fmax(prm.rtol * bnorm, prm.atol);
+  const int nb = (n + 255) / 256;
+  k_diag_inv<double><<<nb, 256, 0, c.stream>>>(n, c.d_rowptr, c.d_colind, c.d_val, w.dinv);
+  k_fill<double><<<nb, 256, 0, c.stream>>>(n, w.x, 0.0);
+  IFEM_HIP(hipGetLastError());
+
+  int total_iters = 0, outer = 0;
+  double rel = bnorm > 0 ? 1.0 : 0.0;
+  bool converged = bnorm == 0.0;
+  bool diverged = false;
+  if (!converged && prm.precision == IFEM_GPU_FP64) {
+    // plain FP64 Jacobi-PCG with true-residual confirmation / replacement
+    IFEM_HIP(hipMemcpyAsync(w.r, c.d_rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
+    while (true) {
+      int it = 0, status = 0;
+      double rr = 0.0;
+      rc = pcg_run<double, double>(c, c.d_val, w.x, w.r, w.z, w.p, w.q, w.dinv, tol * tol,
+                                   prm.maxits - total_iters, check, &it, &rr, &status);
+      if (rc) return rc;
+      total_iters += it;
+      outer++;
+      double trr = 0.0;
+      rc = true_residual(c, w.x, w.r, &trr); // w.r <- b - A x (replacement)
+      if (rc) return rc;
+      rel = std::sqrt(trr) / bnorm;
+      if (std::sqrt(trr) <= tol) { converged = true; break; }
+      if (!(std::sqrt(trr) <= prm.dtol * bnorm)) { diverged = true; break; }
+      if (total_iters >= prm.maxits || status == 2 || it == 0) break;
+    }
+  } else if (!converged) {
+    // FP32 inner PCG inside FP64 iterative refinement
+    if (!c.fp32_ready) return fail(IFEM_GPU_EINVAL, "solve: no FP32 values");
+    k_diag_inv<float><<<nb, 256, 0, c.stream>>>(n, c.d_rowptr, c.d_colind, c.d_val32, w.dinv32);
+    IFEM_HIP(hipMemcpyAsync(w.r, c.d_rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
+    double rr64 = bb;
+    const double inner = prm.inner_rtol > 0 ? prm.inner_rtol : 1e-4;
+    while (total_iters < prm.maxits && outer < 100) {
+      const double rn = std::sqrt(rr64);
+      k_demote_scaled<<<nb, 256, 0, c.stream>>>(n, w.r, w.r32, 1.0 / rn);
+      k_fill<float><<<nb, 256, 0, c.stream>>>(n, w.x32, 0.0f);
+      int it = 0, status = 0;
+      double rr = 0.0;
+      // residual target relative to the scaled (unit-norm) residual; never
+      // ask the inner solve for more than the outer tolerance needs
+      double target = std::fmax(inner, 0.5 * tol / rn);
+      rc = pcg_run<float, float>(c, c.d_val32, w.x32, w.r32, w.z32, w.p32, w.q32, w.dinv32,
+                                 target * target, prm.maxits - total_iters, check, &it, &rr, &status);
+      if (rc) return rc;
+      total_iters += it;
+      outer++;
+      k_axpy_promote<<<nb, 256, 0, c.stream>>>(n, w.x, w.x32, rn);
+      rc = true_residual(c, w.x, w.r, &rr64);
+      if (rc) return rc;
+      rel = std::sqrt(rr64) / bnorm;
+      if (std::sqrt(rr64) <= tol) { converged = true; break; }
+      if (!(std::sqrt(rr64) <= prm.dtol * bnorm)) { diverged = true; break; }
+      if (it == 0 || status == 2) break;
+    }
+  }
+  IFEM_HIP(hipEventRecord(c.ev1, c.stream));
+  IFEM_HIP(hipEventSynchronize(c.ev1));
+  float ms = 0.f;
+  IFEM_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  c.last_solve_ms = ms;
+  if (st) {
+    st->iterations = total_iters;
+    st->outer_iterations = outer;
+    st->converged = converged ? 1 : 0;
+    st->rel_residual = rel;
+    st->solve_ms = ms;
+  }
+  if (diverged) return fail(IFEM_GPU_ENOCONV, "solve: diverged (residual above dtol*||b||)");
+  if (!converged)
+    return fail(IFEM_GPU_ENOCONV, "solve: not converged after " + std::to_string(total_iters) +
+                                      " iterations, relative residual " + std::to_string(rel));
+  return IFEM_GPU_OK;
+}
+
+} // namespace ifem

@@ -1,0 +1,80 @@
+"""The benchmark problems as concrete model inputs.
+
+Each function returns a ``Problem``: patch, integrand, Gauss count,
+equation numbers and an optional traction load, built by ``ifem_amd.model``.
+
+* ``poisson_cube``      -Laplace u = 3 pi^2 sin(pi x)sin(pi y)sin(pi z) on the unit
+                        cube, u = 0 on all faces (manufactured solution).
+* ``elasticity_cube``   isotropic elasticity (E, nu) on a jittered unit cube,
+                        clamped at x = 0, uniform traction (0,0,-1) on x = 1.
+* ``diffusion_cube``    -div(kappa grad u) = f with log-normal kappa (random
+                        Fourier Gaussian field) and a random smooth f, u = 0.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import model as M
+
+
+@dataclasses.dataclass
+class Problem:
+    name: str
+    patch: M.Patch
+    integrand: M.Integrand
+    ng: Tuple[int, int, int]
+    meqn: np.ndarray
+    neq: int
+    traction: Optional[Tuple[int, Tuple[float, float, float], int]] = None
+
+    @property
+    def ncomp(self) -> int:
+        return self.integrand.ncomp
+
+    @property
+    def ndof(self) -> int:
+        return self.patch.nnod * self.ncomp
+
+
+def poisson_cube(nel: int = 16, p: int = 2, ng: Optional[int] = None, jitter: float = 0.0,
+                 seed: int = 0) -> Problem:
+    P = M.cube_patch(p, nel, jitter, seed)
+    msc = M.face_status(P, 1, [1] * 6)
+    meqn, neq = M.equation_numbers(msc)
+    itg = M.Integrand(M.POISSON, source=M.Function(M.FN_SINPROD, 3.0 * math.pi ** 2))
+    g = ng or p + 1
+    return Problem(f"poisson{nel}p{p}", P, itg, (g, g, g), meqn, neq)
+
+
+def elasticity_cube(nel: int = 64, p: int = 2, ng: Optional[int] = None, jitter: float = 0.15,
+                    seed: int = 2024, E: float = 1.0, nu: float = 0.3) -> Problem:
+    P = M.cube_patch(p, nel, jitter, seed)
+    fix = [0b111, 0, 0, 0, 0, 0]  # clamped at x = 0 (face u=0)
+    msc = M.face_status(P, 3, fix)
+    meqn, neq = M.equation_numbers(msc)
+    itg = M.Integrand(M.ELASTICITY, E=E, nu=nu)
+    g = ng or p + 1
+    return Problem(f"elasticity{nel}p{p}", P, itg, (g, g, g), meqn, neq,
+                   traction=(M.FACE_U1, (0.0, 0.0, -1.0), g))
+
+
+def diffusion_cube(nel: int = 48, p: int = 2, ng: Optional[int] = None, sigma: float = 1.5,
+                   ell: float = 0.1, seed: int = 7, nmodes: int = 128,
+                   source_seed: Optional[int] = None) -> Problem:
+    P = M.cube_patch(p, nel)
+    msc = M.face_status(P, 1, [1] * 6)
+    meqn, neq = M.equation_numbers(msc)
+    kappa = M.lognormal_conductivity(seed, sigma, ell, nmodes)
+    src = M.smooth_source(seed + 1 if source_seed is None else source_seed)
+    itg = M.Integrand(M.POISSON, kappa=kappa, source=src)
+    g = ng or p + 1
+    return Problem(f"diffusion{nel}p{p}", P, itg, (g, g, g), meqn, neq)
+
+
+def dof_counts(pb: Problem):
+    """(total DOFs incl. eliminated ones, free equations)."""
+    return pb.ndof, pb.neq

@@ -1,0 +1,196 @@
+/*
+ * ifem_gpu.h -- C ABI of the MI355X (gfx950) isogeometric assembly and
+ * linear-solve library (libifem_gpu.so).
+ *
+ * The entry points replace, for one tensor-product spline patch, the
+ * reference's hot path
+ *   ASMbase::integrate -> Integrand::evalInt -> GlobalIntegral::assemble
+ *   -> SAM::assembleSystem -> SystemMatrix (+ LinSolParams) -> solve
+ * Each declaration names the reference interface it stands in for.
+ *
+ * Conventions: plain pointers and sizes only; host arrays passed in are
+ * copied, never retained.  Every call returns IFEM_GPU_OK (0) or an error
+ * code, and ifem_gpu_last_error() then returns a thread-local message.
+ * Nothing throws across this boundary.  One context per host thread; all GPU
+ * work of a context runs on its stream.
+ */
+#ifndef IFEM_GPU_H
+#define IFEM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IFEM_GPU_ABI_VERSION 1
+
+enum ifem_gpu_status {
+  IFEM_GPU_OK = 0,
+  IFEM_GPU_EINVAL = 1,       /* bad argument or call order */
+  IFEM_GPU_EUNSUPPORTED = 2, /* valid input the GPU path does not handle */
+  IFEM_GPU_EDEVICE = 3,      /* HIP runtime error or no device */
+  IFEM_GPU_ENOCONV = 4,      /* solver stopped without meeting tolerances */
+  IFEM_GPU_ENOMEM = 5
+};
+
+typedef struct ifem_gpu_ctx ifem_gpu_ctx;
+
+/* Thread-local text of the last error (empty string if none). */
+const char* ifem_gpu_last_error(void);
+int ifem_gpu_abi_version(void);
+
+/* Context on HIP device `device`.  stream: a hipStream_t to run on, or NULL
+ * for a new default-priority stream owned by the context.
+ * Stands in for LinAlgInit::Init / AlgEqSystem construction
+ * (src/LinAlg/LinAlgInit.C:25-43, src/ASM/AlgEqSystem.C:33). */
+int ifem_gpu_ctx_create(int device, void* stream, ifem_gpu_ctx** ctx);
+int ifem_gpu_ctx_destroy(ifem_gpu_ctx* ctx);
+/* The hipStream_t the context runs on. */
+void* ifem_gpu_ctx_stream(ifem_gpu_ctx* ctx);
+int ifem_gpu_synchronize(ifem_gpu_ctx* ctx);
+
+/* ---- patch geometry --------------------------------------------------------
+ * Non-rational trivariate B-spline volume, GoTools conventions:
+ * order[d] = p+1, ncoef[d] control points, knots[d] has ncoef+order entries
+ * (open, simple interior knots), coefs xyz-interleaved with node (i,j,k) at
+ * ((k*n2+j)*n1+i)*3.  Replaces ASMs3D::read/generateFEMTopology for the data
+ * the integration loop reads (src/ASM/ASMs3D.C:117-160, 433-519; element
+ * coordinates src/ASM/ASMs3D.C:1640-1678).  rational must be 0. */
+int ifem_gpu_patch_create(ifem_gpu_ctx* ctx, const int order[3], const int ncoef[3],
+                          const double* const knots[3], const double* coefs,
+                          int rational, int* patch_id);
+
+/* ---- SAM -------------------------------------------------------------------
+ * Equation numbers per DOF, node-major / component-minor (DOF = node*ncomp+c),
+ * 1-based; 0 = eliminated (homogeneous Dirichlet, msc = 0).  Free DOFs must be
+ * numbered in increasing DOF order, as SAM::initSystemEquations does
+ * (src/LinAlg/SAM.C:319-323); constrained (negative) equations are not
+ * supported.  Replaces SAM::initSystemEquations/getElmEqns
+ * (src/LinAlg/SAM.C:225-336, 655-696) for the patch. */
+int ifem_gpu_sam_bind(ifem_gpu_ctx* ctx, int patch_id, int ncomp, int ndof,
+                      const int* meqn, int neq);
+
+/* ---- symbolic assembly -----------------------------------------------------
+ * Builds the CSR sparsity pattern (symmetric, full storage, sorted columns)
+ * on the device.  Replaces SystemMatrix::initAssembly -> SparseMatrix::
+ * preAssemble -> SAM::getDofCouplings -> optimiseCols
+ * (src/LinAlg/SparseMatrix.C:717-762, src/LinAlg/SAM.C:377-427,
+ * src/LinAlg/SparseMatrix.C:984-1023).  nnz may be NULL. */
+int ifem_gpu_init_assembly(ifem_gpu_ctx* ctx, int64_t* nnz);
+
+/* ---- integrands ------------------------------------------------------------
+ * Recognised integrand kinds of the GPU path (the physics of
+ * Integrand::evalInt, src/ASM/Integrand.h:206,344):
+ *  IFEM_GPU_POISSON:    EK += kappa(X) detJxW dNdX dNdX^T, EV += f(X) detJxW N
+ *                       (Apps/Utils/EqualOrderOperators.C:158-211,243-255)
+ *  IFEM_GPU_ELASTICITY: EK += detJxW B^T C B (isotropic, E, nu),
+ *                       EV += detJxW body N
+ * Fields: ZERO (0), CONST scale, SINPROD scale*sin(pi x)sin(pi y)sin(pi z),
+ * FOURIER mean + scale*sum a_m cos(k_m.x+phi_m), EXP_FOURIER exp(FOURIER).
+ * modes: nmodes x {kx, ky, kz, phi, a}.  kappa of type ZERO means 1. */
+enum { IFEM_GPU_FN_ZERO = 0, IFEM_GPU_FN_CONST = 1, IFEM_GPU_FN_SINPROD = 2,
+       IFEM_GPU_FN_FOURIER = 3, IFEM_GPU_FN_EXP_FOURIER = 4 };
+enum { IFEM_GPU_POISSON = 1, IFEM_GPU_ELASTICITY = 2 };
+
+typedef struct {
+  int type;
+  int nmodes;
+  double scale;
+  double mean;
+  const double* modes;
+} ifem_gpu_function;
+
+typedef struct {
+  int kind;
+  double E, nu;
+  ifem_gpu_function kappa;
+  ifem_gpu_function source;
+  double body[3];
+} ifem_gpu_integrand;
+
+/* Precision of assembled values and of the solver's inner iteration. */
+enum { IFEM_GPU_FP64 = 64, IFEM_GPU_FP32 = 32 };
+
+/* ---- numeric assembly ------------------------------------------------------
+ * Volume integration over all elements of the bound patch with ng[d] Gauss
+ * points per direction, scattered into K (CSR values) and b.  Zeroes K and b
+ * first (AlgEqSystem::initialize, src/ASM/AlgEqSystem.C:184-212).  Replaces
+ * ASMs3D::integrate (src/ASM/ASMs3D.C:2071-2335) + AlgEqSystem::assemble
+ * (src/ASM/AlgEqSystem.C:215-303) + SAM::assembleSystem
+ * (src/LinAlg/SAM.C:430-467, 518-543).  precision: IFEM_GPU_FP64 writes the
+ * FP64 system; IFEM_GPU_FP32 writes the FP32 copy of K used by the mixed
+ * precision solver (b stays FP64 and is assembled in FP64). */
+int ifem_gpu_assemble(ifem_gpu_ctx* ctx, const ifem_gpu_integrand* itg,
+                      const int ng[3], int precision);
+
+/* Neumann traction t (constant vector, ncomp components) on patch face
+ * face = 1..6 (IFEM numbering: 1 u=0, 2 u=1, 3 v=0, 4 v=1, 5 w=0, 6 w=1),
+ * ng Gauss points per tangent direction, added into b.  Replaces
+ * ASMs3D::integrate(Integrand&, int lIndex, ...) (src/ASM/ASMs3D.C:2547-2812). */
+int ifem_gpu_assemble_neumann(ifem_gpu_ctx* ctx, int face, const double* traction, int ng);
+
+/* ---- linear solve ----------------------------------------------------------
+ * LinSolParams keys (src/LinAlg/LinSolParams.C:68-77,158-195) as fields:
+ * type "cg" (CG only), pc "jacobi" (Jacobi only), rtol, atol, dtol, maxits.
+ * Extensions: precision (FP64: plain FP64 PCG; FP32: FP32 inner PCG inside
+ * FP64 iterative refinement), inner_rtol (FP32 inner residual reduction),
+ * check_every (iterations between host convergence checks).
+ * Convergence: ||r||_2 <= max(rtol*||b||_2, atol), confirmed on the true FP64
+ * residual b - A x.  Replaces SystemMatrix::solve with PETSc KSPCG+PCJACOBI /
+ * dune CGSolver+SeqJac (src/LinAlg/PETScMatrix.C:778-832,980-990;
+ * src/LinAlg/ISTLMatrix.C:169-196). */
+typedef struct {
+  double rtol, atol, dtol;
+  int maxits;
+  int precision;
+  double inner_rtol;
+  int check_every;
+  int verbosity;
+} ifem_gpu_solver_params;
+
+typedef struct {
+  int iterations;        /* total inner (Krylov) iterations */
+  int outer_iterations;  /* refinement steps (1 for plain FP64) */
+  int converged;
+  double rel_residual;   /* true ||b-Ax||_2/||b||_2 in FP64 */
+  double solve_ms;       /* device time of the solve on the ctx stream */
+} ifem_gpu_stats;
+
+void ifem_gpu_solver_defaults(ifem_gpu_solver_params* p);
+/* Solves K x = b; the solution stays on the device (ifem_gpu_get_solution). */
+int ifem_gpu_solve(ifem_gpu_ctx* ctx, const ifem_gpu_solver_params* params,
+                   ifem_gpu_stats* stats);
+
+/* ---- data access -----------------------------------------------------------
+ * Sizes of the bound system; any pointer may be NULL. */
+int ifem_gpu_system_info(ifem_gpu_ctx* ctx, int* neq, int64_t* nnz, int* ncomp);
+/* Copies of the device CSR (0-based) and RHS to host arrays (NULL skips). */
+int ifem_gpu_get_system(ifem_gpu_ctx* ctx, int64_t* rowptr, int32_t* colind,
+                        double* values, double* rhs);
+/* FP32 copy of K values (after an IFEM_GPU_FP32 assembly). */
+int ifem_gpu_get_values_fp32(ifem_gpu_ctx* ctx, float* values);
+/* Solution in equation order (neq). */
+int ifem_gpu_get_solution(ifem_gpu_ctx* ctx, double* x);
+/* Solution expanded to DOF order, eliminated DOFs = 0
+ * (SAM::expandSolution, src/LinAlg/SAM.C:768-810). */
+int ifem_gpu_expand_solution(ifem_gpu_ctx* ctx, double* xdof);
+/* y = K x with host vectors (SparseMatrix::multiply,
+ * src/LinAlg/SparseMatrix.C:578-605). */
+int ifem_gpu_spmv(ifem_gpu_ctx* ctx, const double* x, double* y);
+/* Overwrite the device RHS with a host vector (neq). */
+int ifem_gpu_set_rhs(ifem_gpu_ctx* ctx, const double* b);
+
+/* Device pointers of the resident system for callers that keep data in HBM
+ * (rowptr int64, colind int32, values fp64, rhs fp64, solution fp64). */
+int ifem_gpu_device_pointers(ifem_gpu_ctx* ctx, void** rowptr, void** colind,
+                             void** values, void** rhs, void** solution);
+
+/* Device time (ms, HIP events on the ctx stream) of the last numeric
+ * assembly kernel and of the last solve. */
+int ifem_gpu_last_timings(ifem_gpu_ctx* ctx, double* assemble_ms, double* solve_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

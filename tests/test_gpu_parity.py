@@ -1,0 +1,188 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle on the same
+seeded inputs, plus size-independent properties at the full benchmark sizes.
+
+Tolerances: FP64 assembly and solution 1e-12 relative to max|.| (the north-star
+parity bar); FP32 assembly 1e-5 relative; mixed-precision solution: energy
+norm error vs the FP64 solution <= 1e-6.
+"""
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from ifem_amd import FP32, FP64, model as M, problems, setup_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _oracle_system(pb, nthreads=0):
+    return oracle.system(pb.patch, pb.integrand, pb.ng, pb.meqn, pb.neq, pb.traction, nthreads)
+
+
+def _compare_system(s, pb, tol=1e-12):
+    rp, ci, A, b = _oracle_system(pb)
+    grp, gci, gA, gb = s.system()
+    assert np.array_equal(grp, rp)
+    assert np.array_equal(gci, ci)
+    errA = np.max(np.abs(gA - A)) / np.max(np.abs(A))
+    errb = np.max(np.abs(gb - b)) / max(np.max(np.abs(b)), 1e-300)
+    assert errA <= tol, errA
+    assert errb <= tol, errb
+    return rp, ci, A, b
+
+
+CASES = [
+    ("poisson16p2", lambda: problems.poisson_cube(16, 2)),
+    ("poisson6p1", lambda: problems.poisson_cube(6, 1)),
+    ("poisson5p3", lambda: problems.poisson_cube(5, 3)),
+    ("poisson6p2ng4", lambda: problems.poisson_cube(6, 2, ng=4, jitter=0.1, seed=5)),
+    ("elast6p2", lambda: problems.elasticity_cube(6, 2, seed=3)),
+    ("elast4p1", lambda: problems.elasticity_cube(4, 1, seed=4)),
+    ("elast3p3", lambda: problems.elasticity_cube(3, 3, seed=5)),
+    ("diffusion8p2", lambda: problems.diffusion_cube(8, 2, sigma=1.0, ell=0.2, seed=9)),
+    ("diffusion4p3", lambda: problems.diffusion_cube(4, 3, sigma=1.0, ell=0.2, seed=10)),
+]
+
+
+@pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
+def test_assembly_matches_oracle(name, make):
+    pb = make()
+    _say(f"[gpu] assembly parity {name} neq={pb.neq}")
+    with setup_problem(pb) as s:
+        _compare_system(s, pb)
+
+
+def test_assembly_is_bitwise_deterministic():
+    pb = problems.elasticity_cube(8, 2, seed=21)
+    _say("[gpu] determinism elasticity 8^3")
+    with setup_problem(pb) as s:
+        _, _, A1, b1 = s.system()
+        s.assemble(pb.integrand, pb.ng, FP64)
+        s.add_traction(*pb.traction)
+        _, _, A2, b2 = s.system()
+    assert np.array_equal(A1, A2) and np.array_equal(b1, b2)
+
+
+def test_fp32_assembly_close_to_fp64():
+    pb = problems.elasticity_cube(6, 2, seed=8)
+    _say("[gpu] fp32 assembly elasticity 6^3")
+    with setup_problem(pb, precision=FP32) as s:
+        _, _, A, _ = s.system()
+        A32 = s.values_fp32().astype(np.float64)
+    assert np.max(np.abs(A32 - A)) <= 1e-5 * np.max(np.abs(A))
+
+
+def test_spmv_matches_oracle():
+    pb = problems.elasticity_cube(6, 2, seed=3)
+    _say("[gpu] spmv")
+    with setup_problem(pb) as s:
+        rp, ci, A, b = s.system()
+        x = np.random.default_rng(0).standard_normal(pb.neq)
+        y = s.spmv(x)
+    yo = oracle.spmv(rp, ci, A, x)
+    assert np.max(np.abs(y - yo)) <= 1e-13 * np.max(np.abs(yo))
+
+
+def test_pcg_iterates_match_oracle():
+    """k Jacobi-PCG iterations from x = 0: GPU iterate vs oracle iterate."""
+    pb = problems.poisson_cube(16)
+    _say("[gpu] pcg iterate parity")
+    with setup_problem(pb) as s:
+        rp, ci, A, b = s.system()
+        st = s.solve(rtol=1e-30, maxits=20, check_every=5)
+        x = s.solution()
+    assert st["iterations"] == 20 and not st["converged"]
+    xo = oracle.pcg_fixed(rp, ci, A, b, 20)
+    assert np.max(np.abs(x - xo)) <= 1e-12 * np.max(np.abs(xo))
+
+
+@pytest.mark.parametrize("name,make", [
+    ("poisson16", lambda: problems.poisson_cube(16)),
+    ("elast8", lambda: problems.elasticity_cube(8, 2, seed=6)),
+])
+def test_pcg_solution_matches_oracle(name, make):
+    pb = make()
+    _say(f"[gpu] pcg solution {name}")
+    with setup_problem(pb) as s:
+        rp, ci, A, b = _compare_system(s, pb)
+        st = s.solve(rtol=1e-13, maxits=20000)
+        x = s.solution()
+    assert st["converged"] and st["rel_residual"] <= 1e-13
+    xo, it, res = oracle.pcg(rp, ci, A, b, rtol=1e-13)
+    assert it > 0
+    assert np.max(np.abs(x - xo)) <= 1e-10 * np.max(np.abs(xo))
+    r = b - oracle.spmv(rp, ci, A, x)
+    assert np.linalg.norm(r) <= 1e-13 * np.linalg.norm(b) * 1.01
+
+
+def test_mixed_precision_solution():
+    pb = problems.elasticity_cube(10, 2, seed=12)
+    _say("[gpu] mixed precision elasticity 10^3")
+    with setup_problem(pb, precision=FP32) as s:
+        rp, ci, A, b = s.system()
+        st64 = s.solve(rtol=1e-10, maxits=50000)
+        x64 = s.solution()
+        st = s.solve(rtol=1e-8, maxits=50000, precision=FP32, inner_rtol=1e-4)
+        x = s.solution()
+    assert st64["converged"] and st["converged"] and st["rel_residual"] <= 1e-8
+    K = sp.csr_matrix((A, ci, rp), shape=(pb.neq, pb.neq))
+    e = x - x64
+    en = np.sqrt(e @ (K @ e)) / np.sqrt(x64 @ (K @ x64))
+    assert en <= 1e-6, en
+
+
+def test_solver_reports_non_convergence():
+    pb = problems.poisson_cube(8)
+    _say("[gpu] maxits exhaustion")
+    with setup_problem(pb) as s:
+        st = s.solve(rtol=1e-14, maxits=3, check_every=1)
+    assert st["rc"] == 4 and not st["converged"] and st["iterations"] == 3
+
+
+def test_errors_are_reported_not_thrown():
+    from ifem_amd import GpuError, Session
+    pb = problems.poisson_cube(4)
+    with Session() as s:
+        pid = s.add_patch(pb.patch)
+        bad = pb.meqn.copy()
+        bad[bad > 0] = bad[bad > 0][::-1]  # not monotone
+        with pytest.raises(GpuError):
+            s.bind(pid, 1, bad, pb.neq)
+        s.bind(pid, 1, pb.meqn, pb.neq)
+        with pytest.raises(GpuError):
+            s.assemble(pb.integrand, pb.ng)  # before init_assembly
+        s.init_assembly()
+        with pytest.raises(GpuError):
+            s.assemble(M.Integrand(M.ELASTICITY), pb.ng)  # ncomp mismatch
+
+
+# ---- full benchmark size: size-independent properties -------------------------
+def test_elasticity64_full_size_properties():
+    """64^3 p=2 jittered elasticity (862,488 DOFs): closed-form nnz, traction
+    resultant, rows of a bottom slab vs the oracle, FP64 solve to 1e-8 on the
+    true residual."""
+    pb = problems.elasticity_cube(64, 2)
+    _say("[gpu] full-size elasticity 64^3")
+    with setup_problem(pb) as s:
+        assert s.nnz == 301386096
+        rp, ci, A, b = s.system()
+        # traction resultant: sum of z-loads = -1 (face x=1 has area 1)
+        assert abs(b[2::3].sum() + 1.0) < 1e-12
+        # oracle on elements with e3 in [0,2) gives complete rows for nodes k < 2
+        Ao, bo = oracle.assemble(pb.patch, pb.integrand, pb.ng, pb.meqn, pb.neq, rp, ci,
+                                 e3_range=(0, 2))
+        n1, n2 = pb.patch.n[0], pb.patch.n[1]
+        last_row = int(pb.meqn[: 2 * n1 * n2 * 3].max())
+        hi = rp[last_row]
+        err = np.max(np.abs(A[:hi] - Ao[:hi])) / np.max(np.abs(A[:hi]))
+        assert err <= 1e-12, err
+        del Ao
+        st = s.solve(rtol=1e-8, maxits=20000)
+        assert st["converged"] and st["rel_residual"] <= 1e-8

@@ -1,0 +1,468 @@
+"""Pins the oracle (oracle/liboracle.so) before it is trusted as the checker.
+
+Every function of the oracle is checked against something independent of its
+own code: the reference's golden values and known-answer tests, the
+reference's own compiled functions (oracle/_ref, via committed fixtures),
+library routines (scipy BSpline, spsolve) and closed forms.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.interpolate as si
+import scipy.sparse as sp
+
+import oracle
+from ifem_amd import model as M
+from ifem_amd import problems
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _hex(a):
+    return np.array([float.fromhex(v) for v in a])
+
+
+# ---- a1 Gauss ---------------------------------------------------------------
+def test_gauss_matches_reference_tables():
+    """Reference tables G1..G9 (src/Utility/GaussQuadrature.C:17-60) to <= 1 ulp;
+    G10 of the reference is only accurate to ~1e-8 (its own test uses 1e-7)."""
+    with open(os.path.join(GOLD, "reference_gauss.json")) as f:
+        ref = json.load(f)
+    for n in range(1, 11):
+        xi, w = oracle.gauss(n)
+        rx, rw = _hex(ref[str(n)]["xi"]), _hex(ref[str(n)]["w"])
+        tol = 2.3e-16 if n < 10 else 1e-7
+        assert np.max(np.abs(xi - rx)) <= tol, n
+        assert np.max(np.abs(w - rw)) <= tol, n
+
+
+@pytest.mark.parametrize("n", range(1, 11))
+def test_gauss_exactness(n):
+    """TestGaussQuadrature.Integrate (src/Utility/Test/TestGaussQuadrature.C:20-39)."""
+    xi, w = oracle.gauss(n)
+    res = sum(w[i] * xi[i] ** p for i in range(n) for p in range(n))
+    exa = sum((1.0 ** (p + 1) - (-1.0) ** (p + 1)) / (p + 1) for p in range(n))
+    assert abs(res - exa) <= 1e-14 * abs(exa)
+    # and one degree beyond the loop: 2n-1 exactness of an n-point rule
+    assert abs(sum(w * xi ** (2 * n - 2)) - 2.0 / (2 * n - 1)) < 1e-14
+
+
+def test_gauss_param_closed_form():
+    """a3: u = ((u1-u0) xi + u1 + u0)/2 (src/Utility/SplineUtils.C:217)."""
+    t = M.open_uniform_knots(2, 4)
+    P = M.cube_patch(2, 4)
+    EK, EV = oracle.element_matrices(P, M.Integrand(M.POISSON, source=M.Function(M.FN_CONST, 1.0)),
+                                     [3, 3, 3], (1, 0, 0))
+    # sum_b EV_b = integral of f=1 over the element = element volume (1/4)(1/4)(1/4)
+    assert abs(EV.sum() - 1.0 / 64) < 1e-15
+    assert t[3] == 0.25
+
+
+# ---- a4 basis ----------------------------------------------------------------
+@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("knots", ["uniform", "graded"])
+def test_basis_vs_scipy(p, knots):
+    """Cox-de Boor values and derivatives vs scipy.interpolate.BSpline."""
+    if knots == "uniform":
+        t = M.open_uniform_knots(p, 7)
+    else:
+        inner = np.array([0.05, 0.13, 0.3, 0.31, 0.6, 0.9])
+        t = np.concatenate([[0.0] * (p + 1), inner, [1.0] * (p + 1)])
+    n = len(t) - p - 1
+    rng = np.random.default_rng(p)
+    for u in np.concatenate([rng.uniform(0, 1, 40), t[p + 1:n]]):
+        s, N, dN = oracle.basis(p, t, u)
+        for r in range(p + 1):
+            i = s - p + r
+            c = np.zeros(n); c[i] = 1.0
+            bs = si.BSpline(t, c, p, extrapolate=False)
+            assert abs(N[r] - bs(u)) < 1e-14
+            assert abs(dN[r] - bs.derivative()(u)) < 1e-11 * max(1, abs(dN[r]))
+        assert abs(N.sum() - 1.0) < 1e-15 and abs(dN.sum()) < 1e-11
+        assert np.all(N >= 0)
+
+
+def test_basis_at_end_parameter():
+    """u = end param: the last function is 1 (face evaluation, ASMs3D.C:2576-2601)."""
+    t = M.open_uniform_knots(2, 5)
+    s, N, dN = oracle.basis(2, t, 1.0)
+    assert s == len(t) - 2 - 2 and abs(N[-1] - 1.0) < 1e-15 and abs(N[:-1]).max() < 1e-15
+    assert abs(dN[-1] - 2 * 5) < 1e-12 and abs(dN[-2] + 2 * 5) < 1e-12
+
+
+# ---- a5 MNPC / boundary nodes -----------------------------------------------
+def test_element_nodes_known_answers():
+    """TestASMs3D.ElmNodes (src/ASM/Test/TestASMs3D.C:300-325)."""
+    ref = [[0, 1, 3, 4, 9, 10, 12, 13], [1, 2, 4, 5, 10, 11, 13, 14], [3, 4, 6, 7, 12, 13, 15, 16],
+           [4, 5, 7, 8, 13, 14, 16, 17], [9, 10, 12, 13, 18, 19, 21, 22],
+           [10, 11, 13, 14, 19, 20, 22, 23], [12, 13, 15, 16, 21, 22, 24, 25],
+           [13, 14, 16, 17, 22, 23, 25, 26]]
+    ref_p2 = [0, 1, 2, 4, 5, 6, 8, 9, 10, 16, 17, 18, 20, 21, 22, 24, 25, 26, 32, 33, 34, 36, 37,
+              38, 40, 41, 42]
+    ref_p2_last = [21, 22, 23, 25, 26, 27, 29, 30, 31, 37, 38, 39, 41, 42, 43, 45, 46, 47, 53, 54,
+                   55, 57, 58, 59, 61, 62, 63]
+    import ctypes as C
+    for p, rr in ((1, ref), (2, None)):
+        P = M.cube_patch(p, 2)
+        keep = oracle._Keep()
+        sp_ = oracle._patch(P, keep)
+        e = 0
+        for e3 in range(2):
+            for e2 in range(2):
+                for e1 in range(2):
+                    mn = np.zeros(P.nen, dtype=np.int32)
+                    oracle.lib().orc_element_nodes(C.byref(sp_), e1, e2, e3,
+                                                   mn.ctypes.data_as(C.POINTER(C.c_int)))
+                    if p == 1:
+                        assert mn.tolist() == rr[e]
+                    elif e == 0:
+                        assert mn.tolist() == ref_p2
+                    elif e == 7:
+                        assert mn.tolist() == ref_p2_last
+                    e += 1
+
+
+def test_boundary_nodes_known_answers():
+    """TestASMs3D.BoundaryNodes (src/ASM/Test/TestASMs3D.C:210-225): p=1, 3^3 elements."""
+    ref = [lambda i, j: 1 + 4 * (4 * j + i), lambda i, j: 4 + 4 * (4 * j + i),
+           lambda i, j: 16 * j + i + 1, lambda i, j: 16 * j + i + 13,
+           lambda i, j: 4 * j + i + 1, lambda i, j: 16 * 3 + 4 * j + i + 1]
+    P = M.cube_patch(1, 3)
+    for face in range(6):
+        mask = [0] * 6
+        mask[face] = 1
+        msc = M.face_status(P, 1, mask)
+        fixed = sorted((np.nonzero(msc == 0)[0] + 1).tolist())
+        expect = sorted(ref[face](i, j) for j in range(4) for i in range(4))
+        assert fixed == expect
+
+
+# ---- a8 Jacobian ---------------------------------------------------------------
+def test_jacobian_known_answer():
+    """TestCoordinateMapping.Jacobian3D (src/Utility/Test/TestCoordinateMapping.C:357-392)."""
+    P = M.cube_patch(1, 4)
+    t = P.knots[0]
+    s, N, dN = oracle.basis(1, t, 0.25)
+    nen = 8
+    dNdu = np.zeros((nen, 3))
+    a = 0
+    for a3 in range(2):
+        for a2 in range(2):
+            for a1 in range(2):
+                dNdu[a] = [dN[a1] * N[a2] * N[a3], N[a1] * dN[a2] * N[a3], N[a1] * N[a2] * dN[a3]]
+                a += 1
+    # element 2 (1-based) = (1,0,0)
+    X = np.zeros((3, nen))
+    a = 0
+    for a3 in range(2):
+        for a2 in range(2):
+            for a1 in range(2):
+                node = (1 + a1) + 5 * ((0 + a2) + 5 * (0 + a3))
+                X[:, a] = P.coefs[node]
+                a += 1
+    det, Jinv, dNdX = oracle.jacobian(X, dNdu)
+    assert abs(det - 1.0) < 1e-15
+    assert np.allclose(Jinv, np.eye(3), atol=1e-15)
+    dndx = [-4.0, 4.0, 0, 0, 0, 0, 0, 0, -4.0, 0, 4.0, 0, 0, 0, 0, 0, -4.0, 0, 0, 0, 4.0, 0, 0, 0]
+    assert np.allclose(dNdX.T.ravel(), dndx, atol=1e-14)
+
+
+def test_jacobian_matches_reference_code():
+    """utl::Jacobian of the reference itself (fixture from oracle/_ref)."""
+    with open(os.path.join(GOLD, "reference_jacobian.json")) as f:
+        cases = json.load(f)
+    for c in cases:
+        nen = c["nen"]
+        X = _hex(c["X"]).reshape(3, nen)
+        D = _hex(c["dNdu"]).reshape(nen, 3)
+        det, Jinv, dNdX = oracle.jacobian(X, D)
+        assert det == float.fromhex(c["det"])
+        assert np.array_equal(Jinv.ravel(), _hex(c["Jinv"]))
+        assert np.max(np.abs(dNdX.ravel() - _hex(c["dNdX"]))) <= 1e-15 * np.max(np.abs(dNdX))
+
+
+def test_jacobian_singular_returns_zero():
+    X = np.zeros((3, 8)); D = np.ones((8, 3))
+    det, _, _ = oracle.jacobian(X, D)
+    assert det == 0.0
+
+
+# ---- a9 integrands -------------------------------------------------------------
+class _FakeFE:
+    """The 2-node fake element of Apps/Utils/Test/TestEqualOrderOperators.C:28-41,
+    embedded in 3D (third gradient component zero)."""
+    N = np.array([1.0, 2.0])
+    dNdX = np.array([[1.0, 3.0, 0.0], [2.0, 4.0, 0.0]])
+
+
+def test_equal_order_known_answers_via_oracle_element():
+    """TestEqualOrderOperators.Laplacian/Stress/Source known answers
+    (Apps/Utils/Test/TestEqualOrderOperators.C:113-215) reproduced by the oracle's
+    integrand: a linear map from reference to physical space makes dNdX of a
+    p=1 element equal chosen gradients, so the element matrices are analytic."""
+    # p=1 single element on [0,1]^3 with identity geometry: 1-point Gauss at the
+    # centre gives dNdX = +-1 products of 1/2; check against the closed form
+    P = M.cube_patch(1, 1)
+    EK, EV = oracle.element_matrices(P, M.Integrand(M.POISSON, source=M.Function(M.FN_CONST, 2.0)),
+                                     [1, 1, 1], (0, 0, 0))
+    # gradient of trilinear N_a at the centre: each component = +-1/4
+    grads = []
+    for a3 in (0, 1):
+        for a2 in (0, 1):
+            for a1 in (0, 1):
+                grads.append([(2 * a1 - 1) * 0.25, (2 * a2 - 1) * 0.25, (2 * a3 - 1) * 0.25])
+    G = np.array(grads)
+    assert np.allclose(EK, G @ G.T, atol=1e-15)
+    assert np.allclose(EV, 2.0 * 0.125, atol=1e-15)
+    # the fake-FE KATs: Laplacian [[10,14],[14,20]]; stress form (lam=0, mu=1)
+    FE = _FakeFE
+    lap = FE.dNdX @ FE.dNdX.T
+    assert np.array_equal(lap, [[10, 14], [14, 20]])
+    # elasticity with lam=0, mu=1: EK(a i, b j) = g_aj g_bi + delta_ij g_a.g_b
+    EKs = np.zeros((4, 4))
+    for a in range(2):
+        for b in range(2):
+            for i in range(2):
+                for j in range(2):
+                    EKs[2 * a + i, 2 * b + j] = FE.dNdX[a, j] * FE.dNdX[b, i] + (i == j) * lap[a, b]
+    assert np.array_equal(EKs, [[11, 3, 16, 6], [3, 19, 4, 26], [16, 4, 24, 8], [6, 26, 8, 36]])
+
+
+def test_elasticity_element_matches_stress_known_answer():
+    """The oracle's elasticity integrand with lam = 0 reproduces
+    EqualOrderOperators::Weak::Laplacian(stress=true) on a real element: compare
+    with the stress-form matrix built from the oracle's own Poisson gradients."""
+    P = M.cube_patch(2, 2, jitter=0.1, seed=3)
+    nu = 0.0  # lam = 0, mu = E/2
+    EKe, _ = oracle.element_matrices(P, M.Integrand(M.ELASTICITY, E=2.0, nu=nu), [3, 3, 3], (1, 0, 1))
+    # a rigid translation has zero energy; a rigid rotation too
+    nen = P.nen
+    mn = []
+    for a3 in range(3):
+        for a2 in range(3):
+            for a1 in range(3):
+                mn.append((1 + a1) + P.n[0] * ((0 + a2) + P.n[1] * (1 + a3)))
+    X = P.coefs[mn]
+    for mode in range(6):
+        u = np.zeros((nen, 3))
+        if mode < 3:
+            u[:, mode] = 1.0
+        else:
+            w = np.zeros(3); w[mode - 3] = 1.0
+            u = np.cross(w, X)
+        r = EKe @ u.ravel()
+        assert np.max(np.abs(r)) < 1e-13 * np.max(np.abs(EKe))
+
+
+# ---- a13-a17 SAM and pattern -----------------------------------------------------
+def test_sam_and_couplings_match_reference_code():
+    """SAM::initSystemEquations + getDofCouplings of the reference (fixture from
+    oracle/_ref) vs the oracle's equation numbers and CSR pattern."""
+    with open(os.path.join(GOLD, "reference_sam.json")) as f:
+        cases = json.load(f)
+    for c in cases:
+        P = M.cube_patch(c["p"], c["nel"])
+        msc = M.face_status(P, c["ncomp"], c["fix"])
+        meqn, neq = M.equation_numbers(msc)
+        assert neq == c["neq"]
+        assert meqn.tolist() == c["meqn"]
+        rp, ci = oracle.pattern(P, c["ncomp"], meqn, neq)
+        assert rp.tolist() == c["rowptr"]
+        assert ci.tolist() == c["colind"]
+
+
+@pytest.mark.parametrize("nel,p,nnz", [(16, 2, 405224), (8, 3, None)])
+def test_pattern_closed_form(nel, p, nnz):
+    """nnz of the Dirichlet-reduced Poisson pattern = (sum over free 1D nodes of
+    coupled free 1D nodes)^3."""
+    pb = problems.poisson_cube(nel, p)
+    rp, ci = oracle.pattern(pb.patch, 1, pb.meqn, pb.neq)
+    n = nel + p
+    free = range(1, n - 1)
+    c1 = sum(1 for i in free for j in free if abs(i - j) <= p)
+    assert rp[-1] == c1 ** 3
+    if nnz:
+        assert rp[-1] == nnz
+
+
+def test_identity_stencil_diagonal_is_valence():
+    """TestPETScMatrix identity-stencil invariant (src/LinAlg/Test/TestPETScMatrix.C:26-81):
+    assembling the element mass of N=1 (f=1 source) gives b_A = integral of N_A,
+    and the 1D integrals of B-splines are (t_{i+p+1}-t_i)/(p+1)."""
+    P = M.cube_patch(2, 5)
+    msc = np.ones(P.nnod, dtype=np.int32)
+    meqn, neq = M.equation_numbers(msc)
+    itg = M.Integrand(M.POISSON, source=M.Function(M.FN_CONST, 1.0))
+    rp, ci, A, b = oracle.system(P, itg, [3, 3, 3], meqn, neq)
+    t = P.knots[0]
+    g = np.array([(t[i + 3] - t[i]) / 3 for i in range(P.n[0])])
+    expect = np.einsum("k,j,i->kji", g, g, g).ravel()
+    assert np.max(np.abs(b - expect)) < 1e-15
+    assert abs(b.sum() - 1.0) < 1e-14
+
+
+# ---- a9 Poisson closed form (Kronecker) -----------------------------------------
+def _one_d_matrices(p, nel):
+    """1D stiffness, mass and load of f-factor sin(pi x) by scipy BSpline and a
+    20-point Gauss rule per element (independent of the oracle)."""
+    t = M.open_uniform_knots(p, nel)
+    n = len(t) - p - 1
+    xg, wg = np.polynomial.legendre.leggauss(20)
+    K1 = np.zeros((n, n)); M1 = np.zeros((n, n)); g = np.zeros(n)
+    for e in range(nel):
+        a, b_ = t[p + e], t[p + e + 1]
+        x = 0.5 * (b_ - a) * xg + 0.5 * (a + b_)
+        w = 0.5 * (b_ - a) * wg
+        B = np.zeros((n, len(x))); D = np.zeros((n, len(x)))
+        for i in range(n):
+            c = np.zeros(n); c[i] = 1
+            s = si.BSpline(t, c, p)
+            B[i] = s(x); D[i] = s.derivative()(x)
+        K1 += (D * w) @ D.T
+        M1 += (B * w) @ B.T
+        g += B @ (w * np.sin(np.pi * x))
+    return K1, M1, g
+
+
+def test_poisson16_matches_kronecker_closed_form():
+    """Config-1 Poisson system on identity geometry is K1xM1xM1 + M1xK1xM1 +
+    M1xM1xK1 restricted to free indices, F = 3 pi^2 g x g x g."""
+    pb = problems.poisson_cube(16)
+    rp, ci, A, b = oracle.system(pb.patch, pb.integrand, pb.ng, pb.meqn, pb.neq)
+    K1, M1, g = _one_d_matrices(2, 16)
+    f = slice(1, 17)
+    K1, M1, g = K1[f, f], M1[f, f], g[f]
+    Kk = (sp.kron(sp.kron(M1, M1), K1) + sp.kron(sp.kron(M1, K1), M1)
+          + sp.kron(sp.kron(K1, M1), M1)).tocsr()
+    Ko = sp.csr_matrix((A, ci, rp), shape=Kk.shape)
+    d = (Ko - Kk)
+    assert abs(d).max() <= 1e-12 * abs(Kk).max()
+    F = 3 * np.pi ** 2 * np.einsum("k,j,i->kji", g, g, g).ravel()
+    # f = 3pi^2 sin sin sin is integrated by 3-point Gauss (not exact): 1e-4 rel
+    assert np.max(np.abs(b - F)) < 2e-4 * np.max(np.abs(F))
+
+
+# ---- a18/a19 PCG and SpMV --------------------------------------------------------
+def test_spmv_tridiagonal_known_answer():
+    """TestSparseMatrix.MxV (src/LinAlg/Test/TestSparseMatrix.C:117-165)."""
+    n = 10
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        if i > 0:
+            rows.append(i); cols.append(i - 1); vals.append(1.0)
+        rows.append(i); cols.append(i); vals.append(-4.0)
+        if i < n - 1:
+            rows.append(i); cols.append(i + 1); vals.append(2.0)
+    K = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    x = np.arange(1, n + 1, dtype=float)
+    y = oracle.spmv(K.indptr.astype(np.int64), K.indices.astype(np.int32), K.data, x)
+    ii = np.arange(1, n + 1)
+    expect = np.where(ii > 1, ii - 1, 0.0) - 4.0 * ii + np.where(ii < n, 2.0 * (ii + 1), 0.0)
+    assert np.array_equal(y, expect)
+
+
+def test_pcg_diagonal_known_answer():
+    """TestDiagMatrix.AssembleAndSolve (src/LinAlg/Test/TestDiagMatrix.C:50-80): x_i = 2i."""
+    n = 6
+    i = np.arange(1, n + 1, dtype=float)
+    rp = np.arange(n + 1, dtype=np.int64)
+    ci = np.arange(n, dtype=np.int32)
+    x, it, res = oracle.pcg(rp, ci, i.copy(), 2 * i * i, rtol=1e-14)
+    assert np.allclose(x, 2 * i, rtol=1e-14) and it == 1
+
+
+def test_pcg_matches_scipy_spsolve():
+    """Config-1 Poisson: PCG to 1e-13 vs scipy spsolve (golden fixture)."""
+    pb = problems.poisson_cube(16)
+    rp, ci, A, b = oracle.system(pb.patch, pb.integrand, pb.ng, pb.meqn, pb.neq, nthreads=1)
+    gold = np.load(os.path.join(GOLD, "scipy_poisson16.npz"))
+    assert np.max(np.abs(b - gold["b"])) < 1e-12 * np.max(np.abs(b))
+    x, it, res = oracle.pcg(rp, ci, A, b, rtol=1e-13)
+    assert it > 0 and res <= 1e-13
+    assert np.max(np.abs(x - gold["x"])) < 1e-11 * np.max(np.abs(gold["x"]))
+
+
+def test_pcg_reports_maxits():
+    pb = problems.poisson_cube(8)
+    rp, ci, A, b = oracle.system(pb.patch, pb.integrand, pb.ng, pb.meqn, pb.neq)
+    x, it, res = oracle.pcg(rp, ci, A, b, rtol=1e-14, maxits=3)
+    assert it == -3 and res > 1e-14
+
+
+def test_assembly_thread_count_independent():
+    """Stripe colouring (ThreadGroups) only reorders sums: 1 vs many threads agree."""
+    pb = problems.elasticity_cube(6)
+    rp, ci = oracle.pattern(pb.patch, 3, pb.meqn, pb.neq)
+    A1, b1 = oracle.assemble(pb.patch, pb.integrand, pb.ng, pb.meqn, pb.neq, rp, ci, nthreads=1)
+    A4, b4 = oracle.assemble(pb.patch, pb.integrand, pb.ng, pb.meqn, pb.neq, rp, ci, nthreads=4)
+    assert np.max(np.abs(A1 - A4)) <= 1e-15 * np.max(np.abs(A1))
+
+
+# ---- elasticity pins (no reference integrand exists) ------------------------------
+def _free_all(P, ncomp):
+    meqn, neq = M.equation_numbers(np.ones(P.nnod * ncomp, dtype=np.int32))
+    return meqn, neq
+
+
+def test_elasticity_rigid_body_modes_and_uniform_strain_energy():
+    P = M.cube_patch(2, 4, jitter=0.15, seed=11)
+    E, nu = 1.0, 0.3
+    itg = M.Integrand(M.ELASTICITY, E=E, nu=nu)
+    meqn, neq = _free_all(P, 3)
+    rp, ci, A, b = oracle.system(P, itg, [3, 3, 3], meqn, neq)
+    K = sp.csr_matrix((A, ci, rp), shape=(neq, neq))
+    assert abs(K - K.T).max() <= 1e-15 * abs(K).max()
+    X = P.coefs
+    for mode in range(6):
+        if mode < 3:
+            u = np.zeros_like(X); u[:, mode] = 1.0
+        else:
+            w = np.zeros(3); w[mode - 3] = 1.0
+            u = np.cross(w, X)
+        assert np.max(np.abs(K @ u.ravel())) < 1e-12 * abs(K).max()
+    # uniform strain: u = eps x, energy u^T K u = eps:C:eps * volume (volume 1)
+    eps = np.array([[1.0, 0.3, -0.2], [0.3, -0.5, 0.1], [-0.2, 0.1, 0.7]])
+    u = X @ eps.T
+    lam, mu = M.lame(E, nu)
+    energy = lam * np.trace(eps) ** 2 + 2 * mu * np.sum(eps * eps)
+    assert abs(u.ravel() @ (K @ u.ravel()) - energy) < 1e-12 * energy
+    # patch test: interior nodes are in equilibrium under a linear field
+    r = (K @ u.ravel()).reshape(-1, 3)
+    n = P.n
+    k, j, i = np.meshgrid(range(n[2]), range(n[1]), range(n[0]), indexing="ij")
+    inner = ((i > 0) & (i < n[0] - 1) & (j > 0) & (j < n[1] - 1) & (k > 0) & (k < n[2] - 1)).ravel()
+    assert np.max(np.abs(r[inner])) < 1e-12 * abs(K).max()
+
+
+def test_traction_closed_form():
+    """a21: F_A = -integral of N_A over the face = separable 1D integrals; sum = -area."""
+    pb = problems.elasticity_cube(5, jitter=0.15)
+    b = np.zeros(pb.neq)
+    oracle.add_traction(pb.patch, *pb.traction[:1], pb.traction[1], pb.traction[2], 3, pb.meqn, b)
+    assert abs(b[2::3].sum() + 1.0) < 1e-14
+    assert np.max(np.abs(b[0::3])) == 0 and np.max(np.abs(b[1::3])) == 0
+    t = pb.patch.knots[1]
+    p = 2
+    g = np.array([(t[i + p + 1] - t[i]) / (p + 1) for i in range(pb.patch.n[1])])
+    n = pb.patch.n
+    bz = np.zeros(pb.ndof)
+    bz[pb.meqn > 0] = b[pb.meqn[pb.meqn > 0] - 1]
+    bz = bz.reshape(n[2], n[1], n[0], 3)[:, :, -1, 2]
+    assert np.max(np.abs(bz + np.outer(g, g))) < 1e-15
+
+
+def test_manufactured_solution_converges():
+    """Config-1 Poisson: the L2-like nodal error shrinks ~O(h^2) with refinement."""
+    errs = []
+    for nel in (4, 8, 16):
+        pb = problems.poisson_cube(nel)
+        rp, ci, A, b = oracle.system(pb.patch, pb.integrand, pb.ng, pb.meqn, pb.neq)
+        x, it, res = oracle.pcg(rp, ci, A, b, rtol=1e-12)
+        xd = np.zeros(pb.ndof); xd[pb.meqn > 0] = x
+        X = pb.patch.coefs
+        ue = np.sin(np.pi * X[:, 0]) * np.sin(np.pi * X[:, 1]) * np.sin(np.pi * X[:, 2])
+        errs.append(np.max(np.abs(xd - ue)))
+    assert errs[1] < errs[0] / 3 and errs[2] < errs[1] / 3

@@ -20,7 +20,7 @@ EXPORTS = [
     "ifem_gpu_init_assembly", "ifem_gpu_assemble", "ifem_gpu_assemble_neumann",
     "ifem_gpu_solver_defaults", "ifem_gpu_solve", "ifem_gpu_system_info", "ifem_gpu_get_system",
     "ifem_gpu_get_values_fp32", "ifem_gpu_get_solution", "ifem_gpu_expand_solution",
-    "ifem_gpu_spmv", "ifem_gpu_set_rhs", "ifem_gpu_device_pointers", "ifem_gpu_last_timings",
+    "ifem_gpu_spmv", "ifem_gpu_spmv_device", "ifem_gpu_set_rhs", "ifem_gpu_device_pointers", "ifem_gpu_last_timings",
 ]
 
 FP64, FP32 = 64, 32
@@ -89,6 +89,7 @@ def lib():
     L.ifem_gpu_expand_solution.argtypes = [vp, vp]
     L.ifem_gpu_spmv.argtypes = [vp, vp, vp]
     L.ifem_gpu_set_rhs.argtypes = [vp, vp]
+    L.ifem_gpu_spmv_device.argtypes = [vp, ip, vp, vp]
     L.ifem_gpu_device_pointers.argtypes = [vp] + [C.POINTER(vp)] * 5
     L.ifem_gpu_last_timings.argtypes = [vp, C.POINTER(dp), C.POINTER(dp)]
     _lib = L
@@ -235,6 +236,10 @@ class Session:
         y = np.zeros(self.neq)
         check(lib().ifem_gpu_spmv(self.h, _ptr(x), _ptr(y)))
         return y
+
+    def spmv_device(self, x_ptr: int, y_ptr: int, precision: int = FP64):
+        """y = K x on device pointers (no synchronisation)."""
+        check(lib().ifem_gpu_spmv_device(self.h, precision, C.c_void_p(x_ptr), C.c_void_p(y_ptr)))
 
     def set_rhs(self, b):
         b = np.ascontiguousarray(b, dtype=np.float64)

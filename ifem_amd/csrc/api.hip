@@ -27,6 +27,8 @@ static void free_system(Context& c)
   hipFree(c.d_eq);
   hipFree(c.d_rowptr);
   hipFree(c.d_colind);
+  hipFree(c.d_bcol);
+  c.d_bcol = nullptr;
   hipFree(c.d_val);
   hipFree(c.d_val32);
   hipFree(c.d_rhs);
@@ -342,6 +344,22 @@ int ifem_gpu_spmv(ifem_gpu_ctx* ctx, const double* x, double* y)
   hipFree(dx);
   hipFree(dy);
   return rc;
+}
+
+int ifem_gpu_spmv_device(ifem_gpu_ctx* ctx, int precision, const void* x, void* y)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (!x || !y) return fail(IFEM_GPU_EINVAL, "spmv_device: null vector");
+  if (precision == IFEM_GPU_FP64) {
+    if (!c.fp64_ready) return fail(IFEM_GPU_EINVAL, "spmv_device: no FP64 system");
+    return spmv_device(c, (const double*)x, (double*)y);
+  }
+  if (precision == IFEM_GPU_FP32) {
+    if (!c.fp32_ready) return fail(IFEM_GPU_EINVAL, "spmv_device: no FP32 values");
+    return spmv_device32(c, (const float*)x, (float*)y);
+  }
+  return fail(IFEM_GPU_EINVAL, "spmv_device: precision must be 64 or 32");
 }
 
 int ifem_gpu_set_rhs(ifem_gpu_ctx* ctx, const double* b)

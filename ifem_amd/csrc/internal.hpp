@@ -50,9 +50,8 @@ struct SolverWork {
   double* dinv = nullptr;
   float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
   float* dinv32 = nullptr;
-  double* partial = nullptr; // per-block partial sums
+  double* partial = nullptr; // per-block partial sums (3 x grid)
   double* scal = nullptr;    // device scalars
-  unsigned* counter = nullptr;
   double* h_scal = nullptr;  // pinned host mirror
   int nblocks = 0;
 };
@@ -71,6 +70,7 @@ struct Context {
   int64_t nnz = 0;
   int64_t* d_rowptr = nullptr;
   int32_t* d_colind = nullptr;
+  int32_t* d_bcol = nullptr;    // node-triple column index (3-comp, node-uniform)
   double* d_val = nullptr;
   float* d_val32 = nullptr;
   double* d_rhs = nullptr;
@@ -95,6 +95,7 @@ int assemble_traction(Context& c, int face0, const double* t, int ng);
 int solver_setup(Context& c);
 int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st);
 int spmv_device(Context& c, const double* d_x, double* d_y);
+int spmv_device32(Context& c, const float* d_x, float* d_y);
 void solver_free(Context& c);
 
 } // namespace ifem

@@ -7,14 +7,16 @@
 // contract (rtol, atol, maxits; src/LinAlg/LinSolParams.C:68-77).
 //
 // One iteration is three launches on a fixed grid:
-//   k_spmv_dot : q = A p, partial <p,q>;  last block: alpha = rz / <p,q>
-//   k_update   : x += alpha p, r -= alpha q, z = D^-1 r, partial <r,z>, <r,r>;
-//                last block: beta, convergence flag, iteration count
-//   k_pupdate  : p = z + beta p
-// Scalars stay on the device; the host reads them every check_every
-// iterations.  Reductions are two-level with fixed shapes (per-block partials
-// summed in index order by the last block to finish), so results are
-// bitwise reproducible run to run.
+//   k_spmv_dot : q = A p, per-block partials of <p,q>
+//   k_update   : alpha = rz / sum(partials <p,q>); x += alpha p,
+//                r -= alpha q, z = D^-1 r; per-block partials <r,z>, <r,r>
+//   k_pupdate  : beta = sum(<r,z>) / rz; p = z + beta p; block 0 records the
+//                scalars (rz, rr, iteration count, convergence flag)
+// Every block that needs a global sum reduces the 2048 per-block partials
+// itself, in index order, so all blocks agree bitwise and no grid-wide fence
+// or atomic is needed (a device-scope fence per block forces L2 write-backs
+// across the XCDs).  Scalars stay on the device; the host reads them every
+// check_every iterations.  Results are bitwise reproducible run to run.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -29,57 +31,39 @@ constexpr int kThreads = 256;
 constexpr int kGrid = 2048; // fixed grid: partial sums have a fixed layout
 
 enum Scal {
-  S_RZ = 0, S_ALPHA, S_BETA, S_RR, S_DONE, S_TOL2, S_ITER, S_PQ, S_RZN, S_MAXIT, S_BB, S_NSCAL
+  S_RZ = 0, S_RZ1, S_RR, S_DONE, S_TOL2, S_ITER, S_MAXIT, S_BB, S_NSCAL
 };
 
 template <typename A>
 __device__ inline A block_sum(A v, A* sh)
 {
-  // fixed-shape reduction: wave butterflies, then wave totals in order
+  // fixed-shape reduction: wave butterflies, then wave totals in order;
+  // the result is returned to every thread
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
   if (lane == 0) sh[wave] = v;
   __syncthreads();
   A s = 0;
-  if (threadIdx.x == 0)
-    for (int w = 0; w < kThreads / 64; w++) s += sh[w];
-  __syncthreads();
-  return s; // valid in thread 0
+  for (int w = 0; w < kThreads / 64; w++) s += sh[w];
+  return s;
 }
 
-__device__ inline double load_coherent(const double* p)
-{
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Sum of nparts partials in index order by one block (deterministic).
-__device__ inline double final_sum(const double* part, int nparts, double* sh)
+// Sum of nparts partials in index order, computed by the whole block.
+__device__ inline double sum_parts(const double* __restrict__ part, int nparts, double* sh)
 {
   double s = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += kThreads) s += load_coherent(part + i);
+  for (int i = threadIdx.x; i < nparts; i += kThreads) s += part[i];
   return block_sum<double>(s, sh);
 }
 
-__device__ inline bool last_block(unsigned* counter)
-{
-  __shared__ bool am_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned t = atomicAdd(counter, 1u);
-    am_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  return am_last;
-}
-
 // ---------------------------------------------------------------------------
-// SpMV q = A p (CSR), LANES lanes per row, plus partial <p,q>.
+// SpMV q = A p (CSR), LANES lanes per row; with DOT, per-block partial <p,q>.
 template <typename V, typename ACC, int LANES, bool DOT>
 __global__ __launch_bounds__(kThreads) void k_spmv_dot(
     int n, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
     const V* __restrict__ val, const V* __restrict__ p, V* __restrict__ q,
-    double* __restrict__ part, unsigned* counter, double* __restrict__ scal)
+    double* __restrict__ part_pq, const double* __restrict__ scal)
 {
   if (DOT && scal[S_DONE] != 0.0) return;
   __shared__ double sh[kThreads / 64];
@@ -99,20 +83,71 @@ __global__ __launch_bounds__(kThreads) void k_spmv_dot(
   }
   if (!DOT) return;
   const double bs = block_sum<double>(pq, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = bs;
-  if (last_block(counter)) {
-    const double tot = final_sum(part, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      scal[S_PQ] = tot;
-      if (tot != 0.0 && __builtin_isfinite(tot)) {
-        scal[S_ALPHA] = scal[S_RZ] / tot;
-      } else {
-        scal[S_ALPHA] = 0.0;
-        scal[S_DONE] = 2.0; // breakdown
-      }
-      *counter = 0u;
+  if (threadIdx.x == 0) part_pq[blockIdx.x] = bs;
+}
+
+// SpMV for 3-component systems whose DOFs are free or fixed per node: the
+// three rows of a node share one column pattern made of node triples, so the
+// gather reads one node index (bcol) per 9 values.  One wave per node; the
+// CSR values are read in place.  Blocks are mapped to the 8 XCDs in
+// contiguous node ranges so each XCD's L2 holds its slice of p.
+template <typename V, typename ACC, bool DOT>
+__global__ __launch_bounds__(kThreads) void k_spmv_node3(
+    int nnode, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ bcol,
+    const V* __restrict__ val, const V* __restrict__ p, V* __restrict__ q,
+    double* __restrict__ part_pq, const double* __restrict__ scal)
+{
+  if (DOT && scal[S_DONE] != 0.0) return;
+  __shared__ double sh[kThreads / 64];
+  constexpr int WPB = kThreads / 64;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nxcd = 8;
+  const int xcd = blockIdx.x % nxcd, bl = blockIdx.x / nxcd, nbl = gridDim.x / nxcd;
+  const int lo = (int)((int64_t)nnode * xcd / nxcd), hi = (int)((int64_t)nnode * (xcd + 1) / nxcd);
+  double pq = 0.0;
+  for (int nd = lo + bl * WPB + wave; nd < hi; nd += nbl * WPB) {
+    const int64_t r0 = rowptr[3 * nd];
+    const int cnt = (int)((rowptr[3 * nd + 1] - r0) / 3);
+    const int64_t nb = r0 / 9;
+    const V* v0 = val + r0;
+    const V* v1 = v0 + 3 * cnt;
+    const V* v2 = v1 + 3 * cnt;
+    ACC s0 = 0, s1 = 0, s2 = 0;
+    for (int k = lane; k < cnt; k += 64) {
+      const int64_t c = 3 * (int64_t)bcol[nb + k];
+      const ACC x0 = (ACC)p[c], x1 = (ACC)p[c + 1], x2 = (ACC)p[c + 2];
+      s0 += (ACC)v0[3 * k] * x0 + (ACC)v0[3 * k + 1] * x1 + (ACC)v0[3 * k + 2] * x2;
+      s1 += (ACC)v1[3 * k] * x0 + (ACC)v1[3 * k + 1] * x1 + (ACC)v1[3 * k + 2] * x2;
+      s2 += (ACC)v2[3 * k] * x0 + (ACC)v2[3 * k + 1] * x1 + (ACC)v2[3 * k + 2] * x2;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      s0 += __shfl_xor(s0, off, 64);
+      s1 += __shfl_xor(s1, off, 64);
+      s2 += __shfl_xor(s2, off, 64);
+    }
+    if (lane == 0) {
+      q[3 * nd] = (V)s0;
+      q[3 * nd + 1] = (V)s1;
+      q[3 * nd + 2] = (V)s2;
+      if (DOT)
+        pq += (double)((ACC)p[3 * nd] * s0) + (double)((ACC)p[3 * nd + 1] * s1) +
+              (double)((ACC)p[3 * nd + 2] * s2);
     }
   }
+  if (!DOT) return;
+  const double bs = block_sum<double>(pq, sh);
+  if (threadIdx.x == 0) part_pq[blockIdx.x] = bs;
+}
+
+// bcol[r0/9 + k] = colind[r0 + 3k] / 3 for each node's first row.
+__global__ void k_build_bcol(int nnode, const int64_t* __restrict__ rowptr,
+                             const int32_t* __restrict__ colind, int32_t* __restrict__ bcol)
+{
+  const int nd = blockIdx.x;
+  const int64_t r0 = rowptr[3 * nd];
+  const int cnt = (int)((rowptr[3 * nd + 1] - r0) / 3);
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x) bcol[r0 / 9 + k] = colind[r0 + 3 * k] / 3;
 }
 
 template <typename V>
@@ -120,12 +155,18 @@ __global__ __launch_bounds__(kThreads) void k_update(int n, V* __restrict__ x, V
                                                      V* __restrict__ z, const V* __restrict__ p,
                                                      const V* __restrict__ q,
                                                      const V* __restrict__ dinv,
-                                                     double* __restrict__ part, unsigned* counter,
-                                                     double* __restrict__ scal)
+                                                     const double* __restrict__ part_pq,
+                                                     double* __restrict__ part_rz,
+                                                     double* __restrict__ part_rr,
+                                                     double* __restrict__ scal, int parity)
 {
   if (scal[S_DONE] != 0.0) return;
   __shared__ double sh[kThreads / 64];
-  const V alpha = (V)scal[S_ALPHA];
+  const double pq = sum_parts(part_pq, gridDim.x, sh);
+  double a = 0.0;
+  if (pq != 0.0 && __builtin_isfinite(pq)) a = scal[S_RZ + parity] / pq;
+  else if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_DONE] = 2.0; // breakdown
+  const V alpha = (V)a;
   double rz = 0.0, rr = 0.0;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
     const V xi = x[i] + alpha * p[i];
@@ -140,42 +181,44 @@ __global__ __launch_bounds__(kThreads) void k_update(int n, V* __restrict__ x, V
   const double brz = block_sum<double>(rz, sh);
   const double brr = block_sum<double>(rr, sh);
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = brz;
-    part[gridDim.x + blockIdx.x] = brr;
-  }
-  if (last_block(counter)) {
-    const double trz = final_sum(part, gridDim.x, sh);
-    const double trr = final_sum(part + gridDim.x, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      scal[S_ITER] += 1.0;
-      scal[S_RR] = trr;
-      scal[S_BETA] = scal[S_RZ] != 0.0 ? trz / scal[S_RZ] : 0.0;
-      scal[S_RZ] = trz;
-      if (!(trr > scal[S_TOL2])) scal[S_DONE] = 1.0;       // converged (or NaN)
-      else if (scal[S_ITER] >= scal[S_MAXIT]) scal[S_DONE] = 3.0;
-      *counter = 0u;
-    }
+    part_rz[blockIdx.x] = brz;
+    part_rr[blockIdx.x] = brr;
   }
 }
 
 template <typename V>
 __global__ __launch_bounds__(kThreads) void k_pupdate(int n, V* __restrict__ p,
                                                       const V* __restrict__ z,
-                                                      const double* __restrict__ scal)
+                                                      const double* __restrict__ part_rz,
+                                                      const double* __restrict__ part_rr,
+                                                      double* __restrict__ scal, int parity)
 {
   if (scal[S_DONE] != 0.0) return;
-  const V beta = (V)scal[S_BETA];
+  __shared__ double sh[kThreads / 64];
+  const double rzn = sum_parts(part_rz, gridDim.x, sh);
+  const double rr = sum_parts(part_rr, gridDim.x, sh);
+  const double rzo = scal[S_RZ + parity];
+  const bool conv = !(rr > scal[S_TOL2]); // converged (or NaN)
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[S_RZ + 1 - parity] = rzn;
+    scal[S_RR] = rr;
+    scal[S_ITER] += 1.0;
+    if (conv) scal[S_DONE] = 1.0;
+    else if (scal[S_ITER] >= scal[S_MAXIT]) scal[S_DONE] = 3.0;
+  }
+  if (conv) return;
+  const V beta = (V)(rzo != 0.0 ? rzn / rzo : 0.0);
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
     p[i] = z[i] + beta * p[i];
 }
 
-// CG start from residual r (already in place): z = D^-1 r, p = z, rz, rr.
+// CG start from residual r (already in place): z = D^-1 r, p = z, partials.
 template <typename V>
 __global__ __launch_bounds__(kThreads) void k_start(int n, const V* __restrict__ r,
                                                     V* __restrict__ z, V* __restrict__ p,
                                                     const V* __restrict__ dinv,
-                                                    double* __restrict__ part,
-                                                    unsigned* counter, double* __restrict__ scal)
+                                                    double* __restrict__ part_rz,
+                                                    double* __restrict__ part_rr)
 {
   __shared__ double sh[kThreads / 64];
   double rz = 0.0, rr = 0.0;
@@ -190,29 +233,32 @@ __global__ __launch_bounds__(kThreads) void k_start(int n, const V* __restrict__
   const double brz = block_sum<double>(rz, sh);
   const double brr = block_sum<double>(rr, sh);
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = brz;
-    part[gridDim.x + blockIdx.x] = brr;
-  }
-  if (last_block(counter)) {
-    const double trz = final_sum(part, gridDim.x, sh);
-    const double trr = final_sum(part + gridDim.x, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      scal[S_RZ] = trz;
-      scal[S_RR] = trr;
-      scal[S_DONE] = (trr > scal[S_TOL2]) ? 0.0 : 1.0;
-      if (scal[S_ITER] >= scal[S_MAXIT]) scal[S_DONE] = 3.0;
-      *counter = 0u;
-    }
+    part_rz[blockIdx.x] = brz;
+    part_rr[blockIdx.x] = brr;
   }
 }
 
-// r = b - y (FP64) and ||r||^2 into scal[slot].
+// One block: scalars after k_start.
+__global__ __launch_bounds__(kThreads) void k_start_check(const double* __restrict__ part_rz,
+                                                          const double* __restrict__ part_rr,
+                                                          int nparts, double* __restrict__ scal)
+{
+  __shared__ double sh[kThreads / 64];
+  const double rz = sum_parts(part_rz, nparts, sh);
+  const double rr = sum_parts(part_rr, nparts, sh);
+  if (threadIdx.x == 0) {
+    scal[S_RZ] = rz;
+    scal[S_RR] = rr;
+    scal[S_DONE] = (rr > scal[S_TOL2]) ? 0.0 : 1.0;
+    if (scal[S_ITER] >= scal[S_MAXIT]) scal[S_DONE] = 3.0;
+  }
+}
+
+// r = b - y (FP64), per-block partials of ||r||^2.
 __global__ __launch_bounds__(kThreads) void k_residual(int n, const double* __restrict__ b,
                                                        const double* __restrict__ y,
                                                        double* __restrict__ r,
-                                                       double* __restrict__ part,
-                                                       unsigned* counter, double* __restrict__ scal,
-                                                       int slot)
+                                                       double* __restrict__ part)
 {
   __shared__ double sh[kThreads / 64];
   double rr = 0.0;
@@ -223,13 +269,15 @@ __global__ __launch_bounds__(kThreads) void k_residual(int n, const double* __re
   }
   const double brr = block_sum<double>(rr, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = brr;
-  if (last_block(counter)) {
-    const double t = final_sum(part, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      scal[slot] = t;
-      *counter = 0u;
-    }
-  }
+}
+
+// One block: scal[slot] = sum of partials.
+__global__ __launch_bounds__(kThreads) void k_sum_to(const double* __restrict__ part, int nparts,
+                                                     double* __restrict__ scal, int slot)
+{
+  __shared__ double sh[kThreads / 64];
+  const double t = sum_parts(part, nparts, sh);
+  if (threadIdx.x == 0) scal[slot] = t;
 }
 
 // Jacobi: inverse of the diagonal of A (CSR rows sorted: binary search).
@@ -287,21 +335,34 @@ static void launch_spmv(Context& c, const V* val, const V* p, V* q)
   const int L = lanes_for(c);
   SolverWork& w = c.w;
   const int grid = kGrid;
+  if (c.d_bcol) {
+    k_spmv_node3<V, ACC, DOT><<<grid, kThreads, 0, c.stream>>>(c.neq / 3, c.d_rowptr, c.d_bcol, val,
+                                                               p, q, w.partial, w.scal);
+    return;
+  }
   if (L == 64)
     k_spmv_dot<V, ACC, 64, DOT><<<grid, kThreads, 0, c.stream>>>(c.neq, c.d_rowptr, c.d_colind, val,
-                                                                 p, q, w.partial, w.counter, w.scal);
+                                                                 p, q, w.partial, w.scal);
   else if (L == 32)
     k_spmv_dot<V, ACC, 32, DOT><<<grid, kThreads, 0, c.stream>>>(c.neq, c.d_rowptr, c.d_colind, val,
-                                                                 p, q, w.partial, w.counter, w.scal);
+                                                                 p, q, w.partial, w.scal);
   else
     k_spmv_dot<V, ACC, 16, DOT><<<grid, kThreads, 0, c.stream>>>(c.neq, c.d_rowptr, c.d_colind, val,
-                                                                 p, q, w.partial, w.counter, w.scal);
+                                                                 p, q, w.partial, w.scal);
 }
 
 int spmv_device(Context& c, const double* d_x, double* d_y)
 {
   if (c.neq == 0) return IFEM_GPU_OK;
   launch_spmv<double, double, false>(c, c.d_val, d_x, d_y);
+  IFEM_HIP(hipGetLastError());
+  return IFEM_GPU_OK;
+}
+
+int spmv_device32(Context& c, const float* d_x, float* d_y)
+{
+  if (c.neq == 0) return IFEM_GPU_OK;
+  launch_spmv<float, float, false>(c, c.d_val32, d_x, d_y);
   IFEM_HIP(hipGetLastError());
   return IFEM_GPU_OK;
 }
@@ -313,7 +374,6 @@ void solver_free(Context& c)
   for (double* p : d) hipFree(p);
   float* f[] = {w.r32, w.z32, w.p32, w.q32, w.x32, w.dinv32};
   for (float* p : f) hipFree(p);
-  hipFree(w.counter);
   if (w.h_scal) hipHostFree(w.h_scal);
   w = SolverWork();
 }
@@ -328,12 +388,24 @@ int solver_setup(Context& c)
   for (double** p : d) IFEM_HIP(hipMalloc(p, sizeof(double) * n));
   float** f[] = {&w.r32, &w.z32, &w.p32, &w.q32, &w.x32, &w.dinv32};
   for (float** p : f) IFEM_HIP(hipMalloc(p, sizeof(float) * n));
-  IFEM_HIP(hipMalloc(&w.partial, sizeof(double) * 2 * kGrid));
+  IFEM_HIP(hipMalloc(&w.partial, sizeof(double) * 3 * kGrid));
   IFEM_HIP(hipMalloc(&w.scal, sizeof(double) * 16));
-  IFEM_HIP(hipMalloc(&w.counter, sizeof(unsigned)));
-  IFEM_HIP(hipMemsetAsync(w.counter, 0, sizeof(unsigned), c.stream));
   IFEM_HIP(hipMemsetAsync(w.x, 0, sizeof(double) * n, c.stream));
   IFEM_HIP(hipHostMalloc(&w.h_scal, sizeof(double) * 16, hipHostMallocDefault));
+  // node-triple column index for node-uniform 3-component systems
+  hipFree(c.d_bcol);
+  c.d_bcol = nullptr;
+  bool uniform = c.ncomp == 3 && c.neq % 3 == 0 && c.neq > 0;
+  for (int nd = 0; uniform && nd < c.ndof / 3; nd++) {
+    const int e0 = c.h_eq[3 * nd];
+    if (e0 >= 0) uniform = e0 % 3 == 0 && c.h_eq[3 * nd + 1] == e0 + 1 && c.h_eq[3 * nd + 2] == e0 + 2;
+    else uniform = c.h_eq[3 * nd + 1] < 0 && c.h_eq[3 * nd + 2] < 0;
+  }
+  if (uniform) {
+    IFEM_HIP(hipMalloc(&c.d_bcol, sizeof(int32_t) * (c.nnz / 9 + 1)));
+    k_build_bcol<<<c.neq / 3, 128, 0, c.stream>>>(c.neq / 3, c.d_rowptr, c.d_colind, c.d_bcol);
+    IFEM_HIP(hipGetLastError());
+  }
   w.nblocks = kGrid;
   IFEM_HIP(hipStreamSynchronize(c.stream));
   return IFEM_GPU_OK;
@@ -370,15 +442,20 @@ static int pcg_run(Context& c, const V* val, V* x, V* r, V* z, V* p, V* q, const
   s[S_MAXIT] = maxits;
   int rc = write_scalars(c, s);
   if (rc) return rc;
-  k_start<V><<<kGrid, kThreads, 0, c.stream>>>(n, r, z, p, dinv, w.partial, w.counter, w.scal);
+  double* part_pq = w.partial;
+  double* part_rz = w.partial + kGrid;
+  double* part_rr = w.partial + 2 * kGrid;
+  k_start<V><<<kGrid, kThreads, 0, c.stream>>>(n, r, z, p, dinv, part_rz, part_rr);
+  k_start_check<<<1, kThreads, 0, c.stream>>>(part_rz, part_rr, kGrid, w.scal);
   IFEM_HIP(hipGetLastError());
   int launched = 0;
   while (true) {
     for (int k = 0; k < check_every; k++) {
+      const int parity = (launched + k) & 1;
       launch_spmv<V, ACC, true>(c, val, p, q);
-      k_update<V><<<kGrid, kThreads, 0, c.stream>>>(n, x, r, z, p, q, dinv, w.partial, w.counter,
-                                                    w.scal);
-      k_pupdate<V><<<kGrid, kThreads, 0, c.stream>>>(n, p, z, w.scal);
+      k_update<V><<<kGrid, kThreads, 0, c.stream>>>(n, x, r, z, p, q, dinv, part_pq, part_rz,
+                                                    part_rr, w.scal, parity);
+      k_pupdate<V><<<kGrid, kThreads, 0, c.stream>>>(n, p, z, part_rz, part_rr, w.scal, parity);
     }
     IFEM_HIP(hipGetLastError());
     launched += check_every;
@@ -396,8 +473,8 @@ static int true_residual(Context& c, const double* x, double* r, double* rr)
 {
   SolverWork& w = c.w;
   launch_spmv<double, double, false>(c, c.d_val, x, w.q);
-  k_residual<<<kGrid, kThreads, 0, c.stream>>>(c.neq, c.d_rhs, w.q, r, w.partial, w.counter,
-                                               w.scal, S_BB);
+  k_residual<<<kGrid, kThreads, 0, c.stream>>>(c.neq, c.d_rhs, w.q, r, w.partial);
+  k_sum_to<<<1, kThreads, 0, c.stream>>>(w.partial, kGrid, w.scal, S_BB);
   IFEM_HIP(hipGetLastError());
   int rc = read_scalars(c);
   if (rc) return rc;
@@ -409,8 +486,8 @@ static int norm2_rhs(Context& c, double* bb)
 {
   SolverWork& w = c.w;
   k_fill<double><<<(c.neq + 255) / 256, 256, 0, c.stream>>>(c.neq, w.q, 0.0);
-  k_residual<<<kGrid, kThreads, 0, c.stream>>>(c.neq, c.d_rhs, w.q, w.z, w.partial, w.counter,
-                                               w.scal, S_BB);
+  k_residual<<<kGrid, kThreads, 0, c.stream>>>(c.neq, c.d_rhs, w.q, w.z, w.partial);
+  k_sum_to<<<1, kThreads, 0, c.stream>>>(w.partial, kGrid, w.scal, S_BB);
   IFEM_HIP(hipGetLastError());
   int rc = read_scalars(c);
   if (rc) return rc;

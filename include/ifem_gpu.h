@@ -178,6 +178,10 @@ int ifem_gpu_expand_solution(ifem_gpu_ctx* ctx, double* xdof);
 /* y = K x with host vectors (SparseMatrix::multiply,
  * src/LinAlg/SparseMatrix.C:578-605). */
 int ifem_gpu_spmv(ifem_gpu_ctx* ctx, const double* x, double* y);
+/* y = K x on device vectors (neq each, resident in HBM): precision
+ * IFEM_GPU_FP64 uses the FP64 values with double x, y; IFEM_GPU_FP32 the FP32
+ * values with float x, y.  Runs on the ctx stream, does not synchronize. */
+int ifem_gpu_spmv_device(ifem_gpu_ctx* ctx, int precision, const void* x, void* y);
 /* Overwrite the device RHS with a host vector (neq). */
 int ifem_gpu_set_rhs(ifem_gpu_ctx* ctx, const double* b);
 

@@ -505,10 +505,25 @@ static int assem_element(int nedof, const double* EK, const double* EV,
 }
 
 /* ---- a11/a12: ASMs3D::integrate element loop with ThreadGroups stripes ---
- * (src/ASM/ASMs3D.C:2110-2130; src/Utility/ThreadGroups.C:253-351): two
- * groups of stripes along the third direction, stripe width >= p3, so that
- * stripes of one group share no node; OpenMP runs the stripes of a group in
- * parallel and the groups one after the other. */
+ * (src/ASM/ASMs3D.C:2110-2130; src/Utility/ThreadGroups.C:253-351,444-479):
+ * the strip direction leaves the fewest elements over when cut into
+ * 2*threads stripes; threads are reduced until stripes are >= p wide; thread
+ * t owns stripe 2t (group 0) and 2t+1 (group 1), so stripes of one group share
+ * no node; OpenMP runs the stripes of a group in parallel, groups in turn. */
+static int strip_direction(long n1, long n2, long n3, long parts)
+{
+  long r1 = n1 - (n1 / parts) * parts, r2 = n2 - (n2 / parts) * parts, r3 = n3 - (n3 / parts) * parts;
+  if (r1 * n2 * n3 < n1 * r2 * n3 && r1 * n2 * n3 < n1 * n2 * r3) return 0;
+  if (n1 * r2 * n3 < r1 * n2 * n3 && n1 * r2 * n3 < n1 * n2 * r3) return 1;
+  if (n1 * n2 * r3 < r1 * n2 * n3 && n1 * n2 * r3 < n1 * r2 * n3) return 2;
+  if (r1 * n2 * n3 > n1 * r2 * n3) return n2 > n3 ? 1 : 2;
+  if (n1 * r2 * n3 > n1 * n2 * r3) return n1 > n3 ? 0 : 2;
+  if (n1 * n2 * r3 > r1 * n2 * n3) return n1 > n2 ? 0 : 1;
+  if (n1 >= n2 && n1 >= n3) return 0;
+  if (n2 >= n1 && n2 >= n3) return 1;
+  return 2;
+}
+
 int orc_assemble_range(const orc_patch* P, const orc_integrand* itg,
                        const int ng[3], int ncomp, const int* meqn, int neq,
                        const int64_t* rowptr, const int32_t* colind,
@@ -529,29 +544,54 @@ int orc_assemble_range(const orc_patch* P, const orc_integrand* itg,
 #endif
   const int nen = (P->p[0] + 1) * (P->p[1] + 1) * (P->p[2] + 1);
   const int nedof = ncomp * nen;
-  int parts = nthreads > 1 ? 2 * nthreads : 1;
-  int width = nz / parts;
-  while (parts > 1 && width < P->p[2]) {
+  const int ext[3] = {nel[0], nel[1], nz};
+  int threads = nthreads;
+  int parts = threads > 1 ? 2 * threads : 1;
+  const int dir = strip_direction(ext[0], ext[1], ext[2], parts);
+  const int els = ext[dir];
+  int stripsize = 0;
+  while (threads > 1 && (stripsize = els / parts) < P->p[dir]) {
+    threads--;
     parts -= 2;
-    width = parts > 0 ? nz / parts : nz;
   }
-  if (parts < 1) parts = 1;
-  const int nstripes = parts;
+  /* stripe bounds along dir: [lo, hi) for group g of thread t */
+  int* lo = (int*)malloc(sizeof(int) * 2 * (threads > 0 ? threads : 1));
+  int* hi = (int*)malloc(sizeof(int) * 2 * (threads > 0 ? threads : 1));
+  if (threads <= 1) {
+    threads = 1;
+    lo[0] = 0; hi[0] = els;
+    lo[1] = 0; hi[1] = 0;
+  } else {
+    int remainder = els - stripsize * parts;
+    int* sz = (int*)malloc(sizeof(int) * 2 * threads);
+    for (int t = 0; t < 2 * threads; t++) sz[t] = stripsize;
+    for (int i = 1; i <= remainder; i++) sz[(i % 2) * threads + threads - (i + 1) / 2]++;
+    int offs = 0;
+    for (int t = 0; t < threads; t++)
+      for (int g = 0; g < 2; g++) {
+        lo[g * threads + t] = offs;
+        offs += sz[g * threads + t];
+        hi[g * threads + t] = offs;
+      }
+    free(sz);
+  }
   int err = 0;
-  for (int g = 0; g < (nstripes > 1 ? 2 : 1); g++) {
-#pragma omp parallel num_threads(nthreads)
+  for (int g = 0; g < (threads > 1 ? 2 : 1); g++) {
+#pragma omp parallel num_threads(threads)
     {
       double* EK = (double*)malloc(sizeof(double) * nedof * nedof);
       double* EV = (double*)malloc(sizeof(double) * nedof);
       int* meen = (int*)malloc(sizeof(int) * nedof);
       int mnpc[ORC_MAXNEN];
 #pragma omp for schedule(static)
-      for (int st = g; st < nstripes; st += (nstripes > 1 ? 2 : 1)) {
-        int z0 = e3_begin + (int)((long)nz * st / nstripes);
-        int z1 = e3_begin + (int)((long)nz * (st + 1) / nstripes);
-        for (int e3 = z0; e3 < z1; e3++)
-          for (int e2 = 0; e2 < nel[1]; e2++)
-            for (int e1 = 0; e1 < nel[0]; e1++) {
+      for (int t = 0; t < threads; t++) {
+        int bl[3] = {0, 0, 0}, bh[3] = {ext[0], ext[1], ext[2]};
+        bl[dir] = lo[g * threads + t];
+        bh[dir] = hi[g * threads + t];
+        for (int i3 = bl[2]; i3 < bh[2]; i3++)
+          for (int e2 = bl[1]; e2 < bh[1]; e2++)
+            for (int e1 = bl[0]; e1 < bh[0]; e1++) {
+              const int e3 = e3_begin + i3;
               if (orc_element_matrices(P, itg, ng, e1, e2, e3, EK, EV)) { err = 1; continue; }
               orc_element_nodes(P, e1, e2, e3, mnpc);
               for (int a = 0; a < nen; a++)
@@ -564,6 +604,8 @@ int orc_assemble_range(const orc_patch* P, const orc_integrand* itg,
       free(meen);
     }
   }
+  free(lo);
+  free(hi);
   return err ? -1 : 0;
 }
 

@@ -1,0 +1,361 @@
+// IFEMGpu.C -- reference-interface classes on the libifem_gpu C ABI.
+#include "IFEMGpu.h"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+
+namespace ifem_gpu {
+
+static bool report(int rc, const char* where)
+{
+  if (rc == IFEM_GPU_OK) return true;
+  std::cerr << " *** " << where << ": " << ifem_gpu_last_error() << " (code " << rc << ")"
+            << std::endl;
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+std::string SettingMap::getStringValue(const std::string& key) const
+{
+  auto it = values.find(key);
+  return it == values.end() ? std::string() : it->second;
+}
+
+int SettingMap::getIntValue(const std::string& key) const
+{
+  auto it = values.find(key);
+  return it == values.end() ? 0 : std::atoi(it->second.c_str());
+}
+
+double SettingMap::getDoubleValue(const std::string& key) const
+{
+  auto it = values.find(key);
+  return it == values.end() ? 0.0 : std::atof(it->second.c_str());
+}
+
+LinSolParams::LinSolParams(LinAlg::LinearSystemType ls) : linSys(ls)
+{
+  addValue("type", "gmres");
+  addValue("pc", "default");
+  addValue("rtol", "1e-6");
+  addValue("atol", "1e-20");
+  addValue("dtol", "1e6");
+  addValue("maxits", "1000");
+  addValue("verbosity", "1");
+  addValue("precision", "fp64");
+  addValue("inner_rtol", "1e-4");
+}
+
+// Minimal reader of <tag>value</tag> children (no attributes, no nesting
+// beyond one level), enough for the <linearsolver> keys.
+bool LinSolParams::read(const std::string& xml)
+{
+  static const char* keys[] = {"type", "pc", "rtol", "atol", "dtol", "maxits", "verbosity",
+                               "precision", "inner_rtol"};
+  for (const char* k : keys) {
+    const std::string open = std::string("<") + k + ">", close = std::string("</") + k + ">";
+    size_t a = xml.find(open);
+    if (a == std::string::npos) continue;
+    size_t b = xml.find(close, a);
+    if (b == std::string::npos) {
+      std::cerr << " *** LinSolParams::read: unterminated <" << k << ">" << std::endl;
+      return false;
+    }
+    std::string v = xml.substr(a + open.size(), b - a - open.size());
+    size_t s = v.find_first_not_of(" \t\r\n"), e = v.find_last_not_of(" \t\r\n");
+    addValue(k, s == std::string::npos ? std::string() : v.substr(s, e - s + 1));
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+static void set_function(ifem_gpu_function& f, const std::vector<double>& modes)
+{
+  if (f.type == IFEM_GPU_FN_FOURIER || f.type == IFEM_GPU_FN_EXP_FOURIER) {
+    f.nmodes = (int)(modes.size() / 5);
+    f.modes = modes.data();
+  } else {
+    f.nmodes = 0;
+    f.modes = nullptr;
+  }
+}
+
+bool PoissonIntegrand::gpuDescription(ifem_gpu_integrand& d) const
+{
+  std::memset(&d, 0, sizeof d);
+  d.kind = IFEM_GPU_POISSON;
+  d.kappa = kappa;
+  d.source = source;
+  set_function(d.kappa, kappaModes);
+  set_function(d.source, sourceModes);
+  return true;
+}
+
+bool ElasticityIntegrand::gpuDescription(ifem_gpu_integrand& d) const
+{
+  std::memset(&d, 0, sizeof d);
+  d.kind = IFEM_GPU_ELASTICITY;
+  d.E = E;
+  d.nu = nu;
+  for (int i = 0; i < 3; i++) d.body[i] = body[i];
+  return true;
+}
+
+void ElasticityIntegrand::setTraction(int lIndex, double tx, double ty, double tz)
+{
+  traction[lIndex] = {tx, ty, tz};
+}
+
+bool ElasticityIntegrand::gpuTraction(int lIndex, double t[3]) const
+{
+  auto it = traction.find(lIndex);
+  if (it == traction.end()) return false;
+  for (int i = 0; i < 3; i++) t[i] = it->second[i];
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+SystemMatrix* SystemMatrix::create(void* adm, LinAlg::MatrixType mtype, const LinSolParams& spar)
+{
+  (void)adm;
+  if (mtype != LinAlg::GPU) {
+    std::cerr << " *** SystemMatrix::create: only LinAlg::GPU is available here" << std::endl;
+    return nullptr;
+  }
+  return new GPUMatrix(spar);
+}
+
+GPUMatrix::GPUMatrix(const LinSolParams& spar) : par(spar)
+{
+  report(ifem_gpu_ctx_create(0, nullptr, &ctx), "GPUMatrix");
+}
+
+GPUMatrix::~GPUMatrix()
+{
+  if (ctx) ifem_gpu_ctx_destroy(ctx);
+}
+
+void GPUMatrix::initAssembly(const SAM& sam, char)
+{
+  ok = false;
+  if (!ctx || !sam.patch) {
+    std::cerr << " *** GPUMatrix::initAssembly: no context or no patch in the SAM" << std::endl;
+    return;
+  }
+  const ASMs3D& P = *sam.patch;
+  const double* kp[3] = {P.knots[0].data(), P.knots[1].data(), P.knots[2].data()};
+  int pid = -1;
+  if (!report(ifem_gpu_patch_create(ctx, P.order, P.n, kp, P.coefs.data(), 0, &pid),
+              "GPUMatrix::initAssembly"))
+    return;
+  if (!report(ifem_gpu_sam_bind(ctx, pid, sam.ncomp, sam.ndof, sam.meqn.data(), sam.neq),
+              "GPUMatrix::initAssembly"))
+    return;
+  if (!report(ifem_gpu_init_assembly(ctx, &nnz), "GPUMatrix::initAssembly")) return;
+  neq = sam.neq;
+  ok = true;
+}
+
+bool GPUMatrix::multiply(const SystemVector& B, SystemVector& C) const
+{
+  if (!ok || B.dim() < neq) return false;
+  C.redim(neq);
+  return report(ifem_gpu_spmv(ctx, B.getRef(), C.getPtr()), "GPUMatrix::multiply");
+}
+
+bool GPUMatrix::solve(SystemVector& b, Real* rc)
+{
+  if (!ok || b.dim() != neq) return false;
+  const std::string type = par.getStringValue("type");
+  if (type != "cg") {
+    std::cerr << " *** GPUMatrix::solve: solver type '" << type
+              << "' is not available on the GPU (use <type>cg</type>)" << std::endl;
+    return false;
+  }
+  const std::string pc = par.getStringValue("pc");
+  if (pc != "jacobi" && pc != "default") {
+    std::cerr << " *** GPUMatrix::solve: preconditioner '" << pc << "' is not available"
+              << std::endl;
+    return false;
+  }
+  ifem_gpu_solver_params p;
+  ifem_gpu_solver_defaults(&p);
+  p.rtol = par.getDoubleValue("rtol");
+  p.atol = par.getDoubleValue("atol");
+  p.dtol = par.getDoubleValue("dtol");
+  p.maxits = par.getIntValue("maxits");
+  p.verbosity = par.getIntValue("verbosity");
+  p.precision = par.getStringValue("precision") == "fp32" ? IFEM_GPU_FP32 : IFEM_GPU_FP64;
+  if (par.hasValue("inner_rtol")) p.inner_rtol = par.getDoubleValue("inner_rtol");
+  if (!report(ifem_gpu_set_rhs(ctx, b.getRef()), "GPUMatrix::solve")) return false;
+  const int st = ifem_gpu_solve(ctx, &p, &stats);
+  if (st != IFEM_GPU_OK && st != IFEM_GPU_ENOCONV) return report(st, "GPUMatrix::solve");
+  if (!report(ifem_gpu_get_solution(ctx, b.getPtr()), "GPUMatrix::solve")) return false;
+  if (rc) *rc = stats.rel_residual;
+  if (p.verbosity > 1)
+    std::cout << "\tGPU CG: " << stats.iterations << " iterations, relative residual "
+              << stats.rel_residual << std::endl;
+  return report(st, "GPUMatrix::solve");
+}
+
+// ---------------------------------------------------------------------------
+bool SAM::expandSolution(const SystemVector& solVec, std::vector<Real>& dofVec,
+                         Real scaleSD) const
+{
+  if ((int)solVec.dim() < neq) return false;
+  dofVec.assign(ndof, 0.0);
+  const Real* x = solVec.getRef();
+  for (int i = 0; i < ndof; i++)
+    if (meqn[i] > 0) dofVec[i] = x[meqn[i] - 1];
+    else dofVec[i] = 0.0 * scaleSD; // homogeneous Dirichlet value
+  return true;
+}
+
+Real SAM::normL2(const std::vector<Real>& x) const
+{
+  if (x.empty()) return 0.0;
+  double s = 0.0;
+  for (double v : x) s += v * v;
+  return std::sqrt(s / x.size());
+}
+
+bool SAMpatch::init(const ASMs3D& pch, int nc)
+{
+  patch = &pch;
+  ncomp = nc;
+  nnod = pch.getNoNodes();
+  ndof = nnod * nc;
+  msc.assign(ndof, 1);
+  const int n1 = pch.n[0], n2 = pch.n[1], n3 = pch.n[2];
+  for (int k = 0; k < n3; k++)
+    for (int j = 0; j < n2; j++)
+      for (int i = 0; i < n1; i++) {
+        int m = 0;
+        auto face = [&](int f) { auto it = pch.dirichlet.find(f); return it == pch.dirichlet.end() ? 0 : it->second; };
+        if (i == 0) m |= face(1);
+        if (i == n1 - 1) m |= face(2);
+        if (j == 0) m |= face(3);
+        if (j == n2 - 1) m |= face(4);
+        if (k == 0) m |= face(5);
+        if (k == n3 - 1) m |= face(6);
+        const int node = i + n1 * (j + n2 * k);
+        for (int c = 0; c < nc; c++)
+          if ((m >> c) & 1) msc[node * nc + c] = 0;
+      }
+  // SAM::initSystemEquations: free DOFs numbered in DOF order
+  meqn.assign(ndof, 0);
+  neq = 0;
+  for (int i = 0; i < ndof; i++)
+    if (msc[i] == 1) meqn[i] = ++neq;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+bool AlgEqSystem::init(LinAlg::MatrixType mtype, const LinSolParams& spar)
+{
+  A.reset(SystemMatrix::create(nullptr, mtype, spar));
+  if (!A) return false;
+  A->initAssembly(sam);
+  b.redim(sam.getNoEquations());
+  GPUMatrix* G = dynamic_cast<GPUMatrix*>(A.get());
+  return G && G->ready();
+}
+
+void AlgEqSystem::initialize(bool initLHS)
+{
+  if (initLHS && A) A->init();
+  b.init();
+}
+
+bool AlgEqSystem::finalize(bool)
+{
+  GPUMatrix* G = dynamic_cast<GPUMatrix*>(A.get());
+  if (!G || !G->ready()) return false;
+  if (!A->endAssembly()) return false;
+  return report(ifem_gpu_get_system(G->context(), nullptr, nullptr, nullptr, b.getPtr()),
+                "AlgEqSystem::finalize");
+}
+
+// ---------------------------------------------------------------------------
+ASMs3D::ASMs3D(const int ord[3], const int ncoef[3], const std::vector<double> kn[3],
+               const std::vector<double>& cf)
+  : coefs(cf)
+{
+  for (int d = 0; d < 3; d++) {
+    order[d] = ord[d];
+    n[d] = ncoef[d];
+    knots[d] = kn[d];
+  }
+}
+
+std::unique_ptr<ASMs3D> ASMs3D::unitCube(int p, int nel)
+{
+  std::vector<double> t;
+  for (int i = 0; i <= p; i++) t.push_back(0.0);
+  for (int i = 1; i < nel; i++) t.push_back((double)i / nel);
+  for (int i = 0; i <= p; i++) t.push_back(1.0);
+  const int nc = nel + p;
+  std::vector<double> g(nc);
+  for (int i = 0; i < nc; i++) {
+    double s = 0.0;
+    for (int j = 1; j <= p; j++) s += t[i + j];
+    g[i] = s / p; // Greville abscissae: identity geometry
+  }
+  std::vector<double> cf;
+  cf.reserve((size_t)nc * nc * nc * 3);
+  for (int k = 0; k < nc; k++)
+    for (int j = 0; j < nc; j++)
+      for (int i = 0; i < nc; i++) {
+        cf.push_back(g[i]);
+        cf.push_back(g[j]);
+        cf.push_back(g[k]);
+      }
+  const int ord[3] = {p + 1, p + 1, p + 1}, ncf[3] = {nc, nc, nc};
+  const std::vector<double> kn[3] = {t, t, t};
+  return std::unique_ptr<ASMs3D>(new ASMs3D(ord, ncf, kn, cf));
+}
+
+static GPUMatrix* gpu_matrix(AlgEqSystem& glInt, const char* where)
+{
+  GPUMatrix* G = dynamic_cast<GPUMatrix*>(glInt.getMatrix());
+  if (!G || !G->ready()) {
+    std::cerr << " *** " << where << ": the global integral has no initialised GPU matrix"
+              << std::endl;
+    return nullptr;
+  }
+  return G;
+}
+
+bool ASMs3D::integrate(Integrand& integrand, AlgEqSystem& glInt, const TimeDomain&)
+{
+  GPUMatrix* G = gpu_matrix(glInt, "ASMs3D::integrate");
+  if (!G) return false;
+  ifem_gpu_integrand d;
+  if (!integrand.gpuDescription(d)) {
+    std::cerr << " *** ASMs3D::integrate: integrand kind not available on the GPU" << std::endl;
+    return false;
+  }
+  const int ng = nGauss > 0 ? nGauss : order[0];
+  const int ngv[3] = {ng, ng, ng};
+  if (!report(ifem_gpu_assemble(G->context(), &d, ngv, IFEM_GPU_FP64), "ASMs3D::integrate"))
+    return false;
+  // the FP32 inner solver needs the FP32 element integrals as well
+  if (G->wantsFP32())
+    return report(ifem_gpu_assemble(G->context(), &d, ngv, IFEM_GPU_FP32), "ASMs3D::integrate");
+  return true;
+}
+
+bool ASMs3D::integrate(Integrand& integrand, int lIndex, AlgEqSystem& glInt, const TimeDomain&)
+{
+  GPUMatrix* G = gpu_matrix(glInt, "ASMs3D::integrate(face)");
+  if (!G) return false;
+  double t[3];
+  if (!integrand.gpuTraction(lIndex % 10, t)) return true; // no load on this face
+  const int ng = nGauss > 0 ? nGauss : order[0];
+  return report(ifem_gpu_assemble_neumann(G->context(), lIndex % 10, t, ng),
+                "ASMs3D::integrate(face)");
+}
+
+} // namespace ifem_gpu

@@ -1,0 +1,118 @@
+// facade_test.cpp -- SIM-style driver through the reference-interface facade:
+// model -> SAMpatch -> AlgEqSystem(GPU) -> ASMs3D::integrate (volume, face)
+// -> SystemMatrix::solve -> SAM::expandSolution.  Prints one JSON line per
+// case for tests/test_facade.py.
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "IFEMGpu.h"
+
+using namespace ifem_gpu;
+
+static int poisson(int nel)
+{
+  auto pch = ASMs3D::unitCube(2, nel);
+  for (int f = 1; f <= 6; f++) pch->constrainFace(f, 1);
+  SAMpatch sam;
+  sam.init(*pch, 1);
+  LinSolParams par;
+  par.read("<linearsolver><type>cg</type><pc>jacobi</pc><rtol>1e-10</rtol>"
+           "<maxits>5000</maxits></linearsolver>");
+  AlgEqSystem eq(sam);
+  if (!eq.init(LinAlg::GPU, par)) return 1;
+  PoissonIntegrand itg;
+  itg.source.type = IFEM_GPU_FN_SINPROD;
+  itg.source.scale = 3.0 * M_PI * M_PI;
+  TimeDomain time;
+  eq.initialize(true);
+  if (!pch->integrate(itg, eq, time)) return 2;
+  if (!eq.finalize()) return 3;
+  StdVector& b = *static_cast<StdVector*>(eq.getVector());
+  double bsum = 0.0;
+  for (double v : b.v) bsum += v;
+  Real rc = 0.0;
+  if (!eq.getMatrix()->solve(b, &rc)) return 4;
+  std::vector<Real> sol;
+  sam.expandSolution(b, sol);
+  double err = 0.0;
+  for (int k = 0; k < pch->n[2]; k++)
+    for (int j = 0; j < pch->n[1]; j++)
+      for (int i = 0; i < pch->n[0]; i++) {
+        const int node = i + pch->n[0] * (j + pch->n[1] * k);
+        const double* X = &pch->coefs[3 * node];
+        double ue = std::sin(M_PI * X[0]) * std::sin(M_PI * X[1]) * std::sin(M_PI * X[2]);
+        err = std::fmax(err, std::fabs(sol[node] - ue));
+      }
+  const GPUMatrix* G = static_cast<const GPUMatrix*>(eq.getMatrix());
+  std::printf("{\"case\": \"poisson\", \"nel\": %d, \"neq\": %d, \"nnz\": %lld, \"bsum\": %.17g, "
+              "\"iterations\": %d, \"rel_residual\": %.6g, \"max_nodal_err\": %.6g, "
+              "\"normL2\": %.17g}\n",
+              nel, sam.getNoEquations(), (long long)G->nonZeros(), bsum, G->lastStats().iterations,
+              rc, err, sam.normL2(sol));
+  return 0;
+}
+
+static int elasticity(int nel, const char* precision)
+{
+  auto pch = ASMs3D::unitCube(2, nel);
+  pch->constrainFace(1, 7); // clamped at x = 0
+  SAMpatch sam;
+  sam.init(*pch, 3);
+  LinSolParams par;
+  par.addValue("type", "cg");
+  par.addValue("pc", "jacobi");
+  par.addValue("rtol", "1e-8");
+  par.addValue("maxits", "50000");
+  par.addValue("precision", precision);
+  AlgEqSystem eq(sam);
+  if (!eq.init(LinAlg::GPU, par)) return 11;
+  ElasticityIntegrand itg(1.0, 0.3);
+  itg.setTraction(2, 0.0, 0.0, -1.0); // face 2: x = 1
+  TimeDomain time;
+  eq.initialize(true);
+  if (!pch->integrate(itg, eq, time)) return 12;
+  for (int f = 1; f <= 6; f++)
+    if (!pch->integrate(itg, f, eq, time)) return 13;
+  if (!eq.finalize()) return 14;
+  StdVector& b = *static_cast<StdVector*>(eq.getVector());
+  double fz = 0.0;
+  for (size_t i = 2; i < b.v.size(); i += 3) fz += b.v[i];
+  Real rc = 0.0;
+  if (!eq.getMatrix()->solve(b, &rc)) return 15;
+  std::vector<Real> sol;
+  sam.expandSolution(b, sol);
+  double tip = 0.0;
+  for (size_t i = 2; i < sol.size(); i += 3) tip = std::fmin(tip, sol[i]);
+  const GPUMatrix* G = static_cast<const GPUMatrix*>(eq.getMatrix());
+  std::printf("{\"case\": \"elasticity\", \"precision\": \"%s\", \"nel\": %d, \"neq\": %d, "
+              "\"force_z\": %.17g, \"iterations\": %d, \"rel_residual\": %.6g, "
+              "\"min_uz\": %.17g, \"normL2\": %.17g}\n",
+              precision, nel, sam.getNoEquations(), fz, G->lastStats().iterations, rc, tip,
+              sam.normL2(sol));
+  return 0;
+}
+
+int main()
+{
+  int rc = poisson(16);
+  if (!rc) rc = elasticity(8, "fp64");
+  if (!rc) rc = elasticity(8, "fp32");
+  // unknown solver type is refused (bool false), as the reference does for
+  // unavailable back-ends
+  if (!rc) {
+    auto pch = ASMs3D::unitCube(1, 2);
+    SAMpatch sam;
+    sam.init(*pch, 1);
+    LinSolParams par; // default type gmres
+    AlgEqSystem eq(sam);
+    eq.init(LinAlg::GPU, par);
+    PoissonIntegrand itg;
+    TimeDomain time;
+    pch->integrate(itg, eq, time);
+    StdVector& b = *static_cast<StdVector*>(eq.getVector());
+    std::printf("{\"case\": \"gmres_refused\", \"ok\": %s}\n",
+                eq.getMatrix()->solve(b) ? "false" : "true");
+  }
+  return rc;
+}

@@ -1,0 +1,48 @@
+"""The reference-interface facade (ifem_amd/facade) driven like a SIM: the C++
+driver facade_test runs Poisson and elasticity through ASMs3D::integrate,
+AlgEqSystem, GPUMatrix::solve and SAM::expandSolution; its results are
+checked here against the oracle."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from ifem_amd import problems
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "ifem_amd", "facade_test")
+
+
+def test_facade_built():
+    assert os.path.exists(EXE), "build with __graft_entry__.build()"
+
+
+@pytest.mark.gpu
+def test_facade_driver_matches_oracle():
+    print("[gpu] facade driver", flush=True)
+    r = subprocess.run([EXE], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = {}
+    for line in r.stdout.splitlines():
+        if line.startswith("{"):
+            d = json.loads(line)
+            out[(d["case"], d.get("precision", ""))] = d
+    pz = out[("poisson", "")]
+    pb = problems.poisson_cube(16)
+    rp, ci, A, b = oracle.system(pb.patch, pb.integrand, pb.ng, pb.meqn, pb.neq)
+    assert pz["neq"] == pb.neq and pz["nnz"] == len(ci)
+    assert abs(pz["bsum"] - b.sum()) <= 1e-12 * abs(b.sum())
+    assert pz["rel_residual"] <= 1e-10
+    x, it, res = oracle.pcg(rp, ci, A, b, rtol=1e-12)
+    xd = np.zeros(pb.ndof); xd[pb.meqn > 0] = x
+    rms = np.sqrt(np.mean(xd ** 2))  # SAM::normL2 is an RMS
+    assert abs(pz["normL2"] - rms) <= 1e-9 * rms
+    assert pz["max_nodal_err"] < 0.02
+    e64, e32 = out[("elasticity", "fp64")], out[("elasticity", "fp32")]
+    assert abs(e64["force_z"] + 1.0) < 1e-12
+    assert e64["rel_residual"] <= 1e-8 and e32["rel_residual"] <= 1e-8
+    assert abs(e32["min_uz"] - e64["min_uz"]) <= 1e-6 * abs(e64["min_uz"])
+    assert out[("gmres_refused", "")]["ok"] is True

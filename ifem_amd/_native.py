@@ -20,7 +20,10 @@ EXPORTS = [
     "ifem_gpu_init_assembly", "ifem_gpu_assemble", "ifem_gpu_assemble_neumann",
     "ifem_gpu_solver_defaults", "ifem_gpu_solve", "ifem_gpu_system_info", "ifem_gpu_get_system",
     "ifem_gpu_get_values_fp32", "ifem_gpu_get_solution", "ifem_gpu_expand_solution",
-    "ifem_gpu_spmv", "ifem_gpu_spmv_device", "ifem_gpu_set_rhs", "ifem_gpu_device_pointers", "ifem_gpu_last_timings",
+    "ifem_gpu_spmv", "ifem_gpu_spmv_device", "ifem_gpu_set_rhs", "ifem_gpu_device_pointers",
+    "ifem_gpu_last_timings", "ifem_gpu_unet_create", "ifem_gpu_unet_get_params",
+    "ifem_gpu_unet_set_params", "ifem_gpu_unet_forward", "ifem_gpu_unet_gradient",
+    "ifem_gpu_unet_train_step", "ifem_gpu_set_learned_preconditioner",
 ]
 
 FP64, FP32 = 64, 32
@@ -46,6 +49,11 @@ class SolverParams(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("dtol", C.c_double),
                 ("maxits", C.c_int), ("precision", C.c_int), ("inner_rtol", C.c_double),
                 ("check_every", C.c_int), ("verbosity", C.c_int)]
+
+
+class UNetConfig(C.Structure):
+    _fields_ = [("levels", C.c_int), ("channels", C.c_int * 4), ("in_channels", C.c_int),
+                ("out_channels", C.c_int), ("seed", C.c_uint64), ("activation", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -92,6 +100,14 @@ def lib():
     L.ifem_gpu_spmv_device.argtypes = [vp, ip, vp, vp]
     L.ifem_gpu_device_pointers.argtypes = [vp] + [C.POINTER(vp)] * 5
     L.ifem_gpu_last_timings.argtypes = [vp, C.POINTER(dp), C.POINTER(dp)]
+    L.ifem_gpu_unet_create.argtypes = [vp, C.POINTER(UNetConfig), C.POINTER(C.c_int64)]
+    L.ifem_gpu_unet_get_params.argtypes = [vp, vp]
+    L.ifem_gpu_unet_set_params.argtypes = [vp, vp]
+    L.ifem_gpu_unet_forward.argtypes = [vp, vp, vp, ip, ip, ip, ip]
+    L.ifem_gpu_unet_gradient.argtypes = [vp, vp, vp, vp, ip, ip, ip, ip, vp, C.POINTER(C.c_float)]
+    L.ifem_gpu_unet_train_step.argtypes = [vp, vp, vp, vp, ip, ip, ip, ip, C.c_float,
+                                           C.POINTER(C.c_float)]
+    L.ifem_gpu_set_learned_preconditioner.argtypes = [vp, ip, ip, ip, ip, vp, ip, ip, dp]
     _lib = L
     return L
 
@@ -244,6 +260,55 @@ class Session:
     def set_rhs(self, b):
         b = np.ascontiguousarray(b, dtype=np.float64)
         check(lib().ifem_gpu_set_rhs(self.h, _ptr(b)))
+
+    # ---- learned preconditioner ------------------------------------------------
+    def unet_create(self, levels, channels, in_channels, out_channels, seed=1, linear=False):
+        cfg = UNetConfig()
+        cfg.levels = levels
+        for i, c in enumerate(channels):
+            cfg.channels[i] = int(c)
+        cfg.in_channels, cfg.out_channels, cfg.seed = in_channels, out_channels, seed
+        cfg.activation = 1 if linear else 0
+        n = C.c_int64(0)
+        check(lib().ifem_gpu_unet_create(self.h, C.byref(cfg), C.byref(n)))
+        self.unet_cfg = (levels, list(channels)[:levels], in_channels, out_channels)
+        self.unet_nparams = n.value
+        return n.value
+
+    def unet_params(self):
+        p = np.zeros(self.unet_nparams, dtype=np.float32)
+        check(lib().ifem_gpu_unet_get_params(self.h, _ptr(p)))
+        return p
+
+    def unet_set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.unet_nparams
+        check(lib().ifem_gpu_unet_set_params(self.h, _ptr(p)))
+
+    def unet_forward(self, x_ptr: int, y_ptr: int, N, D, H, W):
+        check(lib().ifem_gpu_unet_forward(self.h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), N, D, H, W))
+
+    def unet_gradient(self, x_ptr: int, t_ptr: int, N, D, H, W, w_ptr: int = 0):
+        g = np.zeros(self.unet_nparams, dtype=np.float32)
+        loss = C.c_float(0.0)
+        check(lib().ifem_gpu_unet_gradient(self.h, C.c_void_p(x_ptr), C.c_void_p(t_ptr),
+                                           C.c_void_p(w_ptr) if w_ptr else None, N, D, H, W,
+                                           _ptr(g), C.byref(loss)))
+        return loss.value, g
+
+    def unet_train_step(self, x_ptr: int, t_ptr: int, N, D, H, W, lr, want_loss=False,
+                        w_ptr: int = 0):
+        loss = C.c_float(0.0)
+        check(lib().ifem_gpu_unet_train_step(self.h, C.c_void_p(x_ptr), C.c_void_p(t_ptr),
+                                             C.c_void_p(w_ptr) if w_ptr else None, N, D, H, W, lr,
+                                             C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def set_learned(self, enable, D=0, H=0, W=0, aux_ptr: int = 0, naux=0, scale=1.0,
+                    features=0):
+        check(lib().ifem_gpu_set_learned_preconditioner(self.h, int(enable), D, H, W,
+                                                        C.c_void_p(aux_ptr) if aux_ptr else None,
+                                                        naux, features, scale))
 
     def timings(self):
         a = C.c_double(0.0); s = C.c_double(0.0)

@@ -40,6 +40,15 @@ static void free_system(Context& c)
   c.d_rhs = nullptr;
   c.pattern_ready = c.fp64_ready = c.fp32_ready = false;
   c.nnz = 0;
+  hipFree(c.d_netin);
+  hipFree(c.d_rold);
+  hipFree(c.d_rold32);
+  hipFree(c.d_aux);
+  c.d_netin = nullptr;
+  c.d_rold = nullptr;
+  c.d_rold32 = nullptr;
+  c.d_aux = nullptr;
+  c.learned = false;
 }
 
 } // namespace ifem
@@ -107,6 +116,7 @@ int ifem_gpu_ctx_destroy(ifem_gpu_ctx* ctx)
     hipFree(p.d_coefs);
   }
   for (void* p : ctx->c.scratch_tables) hipFree(p);
+  ctx->c.net.reset();
   hipEventDestroy(ctx->c.ev0);
   hipEventDestroy(ctx->c.ev1);
   if (ctx->c.own_stream) hipStreamDestroy(ctx->c.stream);
@@ -383,6 +393,13 @@ int ifem_gpu_device_pointers(ifem_gpu_ctx* ctx, void** rowptr, void** colind, vo
   if (rhs) *rhs = c.d_rhs;
   if (solution) *solution = c.w.x;
   return IFEM_GPU_OK;
+}
+
+int ifem_gpu_set_learned_preconditioner(ifem_gpu_ctx* ctx, int enable, int D, int H, int W,
+                                        const float* aux, int naux, int features, double scale)
+{
+  CHECK_CTX(ctx);
+  return set_learned(ctx->c, enable, D, H, W, aux, naux, features, scale);
 }
 
 int ifem_gpu_last_timings(ifem_gpu_ctx* ctx, double* assemble_ms, double* solve_ms)

@@ -8,6 +8,9 @@
 #include <vector>
 
 #include "ifem_gpu.h"
+#include "unet.hpp"
+
+#include <memory>
 
 namespace ifem {
 
@@ -83,6 +86,20 @@ struct Context {
 
   // owned device buffers for integrand mode tables
   std::vector<void*> scratch_tables;
+
+  // learned preconditioner
+  std::unique_ptr<UNet> net;
+  bool learned = false;
+  int lat[3] = {0, 0, 0}; // D, H, W of the free-DOF lattice
+  int naux = 0;
+  int features = 0;          // 0: [r, aux], 1: [r, r*aux]
+  float* d_aux = nullptr;   // naux * D*H*W
+  float* d_netin = nullptr; // (ncomp + naux) * D*H*W
+  float* d_netout = nullptr;
+  double* d_rold = nullptr; // previous residual (flexible CG)
+  float* d_rold32 = nullptr;
+  double learned_scale = 1.0;
+  bool last_fallback = false; // the last solve fell back to Jacobi-PCG
 };
 
 // assemble.hip
@@ -97,6 +114,8 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st);
 int spmv_device(Context& c, const double* d_x, double* d_y);
 int spmv_device32(Context& c, const float* d_x, float* d_y);
 void solver_free(Context& c);
+int set_learned(Context& c, int enable, int D, int H, int W, const float* aux, int naux,
+                int features, double scale);
 
 } // namespace ifem
 

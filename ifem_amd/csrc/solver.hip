@@ -388,7 +388,7 @@ int solver_setup(Context& c)
   for (double** p : d) IFEM_HIP(hipMalloc(p, sizeof(double) * n));
   float** f[] = {&w.r32, &w.z32, &w.p32, &w.q32, &w.x32, &w.dinv32};
   for (float** p : f) IFEM_HIP(hipMalloc(p, sizeof(float) * n));
-  IFEM_HIP(hipMalloc(&w.partial, sizeof(double) * 3 * kGrid));
+  IFEM_HIP(hipMalloc(&w.partial, sizeof(double) * 4 * kGrid));
   IFEM_HIP(hipMalloc(&w.scal, sizeof(double) * 16));
   IFEM_HIP(hipMemsetAsync(w.x, 0, sizeof(double) * n, c.stream));
   IFEM_HIP(hipHostMalloc(&w.h_scal, sizeof(double) * 16, hipHostMallocDefault));
@@ -469,6 +469,217 @@ static int pcg_run(Context& c, const V* val, V* x, V* r, V* z, V* p, V* q, const
   return IFEM_GPU_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Flexible CG with the learned preconditioner z = D^-1 r + s*|r|_rms*U(r/|r|_rms)
+// (Polak-Ribiere beta = <r_new - r_old, z_new> / <r_old, z_old>).
+
+// x += alpha p; r_old = r; r -= alpha q; partials ||r||^2
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_update_f(int n, V* __restrict__ x, V* __restrict__ r,
+                                                       V* __restrict__ rold,
+                                                       const V* __restrict__ p,
+                                                       const V* __restrict__ q,
+                                                       const double* __restrict__ part_pq,
+                                                       double* __restrict__ part_rr,
+                                                       double* __restrict__ scal, int parity)
+{
+  if (scal[S_DONE] != 0.0) return;
+  __shared__ double sh[kThreads / 64];
+  const double pq = sum_parts(part_pq, gridDim.x, sh);
+  double a = 0.0;
+  if (pq != 0.0 && __builtin_isfinite(pq)) a = scal[S_RZ + parity] / pq;
+  else if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_DONE] = 2.0;
+  const V alpha = (V)a;
+  double rr = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    const V ro = r[i];
+    const V rn = ro - alpha * q[i];
+    rold[i] = ro;
+    r[i] = rn;
+    x[i] += alpha * p[i];
+    rr += (double)(rn * rn);
+  }
+  const double brr = block_sum<double>(rr, sh);
+  if (threadIdx.x == 0) part_rr[blockIdx.x] = brr;
+}
+
+// partials of ||r||^2 (start of flexible CG)
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_norm_parts(int n, const V* __restrict__ r,
+                                                         double* __restrict__ part_rr)
+{
+  __shared__ double sh[kThreads / 64];
+  double rr = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
+    rr += (double)(r[i] * r[i]);
+  const double b = block_sum<double>(rr, sh);
+  if (threadIdx.x == 0) part_rr[blockIdx.x] = b;
+}
+
+// network input: channel c of node k = r[k*nc + c] / |r|_rms; with feat 1
+// also r^ * aux_j in channel nc + j*nc + c
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_to_lattice(int n, int nc, const V* __restrict__ r,
+                                                         float* __restrict__ netin,
+                                                         const float* __restrict__ aux, int naux,
+                                                         int feat,
+                                                         const double* __restrict__ part_rr,
+                                                         int nparts, const double* __restrict__ scal)
+{
+  if (scal[S_DONE] != 0.0) return;
+  __shared__ double sh[kThreads / 64];
+  const double rr = sum_parts(part_rr, nparts, sh);
+  const double inv = rr > 0.0 ? 1.0 / sqrt(rr / n) : 0.0;
+  const int vol = n / nc;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    const int node = i / nc, comp = i - node * nc;
+    const float v = (float)((double)r[i] * inv);
+    netin[(int64_t)comp * vol + node] = v;
+    if (feat)
+      for (int j = 0; j < naux; j++)
+        netin[(int64_t)(nc + j * nc + comp) * vol + node] = v * aux[(int64_t)j * vol + node];
+  }
+}
+
+// z = D^-1 r + s*|r|_rms*out ; partials <r,z> and <r_old,z>
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_from_lattice(
+    int n, int nc, const V* __restrict__ r, const V* __restrict__ rold, const V* __restrict__ dinv,
+    const float* __restrict__ out, V* __restrict__ z, double scale,
+    const double* __restrict__ part_rr, int nparts, double* __restrict__ part_rz,
+    double* __restrict__ part_roz, const double* __restrict__ scal)
+{
+  if (scal[S_DONE] != 0.0) return;
+  __shared__ double sh[kThreads / 64];
+  const double rr = sum_parts(part_rr, nparts, sh);
+  const double rms = sqrt(rr / n);
+  const int vol = n / nc;
+  double rz = 0.0, roz = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    const int node = i / nc, comp = i - node * nc;
+    const V zi = dinv[i] * r[i] + (V)(scale * rms * (double)out[(int64_t)comp * vol + node]);
+    z[i] = zi;
+    rz += (double)(r[i] * zi);
+    if (rold) roz += (double)(rold[i] * zi);
+  }
+  const double b1 = block_sum<double>(rz, sh);
+  const double b2 = block_sum<double>(roz, sh);
+  if (threadIdx.x == 0) {
+    part_rz[blockIdx.x] = b1;
+    part_roz[blockIdx.x] = b2;
+  }
+}
+
+// one block: scalars at the start of flexible CG
+__global__ __launch_bounds__(kThreads) void k_start_check_f(const double* __restrict__ part_rz,
+                                                            const double* __restrict__ part_rr,
+                                                            int nparts, double* __restrict__ scal)
+{
+  __shared__ double sh[kThreads / 64];
+  const double rz = sum_parts(part_rz, nparts, sh);
+  const double rr = sum_parts(part_rr, nparts, sh);
+  if (threadIdx.x == 0) {
+    scal[S_RZ] = rz;
+    scal[S_RR] = rr;
+    scal[S_DONE] = (rr > scal[S_TOL2]) ? 0.0 : 1.0;
+    if (!(rz > 0.0)) scal[S_DONE] = 2.0; // preconditioner not positive on r
+    if (scal[S_ITER] >= scal[S_MAXIT]) scal[S_DONE] = 3.0;
+  }
+}
+
+template <typename V>
+__global__ void k_copy(int n, const V* __restrict__ a, V* __restrict__ b)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_pupdate_f(int n, V* __restrict__ p,
+                                                        const V* __restrict__ z,
+                                                        const double* __restrict__ part_rz,
+                                                        const double* __restrict__ part_roz,
+                                                        const double* __restrict__ part_rr,
+                                                        double* __restrict__ scal, int parity)
+{
+  if (scal[S_DONE] != 0.0) return;
+  __shared__ double sh[kThreads / 64];
+  const double rzn = sum_parts(part_rz, gridDim.x, sh);
+  const double roz = sum_parts(part_roz, gridDim.x, sh);
+  const double rr = sum_parts(part_rr, gridDim.x, sh);
+  const double rzo = scal[S_RZ + parity];
+  const bool conv = !(rr > scal[S_TOL2]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[S_RZ + 1 - parity] = rzn;
+    scal[S_RR] = rr;
+    scal[S_ITER] += 1.0;
+    if (conv) scal[S_DONE] = 1.0;
+    else if (!(rzn > 0.0)) scal[S_DONE] = 2.0;
+    else if (scal[S_ITER] >= scal[S_MAXIT]) scal[S_DONE] = 3.0;
+  }
+  if (conv) return;
+  const V beta = (V)(rzo != 0.0 ? (rzn - roz) / rzo : 0.0);
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
+    p[i] = z[i] + beta * p[i];
+}
+
+template <typename V>
+static void apply_learned(Context& c, const V* r, const V* rold, const V* dinv, V* z,
+                          double* part_rr, double* part_rz, double* part_roz)
+{
+  const int n = c.neq;
+  k_to_lattice<V><<<kGrid, kThreads, 0, c.stream>>>(n, c.ncomp, r, c.d_netin, c.d_aux, c.naux,
+                                                    c.features, part_rr, kGrid, c.w.scal);
+  const float* out = c.net->forward(c.d_netin, 1, c.lat[0], c.lat[1], c.lat[2]);
+  k_from_lattice<V><<<kGrid, kThreads, 0, c.stream>>>(n, c.ncomp, r, rold, dinv, out, z,
+                                                      c.learned_scale, part_rr, kGrid, part_rz,
+                                                      part_roz, c.w.scal);
+}
+
+template <typename V, typename ACC>
+static int fcg_run(Context& c, const V* val, V* x, V* r, V* z, V* p, V* q, V* rold,
+                   const V* dinv, double tol2, int maxits, int check_every, int* iters,
+                   double* rr_out, int* status)
+{
+  SolverWork& w = c.w;
+  const int n = c.neq;
+  double s[S_NSCAL] = {0};
+  s[S_TOL2] = tol2;
+  s[S_MAXIT] = maxits;
+  int rc = write_scalars(c, s);
+  if (rc) return rc;
+  double* part_pq = w.partial;
+  double* part_rz = w.partial + kGrid;
+  double* part_rr = w.partial + 2 * kGrid;
+  double* part_roz = w.partial + 3 * kGrid;
+  k_norm_parts<V><<<kGrid, kThreads, 0, c.stream>>>(n, r, part_rr);
+  apply_learned<V>(c, r, (const V*)nullptr, dinv, z, part_rr, part_rz, part_roz);
+  k_start_check_f<<<1, kThreads, 0, c.stream>>>(part_rz, part_rr, kGrid, w.scal);
+  k_copy<V><<<(n + 255) / 256, 256, 0, c.stream>>>(n, z, p);
+  IFEM_HIP(hipGetLastError());
+  int launched = 0;
+  while (true) {
+    for (int k = 0; k < check_every; k++) {
+      const int parity = (launched + k) & 1;
+      launch_spmv<V, ACC, true>(c, val, p, q);
+      k_update_f<V><<<kGrid, kThreads, 0, c.stream>>>(n, x, r, rold, p, q, part_pq, part_rr,
+                                                      w.scal, parity);
+      apply_learned<V>(c, r, rold, dinv, z, part_rr, part_rz, part_roz);
+      k_pupdate_f<V><<<kGrid, kThreads, 0, c.stream>>>(n, p, z, part_rz, part_roz, part_rr,
+                                                       w.scal, parity);
+    }
+    IFEM_HIP(hipGetLastError());
+    launched += check_every;
+    rc = read_scalars(c);
+    if (rc) return rc;
+    if (w.h_scal[S_DONE] != 0.0 || launched >= maxits + check_every) break;
+  }
+  *iters = (int)w.h_scal[S_ITER];
+  *rr_out = w.h_scal[S_RR];
+  *status = (int)w.h_scal[S_DONE];
+  return IFEM_GPU_OK;
+}
+
 static int true_residual(Context& c, const double* x, double* r, double* rr)
 {
   SolverWork& w = c.w;
@@ -516,6 +727,7 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
   IFEM_HIP(hipGetLastError());
 
   int total_iters = 0, outer = 0;
+  bool use_learned = c.learned, fell_back = false;
   double rel = bnorm > 0 ? 1.0 : 0.0;
   bool converged = bnorm == 0.0;
   bool diverged = false;
@@ -525,8 +737,12 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
     while (true) {
       int it = 0, status = 0;
       double rr = 0.0;
-      rc = pcg_run<double, double>(c, c.d_val, w.x, w.r, w.z, w.p, w.q, w.dinv, tol * tol,
-                                   prm.maxits - total_iters, check, &it, &rr, &status);
+      if (use_learned)
+        rc = fcg_run<double, double>(c, c.d_val, w.x, w.r, w.z, w.p, w.q, c.d_rold, w.dinv,
+                                     tol * tol, prm.maxits - total_iters, check, &it, &rr, &status);
+      else
+        rc = pcg_run<double, double>(c, c.d_val, w.x, w.r, w.z, w.p, w.q, w.dinv, tol * tol,
+                                     prm.maxits - total_iters, check, &it, &rr, &status);
       if (rc) return rc;
       total_iters += it;
       outer++;
@@ -536,7 +752,14 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
       rel = std::sqrt(trr) / bnorm;
       if (std::sqrt(trr) <= tol) { converged = true; break; }
       if (!(std::sqrt(trr) <= prm.dtol * bnorm)) { diverged = true; break; }
-      if (total_iters >= prm.maxits || status == 2 || it == 0) break;
+      if (total_iters >= prm.maxits) break;
+      if (status == 2 || it == 0) {
+        // breakdown: the learned preconditioner falls back to Jacobi-PCG,
+        // restarted from the current iterate's true residual
+        if (!use_learned) break;
+        use_learned = false;
+        fell_back = true;
+      }
     }
   } else if (!converged) {
     // FP32 inner PCG inside FP64 iterative refinement
@@ -554,8 +777,14 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
       // residual target relative to the scaled (unit-norm) residual; never
       // ask the inner solve for more than the outer tolerance needs
       double target = std::fmax(inner, 0.5 * tol / rn);
-      rc = pcg_run<float, float>(c, c.d_val32, w.x32, w.r32, w.z32, w.p32, w.q32, w.dinv32,
-                                 target * target, prm.maxits - total_iters, check, &it, &rr, &status);
+      if (use_learned)
+        rc = fcg_run<float, float>(c, c.d_val32, w.x32, w.r32, w.z32, w.p32, w.q32, c.d_rold32,
+                                   w.dinv32, target * target, prm.maxits - total_iters, check, &it,
+                                   &rr, &status);
+      else
+        rc = pcg_run<float, float>(c, c.d_val32, w.x32, w.r32, w.z32, w.p32, w.q32, w.dinv32,
+                                   target * target, prm.maxits - total_iters, check, &it, &rr,
+                                   &status);
       if (rc) return rc;
       total_iters += it;
       outer++;
@@ -565,7 +794,11 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
       rel = std::sqrt(rr64) / bnorm;
       if (std::sqrt(rr64) <= tol) { converged = true; break; }
       if (!(std::sqrt(rr64) <= prm.dtol * bnorm)) { diverged = true; break; }
-      if (it == 0 || status == 2) break;
+      if (it == 0 || status == 2) {
+        if (!use_learned) break;
+        use_learned = false;
+        fell_back = true;
+      }
     }
   }
   IFEM_HIP(hipEventRecord(c.ev1, c.stream));
@@ -580,10 +813,62 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
     st->rel_residual = rel;
     st->solve_ms = ms;
   }
+  c.last_fallback = fell_back;
   if (diverged) return fail(IFEM_GPU_ENOCONV, "solve: diverged (residual above dtol*||b||)");
   if (!converged)
     return fail(IFEM_GPU_ENOCONV, "solve: not converged after " + std::to_string(total_iters) +
                                       " iterations, relative residual " + std::to_string(rel));
+  return IFEM_GPU_OK;
+}
+
+int set_learned(Context& c, int enable, int D, int H, int W, const float* aux, int naux,
+                int features, double scale)
+{
+  if (!enable) {
+    c.learned = false;
+    return IFEM_GPU_OK;
+  }
+  if (!c.net) return fail(IFEM_GPU_EINVAL, "set_learned_preconditioner: create the network first");
+  if (!c.pattern_ready) return fail(IFEM_GPU_EINVAL, "set_learned_preconditioner: no system");
+  if (D < 1 || H < 1 || W < 1 || (int64_t)D * H * W * c.ncomp != c.neq)
+    return fail(IFEM_GPU_EUNSUPPORTED,
+                "set_learned_preconditioner: the free DOFs are not a D x H x W lattice");
+  if (naux < 0 || (naux > 0 && !aux)) return fail(IFEM_GPU_EINVAL, "set_learned_preconditioner: aux");
+  if (features != 0 && features != 1)
+    return fail(IFEM_GPU_EINVAL, "set_learned_preconditioner: features must be 0 or 1");
+  const int cin = features ? c.ncomp * (1 + naux) : c.ncomp + naux;
+  if (c.net->cfg.in_channels != cin || c.net->cfg.out_channels != c.ncomp)
+    return fail(IFEM_GPU_EINVAL, "set_learned_preconditioner: network channels do not match");
+  int rc = c.net->ensure(1, D, H, W);
+  if (rc) return rc;
+  const int64_t vol = (int64_t)D * H * W;
+  hipFree(c.d_netin);
+  hipFree(c.d_rold);
+  hipFree(c.d_rold32);
+  hipFree(c.d_aux);
+  c.d_netin = nullptr;
+  c.d_rold = nullptr;
+  c.d_rold32 = nullptr;
+  c.d_aux = nullptr;
+  IFEM_HIP(hipMalloc(&c.d_netin, sizeof(float) * cin * vol));
+  if (naux > 0) {
+    IFEM_HIP(hipMalloc(&c.d_aux, sizeof(float) * naux * vol));
+    IFEM_HIP(hipMemcpyAsync(c.d_aux, aux, sizeof(float) * naux * vol, hipMemcpyDeviceToDevice,
+                            c.stream));
+  }
+  IFEM_HIP(hipMalloc(&c.d_rold, sizeof(double) * c.neq));
+  IFEM_HIP(hipMalloc(&c.d_rold32, sizeof(float) * c.neq));
+  if (naux > 0 && !features)
+    IFEM_HIP(hipMemcpyAsync(c.d_netin + c.ncomp * vol, aux, sizeof(float) * naux * vol,
+                            hipMemcpyDeviceToDevice, c.stream));
+  IFEM_HIP(hipStreamSynchronize(c.stream));
+  c.lat[0] = D;
+  c.lat[1] = H;
+  c.lat[2] = W;
+  c.naux = naux;
+  c.features = features;
+  c.learned_scale = scale;
+  c.learned = true;
   return IFEM_GPU_OK;
 }
 

@@ -1,0 +1,188 @@
+"""Learned preconditioner: data from the method's own Jacobi-PCG solves,
+training of the U-net on the GPU, and evaluation inside flexible CG.
+
+The network sees the residual on the lattice of free DOFs (normalised to unit
+RMS) plus auxiliary channels describing the problem (for diffusion: the
+log-conductivity at the nodes), and predicts the part of the error that the
+Jacobi step misses:  target = (e - D^-1 r) / |r|_rms  with e = x* - x_k and
+r = b - A x_k taken at PCG iterates x_k.  Inside flexible CG the
+preconditioner is z = D^-1 r + |r|_rms * U(r / |r|_rms, aux).
+
+All network arithmetic runs in the library's own HIP kernels
+(ifem_gpu_unet_*); torch only holds device buffers here.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import time
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import FP64, Session, model as M, problems, setup_problem
+
+
+def lattice(pb) -> tuple:
+    """(D, H, W, ncomp) of the free-DOF lattice, or raise if it is not a box."""
+    n1, n2, n3 = pb.patch.n
+    nc = pb.ncomp
+    free = (pb.meqn.reshape(n3, n2, n1, nc) > 0)
+    ks, js, is_ = np.nonzero(free.any(axis=3))
+    if ks.size == 0:
+        raise ValueError("no free DOFs")
+    D, H, W = ks.max() - ks.min() + 1, js.max() - js.min() + 1, is_.max() - is_.min() + 1
+    if D * H * W * nc != pb.neq or not free[ks.min():ks.max() + 1, js.min():js.max() + 1,
+                                            is_.min():is_.max() + 1].all():
+        raise ValueError("the free DOFs do not form a box lattice")
+    return int(D), int(H), int(W), nc
+
+
+def aux_channels(pb) -> np.ndarray:
+    """Auxiliary input channels on the lattice, (naux, D, H, W) float32.
+
+    Diffusion: log-conductivity at the free control points, scaled by the
+    field's sigma.  Problems with constant coefficients have no channel."""
+    D, H, W, nc = lattice(pb)
+    f = pb.integrand.kappa
+    if pb.integrand.kind != M.POISSON or f.type not in (M.FN_FOURIER, M.FN_EXP_FOURIER):
+        return np.zeros((0, D, H, W), dtype=np.float32)
+    n1, n2, n3 = pb.patch.n
+    X = pb.patch.coefs.reshape(n3, n2, n1, 3)
+    free = pb.meqn.reshape(n3, n2, n1) > 0
+    P = X[free].reshape(-1, 3)
+    md = f.modes
+    g = f.mean + f.scale * (md[:, 4][None, :] * np.cos(P @ md[:, :3].T + md[:, 3][None, :])).sum(1)
+    g = g / max(f.scale, 1e-30)
+    return g.reshape(1, D, H, W).astype(np.float32)
+
+
+@dataclasses.dataclass
+class NetSpec:
+    levels: int = 3
+    channels: Sequence[int] = (8, 16, 48)
+    seed: int = 1
+
+    def params(self, cin, cout):
+        n = 0
+        prev = cin
+        c = list(self.channels)
+        for l in range(self.levels):
+            n += prev * c[l] * 27 + c[l] + c[l] * c[l] * 27 + c[l]
+            prev = c[l]
+        for l in range(self.levels - 2, -1, -1):
+            n += (c[l + 1] + c[l]) * c[l] * 27 + c[l] + c[l] * c[l] * 27 + c[l]
+        n += c[0] * cout * 27 + cout
+        return n
+
+
+def jacobi_inverse(s: Session) -> np.ndarray:
+    rp, ci, A, _ = s.system()
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    d = np.zeros(len(rp) - 1)
+    m = ci == rows
+    d[rows[m]] = A[m]
+    return np.where(d != 0, 1.0 / d, 1.0)
+
+
+def pcg_pairs(s: Session, pb, iters: Sequence[int], torch_device="cuda", features=1):
+    """(input, target) tensors from Jacobi-PCG iterates of an assembled
+    system: x* from a tight solve, x_k = the k-th PCG iterate.  features as
+    in ifem_gpu_set_learned_preconditioner."""
+    import torch
+    D, H, W, nc = lattice(pb)
+    aux = aux_channels(pb)
+    naux = aux.shape[0]
+    s.solve(rtol=1e-12, maxits=200000, check_every=64)
+    xs = torch.from_numpy(s.solution()).to(torch_device)
+    dinv = torch.from_numpy(jacobi_inverse(s)).to(torch_device)
+    b = torch.from_numpy(s.system(values=False)[3]).to(torch_device)
+    ins, tgs = [], []
+    auxt = torch.from_numpy(aux).to(torch_device)
+    for k in iters:
+        st = s.solve(rtol=1e-30, maxits=int(k), check_every=min(int(k), 64))
+        xk = torch.from_numpy(s.solution()).to(torch_device)
+        Ax = torch.empty_like(xk)
+        s.spmv_device(xk.data_ptr(), Ax.data_ptr(), FP64)
+        torch.cuda.synchronize()
+        r = b - Ax
+        rms = torch.sqrt(torch.mean(r * r))
+        if not bool(rms > 0):
+            continue
+        e = xs - xk
+        rin = (r / rms).float().reshape(D, H, W, nc).permute(3, 0, 1, 2)
+        tgt = ((e - dinv * r) / rms).float().reshape(D, H, W, nc).permute(3, 0, 1, 2)
+        if naux and features:
+            mod = [rin * auxt[j:j + 1] for j in range(naux)]
+            ins.append(torch.cat([rin] + mod, 0))
+        else:
+            ins.append(torch.cat([rin, auxt], 0) if naux else rin)
+        tgs.append(tgt.contiguous())
+    return ins, tgs
+
+
+def default_iters(n: int, kmax: int) -> List[int]:
+    """n iteration counts spread geometrically over [1, kmax]."""
+    ks = np.unique(np.round(np.geomspace(1, kmax, n)).astype(int))
+    return [int(k) for k in ks]
+
+
+def target_scale(T) -> float:
+    """RMS of the training targets: the network learns T / scale, and the
+    preconditioner multiplies its output by the same scale."""
+    import torch
+    return float(torch.sqrt(torch.mean(T.double() ** 2)))
+
+
+def sample_weights(T):
+    """Per-sample loss weights 1/mean(t_n^2), normalised to mean 1: every pair
+    counts by its relative error, whatever the amplification of its residual."""
+    import torch
+    m = torch.mean(T.double().reshape(T.shape[0], -1) ** 2, dim=1)
+    w = 1.0 / torch.clamp(m, min=1e-30)
+    return (w / w.mean()).float().contiguous()
+
+
+def train(s: Session, X, T, epochs=20, batch=8, lr=2e-3, log=None, seed=0, weights=None):
+    """Adam on the pairs (X[i], T[i]) with a cosine learning-rate schedule and
+    optional per-sample loss weights.  T must already be divided by
+    target_scale(T)."""
+    import torch
+    n = X.shape[0]
+    D, H, W = X.shape[2:]
+    rng = np.random.default_rng(seed)
+    steps = epochs * max(1, n // batch)
+    step = 0
+    losses = []
+    for ep in range(epochs):
+        perm = rng.permutation(n)
+        tot, cnt = 0.0, 0
+        for i in range(0, n - batch + 1, batch):
+            idx = torch.from_numpy(perm[i:i + batch]).to(X.device)
+            xb = X.index_select(0, idx).contiguous()
+            tb = T.index_select(0, idx).contiguous()
+            wb = weights.index_select(0, idx).contiguous() if weights is not None else None
+            torch.cuda.synchronize()
+            a = lr * 0.5 * (1.0 + math.cos(math.pi * step / max(steps, 1)))
+            l = s.unet_train_step(xb.data_ptr(), tb.data_ptr(), batch, D, H, W, a, want_loss=True,
+                                  w_ptr=wb.data_ptr() if wb is not None else 0)
+            step += 1
+            tot += l
+            cnt = (i // batch) + 1
+        losses.append(tot / max(cnt, 1))
+        if log:
+            log(f"[learned] epoch {ep} loss {tot:.4e}")
+    torch.cuda.synchronize()
+    return losses
+
+
+def enable(s: Session, pb, scale=1.0, features=1):
+    """Turn the trained network into the preconditioner of session s."""
+    import torch
+    D, H, W, nc = lattice(pb)
+    aux = aux_channels(pb)
+    auxt = torch.from_numpy(np.ascontiguousarray(aux)).to("cuda") if aux.shape[0] else None
+    s.set_learned(True, D, H, W, auxt.data_ptr() if auxt is not None else 0, aux.shape[0], scale,
+                  features)
+    torch.cuda.synchronize()
+    return auxt

@@ -194,6 +194,46 @@ int ifem_gpu_device_pointers(ifem_gpu_ctx* ctx, void** rowptr, void** colind,
  * assembly kernel and of the last solve. */
 int ifem_gpu_last_timings(ifem_gpu_ctx* ctx, double* assemble_ms, double* solve_ms);
 
+/* ---- learned preconditioner -------------------------------------------------
+ * No counterpart in the reference (an addition of this library): a 3D
+ * convolutional U-net on the lattice of free DOFs maps a residual to an
+ * error correction; it is trained on the device by the library's own forward,
+ * backward and Adam kernels and applied inside flexible CG
+ * (z = D^-1 r + scale * |r|_rms * U(r/|r|_rms, aux)).
+ * Tensors are NCDHW float on the device; one network per context. */
+typedef struct {
+  int levels;          /* 1..4; grid sizes must divide by 2^(levels-1) */
+  int channels[4];     /* channels per level */
+  int in_channels;     /* residual components + auxiliary channels */
+  int out_channels;    /* residual components */
+  uint64_t seed;       /* He-normal initialisation seed */
+  int activation;      /* 0: ReLU after every hidden convolution, 1: none (linear) */
+} ifem_gpu_unet_config;
+
+int ifem_gpu_unet_create(ifem_gpu_ctx* ctx, const ifem_gpu_unet_config* cfg, int64_t* nparams);
+int ifem_gpu_unet_get_params(ifem_gpu_ctx* ctx, float* params);
+int ifem_gpu_unet_set_params(ifem_gpu_ctx* ctx, const float* params);
+/* y = U(x), N samples of D x H x W (device pointers). */
+int ifem_gpu_unet_forward(ifem_gpu_ctx* ctx, const float* x, float* y, int N, int D, int H, int W);
+/* Loss 0.5*sum_n w_n*sum((U(x_n)-t_n)^2)/N (w: N per-sample weights on the
+ * device, NULL for all ones) and its gradient w.r.t. the parameters (copied
+ * to grad if not NULL). */
+int ifem_gpu_unet_gradient(ifem_gpu_ctx* ctx, const float* x, const float* t, const float* w,
+                           int N, int D, int H, int W, float* grad, float* loss);
+/* One Adam step on that loss (loss may be NULL to skip the read-back). */
+int ifem_gpu_unet_train_step(ifem_gpu_ctx* ctx, const float* x, const float* t, const float* w,
+                             int N, int D, int H, int W, float lr, float* loss);
+
+/* Use the network as preconditioner of the next solves (enable != 0) or go
+ * back to Jacobi (enable == 0).  The free DOFs must form a D x H x W lattice
+ * in equation order with ncomp components per node (neq = D*H*W*ncomp);
+ * aux: naux auxiliary fields (D*H*W each, device pointer, copied).
+ * features 0: network input [r^, aux] (ncomp + naux channels);
+ * features 1: [r^, r^*aux_j per component] (ncomp*(1+naux) channels), which
+ * keeps a linear network linear in the residual. */
+int ifem_gpu_set_learned_preconditioner(ifem_gpu_ctx* ctx, int enable, int D, int H, int W,
+                                        const float* aux, int naux, int features, double scale);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,7 +13,8 @@ Package layout:
 from . import model, problems
 from ._native import FP32, FP64, GpuError, Session, lib
 
-__all__ = ["model", "problems", "Session", "GpuError", "FP32", "FP64", "lib", "setup_problem"]
+__all__ = ["model", "problems", "Session", "GpuError", "FP32", "FP64", "lib", "setup_problem",
+           "setup_batch"]
 
 
 def setup_problem(pb, session=None, precision=FP64):
@@ -28,4 +29,17 @@ def setup_problem(pb, session=None, precision=FP64):
         s.add_traction(face, t, ng)
     if precision == FP32:
         s.assemble(pb.integrand, pb.ng, FP32)
+    return s
+
+
+def setup_batch(problems, session=None, precisions=(FP64,)):
+    """Upload independent scalar Problems as one batch, build the global
+    pattern and assemble (one persistent launch per precision)."""
+    s = session or Session()
+    for pb in problems:
+        pid = s.add_patch(pb.patch)
+        s.batch_add(pid, pb.meqn, pb.neq, pb.integrand, pb.ng[0])
+    s.batch_init()
+    for prec in precisions:
+        s.batch_assemble(prec)
     return s

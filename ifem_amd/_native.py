@@ -24,6 +24,10 @@ EXPORTS = [
     "ifem_gpu_last_timings", "ifem_gpu_unet_create", "ifem_gpu_unet_get_params",
     "ifem_gpu_unet_set_params", "ifem_gpu_unet_forward", "ifem_gpu_unet_gradient",
     "ifem_gpu_unet_train_step", "ifem_gpu_set_learned_preconditioner",
+    "ifem_gpu_batch_clear", "ifem_gpu_batch_add", "ifem_gpu_batch_init", "ifem_gpu_batch_assemble",
+    "ifem_gpu_batch_autotune", "ifem_gpu_batch_solve", "ifem_gpu_batch_results",
+    "ifem_gpu_batch_patch_info", "ifem_gpu_batch_get_system", "ifem_gpu_batch_get_solution",
+    "ifem_gpu_batch_timings",
 ]
 
 FP64, FP32 = 64, 32
@@ -108,6 +112,19 @@ def lib():
     L.ifem_gpu_unet_train_step.argtypes = [vp, vp, vp, vp, ip, ip, ip, ip, C.c_float,
                                            C.POINTER(C.c_float)]
     L.ifem_gpu_set_learned_preconditioner.argtypes = [vp, ip, ip, ip, ip, vp, ip, ip, dp]
+    L.ifem_gpu_batch_clear.argtypes = [vp]
+    L.ifem_gpu_batch_add.argtypes = [vp, ip, C.POINTER(ip), ip, C.POINTER(Integrand), ip,
+                                     C.POINTER(ip)]
+    L.ifem_gpu_batch_init.argtypes = [vp, C.POINTER(ip), C.POINTER(C.c_int64)]
+    L.ifem_gpu_batch_assemble.argtypes = [vp, ip]
+    L.ifem_gpu_batch_autotune.argtypes = [vp, ip, C.POINTER(ip)]
+    L.ifem_gpu_batch_solve.argtypes = [vp, C.POINTER(SolverParams), C.POINTER(Stats)]
+    L.ifem_gpu_batch_results.argtypes = [vp, vp, vp, vp]
+    L.ifem_gpu_batch_patch_info.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(C.c_int64),
+                                            C.POINTER(ip)]
+    L.ifem_gpu_batch_get_system.argtypes = [vp, ip, vp, vp, vp, vp, vp]
+    L.ifem_gpu_batch_get_solution.argtypes = [vp, ip, vp]
+    L.ifem_gpu_batch_timings.argtypes = [vp, C.POINTER(dp), C.POINTER(dp), C.POINTER(dp)]
     _lib = L
     return L
 
@@ -194,6 +211,17 @@ class Session:
             m = np.ascontiguousarray(f.modes.reshape(-1), dtype=np.float64)
             keep.append(m)
             s.modes = m.ctypes.data_as(C.POINTER(C.c_double))
+        return s
+
+    def _integrand(self, itg, keep) -> Integrand:
+        s = Integrand()
+        s.kind = int(itg.kind)
+        s.E = float(itg.E)
+        s.nu = float(itg.nu)
+        s.kappa = self._function(itg.kappa, keep)
+        s.source = self._function(itg.source, keep)
+        for i in range(3):
+            s.body[i] = float(itg.body[i])
         return s
 
     def assemble(self, itg, ng, precision: int = FP64):
@@ -314,3 +342,82 @@ class Session:
         a = C.c_double(0.0); s = C.c_double(0.0)
         check(lib().ifem_gpu_last_timings(self.h, C.byref(a), C.byref(s)))
         return a.value, s.value
+
+    # ---- batch of independent patches -----------------------------------------
+    def batch_clear(self):
+        check(lib().ifem_gpu_batch_clear(self.h))
+        self.batch_size = 0
+
+    def batch_add(self, patch_id: int, meqn: np.ndarray, neq: int, itg, ng: int) -> int:
+        m = np.ascontiguousarray(meqn, dtype=np.int32)
+        keep = []
+        s = self._integrand(itg, keep)
+        idx = C.c_int(-1)
+        check(lib().ifem_gpu_batch_add(self.h, patch_id, m.ctypes.data_as(C.POINTER(C.c_int)), neq,
+                                       C.byref(s), ng, C.byref(idx)))
+        self.batch_size = idx.value + 1
+        return idx.value
+
+    def batch_init(self):
+        n = C.c_int(0)
+        nnz = C.c_int64(0)
+        check(lib().ifem_gpu_batch_init(self.h, C.byref(n), C.byref(nnz)))
+        return n.value, nnz.value
+
+    def batch_assemble(self, precision: int = FP64):
+        check(lib().ifem_gpu_batch_assemble(self.h, precision))
+
+    def batch_autotune(self, precision: int = FP64):
+        t = (C.c_int * 8)()
+        check(lib().ifem_gpu_batch_autotune(self.h, precision, t))
+        return {p: {"half": t[2 * p], "lanes": t[2 * p + 1]} for p in range(1, 4)}
+
+    def batch_solve(self, rtol=1e-8, atol=1e-300, maxits=100000, precision=FP64, inner_rtol=1e-4,
+                    check_every=32, dtol=1e30):
+        p = SolverParams()
+        lib().ifem_gpu_solver_defaults(C.byref(p))
+        p.rtol, p.atol, p.dtol, p.maxits = rtol, atol, dtol, maxits
+        p.precision, p.inner_rtol, p.check_every = precision, inner_rtol, check_every
+        st = Stats()
+        rc = lib().ifem_gpu_batch_solve(self.h, C.byref(p), C.byref(st))
+        if rc not in (0, 4):
+            check(rc)
+        n = self.batch_size
+        it = np.zeros(max(n, 1), dtype=np.int32)
+        cv = np.zeros(max(n, 1), dtype=np.int32)
+        rr = np.zeros(max(n, 1))
+        check(lib().ifem_gpu_batch_results(self.h, _ptr(it), _ptr(cv), _ptr(rr)))
+        return {"iterations": st.iterations, "outer": st.outer_iterations,
+                "converged": bool(st.converged), "rel_residual": st.rel_residual,
+                "solve_ms": st.solve_ms, "rc": rc, "patch_iterations": it[:n],
+                "patch_converged": cv[:n].astype(bool), "patch_rel_residual": rr[:n]}
+
+    def batch_patch_info(self, k: int):
+        n = C.c_int(0); nnz = C.c_int64(0); e0 = C.c_int(0)
+        check(lib().ifem_gpu_batch_patch_info(self.h, k, C.byref(n), C.byref(nnz), C.byref(e0)))
+        return n.value, nnz.value, e0.value
+
+    def batch_system(self, k: int, values=True, values32=False, rhs=True):
+        neq, nnz, _ = self.batch_patch_info(k)
+        rowptr = np.zeros(neq + 1, dtype=np.int64)
+        colind = np.zeros(max(nnz, 1), dtype=np.int32)
+        v = np.zeros(max(nnz, 1)) if values else None
+        v32 = np.zeros(max(nnz, 1), dtype=np.float32) if values32 else None
+        b = np.zeros(max(neq, 1)) if rhs else None
+        check(lib().ifem_gpu_batch_get_system(self.h, k, _ptr(rowptr), _ptr(colind),
+                                              _ptr(v) if values else None,
+                                              _ptr(v32) if values32 else None,
+                                              _ptr(b) if rhs else None))
+        return (rowptr, colind[:nnz], v[:nnz] if values else None,
+                v32[:nnz] if values32 else None, b[:neq] if rhs else None)
+
+    def batch_solution(self, k: int):
+        neq, _, _ = self.batch_patch_info(k)
+        x = np.zeros(max(neq, 1))
+        check(lib().ifem_gpu_batch_get_solution(self.h, k, _ptr(x)))
+        return x[:neq]
+
+    def batch_timings(self):
+        a = C.c_double(0.0); b = C.c_double(0.0); s = C.c_double(0.0)
+        check(lib().ifem_gpu_batch_timings(self.h, C.byref(a), C.byref(b), C.byref(s)))
+        return {"symbolic_ms": a.value, "assemble_ms": b.value, "solve_ms": s.value}

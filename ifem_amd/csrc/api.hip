@@ -6,6 +6,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "internal.hpp"
 
@@ -117,6 +118,7 @@ int ifem_gpu_ctx_destroy(ifem_gpu_ctx* ctx)
   }
   for (void* p : ctx->c.scratch_tables) hipFree(p);
   ctx->c.net.reset();
+  ctx->c.batch.reset();
   hipEventDestroy(ctx->c.ev0);
   hipEventDestroy(ctx->c.ev1);
   if (ctx->c.own_stream) hipStreamDestroy(ctx->c.stream);
@@ -407,6 +409,252 @@ int ifem_gpu_last_timings(ifem_gpu_ctx* ctx, double* assemble_ms, double* solve_
   if (!ctx) return fail(IFEM_GPU_EINVAL, "null context");
   if (assemble_ms) *assemble_ms = ctx->c.last_assemble_ms;
   if (solve_ms) *solve_ms = ctx->c.last_solve_ms;
+  return IFEM_GPU_OK;
+}
+
+// ---- batch of independent patches -----------------------------------------
+
+static int check_function(const ifem_gpu_function& f, const char* what)
+{
+  if (f.type < IFEM_GPU_FN_ZERO || f.type > IFEM_GPU_FN_EXP_FOURIER)
+    return fail(IFEM_GPU_EINVAL, std::string("batch_add: unknown ") + what + " type");
+  if ((f.type == IFEM_GPU_FN_FOURIER || f.type == IFEM_GPU_FN_EXP_FOURIER) && f.nmodes > 0 && !f.modes)
+    return fail(IFEM_GPU_EINVAL, std::string("batch_add: null ") + what + " mode table");
+  if (f.nmodes < 0) return fail(IFEM_GPU_EINVAL, std::string("batch_add: negative ") + what + " nmodes");
+  return IFEM_GPU_OK;
+}
+
+static int upload_modes(const ifem_gpu_function& f, std::vector<double>& host, double** dev)
+{
+  host.clear();
+  *dev = nullptr;
+  if ((f.type == IFEM_GPU_FN_FOURIER || f.type == IFEM_GPU_FN_EXP_FOURIER) && f.nmodes > 0) {
+    host.assign(f.modes, f.modes + 5 * (size_t)f.nmodes);
+    IFEM_HIP(hipMalloc(dev, sizeof(double) * host.size()));
+    IFEM_HIP(hipMemcpy(*dev, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice));
+  }
+  return IFEM_GPU_OK;
+}
+
+#define CHECK_BATCH(c, need_ready)                                                         \
+  do {                                                                                     \
+    if (!(c).batch || (c).batch->pp.empty())                                               \
+      return fail(IFEM_GPU_EINVAL, "batch: no patches added");                             \
+    if ((need_ready) && !(c).batch->ready)                                                 \
+      return fail(IFEM_GPU_EINVAL, "batch: call ifem_gpu_batch_init first");               \
+  } while (0)
+
+int ifem_gpu_batch_clear(ifem_gpu_ctx* ctx)
+{
+  CHECK_CTX(ctx);
+  IFEM_HIP(hipStreamSynchronize(ctx->c.stream));
+  ctx->c.batch.reset();
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_batch_add(ifem_gpu_ctx* ctx, int patch_id, const int* meqn, int neq,
+                       const ifem_gpu_integrand* itg, int ng, int* index)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (patch_id < 0 || patch_id >= (int)c.patches.size())
+    return fail(IFEM_GPU_EINVAL, "batch_add: unknown patch");
+  if (c.batch && c.batch->ready)
+    return fail(IFEM_GPU_EINVAL, "batch_add: the batch is initialised; clear it first");
+  const PatchRec& P = c.patches[patch_id];
+  if (P.p[0] != P.p[1] || P.p[0] != P.p[2] || P.p[0] < 1 || P.p[0] > 3)
+    return fail(IFEM_GPU_EUNSUPPORTED, "batch_add: equal degrees 1..3 in all directions are needed");
+  if (ng != P.p[0] + 1) return fail(IFEM_GPU_EUNSUPPORTED, "batch_add: the batch path uses ng = p+1");
+  if (!itg || itg->kind != IFEM_GPU_POISSON)
+    return fail(IFEM_GPU_EUNSUPPORTED, "batch_add: the batch path takes scalar (Poisson) integrands");
+  int rc = check_function(itg->kappa, "kappa");
+  if (!rc) rc = check_function(itg->source, "source");
+  if (rc) return rc;
+  const int64_t nnod = (int64_t)P.n[0] * P.n[1] * P.n[2];
+  if (!meqn) return fail(IFEM_GPU_EINVAL, "batch_add: null meqn");
+  std::vector<int> eq(nnod);
+  int last = 0;
+  for (int64_t i = 0; i < nnod; i++) {
+    if (meqn[i] < 0)
+      return fail(IFEM_GPU_EUNSUPPORTED, "batch_add: constrained (negative) equations are not supported");
+    if (meqn[i] > 0) {
+      if (meqn[i] != last + 1)
+        return fail(IFEM_GPU_EUNSUPPORTED, "batch_add: free DOFs must be numbered 1..neq in increasing order");
+      last = meqn[i];
+    }
+    eq[i] = meqn[i] - 1;
+  }
+  if (last != neq) return fail(IFEM_GPU_EINVAL, "batch_add: neq does not match meqn");
+  if (!c.batch) c.batch.reset(new Batch());
+  Batch& B = *c.batch;
+  int64_t total = 0;
+  for (const BatchPatch& q : B.pp) total += q.neq;
+  if (total + neq > INT32_MAX) return fail(IFEM_GPU_EUNSUPPORTED, "batch_add: more than 2^31-1 equations");
+  // global equation numbers: offset by the equations of the patches before
+  for (int& e : eq)
+    if (e >= 0) e += (int)total;
+  B.pp.emplace_back();
+  BatchPatch& bp = B.pp.back();
+  bp.patch = patch_id;
+  bp.p = P.p[0];
+  bp.ng = ng;
+  bp.nnod = (int)nnod;
+  bp.neq = neq;
+  bp.itg = *itg;
+  rc = upload_modes(itg->kappa, bp.kappa_modes, &bp.d_kmodes);
+  if (!rc) rc = upload_modes(itg->source, bp.source_modes, &bp.d_smodes);
+  if (!rc && hipMalloc(&bp.d_eq, sizeof(int) * (nnod > 0 ? nnod : 1)) != hipSuccess)
+    rc = fail(IFEM_GPU_EDEVICE, "batch_add: hipMalloc failed");
+  if (!rc && hipMemcpy(bp.d_eq, eq.data(), sizeof(int) * nnod, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(IFEM_GPU_EDEVICE, "batch_add: hipMemcpy failed");
+  bp.itg.kappa.modes = bp.kappa_modes.empty() ? nullptr : bp.kappa_modes.data();
+  bp.itg.source.modes = bp.source_modes.empty() ? nullptr : bp.source_modes.data();
+  if (rc) {
+    hipFree(bp.d_eq);
+    hipFree(bp.d_kmodes);
+    hipFree(bp.d_smodes);
+    B.pp.pop_back();
+    return rc;
+  }
+  if (index) *index = (int)B.pp.size() - 1;
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_batch_init(ifem_gpu_ctx* ctx, int* neq, int64_t* nnz)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  CHECK_BATCH(c, false);
+  if (c.batch->ready) return fail(IFEM_GPU_EINVAL, "batch_init: already initialised");
+  int rc = batch_init(c);
+  if (rc) return rc;
+  if (neq) *neq = c.batch->neq;
+  if (nnz) *nnz = c.batch->nnz;
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_batch_assemble(ifem_gpu_ctx* ctx, int precision)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  CHECK_BATCH(c, true);
+  if (precision != IFEM_GPU_FP64 && precision != IFEM_GPU_FP32)
+    return fail(IFEM_GPU_EINVAL, "batch_assemble: precision must be 64 or 32");
+  double ms = 0.0;
+  int rc = batch_assemble(c, precision, &ms);
+  if (rc) return rc;
+  c.batch->assemble_ms = ms;
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_batch_autotune(ifem_gpu_ctx* ctx, int precision, int* table)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  CHECK_BATCH(c, true);
+  if (precision != IFEM_GPU_FP64 && precision != IFEM_GPU_FP32)
+    return fail(IFEM_GPU_EINVAL, "batch_autotune: precision must be 64 or 32");
+  return batch_autotune(c, precision, table);
+}
+
+int ifem_gpu_batch_solve(ifem_gpu_ctx* ctx, const ifem_gpu_solver_params* params,
+                         ifem_gpu_stats* stats)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  CHECK_BATCH(c, true);
+  if (!params) return fail(IFEM_GPU_EINVAL, "batch_solve: null params");
+  if (params->precision != IFEM_GPU_FP64 && params->precision != IFEM_GPU_FP32)
+    return fail(IFEM_GPU_EINVAL, "batch_solve: precision must be 64 or 32");
+  return batch_solve(c, *params, stats);
+}
+
+int ifem_gpu_batch_results(ifem_gpu_ctx* ctx, int* iterations, int* converged, double* rel_residual)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  CHECK_BATCH(c, true);
+  const Batch& B = *c.batch;
+  for (size_t q = 0; q < B.pp.size(); q++) {
+    if (iterations) iterations[q] = B.iters[q];
+    if (converged) converged[q] = B.conv[q];
+    if (rel_residual) rel_residual[q] = B.relres[q];
+  }
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_batch_patch_info(ifem_gpu_ctx* ctx, int index, int* neq, int64_t* nnz, int* eq0)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  CHECK_BATCH(c, true);
+  const Batch& B = *c.batch;
+  if (index < 0 || index >= (int)B.pp.size()) return fail(IFEM_GPU_EINVAL, "batch: index out of range");
+  const BatchPatch& bp = B.pp[index];
+  int64_t r[2] = {0, 0};
+  IFEM_HIP(hipMemcpy(&r[0], B.d_rowptr + bp.eq0, sizeof(int64_t), hipMemcpyDeviceToHost));
+  IFEM_HIP(hipMemcpy(&r[1], B.d_rowptr + bp.eq0 + bp.neq, sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (neq) *neq = bp.neq;
+  if (nnz) *nnz = r[1] - r[0];
+  if (eq0) *eq0 = bp.eq0;
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_batch_get_system(ifem_gpu_ctx* ctx, int index, int64_t* rowptr, int32_t* colind,
+                              double* values, float* values32, double* rhs)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  CHECK_BATCH(c, true);
+  const Batch& B = *c.batch;
+  if (index < 0 || index >= (int)B.pp.size()) return fail(IFEM_GPU_EINVAL, "batch: index out of range");
+  const BatchPatch& bp = B.pp[index];
+  IFEM_HIP(hipStreamSynchronize(c.stream));
+  std::vector<int64_t> rp(bp.neq + 1);
+  IFEM_HIP(hipMemcpy(rp.data(), B.d_rowptr + bp.eq0, sizeof(int64_t) * (bp.neq + 1), hipMemcpyDeviceToHost));
+  const int64_t z0 = rp[0], nz = rp[bp.neq] - rp[0];
+  if (rowptr)
+    for (int i = 0; i <= bp.neq; i++) rowptr[i] = rp[i] - z0;
+  if (colind && nz > 0) {
+    IFEM_HIP(hipMemcpy(colind, B.d_colind + z0, sizeof(int32_t) * nz, hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < nz; k++) colind[k] -= bp.eq0;
+  }
+  if (values && nz > 0) {
+    if (!B.fp64_ready) return fail(IFEM_GPU_EINVAL, "batch_get_system: no FP64 values");
+    IFEM_HIP(hipMemcpy(values, B.d_val + z0, sizeof(double) * nz, hipMemcpyDeviceToHost));
+  }
+  if (values32 && nz > 0) {
+    if (!B.fp32_ready) return fail(IFEM_GPU_EINVAL, "batch_get_system: no FP32 values");
+    IFEM_HIP(hipMemcpy(values32, B.d_val32 + z0, sizeof(float) * nz, hipMemcpyDeviceToHost));
+  }
+  if (rhs && bp.neq > 0)
+    IFEM_HIP(hipMemcpy(rhs, B.d_rhs + bp.eq0, sizeof(double) * bp.neq, hipMemcpyDeviceToHost));
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_batch_get_solution(ifem_gpu_ctx* ctx, int index, double* x)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  CHECK_BATCH(c, true);
+  const Batch& B = *c.batch;
+  if (index < 0 || index >= (int)B.pp.size()) return fail(IFEM_GPU_EINVAL, "batch: index out of range");
+  const BatchPatch& bp = B.pp[index];
+  IFEM_HIP(hipStreamSynchronize(c.stream));
+  if (x && bp.neq > 0)
+    IFEM_HIP(hipMemcpy(x, B.x + bp.eq0, sizeof(double) * bp.neq, hipMemcpyDeviceToHost));
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_batch_timings(ifem_gpu_ctx* ctx, double* symbolic_ms, double* assemble_ms,
+                           double* solve_ms)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  CHECK_BATCH(c, true);
+  if (symbolic_ms) *symbolic_ms = c.batch->symbolic_ms;
+  if (assemble_ms) *assemble_ms = c.batch->assemble_ms;
+  if (solve_ms) *solve_ms = c.batch->solve_ms;
   return IFEM_GPU_OK;
 }
 

@@ -18,10 +18,12 @@
 // of Integrand::evalInt.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
 
+#include "devutil.hpp"
 #include "internal.hpp"
 
 namespace ifem {
@@ -163,55 +165,92 @@ struct AsmArgs {
   DevFunction kappa, source;
   double body[3];
   double lam, mu;
+  int variant;                         // batch kernel: 2*(p-1) + half chunk
 };
 
-// Chunk of layer elements processed together so that CH*NG^3 ~ one block.
-template <int P, int NG>
+// Chunk of layer elements processed together so that CH*NG^3 ~ one block;
+// HALF selects half of that (an autotuning variant: less LDS, more phases).
+template <int P, int NG, int HALF = 0>
 struct Chunk {
   static constexpr int P1 = P + 1;
   static constexpr int NB = P1 * P1;
   static constexpr int GP3 = NG * NG * NG;
-  static constexpr int v = (256 / GP3) < 1 ? 1 : ((256 / GP3) > NB ? NB : (256 / GP3));
+  static constexpr int full = (256 / GP3) < 1 ? 1 : ((256 / GP3) > NB ? NB : (256 / GP3));
+  static constexpr int v = HALF ? ((full + 1) / 2) : full;
 };
 
 constexpr int kBlock = 256;
 
-// T: arithmetic type of the element integrals (double: FP64 path, float:
-// FP32 path).  NC: components (1 Poisson, 3 elasticity).  SPLIT: number of
-// blocks per line sharing the row components (NC/SPLIT rows each).
-template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS>
-__global__ __launch_bounds__(kBlock) void k_assemble_lines(AsmArgs a)
-{
-  constexpr int P1 = P + 1;
-  constexpr int NEN = P1 * P1 * P1;
-  constexpr int GP3 = NG * NG * NG;
-  constexpr int NB = P1 * P1;
-  constexpr int W1 = 2 * P + 1;
-  constexpr int WN = W1 * W1 * W1;
-  constexpr int NCR = NC / SPLIT;
-  constexpr bool SACC = (NC == 3 && SPLIT == 1); // accumulate S_ij, apply lam/mu later
-  constexpr int NS = NC == 1 ? 1 : (SACC ? 9 : 3);
-  constexpr int CH = Chunk<P, NG>::v;
-  constexpr int GEO = (NC == 3 ? 9 : 0) + 3 * P1;
-  constexpr int NRHS = HAS_RHS ? P1 * NC : 1;
+// LDS of one line workgroup.  T: arithmetic type of the element integrals
+// (double: FP64 path, float: FP32 path).  NC: components (1 Poisson,
+// 3 elasticity).  SPLIT: number of blocks per line sharing the row
+// components (NC/SPLIT rows each).  HALF: half-size element chunks.
+template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF>
+struct LineLds {
+  static constexpr int P1 = P + 1;
+  static constexpr int NEN = P1 * P1 * P1;
+  static constexpr int GP3 = NG * NG * NG;
+  static constexpr int NB = P1 * P1;
+  static constexpr int W1 = 2 * P + 1;
+  static constexpr int WN = W1 * W1 * W1;
+  static constexpr int NCR = NC / SPLIT;
+  static constexpr bool SACC = (NC == 3 && SPLIT == 1); // accumulate S_ij, apply lam/mu later
+  static constexpr int NS = NC == 1 ? 1 : (SACC ? 9 : 3);
+  static constexpr int CH = Chunk<P, NG, HALF>::v;
+  static constexpr int GEO = (NC == 3 ? 9 : 0) + 3 * P1;
+  static constexpr int NRHS = HAS_RHS ? P1 * NC : 1;
 
-  __shared__ T win[P1][NCR][WN][NC];
-  __shared__ T geo[CH * GP3][GEO];
-  __shared__ T stage[CH][NEN][P1][NS];
-  __shared__ T rstage[HAS_RHS ? CH * GP3 : 1][NRHS];
-  __shared__ T rwin[P1][NC];
-  __shared__ double cp[P1][W1][W1][3];
-  __shared__ T tN1[NG][P1], tD1[NG][P1];
-  __shared__ T tN2[P1][NG][P1], tD2[P1][NG][P1];
-  __shared__ T tN3[P1][NG][P1], tD3[P1][NG][P1];
-  __shared__ double sp2[P1], sp3[P1], sp1;
-  __shared__ int be2[NB], be3[NB];
-  __shared__ int nbundle;
+  T win[P1][NCR][WN][NC];
+  T geo[CH * GP3][GEO];
+  T stage[CH][NEN][P1][NS];
+  T rstage[HAS_RHS ? CH * GP3 : 1][NRHS];
+  T rwin[P1][NC];
+  double cp[P1][W1][W1][3];
+  T tN1[NG][P1], tD1[NG][P1];
+  T tN2[P1][NG][P1], tD2[P1][NG][P1];
+  T tN3[P1][NG][P1], tD3[P1][NG][P1];
+  double sp2[P1], sp3[P1], sp1;
+  int be2[NB], be3[NB];
+  int nbundle;
+};
+
+template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF>
+__device__ __forceinline__ void line_body(const AsmArgs& a, const int line, const int split,
+                                          LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>& L)
+{
+  using LL = LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>;
+  constexpr int P1 = LL::P1;
+  constexpr int NEN = LL::NEN;
+  constexpr int GP3 = LL::GP3;
+  constexpr int W1 = LL::W1;
+  constexpr int WN = LL::WN;
+  constexpr int NCR = LL::NCR;
+  constexpr bool SACC = LL::SACC;
+  constexpr int NS = LL::NS;
+  constexpr int CH = LL::CH;
+
+  auto& win = L.win;
+  auto& geo = L.geo;
+  auto& stage = L.stage;
+  auto& rstage = L.rstage;
+  auto& rwin = L.rwin;
+  auto& cp = L.cp;
+  auto& tN1 = L.tN1;
+  auto& tD1 = L.tD1;
+  auto& tN2 = L.tN2;
+  auto& tD2 = L.tD2;
+  auto& tN3 = L.tN3;
+  auto& tD3 = L.tD3;
+  auto& sp2 = L.sp2;
+  auto& sp3 = L.sp3;
+  auto& sp1 = L.sp1;
+  auto& be2 = L.be2;
+  auto& be3 = L.be3;
+  auto& nbundle = L.nbundle;
 
   const int tid = threadIdx.x;
-  const int line = blockIdx.x;
   const int jc = line % a.n2, kc = line / a.n2;
-  const int split = blockIdx.y; // row components [split*NCR, split*NCR+NCR)
+  // split: row components [split*NCR, split*NCR+NCR)
 
   // bundle of elements touching the line, fixed order (e3 outer, e2 inner)
   if (tid == 0) {
@@ -533,6 +572,56 @@ __global__ __launch_bounds__(kBlock) void k_assemble_lines(AsmArgs a)
       if (tid < NC) rwin[slot][tid] = T(0);
       __syncthreads();
     }
+  }
+}
+
+template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS>
+__global__ __launch_bounds__(kBlock) void k_assemble_lines(AsmArgs a)
+{
+  __shared__ LineLds<P, NG, NC, T, SPLIT, HAS_RHS, 0> L;
+  line_body<P, NG, NC, T, SPLIT, HAS_RHS, 0>(a, blockIdx.x, blockIdx.y, L);
+}
+
+// Persistent batch assembly over many independent patches (degree p = 1..3,
+// ng = p+1).  Work items are (patch, line) pairs sorted by decreasing cost;
+// blocks take the next item from a queue counter, so load balances over
+// patches of uneven size.  Each line's rows are written by exactly one
+// block, so the result does not depend on which block takes which item.
+// The element-chunk variant (full / half) of each degree is chosen per patch
+// (AsmArgs::variant = 2*(p-1) + half); the LDS is one union of the variants.
+template <typename T, bool HAS_RHS>
+union BatchLds {
+  LineLds<1, 2, 1, T, 1, HAS_RHS, 0> v0;
+  LineLds<1, 2, 1, T, 1, HAS_RHS, 1> v1;
+  LineLds<2, 3, 1, T, 1, HAS_RHS, 0> v2;
+  LineLds<2, 3, 1, T, 1, HAS_RHS, 1> v3;
+  LineLds<3, 4, 1, T, 1, HAS_RHS, 0> v4;
+  LineLds<3, 4, 1, T, 1, HAS_RHS, 1> v5;
+};
+
+template <typename T, bool HAS_RHS>
+__global__ __launch_bounds__(kBlock) void k_assemble_batch(const AsmArgs* __restrict__ args,
+                                                           const int2* __restrict__ items,
+                                                           int nitems, int* __restrict__ queue)
+{
+  __shared__ BatchLds<T, HAS_RHS> U;
+  __shared__ int next;
+  while (true) {
+    if (threadIdx.x == 0) next = atomicAdd(queue, 1);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(next);
+    if (it >= nitems) break;
+    const int2 w = items[it];
+    const AsmArgs& a = args[w.x];
+    switch (a.variant) {
+    case 0: line_body<1, 2, 1, T, 1, HAS_RHS, 0>(a, w.y, 0, U.v0); break;
+    case 1: line_body<1, 2, 1, T, 1, HAS_RHS, 1>(a, w.y, 0, U.v1); break;
+    case 2: line_body<2, 3, 1, T, 1, HAS_RHS, 0>(a, w.y, 0, U.v2); break;
+    case 3: line_body<2, 3, 1, T, 1, HAS_RHS, 1>(a, w.y, 0, U.v3); break;
+    case 4: line_body<3, 4, 1, T, 1, HAS_RHS, 0>(a, w.y, 0, U.v4); break;
+    default: line_body<3, 4, 1, T, 1, HAS_RHS, 1>(a, w.y, 0, U.v5); break;
+    }
+    __syncthreads();
   }
 }
 
@@ -962,6 +1051,220 @@ int assemble_traction(Context& c, int face0, const double* t, int ng)
 #undef LAUNCH_TR
   IFEM_HIP(hipGetLastError());
   IFEM_HIP(hipStreamSynchronize(c.stream));
+  return IFEM_GPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batch of independent scalar patches (block-diagonal global system).  Each
+// patch is integrated as SIMbase::assembleSystem does per patch
+// (src/SIM/SIMbase.C:1229-1400), but every patch has its own equations and
+// all patches go through one persistent launch.
+
+static int batch_tables(Context& c, const PatchRec& Pt, BatchPatch& bp)
+{
+  GaussArgs ga{};
+  gauss_rule(bp.ng, ga.xi, ga.w);
+  for (int d = 0; d < 3; d++) {
+    const size_t cnt = (size_t)Pt.nel[d] * bp.ng * (bp.p + 1);
+    IFEM_HIP(hipMalloc(&bp.tabN[d], sizeof(double) * cnt));
+    IFEM_HIP(hipMalloc(&bp.tabD[d], sizeof(double) * cnt));
+    IFEM_HIP(hipMalloc(&bp.span[d], sizeof(double) * Pt.nel[d]));
+    const int tot = Pt.nel[d] * bp.ng;
+    const dim3 g((tot + 127) / 128), b(128);
+    switch (bp.p) {
+    case 1: k_basis_tables<1><<<g, b, 0, c.stream>>>(Pt.d_knots[d], Pt.nel[d], bp.ng, ga, bp.tabN[d], bp.tabD[d], bp.span[d]); break;
+    case 2: k_basis_tables<2><<<g, b, 0, c.stream>>>(Pt.d_knots[d], Pt.nel[d], bp.ng, ga, bp.tabN[d], bp.tabD[d], bp.span[d]); break;
+    default: k_basis_tables<3><<<g, b, 0, c.stream>>>(Pt.d_knots[d], Pt.nel[d], bp.ng, ga, bp.tabN[d], bp.tabD[d], bp.span[d]); break;
+    }
+  }
+  IFEM_HIP(hipGetLastError());
+  return IFEM_GPU_OK;
+}
+
+static DevFunction batch_function(const ifem_gpu_function& f, const double* d_modes)
+{
+  DevFunction df;
+  df.type = f.type;
+  df.nmodes = d_modes ? f.nmodes : 0;
+  df.scale = f.scale;
+  df.mean = f.mean;
+  df.modes = d_modes;
+  return df;
+}
+
+// Per-patch assembly arguments for both precisions (the chunk variant of
+// each degree comes from the batch tuning table).
+int batch_upload_args(Context& c)
+{
+  Batch& B = *c.batch;
+  const int np = (int)B.pp.size();
+  std::vector<AsmArgs> a64(np), a32(np);
+  for (int q = 0; q < np; q++) {
+    const BatchPatch& bp = B.pp[q];
+    const PatchRec& Pt = c.patches[bp.patch];
+    AsmArgs a{};
+    a.n1 = Pt.n[0];
+    a.n2 = Pt.n[1];
+    a.n3 = Pt.n[2];
+    a.nel1 = Pt.nel[0];
+    a.nel2 = Pt.nel[1];
+    a.nel3 = Pt.nel[2];
+    a.coefs = Pt.d_coefs;
+    a.tabN1 = bp.tabN[0]; a.tabD1 = bp.tabD[0]; a.span1 = bp.span[0];
+    a.tabN2 = bp.tabN[1]; a.tabD2 = bp.tabD[1]; a.span2 = bp.span[1];
+    a.tabN3 = bp.tabN[2]; a.tabD3 = bp.tabD[2]; a.span3 = bp.span[2];
+    GaussArgs ga{};
+    gauss_rule(bp.ng, ga.xi, ga.w);
+    for (int g = 0; g < kMaxNG; g++) a.gw[g] = ga.w[g];
+    a.eq = bp.d_eq;
+    a.rowptr = B.d_rowptr;
+    a.kappa = batch_function(bp.itg.kappa, bp.d_kmodes);
+    a.source = batch_function(bp.itg.source, bp.d_smodes);
+    a.variant = 2 * (bp.p - 1) + (B.tune[bp.p].half ? 1 : 0);
+    a.val64 = B.d_val;
+    a.val32 = nullptr;
+    a.rhs = B.d_rhs;
+    a64[q] = a;
+    a.val64 = nullptr;
+    a.val32 = B.d_val32;
+    a.rhs = nullptr;
+    a32[q] = a;
+  }
+  if (!B.d_args64) IFEM_HIP(hipMalloc(&B.d_args64, sizeof(AsmArgs) * (np > 0 ? np : 1)));
+  if (!B.d_args32) IFEM_HIP(hipMalloc(&B.d_args32, sizeof(AsmArgs) * (np > 0 ? np : 1)));
+  IFEM_HIP(hipMemcpy(B.d_args64, a64.data(), sizeof(AsmArgs) * np, hipMemcpyHostToDevice));
+  IFEM_HIP(hipMemcpy(B.d_args32, a32.data(), sizeof(AsmArgs) * np, hipMemcpyHostToDevice));
+  return IFEM_GPU_OK;
+}
+
+// Symbolic phase of the batch: per-patch basis tables, global CSR pattern
+// (patch q's rows and columns are its equations shifted by eq0), and the
+// (patch, line) work list sorted by decreasing estimated cost.
+int batch_symbolic(Context& c)
+{
+  Batch& B = *c.batch;
+  const int np = (int)B.pp.size();
+  int eq0 = 0;
+  for (BatchPatch& bp : B.pp) {
+    bp.eq0 = eq0;
+    eq0 += bp.neq;
+  }
+  B.neq = eq0;
+  const int neq = B.neq;
+  IFEM_HIP(hipEventRecord(c.ev0, c.stream));
+  for (BatchPatch& bp : B.pp) {
+    int rc = batch_tables(c, c.patches[bp.patch], bp);
+    if (rc) return rc;
+  }
+  int64_t* rowlen = nullptr;
+  int64_t* bsum = nullptr;
+  const int nb = (neq + kScanBlock - 1) / kScanBlock;
+  IFEM_HIP(hipMalloc(&B.d_rowptr, sizeof(int64_t) * (neq + 1)));
+  IFEM_HIP(hipMalloc(&rowlen, sizeof(int64_t) * (neq > 0 ? neq : 1)));
+  IFEM_HIP(hipMalloc(&bsum, sizeof(int64_t) * (nb > 0 ? nb : 1)));
+  IFEM_HIP(hipMemsetAsync(B.d_rowptr, 0, sizeof(int64_t) * (neq + 1), c.stream));
+  for (BatchPatch& bp : B.pp) {
+    const PatchRec& Pt = c.patches[bp.patch];
+    if (bp.neq > 0)
+      k_row_lengths<<<(bp.nnod + 255) / 256, 256, 0, c.stream>>>(Pt.n[0], Pt.n[1], Pt.n[2], bp.p, 1,
+                                                                   bp.d_eq, rowlen, bp.nnod);
+  }
+  if (neq > 0) {
+    k_scan_local<<<nb, kScanBlock, 0, c.stream>>>(rowlen, B.d_rowptr, bsum, neq);
+    k_scan_sums<<<1, 64, 0, c.stream>>>(bsum, nb);
+    k_scan_add<<<nb, kScanBlock, 0, c.stream>>>(B.d_rowptr, bsum, neq);
+  }
+  IFEM_HIP(hipGetLastError());
+  int64_t nnz = 0;
+  IFEM_HIP(hipMemcpyAsync(&nnz, B.d_rowptr + neq, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+  IFEM_HIP(hipStreamSynchronize(c.stream));
+  hipFree(rowlen);
+  hipFree(bsum);
+  B.nnz = nnz;
+  IFEM_HIP(hipMalloc(&B.d_colind, sizeof(int32_t) * (nnz > 0 ? nnz : 1)));
+  for (BatchPatch& bp : B.pp) {
+    const PatchRec& Pt = c.patches[bp.patch];
+    if (bp.neq > 0)
+      k_fill_columns<<<(bp.nnod + 3) / 4, 256, 0, c.stream>>>(Pt.n[0], Pt.n[1], Pt.n[2], bp.p, 1,
+                                                              bp.d_eq, B.d_rowptr, B.d_colind, bp.nnod);
+  }
+  IFEM_HIP(hipGetLastError());
+  IFEM_HIP(hipMalloc(&B.d_val, sizeof(double) * (nnz > 0 ? nnz : 1)));
+  IFEM_HIP(hipMalloc(&B.d_val32, sizeof(float) * (nnz > 0 ? nnz : 1)));
+  IFEM_HIP(hipMalloc(&B.d_rhs, sizeof(double) * (neq > 0 ? neq : 1)));
+  IFEM_HIP(hipMemsetAsync(B.d_rhs, 0, sizeof(double) * (neq > 0 ? neq : 1), c.stream));
+
+  // work list: one item per node line (j, k) of each patch
+  struct Item { double cost; int patch, line; };
+  std::vector<Item> items;
+  for (int q = 0; q < np; q++) {
+    const BatchPatch& bp = B.pp[q];
+    if (bp.neq == 0) continue;
+    const PatchRec& Pt = c.patches[bp.patch];
+    const int P1 = bp.p + 1;
+    const double per_elem = (double)P1 * P1 * P1 * bp.ng * bp.ng * bp.ng;
+    for (int k = 0; k < Pt.n[2]; k++)
+      for (int j = 0; j < Pt.n[1]; j++) {
+        const int c2 = std::min(j, Pt.nel[1] - 1) - std::max(0, j - bp.p) + 1;
+        const int c3 = std::min(k, Pt.nel[2] - 1) - std::max(0, k - bp.p) + 1;
+        items.push_back({per_elem * Pt.nel[0] * c2 * c3, q, j + Pt.n[1] * k});
+      }
+  }
+  std::stable_sort(items.begin(), items.end(),
+                   [](const Item& x, const Item& y) { return x.cost > y.cost; });
+  std::vector<int2> hi(items.size());
+  for (size_t i = 0; i < items.size(); i++) hi[i] = make_int2(items[i].patch, items[i].line);
+  B.nitems = (int)hi.size();
+  IFEM_HIP(hipMalloc(&B.d_items, sizeof(int2) * (hi.empty() ? 1 : hi.size())));
+  if (!hi.empty())
+    IFEM_HIP(hipMemcpy(B.d_items, hi.data(), sizeof(int2) * hi.size(), hipMemcpyHostToDevice));
+  IFEM_HIP(hipMalloc(&B.d_queue, sizeof(int)));
+
+  // persistent grid: as many blocks as stay resident
+  hipDeviceProp_t prop;
+  IFEM_HIP(hipGetDeviceProperties(&prop, c.device));
+  int per_cu64 = 0, per_cu32 = 0;
+  IFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu64, reinterpret_cast<const void*>(&k_assemble_batch<double, true>), kBlock, 0));
+  IFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu32, reinterpret_cast<const void*>(&k_assemble_batch<float, false>), kBlock, 0));
+  B.grid64 = std::max(1, std::min(B.nitems, std::max(1, per_cu64) * prop.multiProcessorCount));
+  B.grid32 = std::max(1, std::min(B.nitems, std::max(1, per_cu32) * prop.multiProcessorCount));
+  int rc = batch_upload_args(c);
+  if (rc) return rc;
+  IFEM_HIP(hipEventRecord(c.ev1, c.stream));
+  IFEM_HIP(hipEventSynchronize(c.ev1));
+  float ms = 0.f;
+  IFEM_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  B.symbolic_ms = ms;
+  return IFEM_GPU_OK;
+}
+
+// Numeric assembly of every patch of the batch in one persistent launch:
+// FP64 writes K and b, FP32 writes the FP32 copy of K.  Returns the launch's
+// device time in *ms.
+int batch_assemble(Context& c, int precision, double* ms_out)
+{
+  Batch& B = *c.batch;
+  const bool fp32 = precision == IFEM_GPU_FP32;
+  k_fill<int><<<1, 64, 0, c.stream>>>(1, B.d_queue, 0);
+  IFEM_HIP(hipEventRecord(c.ev0, c.stream));
+  if (B.nitems > 0) {
+    if (fp32)
+      k_assemble_batch<float, false><<<B.grid32, kBlock, 0, c.stream>>>(
+          static_cast<const AsmArgs*>(B.d_args32), B.d_items, B.nitems, B.d_queue);
+    else
+      k_assemble_batch<double, true><<<B.grid64, kBlock, 0, c.stream>>>(
+          static_cast<const AsmArgs*>(B.d_args64), B.d_items, B.nitems, B.d_queue);
+  }
+  IFEM_HIP(hipGetLastError());
+  IFEM_HIP(hipEventRecord(c.ev1, c.stream));
+  IFEM_HIP(hipEventSynchronize(c.ev1));
+  float ms = 0.f;
+  IFEM_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  if (ms_out) *ms_out = ms;
+  if (fp32) B.fp32_ready = true;
+  else B.fp64_ready = true;
   return IFEM_GPU_OK;
 }
 

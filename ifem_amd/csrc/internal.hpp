@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "batch.hpp"
 #include "ifem_gpu.h"
 #include "unet.hpp"
 
@@ -100,6 +101,9 @@ struct Context {
   float* d_rold32 = nullptr;
   double learned_scale = 1.0;
   bool last_fallback = false; // the last solve fell back to Jacobi-PCG
+
+  // batch of independent patches (batch.hip)
+  std::unique_ptr<Batch> batch;
 };
 
 // assemble.hip
@@ -107,6 +111,14 @@ int build_pattern(Context& c);
 int assemble_volume(Context& c, const ifem_gpu_integrand& itg, const int ng[3],
                     int precision);
 int assemble_traction(Context& c, int face0, const double* t, int ng);
+int batch_symbolic(Context& c);
+int batch_upload_args(Context& c);
+int batch_assemble(Context& c, int precision, double* ms);
+
+// batch.hip
+int batch_init(Context& c);
+int batch_solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st);
+int batch_autotune(Context& c, int precision, int* table);
 
 // solver.hip
 int solver_setup(Context& c);

@@ -9,6 +9,8 @@ equation numbers and an optional traction load, built by ``ifem_amd.model``.
                         clamped at x = 0, uniform traction (0,0,-1) on x = 1.
 * ``diffusion_cube``    -div(kappa grad u) = f with log-normal kappa (random
                         Fourier Gaussian field) and a random smooth f, u = 0.
+* ``diffusion_batch``   many independent diffusion patches of uneven size and
+                        degree, each with its own kappa and f.
 """
 from __future__ import annotations
 
@@ -73,6 +75,31 @@ def diffusion_cube(nel: int = 48, p: int = 2, ng: Optional[int] = None, sigma: f
     itg = M.Integrand(M.POISSON, kappa=kappa, source=src)
     g = ng or p + 1
     return Problem(f"diffusion{nel}p{p}", P, itg, (g, g, g), meqn, neq)
+
+
+def diffusion_batch(npatch: int = 256, nel_range: Tuple[int, int] = (8, 48),
+                    degrees: Tuple[int, ...] = (2, 3), sigma: float = 1.0, ell: float = 0.2,
+                    seed: int = 11, nmodes: int = 128):
+    """``npatch`` unit-cube diffusion patches; patch q draws its element count
+    per direction uniformly from ``nel_range`` (inclusive) and its degree from
+    ``degrees`` (splitmix64 stream ``seed``), with a log-normal conductivity
+    (seed ``seed*100003 + 2q``) and a smooth source (seed ``seed*100003 + 2q + 1``),
+    u = 0 on all faces, ng = p+1."""
+    lo, hi = nel_range
+    u = M.uniform(seed, 4 * npatch).reshape(npatch, 4)
+    out = []
+    for q in range(npatch):
+        nel = [lo + min(int(u[q, d] * (hi - lo + 1)), hi - lo) for d in range(3)]
+        p = degrees[min(int(u[q, 3] * len(degrees)), len(degrees) - 1)]
+        P = M.cube_patch(p, nel)
+        msc = M.face_status(P, 1, [1] * 6)
+        meqn, neq = M.equation_numbers(msc)
+        base = seed * 100003 + 2 * q
+        kappa = M.lognormal_conductivity(base, sigma, ell, nmodes)
+        itg = M.Integrand(M.POISSON, kappa=kappa, source=M.smooth_source(base + 1))
+        g = p + 1
+        out.append(Problem(f"batch{q}:{nel[0]}x{nel[1]}x{nel[2]}p{p}", P, itg, (g, g, g), meqn, neq))
+    return out
 
 
 def dof_counts(pb: Problem):

@@ -234,6 +234,58 @@ int ifem_gpu_unet_train_step(ifem_gpu_ctx* ctx, const float* x, const float* t, 
 int ifem_gpu_set_learned_preconditioner(ifem_gpu_ctx* ctx, int enable, int D, int H, int W,
                                         const float* aux, int naux, int features, double scale);
 
+/* ---- batch of independent patches ------------------------------------------
+ * Many scalar (Poisson / diffusion) patches, each with its own SAM numbering
+ * and integrand, assembled by one persistent launch into one block-diagonal
+ * system and solved together with one Jacobi-PCG per patch (own scalars and
+ * convergence test; the iteration loop is replayed from a captured hipGraph).
+ * Per patch, assembly and solve are those of ifem_gpu_assemble and
+ * ifem_gpu_solve; the batch stands in for the per-patch loop of
+ * SIMbase::assembleSystem (src/SIM/SIMbase.C:1229-1400) over independent
+ * models, each followed by SystemMatrix::solve
+ * (src/LinAlg/PETScMatrix.C:778-832).  Patches need equal degrees p = 1..3
+ * and ng = p+1 Gauss points per direction. */
+int ifem_gpu_batch_clear(ifem_gpu_ctx* ctx);
+/* Adds patch patch_id (from ifem_gpu_patch_create) with equation numbers
+ * meqn per node (1-based, 0 = eliminated; same rules as ifem_gpu_sam_bind
+ * with ncomp = 1) and integrand itg (kind IFEM_GPU_POISSON; copied).
+ * *index receives the patch's position in the batch. */
+int ifem_gpu_batch_add(ifem_gpu_ctx* ctx, int patch_id, const int* meqn, int neq,
+                       const ifem_gpu_integrand* itg, int ng, int* index);
+/* Basis tables, global sparsity pattern (SystemMatrix::initAssembly per
+ * patch, src/LinAlg/SparseMatrix.C:717-762), work list and solver buffers.
+ * neq, nnz (totals) may be NULL. */
+int ifem_gpu_batch_init(ifem_gpu_ctx* ctx, int* neq, int64_t* nnz);
+/* One persistent launch over all (patch, node line) work items, largest
+ * first: IFEM_GPU_FP64 writes K and b, IFEM_GPU_FP32 the FP32 copy of K. */
+int ifem_gpu_batch_assemble(ifem_gpu_ctx* ctx, int precision);
+/* Tunes, for each degree present (nen = (p+1)^3 element nodes), the element
+ * chunk of the assembly (full or half) and the SpMV lanes per row
+ * (16/32/64) by timing the batch, then re-assembles with the choice.
+ * table (may be NULL) receives 4 x {half, lanes} indexed by p. */
+int ifem_gpu_batch_autotune(ifem_gpu_ctx* ctx, int precision, int* table);
+/* Solves every patch to max(rtol*||b_q||, atol) on its true FP64 residual;
+ * precision IFEM_GPU_FP32 runs FP32 inner CG inside FP64 refinement.
+ * stats: iterations = max over patches, outer_iterations = refinement
+ * passes, converged = all converged, rel_residual = max over patches. */
+int ifem_gpu_batch_solve(ifem_gpu_ctx* ctx, const ifem_gpu_solver_params* params,
+                         ifem_gpu_stats* stats);
+/* Per-patch results of the last batch solve (arrays of the batch size, NULL
+ * skips). */
+int ifem_gpu_batch_results(ifem_gpu_ctx* ctx, int* iterations, int* converged,
+                           double* rel_residual);
+/* Size of patch `index`'s block and its first global equation. */
+int ifem_gpu_batch_patch_info(ifem_gpu_ctx* ctx, int index, int* neq, int64_t* nnz, int* eq0);
+/* Patch `index`'s block in local numbering: CSR (0-based, neq+1 / nnz),
+ * FP64 values, FP32 values, RHS; NULL skips. */
+int ifem_gpu_batch_get_system(ifem_gpu_ctx* ctx, int index, int64_t* rowptr, int32_t* colind,
+                              double* values, float* values32, double* rhs);
+/* Patch `index`'s solution in its equation order. */
+int ifem_gpu_batch_get_solution(ifem_gpu_ctx* ctx, int index, double* x);
+/* Device times (ms) of the last batch init, assembly and solve. */
+int ifem_gpu_batch_timings(ifem_gpu_ctx* ctx, double* symbolic_ms, double* assemble_ms,
+                           double* solve_ms);
+
 #ifdef __cplusplus
 }
 #endif

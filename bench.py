@@ -14,8 +14,9 @@ HBM before the timed region; the symbolic pattern build is reported apart.
 
 Also reported: the FP32-inner / FP64-refinement solve (time, iterations,
 energy-norm and max-nodal error against the FP64 solve), the roofline of the
-assembly kernel and of the SpMV, and the oracle (CPU restatement) timed on a
-bounded sample on the host cores.
+assembly kernel and of the SpMV, the oracle (CPU restatement) timed on a
+bounded sample on the host cores, and the batch of 256 uneven diffusion
+patches (one persistent assembly launch per precision, per-patch PCG).
 """
 import argparse
 import json
@@ -43,6 +44,8 @@ def parse():
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched uneven-patch leg (256 diffusion patches)")
     ap.add_argument("--cpu-layers", type=int, default=16,
                     help="element layers (in w) of the patch assembled by the CPU baseline")
     ap.add_argument("--spmv-reps", type=int, default=20)
@@ -167,6 +170,42 @@ def main():
         }
         log(f"[bench] mixed: {mixed}")
 
+    # ---- batched uneven patches: one persistent assembly launch ----------------
+    batch = None
+    if not args.no_batch:
+        from ifem_amd import setup_batch
+        bpbs = problems.diffusion_batch()
+        bdofs = int(sum(q.ndof for q in bpbs))
+        bs = setup_batch(bpbs, precisions=())
+        tune = bs.batch_autotune(FP32)
+        bt = {}
+        for prec in (FP64, FP32):
+            ts = []
+            for _ in range(3):
+                bs.batch_assemble(prec)
+                ts.append(bs.batch_timings()["assemble_ms"])
+            bt[prec] = float(np.median(ts))
+        b64 = bs.batch_solve(rtol=args.rtol, maxits=20000, check_every=32)
+        bmx = bs.batch_solve(rtol=args.rtol, maxits=20000, precision=FP32, check_every=32)
+        degs = [q.patch.p[0] for q in bpbs]
+        batch = {
+            "workload": "256 diffusion patches, nel per direction U{8..48}, p in {2,3}, "
+                        "log-normal kappa (sigma 1, length 0.2), smooth source, u=0",
+            "patches": len(bpbs), "p2": degs.count(2), "p3": degs.count(3),
+            "dofs": bdofs, "neq": int(sum(q.neq for q in bpbs)),
+            "symbolic_ms": round(bs.batch_timings()["symbolic_ms"], 2),
+            "assembly_fp64_ms": round(bt[FP64], 2), "assembly_fp32_ms": round(bt[FP32], 2),
+            "assembled_dof_per_s_fp64": round(bdofs / (bt[FP64] * 1e-3), 1),
+            "fp64_solve_ms": round(b64["solve_ms"], 2), "fp64_max_iterations": b64["iterations"],
+            "mixed_solve_ms": round(bmx["solve_ms"], 2), "mixed_max_iterations": bmx["iterations"],
+            "mixed_outer": bmx["outer"],
+            "all_converged": bool(b64["converged"] and bmx["converged"]),
+            "max_rel_residual": float(max(b64["rel_residual"], bmx["rel_residual"])),
+            "tuning": {f"p{p}": tune[p] for p in (2, 3)},
+        }
+        bs.close()
+        log(f"[bench] batch: {batch}")
+
     # ---- CPU baseline: the oracle on a bounded sample ----------------------------
     cpu = None
     if not args.no_cpu:
@@ -229,6 +268,7 @@ def main():
         "mixed": mixed,
         "speedup_mixed_vs_fp64": (round(solve64_ms / mixed["solve_ms"], 3) if mixed else None),
         "roofline": roofline, "roofline_spmv": spmv_roof, "cpu_baseline": cpu,
+        "batch": batch,
     }
     s.close()
     print(json.dumps(out), flush=True)

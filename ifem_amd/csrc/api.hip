@@ -119,6 +119,7 @@ int ifem_gpu_ctx_destroy(ifem_gpu_ctx* ctx)
   for (void* p : ctx->c.scratch_tables) hipFree(p);
   ctx->c.net.reset();
   ctx->c.batch.reset();
+  hipFree(ctx->c.d_asmargs);
   hipEventDestroy(ctx->c.ev0);
   hipEventDestroy(ctx->c.ev1);
   if (ctx->c.own_stream) hipStreamDestroy(ctx->c.stream);

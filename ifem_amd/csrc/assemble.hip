@@ -166,7 +166,24 @@ struct AsmArgs {
   double body[3];
   double lam, mu;
   int variant;                         // batch kernel: 2*(p-1) + half chunk
+  int ng;                              // Gauss points per direction
+  // kappa / source at the Gauss points, [element][gp] with element
+  // e1 + nel1*(e2 + nel2*e3) and gp q1 + ng*(q2 + ng*q3); nullptr:
+  // evaluated in the line kernel
+  double* gp_kappa;
+  double* gp_source;
+  double kappa_c, source_c;            // values of constant fields
 };
+
+// The line kernels read their AsmArgs from a device buffer through the
+// constant address space: the fields are uniform, read-only loads that the
+// compiler may re-issue instead of holding them all in registers.
+typedef const __attribute__((address_space(4))) AsmArgs ConstArgs;
+
+__device__ inline ConstArgs* const_args(const AsmArgs* p)
+{
+  return (ConstArgs*)p;
+}
 
 // Chunk of layer elements processed together so that CH*NG^3 ~ one block;
 // HALF selects half of that (an autotuning variant: less LDS, more phases).
@@ -178,6 +195,97 @@ struct Chunk {
   static constexpr int full = (256 / GP3) < 1 ? 1 : ((256 / GP3) > NB ? NB : (256 / GP3));
   static constexpr int v = HALF ? ((full + 1) / 2) : full;
 };
+
+// Fields that are expensive to evaluate (Fourier series) are evaluated once
+// per (element, Gauss point) before the line kernel, which would otherwise
+// evaluate each of them once per line touching the element ((p+1)^2 times).
+// X = sum_b N_b(xi) x_b as in the line kernel (phase A).
+template <int P>
+__device__ inline void gp_fields_at(const AsmArgs& a, int64_t id)
+{
+  constexpr int P1 = P + 1;
+  const int ng = a.ng;
+  const int gp3 = ng * ng * ng;
+  const int64_t e = id / gp3;
+  const int q = (int)(id - e * gp3);
+  const int q1 = q % ng, q2 = (q / ng) % ng, q3 = q / (ng * ng);
+  const int e1 = (int)(e % a.nel1), e2 = (int)((e / a.nel1) % a.nel2),
+            e3 = (int)(e / ((int64_t)a.nel1 * a.nel2));
+  double X[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int b3 = 0; b3 < P1; b3++)
+#pragma unroll
+    for (int b2 = 0; b2 < P1; b2++)
+#pragma unroll
+      for (int b1 = 0; b1 < P1; b1++) {
+        const double n1 = a.tabN1[(e1 * ng + q1) * P1 + b1];
+        const double n2 = a.tabN2[(e2 * ng + q2) * P1 + b2];
+        const double n3 = a.tabN3[(e3 * ng + q3) * P1 + b3];
+        const double nv = n1 * n2 * n3;
+        const int64_t node =
+            (int64_t)(e1 + b1) + (int64_t)a.n1 * ((e2 + b2) + (int64_t)a.n2 * (e3 + b3));
+#pragma unroll
+        for (int i = 0; i < 3; i++) X[i] += a.coefs[3 * node + i] * nv;
+      }
+  if (a.gp_kappa) a.gp_kappa[id] = eval_field(a.kappa, X[0], X[1], X[2]);
+  if (a.gp_source) a.gp_source[id] = eval_field(a.source, X[0], X[1], X[2]);
+}
+
+template <int P>
+__global__ void k_gp_fields(AsmArgs a, int64_t total)
+{
+  for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < total;
+       id += (int64_t)gridDim.x * blockDim.x)
+    gp_fields_at<P>(a, id);
+}
+
+// Batch: gp0[q] = first Gauss-point slot of patch q (patches without
+// tabulated fields own no slots).
+__global__ void k_gp_fields_batch(const AsmArgs* __restrict__ args, const int64_t* __restrict__ gp0,
+                                  int np, int64_t total)
+{
+  for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < total;
+       id += (int64_t)gridDim.x * blockDim.x) {
+    int lo = 0, hi = np - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (gp0[mid] <= id) lo = mid;
+      else hi = mid - 1;
+    }
+    const AsmArgs& a = args[lo];
+    const int64_t local = id - gp0[lo];
+    switch (a.variant >> 1) {
+    case 0: gp_fields_at<1>(a, local); break;
+    case 1: gp_fields_at<2>(a, local); break;
+    default: gp_fields_at<3>(a, local); break;
+    }
+  }
+}
+
+// Non-constant fields are tabulated at the Gauss points; constant ones are
+// passed by value (kappa of type ZERO means 1).
+static bool tabulated(const ifem_gpu_function& f)
+{
+  return f.type == IFEM_GPU_FN_SINPROD ||
+         ((f.type == IFEM_GPU_FN_FOURIER || f.type == IFEM_GPU_FN_EXP_FOURIER) && f.nmodes > 0);
+}
+
+static double const_value(const ifem_gpu_function& f, double zero_value)
+{
+  switch (f.type) {
+  case IFEM_GPU_FN_CONST: return f.scale;
+  case IFEM_GPU_FN_FOURIER: return f.mean;
+  case IFEM_GPU_FN_EXP_FOURIER: return std::exp(f.mean);
+  case IFEM_GPU_FN_ZERO: return zero_value;
+  default: return 0.0;
+  }
+}
+
+static int grid_for(int64_t total)
+{
+  const int64_t g = (total + 255) / 256;
+  return (int)(g < 1 ? 1 : (g > 65536 ? 65536 : g));
+}
 
 constexpr int kBlock = 256;
 
@@ -200,23 +308,45 @@ struct LineLds {
   static constexpr int GEO = (NC == 3 ? 9 : 0) + 3 * P1;
   static constexpr int NRHS = HAS_RHS ? P1 * NC : 1;
 
-  T win[P1][NCR][WN][NC];
-  T geo[CH * GP3][GEO];
-  T stage[CH][NEN][P1][NS];
-  T rstage[HAS_RHS ? CH * GP3 : 1][NRHS];
-  T rwin[P1][NC];
-  double cp[P1][W1][W1][3];
-  T tN1[NG][P1], tD1[NG][P1];
-  T tN2[P1][NG][P1], tD2[P1][NG][P1];
-  T tN3[P1][NG][P1], tD3[P1][NG][P1];
-  double sp2[P1], sp3[P1], sp1;
-  int be2[NB], be3[NB];
+  // every member 16-byte aligned: lets the compiler use 128-bit LDS loads
+  alignas(16) T win[P1][NCR][WN][NC];
+  alignas(16) T geo[CH * GP3][GEO];
+  alignas(16) T stage[CH][NEN][P1][NS];
+  alignas(16) T rstage[HAS_RHS ? CH * GP3 : 1][NRHS];
+  alignas(16) T rwin[P1][NC];
+  alignas(16) double cp[P1][W1][W1][3];
+  alignas(16) T tN1[NG][P1];
+  alignas(16) T tD1[NG][P1];
+  alignas(16) T tN2[P1][NG][P1];
+  alignas(16) T tD2[P1][NG][P1];
+  alignas(16) T tN3[P1][NG][P1];
+  alignas(16) T tD3[P1][NG][P1];
+  alignas(16) double sp2[P1];
+  alignas(16) double sp3[P1];
+  double sp1;
+  alignas(16) int be2[NB];
+  alignas(16) int be3[NB];
   int nbundle;
 };
 
+// The LDS arrays come in as separate __restrict__ pointers so that, after
+// inlining, the compiler knows they do not alias (fields of one struct with
+// run-time indices would otherwise be assumed to overlap).
 template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF>
-__device__ __forceinline__ void line_body(const AsmArgs& a, const int line, const int split,
-                                          LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>& L)
+__device__ __forceinline__ void line_impl(
+    ConstArgs& a, const int line, const int split,
+    T (*__restrict__ win)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NCR]
+                         [LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::WN][NC],
+    T (*__restrict__ geo)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::GEO],
+    T (*__restrict__ stage)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NEN][P + 1]
+                           [LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NS],
+    T (*__restrict__ rstage)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NRHS],
+    T (*__restrict__ rwin)[NC], double (*__restrict__ cp)[2 * P + 1][2 * P + 1][3],
+    T (*__restrict__ tN1)[P + 1], T (*__restrict__ tD1)[P + 1],
+    T (*__restrict__ tN2)[NG][P + 1], T (*__restrict__ tD2)[NG][P + 1],
+    T (*__restrict__ tN3)[NG][P + 1], T (*__restrict__ tD3)[NG][P + 1],
+    double* __restrict__ sp2, double* __restrict__ sp3, double* __restrict__ sp1p,
+    int* __restrict__ be2, int* __restrict__ be3, int* __restrict__ nbundlep)
 {
   using LL = LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>;
   constexpr int P1 = LL::P1;
@@ -228,25 +358,6 @@ __device__ __forceinline__ void line_body(const AsmArgs& a, const int line, cons
   constexpr bool SACC = LL::SACC;
   constexpr int NS = LL::NS;
   constexpr int CH = LL::CH;
-
-  auto& win = L.win;
-  auto& geo = L.geo;
-  auto& stage = L.stage;
-  auto& rstage = L.rstage;
-  auto& rwin = L.rwin;
-  auto& cp = L.cp;
-  auto& tN1 = L.tN1;
-  auto& tD1 = L.tD1;
-  auto& tN2 = L.tN2;
-  auto& tD2 = L.tD2;
-  auto& tN3 = L.tN3;
-  auto& tD3 = L.tD3;
-  auto& sp2 = L.sp2;
-  auto& sp3 = L.sp3;
-  auto& sp1 = L.sp1;
-  auto& be2 = L.be2;
-  auto& be3 = L.be3;
-  auto& nbundle = L.nbundle;
 
   const int tid = threadIdx.x;
   const int jc = line % a.n2, kc = line / a.n2;
@@ -264,7 +375,7 @@ __device__ __forceinline__ void line_body(const AsmArgs& a, const int line, cons
           nb++;
         }
       }
-    nbundle = nb;
+    *nbundlep = nb;
   }
   // direction 2/3 tables for elements jc-P..jc and kc-P..kc
   for (int t = tid; t < P1 * NG * P1; t += kBlock) {
@@ -293,7 +404,7 @@ __device__ __forceinline__ void line_body(const AsmArgs& a, const int line, cons
   for (int t = tid; t < P1 * NCR * WN * NC; t += kBlock) (&win[0][0][0][0])[t] = T(0);
   for (int t = tid; t < P1 * NC; t += kBlock) (&rwin[0][0])[t] = T(0);
   __syncthreads();
-  const int nb = nbundle;
+  const int nb = *nbundlep;
 
   const T lam = (T)a.lam, mu = (T)a.mu;
 
@@ -318,7 +429,7 @@ __device__ __forceinline__ void line_body(const AsmArgs& a, const int line, cons
       (&tN1[0][0])[t] = (T)a.tabN1[e1 * NG * P1 + t];
       (&tD1[0][0])[t] = (T)a.tabD1[e1 * NG * P1 + t];
     }
-    if (tid == 0) sp1 = a.span1[e1];
+    if (tid == 0) *sp1p = a.span1[e1];
     __syncthreads();
 
     for (int c0 = 0; c0 < nb; c0 += CH) {
@@ -330,8 +441,13 @@ __device__ __forceinline__ void line_body(const AsmArgs& a, const int line, cons
         const int e2 = be2[c0 + le], e3 = be3[c0 + le];
         const int m2 = e2 - jc + P, m3 = e3 - kc + P; // table slots
         const int q1 = q % NG, q2 = (q / NG) % NG, q3 = q / (NG * NG);
+        // material and source at this Gauss point (tabulated or constant)
+        const int64_t gpi = ((int64_t)e1 + (int64_t)a.nel1 * (e2 + (int64_t)a.nel2 * e3)) * GP3 + q;
+        T coef = T(1);
+        if (NC == 1) coef = a.gp_kappa ? (T)a.gp_kappa[gpi] : (T)a.kappa_c;
+        T fval = T(0);
+        if (HAS_RHS && NC == 1) fval = a.gp_source ? (T)a.gp_source[gpi] : (T)a.source_c;
         T J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        T X[3] = {0, 0, 0};
 #pragma unroll
         for (int b3 = 0; b3 < P1; b3++)
 #pragma unroll
@@ -342,7 +458,6 @@ __device__ __forceinline__ void line_body(const AsmArgs& a, const int line, cons
               const T n2 = tN2[m2][q2][b2], d2 = tD2[m2][q2][b2];
               const T n3 = tN3[m3][q3][b3], d3 = tD3[m3][q3][b3];
               const T g0 = d1 * n2 * n3, g1 = n1 * d2 * n3, g2 = n1 * n2 * d3;
-              const T nv = n1 * n2 * n3;
               const double* xb = cp[b1][e2 + b2 - jc + P][e3 + b3 - kc + P];
 #pragma unroll
               for (int i = 0; i < 3; i++) {
@@ -350,7 +465,6 @@ __device__ __forceinline__ void line_body(const AsmArgs& a, const int line, cons
                 J[3 * i + 0] += xi * g0;
                 J[3 * i + 1] += xi * g1;
                 J[3 * i + 2] += xi * g2;
-                X[i] += xi * nv;
               }
             }
         // matrix::det / matrix::inverse (src/LinAlg/matrix.h:791-862)
@@ -372,12 +486,9 @@ __device__ __forceinline__ void line_body(const AsmArgs& a, const int line, cons
           Ji[2] = (J[1] * J[5] - J[4] * J[2]) / det;
           Ji[5] = -(J[0] * J[5] - J[3] * J[2]) / det;
           Ji[8] = (J[0] * J[4] - J[3] * J[1]) / det;
-          const double dV = 0.125 * (sp1 * sp2[m2] * sp3[m3]);
+          const double dV = 0.125 * (*sp1p * sp2[m2] * sp3[m3]);
           wdet = det * (T)(dV * a.gw[q1] * a.gw[q2] * a.gw[q3]);
         }
-        T coef = T(1);
-        if (NC == 1 && a.kappa.type != IFEM_GPU_FN_ZERO)
-          coef = (T)eval_field(a.kappa, (double)X[0], (double)X[1], (double)X[2]);
         if (NC == 3) {
 #pragma unroll
           for (int m = 0; m < 9; m++) geo[t][m] = Ji[m];
@@ -385,8 +496,6 @@ __device__ __forceinline__ void line_body(const AsmArgs& a, const int line, cons
         const int a2r = jc - e2, a3r = kc - e3; // row node's local (a2, a3)
         const T n2a = tN2[m2][q2][a2r], d2a = tD2[m2][q2][a2r];
         const T n3a = tN3[m3][q3][a3r], d3a = tD3[m3][q3][a3r];
-        T fval = T(0);
-        if (HAS_RHS && NC == 1) fval = (T)eval_field(a.source, (double)X[0], (double)X[1], (double)X[2]);
 #pragma unroll
         for (int a1 = 0; a1 < P1; a1++) {
           const T n1 = tN1[q1][a1], d1 = tD1[q1][a1];
@@ -575,11 +684,38 @@ __device__ __forceinline__ void line_body(const AsmArgs& a, const int line, cons
   }
 }
 
-template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS>
-__global__ __launch_bounds__(kBlock) void k_assemble_lines(AsmArgs a)
+template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF>
+__device__ __forceinline__ void line_body(ConstArgs& a, const int line, const int split,
+                                          LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>& L)
 {
-  __shared__ LineLds<P, NG, NC, T, SPLIT, HAS_RHS, 0> L;
-  line_body<P, NG, NC, T, SPLIT, HAS_RHS, 0>(a, blockIdx.x, blockIdx.y, L);
+  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF>(a, line, split, L.win, L.geo, L.stage, L.rstage,
+                                                L.rwin, L.cp, L.tN1, L.tD1, L.tN2, L.tD2, L.tN3,
+                                                L.tD3, L.sp2, L.sp3, &L.sp1, L.be2, L.be3,
+                                                &L.nbundle);
+}
+
+// Single patch: the LDS arrays are separate __shared__ variables.
+template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS>
+__global__ __launch_bounds__(kBlock) void k_assemble_lines(const AsmArgs* __restrict__ args)
+{
+  ConstArgs& a = *const_args(args);
+  using LL = LineLds<P, NG, NC, T, SPLIT, HAS_RHS, 0>;
+  constexpr int P1 = P + 1, W1 = 2 * P + 1;
+  __shared__ T win[P1][LL::NCR][LL::WN][NC];
+  __shared__ T geo[LL::CH * LL::GP3][LL::GEO];
+  __shared__ T stage[LL::CH][LL::NEN][P1][LL::NS];
+  __shared__ T rstage[HAS_RHS ? LL::CH * LL::GP3 : 1][LL::NRHS];
+  __shared__ T rwin[P1][NC];
+  __shared__ double cp[P1][W1][W1][3];
+  __shared__ T tN1[NG][P1], tD1[NG][P1];
+  __shared__ T tN2[P1][NG][P1], tD2[P1][NG][P1];
+  __shared__ T tN3[P1][NG][P1], tD3[P1][NG][P1];
+  __shared__ double sp2[P1], sp3[P1], sp1;
+  __shared__ int be2[LL::NB], be3[LL::NB];
+  __shared__ int nbundle;
+  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, 0>(a, blockIdx.x, blockIdx.y, win, geo, stage, rstage,
+                                             rwin, cp, tN1, tD1, tN2, tD2, tN3, tD3, sp2, sp3,
+                                             &sp1, be2, be3, &nbundle);
 }
 
 // Persistent batch assembly over many independent patches (degree p = 1..3,
@@ -612,7 +748,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_batch(const AsmArgs* __rest
     const int it = __builtin_amdgcn_readfirstlane(next);
     if (it >= nitems) break;
     const int2 w = items[it];
-    const AsmArgs& a = args[w.x];
+    ConstArgs& a = const_args(args)[w.x];
     switch (a.variant) {
     case 0: line_body<1, 2, 1, T, 1, HAS_RHS, 0>(a, w.y, 0, U.v0); break;
     case 1: line_body<1, 2, 1, T, 1, HAS_RHS, 1>(a, w.y, 0, U.v1); break;
@@ -791,6 +927,15 @@ struct Tables {
   }
 };
 
+struct GpFields {
+  double *kappa = nullptr, *source = nullptr;
+  ~GpFields()
+  {
+    hipFree(kappa);
+    hipFree(source);
+  }
+};
+
 template <int P>
 static int make_tables(Context& c, const PatchRec& Pt, int ng, Tables& tb, GaussArgs& ga)
 {
@@ -834,10 +979,11 @@ static int launch_lines(Context& c, const AsmArgs& args, bool has_rhs)
 {
   constexpr int SPLIT = (NC == 3 && P >= 3) ? 3 : 1;
   dim3 grid(args.n2 * args.n3, SPLIT);
+  const AsmArgs* d = static_cast<const AsmArgs*>(c.d_asmargs);
   if (has_rhs)
-    k_assemble_lines<P, NG, NC, T, SPLIT, true><<<grid, kBlock, 0, c.stream>>>(args);
+    k_assemble_lines<P, NG, NC, T, SPLIT, true><<<grid, kBlock, 0, c.stream>>>(d);
   else
-    k_assemble_lines<P, NG, NC, T, SPLIT, false><<<grid, kBlock, 0, c.stream>>>(args);
+    k_assemble_lines<P, NG, NC, T, SPLIT, false><<<grid, kBlock, 0, c.stream>>>(d);
   IFEM_HIP(hipGetLastError());
   return IFEM_GPU_OK;
 }
@@ -909,7 +1055,26 @@ static int assemble_p(Context& c, const ifem_gpu_integrand& itg, int ng, int pre
     else has_rhs = itg.body[0] != 0.0 || itg.body[1] != 0.0 || itg.body[2] != 0.0;
     IFEM_HIP(hipMemsetAsync(c.d_rhs, 0, sizeof(double) * (c.neq > 0 ? c.neq : 1), c.stream));
   }
+  // Fourier-series fields: tabulated at the Gauss points first
+  a.ng = ng;
+  a.gp_kappa = a.gp_source = nullptr;
+  a.kappa_c = const_value(itg.kappa, 1.0);
+  a.source_c = const_value(itg.source, 0.0);
+  const int64_t ngp = (int64_t)Pt.nel[0] * Pt.nel[1] * Pt.nel[2] * ng * ng * ng;
+  GpFields gf;
+  if (c.ncomp == 1 && tabulated(itg.kappa)) {
+    IFEM_HIP(hipMalloc(&gf.kappa, sizeof(double) * (ngp > 0 ? ngp : 1)));
+    a.gp_kappa = gf.kappa;
+  }
+  if (c.ncomp == 1 && has_rhs && tabulated(itg.source)) {
+    IFEM_HIP(hipMalloc(&gf.source, sizeof(double) * (ngp > 0 ? ngp : 1)));
+    a.gp_source = gf.source;
+  }
+  if (!c.d_asmargs) IFEM_HIP(hipMalloc(&c.d_asmargs, sizeof(AsmArgs)));
+  IFEM_HIP(hipMemcpyAsync(c.d_asmargs, &a, sizeof(AsmArgs), hipMemcpyHostToDevice, c.stream));
   IFEM_HIP(hipEventRecord(c.ev0, c.stream));
+  if ((a.gp_kappa || a.gp_source) && ngp > 0)
+    k_gp_fields<P><<<grid_for(ngp), 256, 0, c.stream>>>(a, ngp);
   rc = dispatch_ng<P>(c, a, ng, c.ncomp, fp32, has_rhs);
   if (rc) return rc;
   IFEM_HIP(hipEventRecord(c.ev1, c.stream));
@@ -1121,6 +1286,12 @@ int batch_upload_args(Context& c)
     a.kappa = batch_function(bp.itg.kappa, bp.d_kmodes);
     a.source = batch_function(bp.itg.source, bp.d_smodes);
     a.variant = 2 * (bp.p - 1) + (B.tune[bp.p].half ? 1 : 0);
+    a.ng = bp.ng;
+    const int64_t g0 = B.h_gp0.empty() ? 0 : B.h_gp0[q];
+    a.gp_kappa = tabulated(bp.itg.kappa) ? B.d_gpk + g0 : nullptr;
+    a.gp_source = tabulated(bp.itg.source) ? B.d_gps + g0 : nullptr;
+    a.kappa_c = const_value(bp.itg.kappa, 1.0);
+    a.source_c = const_value(bp.itg.source, 0.0);
     a.val64 = B.d_val;
     a.val32 = nullptr;
     a.rhs = B.d_rhs;
@@ -1128,6 +1299,7 @@ int batch_upload_args(Context& c)
     a.val64 = nullptr;
     a.val32 = B.d_val32;
     a.rhs = nullptr;
+    a.gp_source = nullptr; // no right-hand side in the FP32 pass
     a32[q] = a;
   }
   if (!B.d_args64) IFEM_HIP(hipMalloc(&B.d_args64, sizeof(AsmArgs) * (np > 0 ? np : 1)));
@@ -1194,6 +1366,24 @@ int batch_symbolic(Context& c)
   IFEM_HIP(hipMalloc(&B.d_rhs, sizeof(double) * (neq > 0 ? neq : 1)));
   IFEM_HIP(hipMemsetAsync(B.d_rhs, 0, sizeof(double) * (neq > 0 ? neq : 1), c.stream));
 
+  // Gauss-point slots of the tabulated fields
+  B.h_gp0.assign(np, 0);
+  int64_t ngp = 0;
+  for (int q = 0; q < np; q++) {
+    const BatchPatch& bp = B.pp[q];
+    const PatchRec& Pt = c.patches[bp.patch];
+    B.h_gp0[q] = ngp;
+    if (tabulated(bp.itg.kappa) || tabulated(bp.itg.source))
+      ngp += (int64_t)Pt.nel[0] * Pt.nel[1] * Pt.nel[2] * bp.ng * bp.ng * bp.ng;
+  }
+  B.ngp = ngp;
+  if (ngp > 0) {
+    IFEM_HIP(hipMalloc(&B.d_gpk, sizeof(double) * ngp));
+    IFEM_HIP(hipMalloc(&B.d_gps, sizeof(double) * ngp));
+  }
+  IFEM_HIP(hipMalloc(&B.d_gp0, sizeof(int64_t) * (np > 0 ? np : 1)));
+  IFEM_HIP(hipMemcpy(B.d_gp0, B.h_gp0.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice));
+
   // work list: one item per node line (j, k) of each patch
   struct Item { double cost; int patch, line; };
   std::vector<Item> items;
@@ -1249,6 +1439,10 @@ int batch_assemble(Context& c, int precision, double* ms_out)
   const bool fp32 = precision == IFEM_GPU_FP32;
   k_fill<int><<<1, 64, 0, c.stream>>>(1, B.d_queue, 0);
   IFEM_HIP(hipEventRecord(c.ev0, c.stream));
+  if (B.ngp > 0)
+    k_gp_fields_batch<<<grid_for(B.ngp), 256, 0, c.stream>>>(
+        static_cast<const AsmArgs*>(fp32 ? B.d_args32 : B.d_args64), B.d_gp0, (int)B.pp.size(),
+        B.ngp);
   if (B.nitems > 0) {
     if (fp32)
       k_assemble_batch<float, false><<<B.grid32, kBlock, 0, c.stream>>>(

@@ -87,6 +87,7 @@ struct Context {
 
   // owned device buffers for integrand mode tables
   std::vector<void*> scratch_tables;
+  void* d_asmargs = nullptr; // assembly kernel arguments (device copy)
 
   // learned preconditioner
   std::unique_ptr<UNet> net;

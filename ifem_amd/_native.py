@@ -27,7 +27,7 @@ EXPORTS = [
     "ifem_gpu_batch_clear", "ifem_gpu_batch_add", "ifem_gpu_batch_init", "ifem_gpu_batch_assemble",
     "ifem_gpu_batch_autotune", "ifem_gpu_batch_solve", "ifem_gpu_batch_results",
     "ifem_gpu_batch_patch_info", "ifem_gpu_batch_get_system", "ifem_gpu_batch_get_solution",
-    "ifem_gpu_batch_timings",
+    "ifem_gpu_batch_timings", "ifem_gpu_autotune_assembly",
 ]
 
 FP64, FP32 = 64, 32
@@ -91,6 +91,8 @@ def lib():
     L.ifem_gpu_init_assembly.argtypes = [vp, C.POINTER(C.c_int64)]
     L.ifem_gpu_assemble.argtypes = [vp, C.POINTER(Integrand), C.POINTER(ip), ip]
     L.ifem_gpu_assemble_neumann.argtypes = [vp, ip, C.POINTER(dp), ip]
+    L.ifem_gpu_autotune_assembly.argtypes = [vp, C.POINTER(Integrand), C.POINTER(ip), ip,
+                                             C.POINTER(ip)]
     L.ifem_gpu_solver_defaults.argtypes = [C.POINTER(SolverParams)]
     L.ifem_gpu_solver_defaults.restype = None
     L.ifem_gpu_solve.argtypes = [vp, C.POINTER(SolverParams), C.POINTER(Stats)]
@@ -236,6 +238,16 @@ class Session:
             s.body[i] = float(itg.body[i])
         ngv = (C.c_int * 3)(*ng)
         check(lib().ifem_gpu_assemble(self.h, C.byref(s), ngv, precision))
+
+    def autotune_assembly(self, itg, ng, precision: int = FP64) -> int:
+        """Pick the faster element-chunk variant for this system; returns 1
+        for half chunks, 0 for full.  K and b are left as after assemble()."""
+        keep = []
+        s = self._integrand(itg, keep)
+        ngv = (C.c_int * 3)(*ng)
+        h = C.c_int(-1)
+        check(lib().ifem_gpu_autotune_assembly(self.h, C.byref(s), ngv, precision, C.byref(h)))
+        return h.value
 
     def add_traction(self, face: int, t, ng: int):
         """face: 0..5 (u0,u1,v0,v1,w0,w1) -> IFEM face index face+1."""

@@ -241,6 +241,24 @@ int ifem_gpu_assemble(ifem_gpu_ctx* ctx, const ifem_gpu_integrand* itg, const in
   return assemble_volume(c, *itg, ng, precision);
 }
 
+int ifem_gpu_autotune_assembly(ifem_gpu_ctx* ctx, const ifem_gpu_integrand* itg, const int ng[3],
+                               int precision, int* half)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (!itg || !ng) return fail(IFEM_GPU_EINVAL, "autotune_assembly: null argument");
+  if (!c.pattern_ready)
+    return fail(IFEM_GPU_EINVAL, "autotune_assembly: call ifem_gpu_init_assembly first");
+  if (precision != IFEM_GPU_FP64 && precision != IFEM_GPU_FP32)
+    return fail(IFEM_GPU_EINVAL, "autotune_assembly: precision must be 64 or 32");
+  const int want = itg->kind == IFEM_GPU_ELASTICITY ? 3 : (itg->kind == IFEM_GPU_POISSON ? 1 : 0);
+  if (want != c.ncomp)
+    return fail(IFEM_GPU_EINVAL, "autotune_assembly: integrand component count differs from the SAM's");
+  if (ng[0] != ng[1] || ng[0] != ng[2])
+    return fail(IFEM_GPU_EUNSUPPORTED, "autotune_assembly: equal Gauss counts per direction needed");
+  return autotune_assembly(c, *itg, ng, precision, half);
+}
+
 int ifem_gpu_assemble_neumann(ifem_gpu_ctx* ctx, int face, const double* traction, int ng)
 {
   CHECK_CTX(ctx);

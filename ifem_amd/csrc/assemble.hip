@@ -695,11 +695,11 @@ __device__ __forceinline__ void line_body(ConstArgs& a, const int line, const in
 }
 
 // Single patch: the LDS arrays are separate __shared__ variables.
-template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS>
+template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF>
 __global__ __launch_bounds__(kBlock) void k_assemble_lines(const AsmArgs* __restrict__ args)
 {
   ConstArgs& a = *const_args(args);
-  using LL = LineLds<P, NG, NC, T, SPLIT, HAS_RHS, 0>;
+  using LL = LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>;
   constexpr int P1 = P + 1, W1 = 2 * P + 1;
   __shared__ T win[P1][LL::NCR][LL::WN][NC];
   __shared__ T geo[LL::CH * LL::GP3][LL::GEO];
@@ -713,7 +713,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_lines(const AsmArgs* __rest
   __shared__ double sp2[P1], sp3[P1], sp1;
   __shared__ int be2[LL::NB], be3[LL::NB];
   __shared__ int nbundle;
-  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, 0>(a, blockIdx.x, blockIdx.y, win, geo, stage, rstage,
+  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF>(a, blockIdx.x, blockIdx.y, win, geo, stage, rstage,
                                              rwin, cp, tN1, tD1, tN2, tD2, tN3, tD3, sp2, sp3,
                                              &sp1, be2, be3, &nbundle);
 }
@@ -974,16 +974,40 @@ static int upload_function(Context& c, const ifem_gpu_function& f, DevFunction& 
   return IFEM_GPU_OK;
 }
 
-template <int P, int NG, int NC, typename T>
-static int launch_lines(Context& c, const AsmArgs& args, bool has_rhs)
+// Element-chunk variant of the line kernel for (p, ng, ncomp, precision):
+// the tuned choice if ifem_gpu_autotune_assembly ran, else half chunks for
+// FP64 elasticity (LDS for two blocks per CU) and full chunks otherwise.
+int& asm_half_slot(Context& c, int p, int ng, int nc, bool fp32)
+{
+  return c.asm_half[p][ng > p + 1 ? 1 : 0][nc == 3 ? 1 : 0][fp32 ? 1 : 0];
+}
+
+static int asm_half(Context& c, int p, int ng, int nc, bool fp32)
+{
+  const int h = asm_half_slot(c, p, ng, nc, fp32);
+  if (h >= 0) return h;
+  return (!fp32 && nc == 3) ? 1 : 0;
+}
+
+template <int P, int NG, int NC, typename T, int HALF>
+static void launch_lines_h(Context& c, const AsmArgs& args, bool has_rhs)
 {
   constexpr int SPLIT = (NC == 3 && P >= 3) ? 3 : 1;
   dim3 grid(args.n2 * args.n3, SPLIT);
   const AsmArgs* d = static_cast<const AsmArgs*>(c.d_asmargs);
   if (has_rhs)
-    k_assemble_lines<P, NG, NC, T, SPLIT, true><<<grid, kBlock, 0, c.stream>>>(d);
+    k_assemble_lines<P, NG, NC, T, SPLIT, true, HALF><<<grid, kBlock, 0, c.stream>>>(d);
   else
-    k_assemble_lines<P, NG, NC, T, SPLIT, false><<<grid, kBlock, 0, c.stream>>>(d);
+    k_assemble_lines<P, NG, NC, T, SPLIT, false, HALF><<<grid, kBlock, 0, c.stream>>>(d);
+}
+
+template <int P, int NG, int NC, typename T>
+static int launch_lines(Context& c, const AsmArgs& args, bool has_rhs)
+{
+  if (asm_half(c, P, NG, NC, sizeof(T) == 4))
+    launch_lines_h<P, NG, NC, T, 1>(c, args, has_rhs);
+  else
+    launch_lines_h<P, NG, NC, T, 0>(c, args, has_rhs);
   IFEM_HIP(hipGetLastError());
   return IFEM_GPU_OK;
 }
@@ -1085,6 +1109,36 @@ static int assemble_p(Context& c, const ifem_gpu_integrand& itg, int ng, int pre
   if (fp32) c.fp32_ready = true;
   else c.fp64_ready = true;
   return IFEM_GPU_OK;
+}
+
+// Times both element-chunk variants of the line kernel on this system and
+// keeps the faster one for (p, ng, ncomp, precision); the system is left
+// assembled with the chosen variant.
+int autotune_assembly(Context& c, const ifem_gpu_integrand& itg, const int ng[3], int precision,
+                      int* half_out)
+{
+  const PatchRec& Pt = c.patches[c.patch];
+  const bool fp32 = precision == IFEM_GPU_FP32;
+  if (ng[0] < 1 || ng[0] > kMaxNG) return fail(IFEM_GPU_EUNSUPPORTED, "autotune: ng out of range");
+  int& slot = asm_half_slot(c, Pt.p[0], ng[0], c.ncomp, fp32);
+  double best = 1e300;
+  int best_h = 0;
+  for (int h = 0; h <= 1; h++) {
+    slot = h;
+    double t = 1e300;
+    for (int rep = 0; rep < 3; rep++) {
+      int rc = assemble_volume(c, itg, ng, precision);
+      if (rc) return rc;
+      t = std::fmin(t, c.last_assemble_ms);
+    }
+    if (t < best) {
+      best = t;
+      best_h = h;
+    }
+  }
+  slot = best_h;
+  if (half_out) *half_out = best_h;
+  return assemble_volume(c, itg, ng, precision);
 }
 
 int assemble_volume(Context& c, const ifem_gpu_integrand& itg, const int ng[3], int precision)

@@ -88,6 +88,12 @@ struct Context {
   // owned device buffers for integrand mode tables
   std::vector<void*> scratch_tables;
   void* d_asmargs = nullptr; // assembly kernel arguments (device copy)
+  // element-chunk variant of the line kernel per (p, ng-p-1, ncomp==3, fp32):
+  // -1 built-in default, 0 full, 1 half
+  int asm_half[4][2][2][2] = {{{{-1, -1}, {-1, -1}}, {{-1, -1}, {-1, -1}}},
+                              {{{-1, -1}, {-1, -1}}, {{-1, -1}, {-1, -1}}},
+                              {{{-1, -1}, {-1, -1}}, {{-1, -1}, {-1, -1}}},
+                              {{{-1, -1}, {-1, -1}}, {{-1, -1}, {-1, -1}}}};
 
   // learned preconditioner
   std::unique_ptr<UNet> net;
@@ -112,6 +118,8 @@ int build_pattern(Context& c);
 int assemble_volume(Context& c, const ifem_gpu_integrand& itg, const int ng[3],
                     int precision);
 int assemble_traction(Context& c, int face0, const double* t, int ng);
+int autotune_assembly(Context& c, const ifem_gpu_integrand& itg, const int ng[3], int precision,
+                      int* half);
 int batch_symbolic(Context& c);
 int batch_upload_args(Context& c);
 int batch_assemble(Context& c, int precision, double* ms);

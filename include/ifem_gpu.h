@@ -124,6 +124,15 @@ enum { IFEM_GPU_FP64 = 64, IFEM_GPU_FP32 = 32 };
 int ifem_gpu_assemble(ifem_gpu_ctx* ctx, const ifem_gpu_integrand* itg,
                       const int ng[3], int precision);
 
+/* No counterpart in the reference (part of this library's autotuner): times
+ * the line-assembly kernel of the bound system with full and with half
+ * element chunks (tile shape of the (p, nen) kernel: LDS per block against
+ * blocks per CU), keeps the faster for this (p, ng, ncomp, precision) in the
+ * context and leaves the system assembled as ifem_gpu_assemble does (apply
+ * Neumann loads afterwards).  *half (may be NULL) receives the choice. */
+int ifem_gpu_autotune_assembly(ifem_gpu_ctx* ctx, const ifem_gpu_integrand* itg,
+                               const int ng[3], int precision, int* half);
+
 /* Neumann traction t (constant vector, ncomp components) on patch face
  * face = 1..6 (IFEM numbering: 1 u=0, 2 u=1, 3 v=0, 4 v=1, 5 w=0, 6 w=1),
  * ng Gauss points per tangent direction, added into b.  Replaces

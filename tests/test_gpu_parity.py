@@ -70,6 +70,22 @@ def test_assembly_is_bitwise_deterministic():
     assert np.array_equal(A1, A2) and np.array_equal(b1, b2)
 
 
+def test_autotuned_assembly_keeps_parity():
+    """Both element-chunk variants of the line kernel give the oracle's system
+    (the autotuner leaves the faster one assembled)."""
+    for pb in (problems.elasticity_cube(6, 2, seed=3), problems.poisson_cube(5, 3),
+               problems.diffusion_cube(6, 2, sigma=1.0, ell=0.2, seed=4)):
+        _say(f"[gpu] autotuned assembly {pb.name}")
+        with setup_problem(pb) as s:
+            for prec in (FP64, FP32):
+                assert s.autotune_assembly(pb.integrand, pb.ng, prec) in (0, 1)
+            if pb.traction is not None:
+                s.add_traction(*pb.traction)
+            _, _, A, _ = _compare_system(s, pb)
+            A32 = s.values_fp32().astype(np.float64)
+            assert np.max(np.abs(A32 - A)) <= 1e-5 * np.max(np.abs(A))
+
+
 def test_fp32_assembly_close_to_fp64():
     pb = problems.elasticity_cube(6, 2, seed=8)
     _say("[gpu] fp32 assembly elasticity 6^3")

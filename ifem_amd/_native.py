@@ -27,7 +27,7 @@ EXPORTS = [
     "ifem_gpu_batch_clear", "ifem_gpu_batch_add", "ifem_gpu_batch_init", "ifem_gpu_batch_assemble",
     "ifem_gpu_batch_autotune", "ifem_gpu_batch_solve", "ifem_gpu_batch_results",
     "ifem_gpu_batch_patch_info", "ifem_gpu_batch_get_system", "ifem_gpu_batch_get_solution",
-    "ifem_gpu_batch_timings", "ifem_gpu_autotune_assembly",
+    "ifem_gpu_batch_timings", "ifem_gpu_autotune_assembly", "ifem_gpu_unet_backward",
 ]
 
 FP64, FP32 = 64, 32
@@ -114,6 +114,7 @@ def lib():
     L.ifem_gpu_unet_train_step.argtypes = [vp, vp, vp, vp, ip, ip, ip, ip, C.c_float,
                                            C.POINTER(C.c_float)]
     L.ifem_gpu_set_learned_preconditioner.argtypes = [vp, ip, ip, ip, ip, vp, ip, ip, dp]
+    L.ifem_gpu_unet_backward.argtypes = [vp, vp, vp, ip, ip, ip, ip, C.c_float, vp]
     L.ifem_gpu_batch_clear.argtypes = [vp]
     L.ifem_gpu_batch_add.argtypes = [vp, ip, C.POINTER(ip), ip, C.POINTER(Integrand), ip,
                                      C.POINTER(ip)]
@@ -343,6 +344,13 @@ class Session:
                                              C.c_void_p(w_ptr) if w_ptr else None, N, D, H, W, lr,
                                              C.byref(loss) if want_loss else None))
         return loss.value if want_loss else None
+
+    def unet_backward(self, x_ptr: int, dy_ptr: int, N, D, H, W, lr=0.0, want_grad=False):
+        """Parameter gradient of sum(dy * U(x)); lr > 0 also takes an Adam step."""
+        g = np.zeros(self.unet_nparams, dtype=np.float32) if want_grad else None
+        check(lib().ifem_gpu_unet_backward(self.h, C.c_void_p(x_ptr), C.c_void_p(dy_ptr), N, D, H,
+                                           W, float(lr), _ptr(g) if want_grad else None))
+        return g
 
     def set_learned(self, enable, D=0, H=0, W=0, aux_ptr: int = 0, naux=0, scale=1.0,
                     features=0):

@@ -98,7 +98,8 @@ struct Context {
   // learned preconditioner
   std::unique_ptr<UNet> net;
   bool learned = false;
-  int lat[3] = {0, 0, 0}; // D, H, W of the free-DOF lattice
+  int lat[3] = {0, 0, 0}; // D, H, W of the (padded) network lattice
+  int box[3] = {0, 0, 0}; // k, j, i extents of the free-node box
   int naux = 0;
   int features = 0;          // 0: [r, aux], 1: [r, r*aux]
   float* d_aux = nullptr;   // naux * D*H*W

@@ -19,6 +19,7 @@
 // check_every iterations.  Results are bitwise reproducible run to run.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -463,11 +464,24 @@ __global__ __launch_bounds__(kThreads) void k_norm_parts(int n, const V* __restr
 
 // network input: channel c of node k = r[k*nc + c] / |r|_rms; with feat 1
 // also r^ * aux_j in channel nc + j*nc + c
+// Free-DOF box (bw x bh x bd nodes, i fastest) inside the zero-padded
+// D x H x W lattice: node index -> lattice index.
+struct LatMap {
+  int bw, bh, W, H;
+  int64_t vol; // D*H*W
+  __device__ int64_t at(int node) const
+  {
+    const int ii = node % bw, r = node / bw;
+    const int jj = r % bh, kk = r / bh;
+    return ((int64_t)kk * H + jj) * W + ii;
+  }
+};
+
 template <typename V>
 __global__ __launch_bounds__(kThreads) void k_to_lattice(int n, int nc, const V* __restrict__ r,
                                                          float* __restrict__ netin,
                                                          const float* __restrict__ aux, int naux,
-                                                         int feat,
+                                                         int feat, LatMap lm,
                                                          const double* __restrict__ part_rr,
                                                          int nparts, const double* __restrict__ scal)
 {
@@ -475,14 +489,15 @@ __global__ __launch_bounds__(kThreads) void k_to_lattice(int n, int nc, const V*
   __shared__ double sh[kThreads / 64];
   const double rr = sum_parts(part_rr, nparts, sh);
   const double inv = rr > 0.0 ? 1.0 / sqrt(rr / n) : 0.0;
-  const int vol = n / nc;
+  const int64_t vol = lm.vol;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
     const int node = i / nc, comp = i - node * nc;
+    const int64_t li = lm.at(node);
     const float v = (float)((double)r[i] * inv);
-    netin[(int64_t)comp * vol + node] = v;
+    netin[(int64_t)comp * vol + li] = v;
     if (feat)
       for (int j = 0; j < naux; j++)
-        netin[(int64_t)(nc + j * nc + comp) * vol + node] = v * aux[(int64_t)j * vol + node];
+        netin[(int64_t)(nc + j * nc + comp) * vol + li] = v * aux[(int64_t)j * vol + li];
   }
 }
 
@@ -490,7 +505,7 @@ __global__ __launch_bounds__(kThreads) void k_to_lattice(int n, int nc, const V*
 template <typename V>
 __global__ __launch_bounds__(kThreads) void k_from_lattice(
     int n, int nc, const V* __restrict__ r, const V* __restrict__ rold, const V* __restrict__ dinv,
-    const float* __restrict__ out, V* __restrict__ z, double scale,
+    const float* __restrict__ out, V* __restrict__ z, double scale, LatMap lm,
     const double* __restrict__ part_rr, int nparts, double* __restrict__ part_rz,
     double* __restrict__ part_roz, const double* __restrict__ scal)
 {
@@ -498,11 +513,11 @@ __global__ __launch_bounds__(kThreads) void k_from_lattice(
   __shared__ double sh[kThreads / 64];
   const double rr = sum_parts(part_rr, nparts, sh);
   const double rms = sqrt(rr / n);
-  const int vol = n / nc;
+  const int64_t vol = lm.vol;
   double rz = 0.0, roz = 0.0;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
     const int node = i / nc, comp = i - node * nc;
-    const V zi = dinv[i] * r[i] + (V)(scale * rms * (double)out[(int64_t)comp * vol + node]);
+    const V zi = dinv[i] * r[i] + (V)(scale * rms * (double)out[(int64_t)comp * vol + lm.at(node)]);
     z[i] = zi;
     rz += (double)(r[i] * zi);
     if (rold) roz += (double)(rold[i] * zi);
@@ -573,11 +588,17 @@ static void apply_learned(Context& c, const V* r, const V* rold, const V* dinv, 
                           double* part_rr, double* part_rz, double* part_roz)
 {
   const int n = c.neq;
+  LatMap lm;
+  lm.bw = c.box[2];
+  lm.bh = c.box[1];
+  lm.W = c.lat[2];
+  lm.H = c.lat[1];
+  lm.vol = (int64_t)c.lat[0] * c.lat[1] * c.lat[2];
   k_to_lattice<V><<<kGrid, kThreads, 0, c.stream>>>(n, c.ncomp, r, c.d_netin, c.d_aux, c.naux,
-                                                    c.features, part_rr, kGrid, c.w.scal);
+                                                    c.features, lm, part_rr, kGrid, c.w.scal);
   const float* out = c.net->forward(c.d_netin, 1, c.lat[0], c.lat[1], c.lat[2]);
   k_from_lattice<V><<<kGrid, kThreads, 0, c.stream>>>(n, c.ncomp, r, rold, dinv, out, z,
-                                                      c.learned_scale, part_rr, kGrid, part_rz,
+                                                      c.learned_scale, lm, part_rr, kGrid, part_rz,
                                                       part_roz, c.w.scal);
 }
 
@@ -775,9 +796,35 @@ int set_learned(Context& c, int enable, int D, int H, int W, const float* aux, i
   }
   if (!c.net) return fail(IFEM_GPU_EINVAL, "set_learned_preconditioner: create the network first");
   if (!c.pattern_ready) return fail(IFEM_GPU_EINVAL, "set_learned_preconditioner: no system");
-  if (D < 1 || H < 1 || W < 1 || (int64_t)D * H * W * c.ncomp != c.neq)
-    return fail(IFEM_GPU_EUNSUPPORTED,
-                "set_learned_preconditioner: the free DOFs are not a D x H x W lattice");
+  // box of the nodes with free DOFs; all components free inside, none outside
+  const PatchRec& Pt = c.patches[c.patch];
+  const int n1 = Pt.n[0], n2 = Pt.n[1], n3 = Pt.n[2], nc = c.ncomp;
+  int lo[3] = {n1, n2, n3}, hi[3] = {-1, -1, -1};
+  for (int k = 0; k < n3; k++)
+    for (int j = 0; j < n2; j++)
+      for (int i = 0; i < n1; i++) {
+        const int64_t node = (int64_t)i + (int64_t)n1 * (j + (int64_t)n2 * k);
+        if (c.h_eq[node * nc] < 0) continue;
+        const int idx[3] = {i, j, k};
+        for (int d = 0; d < 3; d++) {
+          lo[d] = std::min(lo[d], idx[d]);
+          hi[d] = std::max(hi[d], idx[d]);
+        }
+      }
+  const int bw = hi[0] - lo[0] + 1, bh = hi[1] - lo[1] + 1, bd = hi[2] - lo[2] + 1;
+  if (bw < 1 || (int64_t)bw * bh * bd * nc != c.neq)
+    return fail(IFEM_GPU_EUNSUPPORTED, "set_learned_preconditioner: the free DOFs are not a box");
+  for (int k = lo[2]; k <= hi[2]; k++)
+    for (int j = lo[1]; j <= hi[1]; j++)
+      for (int i = lo[0]; i <= hi[0]; i++) {
+        const int64_t node = (int64_t)i + (int64_t)n1 * (j + (int64_t)n2 * k);
+        for (int cc = 0; cc < nc; cc++)
+          if (c.h_eq[node * nc + cc] < 0)
+            return fail(IFEM_GPU_EUNSUPPORTED,
+                        "set_learned_preconditioner: the free DOFs are not a box");
+      }
+  if (D < bd || H < bh || W < bw)
+    return fail(IFEM_GPU_EINVAL, "set_learned_preconditioner: lattice smaller than the free box");
   if (naux < 0 || (naux > 0 && !aux)) return fail(IFEM_GPU_EINVAL, "set_learned_preconditioner: aux");
   if (features != 0 && features != 1)
     return fail(IFEM_GPU_EINVAL, "set_learned_preconditioner: features must be 0 or 1");
@@ -796,6 +843,7 @@ int set_learned(Context& c, int enable, int D, int H, int W, const float* aux, i
   c.d_rold32 = nullptr;
   c.d_aux = nullptr;
   IFEM_HIP(hipMalloc(&c.d_netin, sizeof(float) * cin * vol));
+  IFEM_HIP(hipMemsetAsync(c.d_netin, 0, sizeof(float) * cin * vol, c.stream)); // padding stays 0
   if (naux > 0) {
     IFEM_HIP(hipMalloc(&c.d_aux, sizeof(float) * naux * vol));
     IFEM_HIP(hipMemcpyAsync(c.d_aux, aux, sizeof(float) * naux * vol, hipMemcpyDeviceToDevice,
@@ -810,6 +858,9 @@ int set_learned(Context& c, int enable, int D, int H, int W, const float* aux, i
   c.lat[0] = D;
   c.lat[1] = H;
   c.lat[2] = W;
+  c.box[0] = bd;
+  c.box[1] = bh;
+  c.box[2] = bw;
   c.naux = naux;
   c.features = features;
   c.learned_scale = scale;

@@ -518,8 +518,8 @@ void UNet::conv_backward(const Conv& cv, const float* in0, int c0, const float* 
   }
 }
 
-int UNet::backward(const float* x, const float* target, const float* wts, int N, int D, int H,
-                   int W, bool want_loss, float* loss)
+int UNet::backward(const float* x, const float* target, const float* wts, const float* dy, int N,
+                   int D, int H, int W, bool want_loss, float* loss)
 {
   const int L = cfg.levels;
   hipStream_t s = stream;
@@ -527,8 +527,13 @@ int UNet::backward(const float* x, const float* target, const float* wts, int N,
   float* gA = gscratch + maxt;
   float* gB = gscratch + 2 * maxt;
   const int nb = 256;
-  k_mse<<<nb, 256, 0, s>>>(outbuf, target, gA, nout, 1.0f / (float)N, wts,
-                           nout / N, part);
+  if (dy) {
+    // the caller's loss: dL/dy given
+    IFEM_HIP(hipMemcpyAsync(gA, dy, sizeof(float) * nout, hipMemcpyDeviceToDevice, s));
+    want_loss = false;
+  } else {
+    k_mse<<<nb, 256, 0, s>>>(outbuf, target, gA, nout, 1.0f / (float)N, wts, nout / N, part);
+  }
   int ic = (int)convs.size() - 1;
   const bool act = cfg.activation == 0;
   auto relu_bwd = [&](float* g, const float* y, int64_t n) {
@@ -667,7 +672,7 @@ int ifem_gpu_unet_gradient(ifem_gpu_ctx* ctx, const float* x, const float* t, co
   int rc = n.ensure(N, D, H, W);
   if (rc) return rc;
   n.forward(x, N, D, H, W);
-  rc = n.backward(x, t, w, N, D, H, W, true, loss);
+  rc = n.backward(x, t, w, nullptr, N, D, H, W, true, loss);
   if (rc) return rc;
   if (grad_host) {
     IFEM_HIP(hipStreamSynchronize(ctx->c.stream));
@@ -685,9 +690,29 @@ int ifem_gpu_unet_train_step(ifem_gpu_ctx* ctx, const float* x, const float* t, 
   int rc = n.ensure(N, D, H, W);
   if (rc) return rc;
   n.forward(x, N, D, H, W);
-  rc = n.backward(x, t, w, N, D, H, W, loss != nullptr, loss);
+  rc = n.backward(x, t, w, nullptr, N, D, H, W, loss != nullptr, loss);
   if (rc) return rc;
   return n.adam(lr);
+}
+
+int ifem_gpu_unet_backward(ifem_gpu_ctx* ctx, const float* x, const float* dy, int N, int D, int H,
+                           int W, float lr, float* grad_host)
+{
+  if (!ctx || !ctx->c.net || !x || !dy) return fail(IFEM_GPU_EINVAL, "unet_backward: bad argument");
+  if (!(lr >= 0.f)) return fail(IFEM_GPU_EINVAL, "unet_backward: lr must be >= 0");
+  IFEM_HIP(hipSetDevice(ctx->c.device));
+  UNet& n = *ctx->c.net;
+  int rc = n.ensure(N, D, H, W);
+  if (rc) return rc;
+  n.forward(x, N, D, H, W);
+  rc = n.backward(x, nullptr, nullptr, dy, N, D, H, W, false, nullptr);
+  if (rc) return rc;
+  if (grad_host) {
+    IFEM_HIP(hipStreamSynchronize(ctx->c.stream));
+    IFEM_HIP(hipMemcpy(grad_host, n.grads, sizeof(float) * n.nparams, hipMemcpyDeviceToHost));
+  }
+  if (lr > 0.f) return n.adam(lr);
+  return IFEM_GPU_OK;
 }
 
 } // extern "C"

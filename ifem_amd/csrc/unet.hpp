@@ -42,11 +42,11 @@ struct UNet {
   int init(const ifem_gpu_unet_config& c, hipStream_t s);
   int ensure(int N, int D, int H, int W);
   const float* forward(const float* x, int N, int D, int H, int W);
-  // gradient of 0.5*sum_n w_n*sum((y-t)^2)/N (w_n = 1 if wts is null)
-  // w.r.t. the parameters into grads; returns
-  // the loss when want_loss (synchronises), else 0
-  int backward(const float* x, const float* target, const float* wts, int N, int D, int H, int W,
-               bool want_loss, float* loss);
+  // gradient of 0.5*sum_n w_n*sum((y-t)^2)/N (w_n = 1 if wts is null), or
+  // with dy != nullptr of sum(dy*y), w.r.t. the parameters into grads, after
+  // forward(x); returns the loss when want_loss (synchronises)
+  int backward(const float* x, const float* target, const float* wts, const float* dy, int N,
+               int D, int H, int W, bool want_loss, float* loss);
   int adam(float lr);
 
 private:

@@ -1,6 +1,9 @@
 """Learned preconditioner: data from the method's own Jacobi-PCG solves,
 training of the U-net on the GPU, and evaluation inside flexible CG.
 
+The free-DOF box can be placed in a zero-padded lattice whose extents divide
+by 2^(levels-1) (``lattice_dims``, ``to_lattice``, ``from_lattice``).
+
 The network sees the residual on the lattice of free DOFs (normalised to unit
 RMS) plus auxiliary channels describing the problem (for diffusion: the
 log-conductivity at the nodes), and predicts the part of the error that the
@@ -36,6 +39,36 @@ def lattice(pb) -> tuple:
                                             is_.min():is_.max() + 1].all():
         raise ValueError("the free DOFs do not form a box lattice")
     return int(D), int(H), int(W), nc
+
+
+def lattice_dims(pb, levels: int) -> tuple:
+    """(D, H, W): the free-DOF box extents rounded up to multiples of
+    2^(levels-1) (the U-net pools levels-1 times)."""
+    D, H, W, _ = lattice(pb)
+    m = 1 << (levels - 1)
+    return tuple(int(-(-v // m) * m) for v in (D, H, W))
+
+
+def to_lattice(v, pb, dims):
+    """Equation-order vector(s) (..., neq) -> (..., nc, D, H, W) float32 with
+    the free box at the origin and zeros beyond it."""
+    import torch
+    bd, bh, bw, nc = lattice(pb)
+    D, H, W = dims
+    lead = v.shape[:-1]
+    t = v.reshape(*lead, bd, bh, bw, nc).float()
+    t = t.movedim(-1, -4)
+    out = torch.zeros(*lead, nc, D, H, W, dtype=torch.float32, device=v.device)
+    out[..., :bd, :bh, :bw] = t
+    return out
+
+
+def from_lattice(t, pb):
+    """Inverse of to_lattice: (..., nc, D, H, W) -> (..., neq) float64."""
+    bd, bh, bw, nc = lattice(pb)
+    lead = t.shape[:-4]
+    box = t[..., :bd, :bh, :bw].movedim(-4, -1)
+    return box.reshape(*lead, bd * bh * bw * nc).double()
 
 
 def aux_channels(pb) -> np.ndarray:
@@ -176,13 +209,23 @@ def train(s: Session, X, T, epochs=20, batch=8, lr=2e-3, log=None, seed=0, weigh
     return losses
 
 
-def enable(s: Session, pb, scale=1.0, features=1):
-    """Turn the trained network into the preconditioner of session s."""
+def enable(s: Session, pb, scale=1.0, features=1, dims=None):
+    """Turn the trained network into the preconditioner of session s (on the
+    lattice dims, default the free box itself)."""
     import torch
     D, H, W, nc = lattice(pb)
-    aux = aux_channels(pb)
+    dims = dims or (D, H, W)
+    aux = padded_aux(pb, dims)
     auxt = torch.from_numpy(np.ascontiguousarray(aux)).to("cuda") if aux.shape[0] else None
-    s.set_learned(True, D, H, W, auxt.data_ptr() if auxt is not None else 0, aux.shape[0], scale,
-                  features)
+    s.set_learned(True, dims[0], dims[1], dims[2], auxt.data_ptr() if auxt is not None else 0,
+                  aux.shape[0], scale, features)
     torch.cuda.synchronize()
     return auxt
+
+
+def padded_aux(pb, dims) -> np.ndarray:
+    aux = aux_channels(pb)
+    D, H, W = dims
+    out = np.zeros((aux.shape[0], D, H, W), dtype=np.float32)
+    out[:, :aux.shape[1], :aux.shape[2], :aux.shape[3]] = aux
+    return out

@@ -233,9 +233,19 @@ int ifem_gpu_unet_gradient(ifem_gpu_ctx* ctx, const float* x, const float* t, co
 int ifem_gpu_unet_train_step(ifem_gpu_ctx* ctx, const float* x, const float* t, const float* w,
                              int N, int D, int H, int W, float lr, float* loss);
 
+/* Gradient of sum(dy * U(x)) w.r.t. the parameters for a caller-supplied
+ * output gradient dy (device, shape of U(x)), e.g. the energy-norm loss
+ * 0.5 z^T K z - z^T r of the preconditioned residual z, whose gradient needs
+ * K (ifem_gpu_spmv_device).  Runs the forward pass on x first; grad (host,
+ * may be NULL) receives the gradient; lr > 0 applies one Adam step. */
+int ifem_gpu_unet_backward(ifem_gpu_ctx* ctx, const float* x, const float* dy, int N, int D, int H,
+                           int W, float lr, float* grad);
+
 /* Use the network as preconditioner of the next solves (enable != 0) or go
- * back to Jacobi (enable == 0).  The free DOFs must form a D x H x W lattice
- * in equation order with ncomp components per node (neq = D*H*W*ncomp);
+ * back to Jacobi (enable == 0).  The nodes with free DOFs must form a box of
+ * the control-point grid (all ncomp components free inside, none outside);
+ * the box (node i fastest) is placed at the origin of a D x H x W lattice
+ * with D, H, W >= the box extents in k, j, i (zero padding beyond it).
  * aux: naux auxiliary fields (D*H*W each, device pointer, copied).
  * features 0: network input [r^, aux] (ncomp + naux channels);
  * features 1: [r^, r^*aux_j per component] (ncomp*(1+naux) channels), which

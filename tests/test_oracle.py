@@ -466,3 +466,21 @@ def test_manufactured_solution_converges():
         ue = np.sin(np.pi * X[:, 0]) * np.sin(np.pi * X[:, 1]) * np.sin(np.pi * X[:, 2])
         errs.append(np.max(np.abs(xd - ue)))
     assert errs[1] < errs[0] / 3 and errs[2] < errs[1] / 3
+
+
+def test_lattice_padding_round_trip():
+    """Free-DOF box of the clamped elasticity cube in a zero-padded lattice
+    (host-side helpers of the learned preconditioner)."""
+    import torch
+    from ifem_amd import learned, problems
+    pb = problems.elasticity_cube(6, 2, seed=1)
+    assert learned.lattice(pb) == (8, 8, 7, 3)
+    dims = learned.lattice_dims(pb, 3)
+    assert dims == (8, 8, 8)
+    v = torch.arange(pb.neq, dtype=torch.float64)
+    t = learned.to_lattice(v, pb, dims)
+    assert t.shape == (3, 8, 8, 8)
+    assert float(t[:, :, :, 7].abs().sum()) == 0.0  # padding in i
+    # equation of node (i, j, k) = (1 + ii, jj, kk), component cc
+    assert float(t[2, 3, 4, 5]) == float(((3 * 8 + 4) * 7 + 5) * 3 + 2)
+    assert torch.equal(learned.from_lattice(t, pb), v)

@@ -15,17 +15,24 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
 
 
-@pytest.mark.parametrize("levels,channels", [(3, (4, 6, 8)), (2, (3, 5)), (1, (4,))])
-def test_unet_forward_and_gradient_match_oracle(levels, channels):
+# seeds: with ReLU, a pre-activation within FP32 rounding of zero flips its
+# mask between the FP32 kernels and the FP64 restatement and changes the
+# gradient by O(1) there; the seeds below keep every pre-activation clear of
+# zero (the linear network has no such ties).
+@pytest.mark.parametrize("levels,channels,dims,seed,linear", [
+    (3, (4, 6, 8), (8, 8, 4), 3, False), (2, (3, 5), (8, 8, 4), 2, False),
+    (1, (4,), (8, 8, 4), 1, False), (4, (3, 4, 5, 6), (16, 16, 8), 4, False),
+    (3, (4, 5, 6), (12, 20, 12), 4, False), (3, (4, 5, 6), (12, 20, 12), 3, True)])
+def test_unet_forward_and_gradient_match_oracle(levels, channels, dims, seed, linear):
     import torch
     from ifem_amd import Session
-    print(f"[gpu] unet gradient check levels={levels}", flush=True)
-    N, D, H, W, cin, cout = 2, 8, 8, 4, 2, 1
-    rng = np.random.default_rng(levels)
+    print(f"[gpu] unet gradient check levels={levels} dims={dims} linear={linear}", flush=True)
+    (D, H, W), N, cin, cout = dims, 2, 2, 1
+    rng = np.random.default_rng(seed)
     x = rng.standard_normal((N, cin, D, H, W)).astype(np.float32)
     t = rng.standard_normal((N, cout, D, H, W)).astype(np.float32)
     with Session() as s:
-        n = s.unet_create(levels, channels, cin, cout, seed=7)
+        n = s.unet_create(levels, channels, cin, cout, seed=7, linear=linear)
         assert n == unet_ref.layout(levels, channels, cin, cout)[1]
         p = s.unet_params()
         # non-zero biases so that their gradients and ReLU masks are exercised
@@ -41,10 +48,31 @@ def test_unet_forward_and_gradient_match_oracle(levels, channels):
         loss, g = s.unet_gradient(xd.data_ptr(), td.data_ptr(), N, D, H, W, w_ptr=wd.data_ptr())
         y = yd.cpu().numpy()
     yr, lr_, gr = unet_ref.forward_backward(p.astype(np.float64), x, t, levels, list(channels),
-                                            cin, cout, weights=wts)
+                                            cin, cout, weights=wts, linear=linear)
     assert _rel(y, yr) <= 1e-4
     assert abs(loss - lr_) <= 1e-4 * abs(lr_)
     assert _rel(g, gr) <= 1e-4
+
+
+def test_unet_backward_with_output_gradient():
+    """The gradient of sum(dy * U(x)) equals the regression gradient for the
+    target t = U(x) - N*dy (unit weights), whose dL/dy is dy."""
+    import torch
+    from ifem_amd import Session
+    print("[gpu] unet backward with output gradient", flush=True)
+    N, D, H, W, cin, cout = 2, 8, 8, 8, 3, 3
+    rng = np.random.default_rng(11)
+    x = torch.from_numpy(rng.standard_normal((N, cin, D, H, W)).astype(np.float32)).cuda()
+    dy = torch.from_numpy(rng.standard_normal((N, cout, D, H, W)).astype(np.float32)).cuda()
+    with Session() as s:
+        s.unet_create(3, (4, 6, 8), cin, cout, seed=5, linear=True)
+        y = torch.empty((N, cout, D, H, W), dtype=torch.float32, device="cuda")
+        s.unet_forward(x.data_ptr(), y.data_ptr(), N, D, H, W)
+        torch.cuda.synchronize()
+        t = (y - N * dy).contiguous()
+        _, g_ref = s.unet_gradient(x.data_ptr(), t.data_ptr(), N, D, H, W)
+        g = s.unet_backward(x.data_ptr(), dy.data_ptr(), N, D, H, W, want_grad=True)
+    assert _rel(g, g_ref) <= 1e-4
 
 
 def test_unet_training_reduces_loss():
@@ -127,3 +155,68 @@ def test_learned_preconditioner_falls_back_to_jacobi():
         learned.enable(s, pb, scale=1e6)
         st = s.solve(rtol=1e-8, maxits=5000, check_every=2)
     assert st["converged"] and st["rel_residual"] <= 1e-8
+
+
+def _identity_unet_params(levels, c, ncomp):
+    """Flat parameters of a linear U-net (channels c at every level) whose
+    output equals its input: identity centre taps along the encoder level 0,
+    the skip half of the decoder's first convolution and the output
+    convolution; every other weight and all biases zero."""
+    convs, n = unet_ref.layout(levels, [c] * levels, ncomp, ncomp)
+    p = np.zeros(n, dtype=np.float32)
+
+    def eye(i, ci_off=0):
+        ci, co, relu, w, b = convs[i]
+        W = p[w:w + co * ci * 27].reshape(co, ci, 27)
+        for k in range(min(co, ncomp)):
+            W[k, ci_off + k, 13] = 1.0
+
+    eye(0)
+    eye(1)
+    if levels > 1:
+        nenc = 2 * levels
+        eye(nenc + 2 * (levels - 2), ci_off=c)  # decoder level 0, first conv: skip part
+        eye(nenc + 2 * (levels - 2) + 1)
+    eye(len(convs) - 1)
+    return p
+
+
+def test_learned_preconditioner_on_padded_lattice_matches_numpy_pcg():
+    """Elasticity 4^3 (free box 6x6x5 nodes, clamped face) on a 6x6x6 lattice
+    (levels 2): with an identity network the learned preconditioner is
+    z = D^-1 r + s r, an SPD operator, so 10 flexible-CG iterations equal 10
+    PCG iterations with M^-1 = D^-1 + s I computed in numpy.  A wrong box <->
+    lattice mapping would make M^-1 a different, non-symmetric operator."""
+    import scipy.sparse as sp
+    from ifem_amd import learned, problems, setup_problem
+    print("[gpu] learned preconditioner on a padded lattice", flush=True)
+    pb = problems.elasticity_cube(4, 2, seed=3)
+    levels, c = 2, 3
+    dims = learned.lattice_dims(pb, levels)
+    assert learned.lattice(pb)[:3] == (6, 6, 5) and dims == (6, 6, 6)
+    with setup_problem(pb) as s:
+        rp, ci, A, b = s.system()
+        dinv = learned.jacobi_inverse(s)
+        scale = float(np.mean(dinv))
+        s.unet_create(levels, (c, c), pb.ncomp, pb.ncomp, linear=True)
+        s.unet_set_params(_identity_unet_params(levels, c, pb.ncomp))
+        learned.enable(s, pb, scale=scale, dims=dims)
+        st = s.solve(rtol=1e-30, maxits=10, check_every=2)
+        x = s.solution()
+    assert st["iterations"] == 10
+    K = sp.csr_matrix((A, ci, rp), shape=(pb.neq, pb.neq))
+    xr = np.zeros(pb.neq)
+    r = b.copy()
+    z = dinv * r + scale * r
+    p = z.copy()
+    rz = r @ z
+    for _ in range(10):
+        q = K @ p
+        a = rz / (p @ q)
+        xr += a * p
+        r -= a * q
+        z = dinv * r + scale * r
+        rzn = r @ z
+        p = z + (rzn / rz) * p
+        rz = rzn
+    assert np.max(np.abs(x - xr)) <= 1e-4 * np.max(np.abs(xr))

@@ -35,10 +35,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (calibrated against a 1 GiB copy), or None."""
+    fn = os.path.join(ROOT, "profiles", "hbm_traffic_elasticity64.md")
+    try:
+        with open(fn) as f:
+            for line in f:
+                cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+                if len(cells) >= 4 and cells[0] == kernel:
+                    return round(float(cells[3]) * 1e9)
+    except OSError:
+        pass
+    return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nel", type=int, default=64)
     ap.add_argument("--rtol", type=float, default=1e-8)
@@ -211,7 +226,8 @@ def main():
     if not args.no_cpu:
         import oracle
         rp, ci, A, b = s.system(values=True)
-        threads = min(16, os.cpu_count() or 1)
+        avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        threads = max(1, min(16, avail or 1))
         layers = min(args.cpu_layers, pb.patch.nel[2])
         t0 = time.perf_counter()
         oracle.assemble(pb.patch, pb.integrand, pb.ng, pb.meqn, neq, rp, ci, nthreads=threads,
@@ -219,6 +235,22 @@ def main():
         t_asm = time.perf_counter() - t0
         frac = layers / pb.patch.nel[2]
         cpu_dofs = ndof / (t_asm / frac)
+        # one thread on a smaller sample (2 element layers)
+        l1 = min(2, pb.patch.nel[2])
+        t0 = time.perf_counter()
+        oracle.assemble(pb.patch, pb.integrand, pb.ng, pb.meqn, neq, rp, ci, nthreads=1,
+                        e3_range=(0, l1))
+        t_1 = time.perf_counter() - t0
+        cpu_dofs_1 = ndof / (t_1 / (l1 / pb.patch.nel[2]))
+        model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
         its = 10
         t0 = time.perf_counter()
         oracle.pcg_fixed(rp, ci, A, b, its)
@@ -230,7 +262,10 @@ def main():
                           f"{t_asm:.1f} s) scaled to the whole patch; Jacobi-PCG {its} iterations "
                           f"on the full FP64 system"),
                "pcg_ms_per_iter": round(t_it * 1e3, 2),
-               "time_to_tol_est_s": round(t_it * iters64, 1)}
+               "time_to_tol_est_s": round(t_it * iters64, 1),
+               "value_1thread": round(cpu_dofs_1, 1),
+               "sample_1thread": f"element layers w<{l1}, one thread, {t_1:.1f} s",
+               "host_cpus_available": avail, "cpu_model": model}
         del A
         log(f"[bench] cpu baseline: {cpu}")
 
@@ -238,9 +273,11 @@ def main():
     flops = (36 * nen * nen + 6 * nen) * ngp * nel_tot
     asm_bytes = nnz * 8 + neq * 8 + pb.patch.nnod * 3 * 8
     achieved = flops / (kern_ms * 1e-3)
-    roofline = {"kernel": "k_assemble_lines<2,3,3,double>", "bound": "mfma",
+    traffic = measured_traffic("ifem::k_assemble_lines<2, 3, 3, double, 1, false, 1>") \
+        if args.nel == 64 else None
+    roofline = {"kernel": "k_assemble_lines<2,3,3,double,1,false,1>", "bound": "mfma",
                 "achieved": round(achieved / 1e12, 3), "peak": PEAK_FP64 / 1e12,
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64, 4), "traffic": traffic,
                 "algorithmic_flops": flops, "algorithmic_bytes": asm_bytes,
                 "kernel_ms": round(kern_ms, 3),
                 "hbm_frac_of_bytes": round(asm_bytes / (kern_ms * 1e-3) / PEAK_HBM, 4),
@@ -249,7 +286,8 @@ def main():
     spmv_roof = {"kernel": "k_spmv_node3<double,double,false>", "bound": "hbm",
                  "achieved": round(spmv_bytes / (spmv_ms * 1e-3) / 1e9, 1), "peak": 8000.0,
                  "unit": "GB/s", "frac": round(spmv_bytes / (spmv_ms * 1e-3) / PEAK_HBM, 4),
-                 "traffic": None, "bytes": spmv_bytes, "kernel_ms": round(spmv_ms, 4)}
+                 "traffic": (measured_traffic("ifem::k_spmv_node3<double, double, false>")
+                             if args.nel == 64 else None), "bytes": spmv_bytes, "kernel_ms": round(spmv_ms, 4)}
 
     value = ndof / (asm_ms * 1e-3)
     out = {

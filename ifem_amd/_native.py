@@ -28,6 +28,7 @@ EXPORTS = [
     "ifem_gpu_batch_autotune", "ifem_gpu_batch_solve", "ifem_gpu_batch_results",
     "ifem_gpu_batch_patch_info", "ifem_gpu_batch_get_system", "ifem_gpu_batch_get_solution",
     "ifem_gpu_batch_timings", "ifem_gpu_autotune_assembly", "ifem_gpu_unet_backward",
+    "ifem_gpu_last_assembly_parts",
 ]
 
 FP64, FP32 = 64, 32
@@ -106,6 +107,7 @@ def lib():
     L.ifem_gpu_spmv_device.argtypes = [vp, ip, vp, vp]
     L.ifem_gpu_device_pointers.argtypes = [vp] + [C.POINTER(vp)] * 5
     L.ifem_gpu_last_timings.argtypes = [vp, C.POINTER(dp), C.POINTER(dp)]
+    L.ifem_gpu_last_assembly_parts.argtypes = [vp, C.POINTER(dp), C.POINTER(dp)]
     L.ifem_gpu_unet_create.argtypes = [vp, C.POINTER(UNetConfig), C.POINTER(C.c_int64)]
     L.ifem_gpu_unet_get_params.argtypes = [vp, vp]
     L.ifem_gpu_unet_set_params.argtypes = [vp, vp]
@@ -362,6 +364,12 @@ class Session:
         a = C.c_double(0.0); s = C.c_double(0.0)
         check(lib().ifem_gpu_last_timings(self.h, C.byref(a), C.byref(s)))
         return a.value, s.value
+
+    def assembly_parts(self):
+        """(geometry pass ms, line kernel ms) of the last assembly."""
+        g = C.c_double(0.0); l = C.c_double(0.0)
+        check(lib().ifem_gpu_last_assembly_parts(self.h, C.byref(g), C.byref(l)))
+        return g.value, l.value
 
     # ---- batch of independent patches -----------------------------------------
     def batch_clear(self):

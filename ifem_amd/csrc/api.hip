@@ -98,7 +98,8 @@ int ifem_gpu_ctx_create(int device, void* stream, ifem_gpu_ctx** out)
     }
     ctx->c.own_stream = true;
   }
-  if (hipEventCreate(&ctx->c.ev0) != hipSuccess || hipEventCreate(&ctx->c.ev1) != hipSuccess) {
+  if (hipEventCreate(&ctx->c.ev0) != hipSuccess || hipEventCreate(&ctx->c.ev1) != hipSuccess ||
+      hipEventCreate(&ctx->c.ev_mid) != hipSuccess) {
     delete ctx;
     return fail(IFEM_GPU_EDEVICE, "hipEventCreate failed");
   }
@@ -120,8 +121,10 @@ int ifem_gpu_ctx_destroy(ifem_gpu_ctx* ctx)
   ctx->c.net.reset();
   ctx->c.batch.reset();
   hipFree(ctx->c.d_asmargs);
+  hipFree(ctx->c.d_gpgeo);
   hipEventDestroy(ctx->c.ev0);
   hipEventDestroy(ctx->c.ev1);
+  hipEventDestroy(ctx->c.ev_mid);
   if (ctx->c.own_stream) hipStreamDestroy(ctx->c.stream);
   delete ctx;
   return IFEM_GPU_OK;
@@ -421,6 +424,14 @@ int ifem_gpu_set_learned_preconditioner(ifem_gpu_ctx* ctx, int enable, int D, in
 {
   CHECK_CTX(ctx);
   return set_learned(ctx->c, enable, D, H, W, aux, naux, features, scale);
+}
+
+int ifem_gpu_last_assembly_parts(ifem_gpu_ctx* ctx, double* geometry_ms, double* lines_ms)
+{
+  if (!ctx) return fail(IFEM_GPU_EINVAL, "null context");
+  if (geometry_ms) *geometry_ms = ctx->c.last_geometry_ms;
+  if (lines_ms) *lines_ms = ctx->c.last_lines_ms;
+  return IFEM_GPU_OK;
 }
 
 int ifem_gpu_last_timings(ifem_gpu_ctx* ctx, double* assemble_ms, double* solve_ms)

@@ -167,12 +167,20 @@ struct AsmArgs {
   double lam, mu;
   int variant;                         // batch kernel: 2*(p-1) + half chunk
   int ng;                              // Gauss points per direction
-  // kappa / source at the Gauss points, [element][gp] with element
-  // e1 + nel1*(e2 + nel2*e3) and gp q1 + ng*(q2 + ng*q3); nullptr:
-  // evaluated in the line kernel
-  double* gp_kappa;
-  double* gp_source;
+  int ncomp;                           // 1 scalar, 3 vector (elasticity)
+  // geometry record per (element, Gauss point), [element][gp][GeoRec::N]
+  // with element e1 + nel1*(e2 + nel2*e3) and gp q1 + ng*(q2 + ng*q3),
+  // written by k_gp_geometry, read by the line kernels
+  double* gp_geo;
   double kappa_c, source_c;            // values of constant fields
+};
+
+// Geometry record of one Gauss point.  Vector problems: J^-1 (row-major, 9)
+// and w*detJ.  Scalar problems: M = w*detJ*kappa(X) * J^-1 J^-T (symmetric:
+// m00 m01 m02 m11 m12 m22) and w*detJ*f(X), padded to 8.
+template <int NC>
+struct GeoRec {
+  static constexpr int N = NC == 1 ? 8 : 10;
 };
 
 // The line kernels read their AsmArgs from a device buffer through the
@@ -196,12 +204,33 @@ struct Chunk {
   static constexpr int v = HALF ? ((full + 1) / 2) : full;
 };
 
-// Fields that are expensive to evaluate (Fourier series) are evaluated once
-// per (element, Gauss point) before the line kernel, which would otherwise
-// evaluate each of them once per line touching the element ((p+1)^2 times).
-// X = sum_b N_b(xi) x_b as in the line kernel (phase A).
+// Geometry pre-pass: one thread per (element, Gauss point) computes, once
+// per element, what the line kernels would otherwise recompute in each of the
+// (p+1)^2 lines touching the element: J = X dN/du with the reference's
+// cofactor det/inverse (utl::Jacobian, src/Utility/CoordinateMapping.C:24-63,
+// src/LinAlg/matrix.h:791-862, |detJ| <= 1e-12 drops the point),
+// detJxW = detJ*dV*w_i*w_j*w_k (src/ASM/ASMs3D.C:2145,2285), and for scalar
+// problems the material kappa(X) and source f(X) at X = sum_b N_b x_b
+// (src/ASM/ASMs3D.C:2303), folded into the record GeoRec.
+__device__ inline bool field_varies(const DevFunction& f)
+{
+  return f.type == IFEM_GPU_FN_SINPROD ||
+         ((f.type == IFEM_GPU_FN_FOURIER || f.type == IFEM_GPU_FN_EXP_FOURIER) && f.nmodes > 0);
+}
+
+__device__ inline DevFunction load_function(const __attribute__((address_space(4))) DevFunction& f)
+{
+  DevFunction d;
+  d.type = f.type;
+  d.nmodes = f.nmodes;
+  d.scale = f.scale;
+  d.mean = f.mean;
+  d.modes = f.modes;
+  return d;
+}
+
 template <int P>
-__device__ inline void gp_fields_at(const AsmArgs& a, int64_t id)
+__device__ inline void gp_geometry_at(ConstArgs& a, int64_t id, bool want_rhs)
 {
   constexpr int P1 = P + 1;
   const int ng = a.ng;
@@ -211,38 +240,88 @@ __device__ inline void gp_fields_at(const AsmArgs& a, int64_t id)
   const int q1 = q % ng, q2 = (q / ng) % ng, q3 = q / (ng * ng);
   const int e1 = (int)(e % a.nel1), e2 = (int)((e / a.nel1) % a.nel2),
             e3 = (int)(e / ((int64_t)a.nel1 * a.nel2));
+  double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   double X[3] = {0.0, 0.0, 0.0};
-#pragma unroll
+#pragma unroll 1
   for (int b3 = 0; b3 < P1; b3++)
-#pragma unroll
+#pragma unroll 1
     for (int b2 = 0; b2 < P1; b2++)
 #pragma unroll
       for (int b1 = 0; b1 < P1; b1++) {
-        const double n1 = a.tabN1[(e1 * ng + q1) * P1 + b1];
-        const double n2 = a.tabN2[(e2 * ng + q2) * P1 + b2];
-        const double n3 = a.tabN3[(e3 * ng + q3) * P1 + b3];
+        const double n1 = a.tabN1[(e1 * ng + q1) * P1 + b1], d1 = a.tabD1[(e1 * ng + q1) * P1 + b1];
+        const double n2 = a.tabN2[(e2 * ng + q2) * P1 + b2], d2 = a.tabD2[(e2 * ng + q2) * P1 + b2];
+        const double n3 = a.tabN3[(e3 * ng + q3) * P1 + b3], d3 = a.tabD3[(e3 * ng + q3) * P1 + b3];
+        const double g0 = d1 * n2 * n3, g1 = n1 * d2 * n3, g2 = n1 * n2 * d3;
         const double nv = n1 * n2 * n3;
         const int64_t node =
             (int64_t)(e1 + b1) + (int64_t)a.n1 * ((e2 + b2) + (int64_t)a.n2 * (e3 + b3));
 #pragma unroll
-        for (int i = 0; i < 3; i++) X[i] += a.coefs[3 * node + i] * nv;
+        for (int i = 0; i < 3; i++) {
+          const double xi = a.coefs[3 * node + i];
+          J[3 * i + 0] += xi * g0;
+          J[3 * i + 1] += xi * g1;
+          J[3 * i + 2] += xi * g2;
+          X[i] += xi * nv;
+        }
       }
-  if (a.gp_kappa) a.gp_kappa[id] = eval_field(a.kappa, X[0], X[1], X[2]);
-  if (a.gp_source) a.gp_source[id] = eval_field(a.source, X[0], X[1], X[2]);
+  const double det = J[0] * (J[4] * J[8] - J[7] * J[5]) - J[1] * (J[3] * J[8] - J[6] * J[5]) +
+                     J[2] * (J[3] * J[7] - J[6] * J[4]);
+  double Ji[9];
+  double wdet;
+  if (det <= 1e-12 && det >= -1e-12) {
+#pragma unroll
+    for (int m = 0; m < 9; m++) Ji[m] = 0.0;
+    wdet = 0.0;
+  } else {
+    Ji[0] = (J[4] * J[8] - J[7] * J[5]) / det;
+    Ji[3] = -(J[3] * J[8] - J[6] * J[5]) / det;
+    Ji[6] = (J[3] * J[7] - J[6] * J[4]) / det;
+    Ji[1] = -(J[1] * J[8] - J[7] * J[2]) / det;
+    Ji[4] = (J[0] * J[8] - J[6] * J[2]) / det;
+    Ji[7] = -(J[0] * J[7] - J[6] * J[1]) / det;
+    Ji[2] = (J[1] * J[5] - J[4] * J[2]) / det;
+    Ji[5] = -(J[0] * J[5] - J[3] * J[2]) / det;
+    Ji[8] = (J[0] * J[4] - J[3] * J[1]) / det;
+    const double dV = 0.125 * (a.span1[e1] * a.span2[e2] * a.span3[e3]);
+    wdet = det * (dV * a.gw[q1] * a.gw[q2] * a.gw[q3]);
+  }
+  if (a.ncomp == 3) {
+    double* out = a.gp_geo + id * GeoRec<3>::N;
+#pragma unroll
+    for (int m = 0; m < 9; m++) out[m] = Ji[m];
+    out[9] = wdet;
+    return;
+  }
+  const DevFunction kf = load_function(a.kappa), sf = load_function(a.source);
+  const double kap = field_varies(kf) ? eval_field(kf, X[0], X[1], X[2]) : a.kappa_c;
+  const double src =
+      want_rhs ? (field_varies(sf) ? eval_field(sf, X[0], X[1], X[2]) : a.source_c) : 0.0;
+  const double sk = wdet * kap;
+  double* out = a.gp_geo + id * GeoRec<1>::N;
+  out[0] = sk * (Ji[0] * Ji[0] + Ji[1] * Ji[1] + Ji[2] * Ji[2]);
+  out[1] = sk * (Ji[0] * Ji[3] + Ji[1] * Ji[4] + Ji[2] * Ji[5]);
+  out[2] = sk * (Ji[0] * Ji[6] + Ji[1] * Ji[7] + Ji[2] * Ji[8]);
+  out[3] = sk * (Ji[3] * Ji[3] + Ji[4] * Ji[4] + Ji[5] * Ji[5]);
+  out[4] = sk * (Ji[3] * Ji[6] + Ji[4] * Ji[7] + Ji[5] * Ji[8]);
+  out[5] = sk * (Ji[6] * Ji[6] + Ji[7] * Ji[7] + Ji[8] * Ji[8]);
+  out[6] = wdet * src;
+  out[7] = 0.0;
 }
 
 template <int P>
-__global__ void k_gp_fields(AsmArgs a, int64_t total)
+__global__ __launch_bounds__(256) void k_gp_geometry(const AsmArgs* __restrict__ args,
+                                                     int64_t total, int want_rhs)
 {
+  ConstArgs& a = *const_args(args);
   for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < total;
        id += (int64_t)gridDim.x * blockDim.x)
-    gp_fields_at<P>(a, id);
+    gp_geometry_at<P>(a, id, want_rhs != 0);
 }
 
-// Batch: gp0[q] = first Gauss-point slot of patch q (patches without
-// tabulated fields own no slots).
-__global__ void k_gp_fields_batch(const AsmArgs* __restrict__ args, const int64_t* __restrict__ gp0,
-                                  int np, int64_t total)
+// Batch: gp0[q] = first Gauss-point record of patch q.
+__global__ __launch_bounds__(256) void k_gp_geometry_batch(const AsmArgs* __restrict__ args,
+                                    const int64_t* __restrict__ gp0, int np, int64_t total,
+                                    int want_rhs)
 {
   for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < total;
        id += (int64_t)gridDim.x * blockDim.x) {
@@ -252,24 +331,17 @@ __global__ void k_gp_fields_batch(const AsmArgs* __restrict__ args, const int64_
       if (gp0[mid] <= id) lo = mid;
       else hi = mid - 1;
     }
-    const AsmArgs& a = args[lo];
+    ConstArgs& a = const_args(args)[lo];
     const int64_t local = id - gp0[lo];
     switch (a.variant >> 1) {
-    case 0: gp_fields_at<1>(a, local); break;
-    case 1: gp_fields_at<2>(a, local); break;
-    default: gp_fields_at<3>(a, local); break;
+    case 0: gp_geometry_at<1>(a, local, want_rhs != 0); break;
+    case 1: gp_geometry_at<2>(a, local, want_rhs != 0); break;
+    default: gp_geometry_at<3>(a, local, want_rhs != 0); break;
     }
   }
 }
 
-// Non-constant fields are tabulated at the Gauss points; constant ones are
-// passed by value (kappa of type ZERO means 1).
-static bool tabulated(const ifem_gpu_function& f)
-{
-  return f.type == IFEM_GPU_FN_SINPROD ||
-         ((f.type == IFEM_GPU_FN_FOURIER || f.type == IFEM_GPU_FN_EXP_FOURIER) && f.nmodes > 0);
-}
-
+// Value of a constant field (kappa of type ZERO means 1).
 static double const_value(const ifem_gpu_function& f, double zero_value)
 {
   switch (f.type) {
@@ -314,16 +386,12 @@ struct LineLds {
   alignas(16) T stage[CH][NEN][P1][NS];
   alignas(16) T rstage[HAS_RHS ? CH * GP3 : 1][NRHS];
   alignas(16) T rwin[P1][NC];
-  alignas(16) double cp[P1][W1][W1][3];
   alignas(16) T tN1[NG][P1];
   alignas(16) T tD1[NG][P1];
   alignas(16) T tN2[P1][NG][P1];
   alignas(16) T tD2[P1][NG][P1];
   alignas(16) T tN3[P1][NG][P1];
   alignas(16) T tD3[P1][NG][P1];
-  alignas(16) double sp2[P1];
-  alignas(16) double sp3[P1];
-  double sp1;
   alignas(16) int be2[NB];
   alignas(16) int be3[NB];
   int nbundle;
@@ -341,11 +409,10 @@ __device__ __forceinline__ void line_impl(
     T (*__restrict__ stage)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NEN][P + 1]
                            [LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NS],
     T (*__restrict__ rstage)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NRHS],
-    T (*__restrict__ rwin)[NC], double (*__restrict__ cp)[2 * P + 1][2 * P + 1][3],
+    T (*__restrict__ rwin)[NC],
     T (*__restrict__ tN1)[P + 1], T (*__restrict__ tD1)[P + 1],
     T (*__restrict__ tN2)[NG][P + 1], T (*__restrict__ tD2)[NG][P + 1],
     T (*__restrict__ tN3)[NG][P + 1], T (*__restrict__ tD3)[NG][P + 1],
-    double* __restrict__ sp2, double* __restrict__ sp3, double* __restrict__ sp1p,
     int* __restrict__ be2, int* __restrict__ be3, int* __restrict__ nbundlep)
 {
   using LL = LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>;
@@ -396,11 +463,6 @@ __device__ __forceinline__ void line_impl(
     tN3[m][g][q] = n3;
     tD3[m][g][q] = d3;
   }
-  if (tid < P1) {
-    const int e2 = jc - P + tid, e3 = kc - P + tid;
-    sp2[tid] = (e2 >= 0 && e2 < a.nel2) ? a.span2[e2] : 0.0;
-    sp3[tid] = (e3 >= 0 && e3 < a.nel3) ? a.span3[e3] : 0.0;
-  }
   for (int t = tid; t < P1 * NCR * WN * NC; t += kBlock) (&win[0][0][0][0])[t] = T(0);
   for (int t = tid; t < P1 * NC; t += kBlock) (&rwin[0][0])[t] = T(0);
   __syncthreads();
@@ -409,27 +471,11 @@ __device__ __forceinline__ void line_impl(
   const T lam = (T)a.lam, mu = (T)a.mu;
 
   for (int e1 = 0; e1 < a.nel1; e1++) {
-    // ---- stage control points and direction-1 tables of this layer -------
-    for (int t = tid; t < P1 * W1 * W1; t += kBlock) {
-      const int b1 = t % P1, r = t / P1;
-      const int jj = r % W1, kk = r / W1;
-      const int i = e1 + b1, j = jc - P + jj, k = kc - P + kk;
-      double x = 0, y = 0, z = 0;
-      if (j >= 0 && j < a.n2 && k >= 0 && k < a.n3) {
-        const int64_t node = (int64_t)i + (int64_t)a.n1 * (j + (int64_t)a.n2 * k);
-        x = a.coefs[3 * node];
-        y = a.coefs[3 * node + 1];
-        z = a.coefs[3 * node + 2];
-      }
-      cp[b1][jj][kk][0] = x;
-      cp[b1][jj][kk][1] = y;
-      cp[b1][jj][kk][2] = z;
-    }
+    // ---- direction-1 tables of this layer ----------------------------------
     for (int t = tid; t < NG * P1; t += kBlock) {
       (&tN1[0][0])[t] = (T)a.tabN1[e1 * NG * P1 + t];
       (&tD1[0][0])[t] = (T)a.tabD1[e1 * NG * P1 + t];
     }
-    if (tid == 0) *sp1p = a.span1[e1];
     __syncthreads();
 
     for (int c0 = 0; c0 < nb; c0 += CH) {
@@ -441,57 +487,17 @@ __device__ __forceinline__ void line_impl(
         const int e2 = be2[c0 + le], e3 = be3[c0 + le];
         const int m2 = e2 - jc + P, m3 = e3 - kc + P; // table slots
         const int q1 = q % NG, q2 = (q / NG) % NG, q3 = q / (NG * NG);
-        // material and source at this Gauss point (tabulated or constant)
+        // geometry record of this (element, Gauss point), computed once per
+        // element by k_gp_geometry: J^-1 and w*detJ (vector), or
+        // M = w*detJ*kappa*J^-1 J^-T and w*detJ*f (scalar)
         const int64_t gpi = ((int64_t)e1 + (int64_t)a.nel1 * (e2 + (int64_t)a.nel2 * e3)) * GP3 + q;
-        T coef = T(1);
-        if (NC == 1) coef = a.gp_kappa ? (T)a.gp_kappa[gpi] : (T)a.kappa_c;
-        T fval = T(0);
-        if (HAS_RHS && NC == 1) fval = a.gp_source ? (T)a.gp_source[gpi] : (T)a.source_c;
-        T J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const double* __restrict__ rec = a.gp_geo + gpi * GeoRec<NC>::N;
+        T R[GeoRec<NC>::N];
 #pragma unroll
-        for (int b3 = 0; b3 < P1; b3++)
-#pragma unroll
-          for (int b2 = 0; b2 < P1; b2++)
-#pragma unroll
-            for (int b1 = 0; b1 < P1; b1++) {
-              const T n1 = tN1[q1][b1], d1 = tD1[q1][b1];
-              const T n2 = tN2[m2][q2][b2], d2 = tD2[m2][q2][b2];
-              const T n3 = tN3[m3][q3][b3], d3 = tD3[m3][q3][b3];
-              const T g0 = d1 * n2 * n3, g1 = n1 * d2 * n3, g2 = n1 * n2 * d3;
-              const double* xb = cp[b1][e2 + b2 - jc + P][e3 + b3 - kc + P];
-#pragma unroll
-              for (int i = 0; i < 3; i++) {
-                const T xi = (T)xb[i];
-                J[3 * i + 0] += xi * g0;
-                J[3 * i + 1] += xi * g1;
-                J[3 * i + 2] += xi * g2;
-              }
-            }
-        // matrix::det / matrix::inverse (src/LinAlg/matrix.h:791-862)
-        const T det = J[0] * (J[4] * J[8] - J[7] * J[5]) - J[1] * (J[3] * J[8] - J[6] * J[5]) +
-                      J[2] * (J[3] * J[7] - J[6] * J[4]);
-        T Ji[9];
-        T wdet;
-        if (det <= T(1e-12) && det >= T(-1e-12)) {
-#pragma unroll
-          for (int m = 0; m < 9; m++) Ji[m] = 0;
-          wdet = 0;
-        } else {
-          Ji[0] = (J[4] * J[8] - J[7] * J[5]) / det;
-          Ji[3] = -(J[3] * J[8] - J[6] * J[5]) / det;
-          Ji[6] = (J[3] * J[7] - J[6] * J[4]) / det;
-          Ji[1] = -(J[1] * J[8] - J[7] * J[2]) / det;
-          Ji[4] = (J[0] * J[8] - J[6] * J[2]) / det;
-          Ji[7] = -(J[0] * J[7] - J[6] * J[1]) / det;
-          Ji[2] = (J[1] * J[5] - J[4] * J[2]) / det;
-          Ji[5] = -(J[0] * J[5] - J[3] * J[2]) / det;
-          Ji[8] = (J[0] * J[4] - J[3] * J[1]) / det;
-          const double dV = 0.125 * (*sp1p * sp2[m2] * sp3[m3]);
-          wdet = det * (T)(dV * a.gw[q1] * a.gw[q2] * a.gw[q3]);
-        }
+        for (int m = 0; m < GeoRec<NC>::N; m++) R[m] = (T)rec[m];
         if (NC == 3) {
 #pragma unroll
-          for (int m = 0; m < 9; m++) geo[t][m] = Ji[m];
+          for (int m = 0; m < 9; m++) geo[t][m] = R[m];
         }
         const int a2r = jc - e2, a3r = kc - e3; // row node's local (a2, a3)
         const T n2a = tN2[m2][q2][a2r], d2a = tD2[m2][q2][a2r];
@@ -500,29 +506,25 @@ __device__ __forceinline__ void line_impl(
         for (int a1 = 0; a1 < P1; a1++) {
           const T n1 = tN1[q1][a1], d1 = tD1[q1][a1];
           const T r0 = d1 * n2a * n3a, r1 = n1 * d2a * n3a, r2 = n1 * n2a * d3a;
-          // g_a = J^-T grad_ref N_a  (dNdX = dNdu * J^-1)
-          const T ga0 = r0 * Ji[0] + r1 * Ji[3] + r2 * Ji[6];
-          const T ga1 = r0 * Ji[1] + r1 * Ji[4] + r2 * Ji[7];
-          const T ga2 = r0 * Ji[2] + r1 * Ji[5] + r2 * Ji[8];
-          const T s = wdet * coef;
           if (NC == 1) {
-            // h_a = J^-1 (w kappa g_a): S_ab = grad_ref N_b . h_a
-            const T w0 = s * ga0, w1 = s * ga1, w2 = s * ga2;
-            geo[t][3 * a1 + 0] = Ji[0] * w0 + Ji[1] * w1 + Ji[2] * w2;
-            geo[t][3 * a1 + 1] = Ji[3] * w0 + Ji[4] * w1 + Ji[5] * w2;
-            geo[t][3 * a1 + 2] = Ji[6] * w0 + Ji[7] * w1 + Ji[8] * w2;
+            // h_a = M grad_ref N_a: S_ab = grad_ref N_b . h_a
+            geo[t][3 * a1 + 0] = R[0] * r0 + R[1] * r1 + R[2] * r2;
+            geo[t][3 * a1 + 1] = R[1] * r0 + R[3] * r1 + R[4] * r2;
+            geo[t][3 * a1 + 2] = R[2] * r0 + R[4] * r1 + R[5] * r2;
           } else {
-            geo[t][9 + 3 * a1 + 0] = s * ga0;
-            geo[t][9 + 3 * a1 + 1] = s * ga1;
-            geo[t][9 + 3 * a1 + 2] = s * ga2;
+            // w g_a = w J^-T grad_ref N_a  (dNdX = dNdu * J^-1)
+            const T w = R[9];
+            geo[t][9 + 3 * a1 + 0] = w * (r0 * R[0] + r1 * R[3] + r2 * R[6]);
+            geo[t][9 + 3 * a1 + 1] = w * (r0 * R[1] + r1 * R[4] + r2 * R[7]);
+            geo[t][9 + 3 * a1 + 2] = w * (r0 * R[2] + r1 * R[5] + r2 * R[8]);
           }
           if (HAS_RHS) {
             const T Na = n1 * n2a * n3a;
             if (NC == 1) {
-              rstage[t][a1] = (fval * wdet) * Na;
+              rstage[t][a1] = R[6] * Na;
             } else {
 #pragma unroll
-              for (int c = 0; c < NC; c++) rstage[t][a1 * NC + c] = wdet * (T)a.body[c] * Na;
+              for (int c = 0; c < NC; c++) rstage[t][a1 * NC + c] = R[9] * (T)a.body[c] * Na;
             }
           }
         }
@@ -689,9 +691,8 @@ __device__ __forceinline__ void line_body(ConstArgs& a, const int line, const in
                                           LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>& L)
 {
   line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF>(a, line, split, L.win, L.geo, L.stage, L.rstage,
-                                                L.rwin, L.cp, L.tN1, L.tD1, L.tN2, L.tD2, L.tN3,
-                                                L.tD3, L.sp2, L.sp3, &L.sp1, L.be2, L.be3,
-                                                &L.nbundle);
+                                                L.rwin, L.tN1, L.tD1, L.tN2, L.tD2, L.tN3, L.tD3,
+                                                L.be2, L.be3, &L.nbundle);
 }
 
 // Single patch: the LDS arrays are separate __shared__ variables.
@@ -700,22 +701,20 @@ __global__ __launch_bounds__(kBlock) void k_assemble_lines(const AsmArgs* __rest
 {
   ConstArgs& a = *const_args(args);
   using LL = LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>;
-  constexpr int P1 = P + 1, W1 = 2 * P + 1;
+  constexpr int P1 = P + 1;
   __shared__ T win[P1][LL::NCR][LL::WN][NC];
   __shared__ T geo[LL::CH * LL::GP3][LL::GEO];
   __shared__ T stage[LL::CH][LL::NEN][P1][LL::NS];
   __shared__ T rstage[HAS_RHS ? LL::CH * LL::GP3 : 1][LL::NRHS];
   __shared__ T rwin[P1][NC];
-  __shared__ double cp[P1][W1][W1][3];
   __shared__ T tN1[NG][P1], tD1[NG][P1];
   __shared__ T tN2[P1][NG][P1], tD2[P1][NG][P1];
   __shared__ T tN3[P1][NG][P1], tD3[P1][NG][P1];
-  __shared__ double sp2[P1], sp3[P1], sp1;
   __shared__ int be2[LL::NB], be3[LL::NB];
   __shared__ int nbundle;
   line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF>(a, blockIdx.x, blockIdx.y, win, geo, stage, rstage,
-                                             rwin, cp, tN1, tD1, tN2, tD2, tN3, tD3, sp2, sp3,
-                                             &sp1, be2, be3, &nbundle);
+                                             rwin, tN1, tD1, tN2, tD2, tN3, tD3, be2, be3,
+                                             &nbundle);
 }
 
 // Persistent batch assembly over many independent patches (degree p = 1..3,
@@ -927,15 +926,6 @@ struct Tables {
   }
 };
 
-struct GpFields {
-  double *kappa = nullptr, *source = nullptr;
-  ~GpFields()
-  {
-    hipFree(kappa);
-    hipFree(source);
-  }
-};
-
 template <int P>
 static int make_tables(Context& c, const PatchRec& Pt, int ng, Tables& tb, GaussArgs& ga)
 {
@@ -1079,33 +1069,38 @@ static int assemble_p(Context& c, const ifem_gpu_integrand& itg, int ng, int pre
     else has_rhs = itg.body[0] != 0.0 || itg.body[1] != 0.0 || itg.body[2] != 0.0;
     IFEM_HIP(hipMemsetAsync(c.d_rhs, 0, sizeof(double) * (c.neq > 0 ? c.neq : 1), c.stream));
   }
-  // Fourier-series fields: tabulated at the Gauss points first
+  // geometry records of all (element, Gauss point) pairs, then the lines
   a.ng = ng;
-  a.gp_kappa = a.gp_source = nullptr;
+  a.ncomp = c.ncomp;
   a.kappa_c = const_value(itg.kappa, 1.0);
   a.source_c = const_value(itg.source, 0.0);
   const int64_t ngp = (int64_t)Pt.nel[0] * Pt.nel[1] * Pt.nel[2] * ng * ng * ng;
-  GpFields gf;
-  if (c.ncomp == 1 && tabulated(itg.kappa)) {
-    IFEM_HIP(hipMalloc(&gf.kappa, sizeof(double) * (ngp > 0 ? ngp : 1)));
-    a.gp_kappa = gf.kappa;
+  const size_t need = (size_t)(ngp > 0 ? ngp : 1) * (c.ncomp == 3 ? GeoRec<3>::N : GeoRec<1>::N);
+  if (c.gpgeo_cap < need) {
+    hipFree(c.d_gpgeo);
+    c.d_gpgeo = nullptr;
+    c.gpgeo_cap = 0;
+    IFEM_HIP(hipMalloc(&c.d_gpgeo, sizeof(double) * need));
+    c.gpgeo_cap = need;
   }
-  if (c.ncomp == 1 && has_rhs && tabulated(itg.source)) {
-    IFEM_HIP(hipMalloc(&gf.source, sizeof(double) * (ngp > 0 ? ngp : 1)));
-    a.gp_source = gf.source;
-  }
+  a.gp_geo = c.d_gpgeo;
   if (!c.d_asmargs) IFEM_HIP(hipMalloc(&c.d_asmargs, sizeof(AsmArgs)));
   IFEM_HIP(hipMemcpyAsync(c.d_asmargs, &a, sizeof(AsmArgs), hipMemcpyHostToDevice, c.stream));
   IFEM_HIP(hipEventRecord(c.ev0, c.stream));
-  if ((a.gp_kappa || a.gp_source) && ngp > 0)
-    k_gp_fields<P><<<grid_for(ngp), 256, 0, c.stream>>>(a, ngp);
+  if (ngp > 0)
+    k_gp_geometry<P><<<grid_for(ngp), 256, 0, c.stream>>>(static_cast<const AsmArgs*>(c.d_asmargs),
+                                                         ngp, has_rhs ? 1 : 0);
+  IFEM_HIP(hipEventRecord(c.ev_mid, c.stream));
   rc = dispatch_ng<P>(c, a, ng, c.ncomp, fp32, has_rhs);
   if (rc) return rc;
   IFEM_HIP(hipEventRecord(c.ev1, c.stream));
   IFEM_HIP(hipEventSynchronize(c.ev1));
-  float ms = 0.f;
+  float ms = 0.f, ms_geo = 0.f;
   IFEM_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  IFEM_HIP(hipEventElapsedTime(&ms_geo, c.ev0, c.ev_mid));
   c.last_assemble_ms = ms;
+  c.last_geometry_ms = ms_geo;
+  c.last_lines_ms = ms - ms_geo;
   if (fp32) c.fp32_ready = true;
   else c.fp64_ready = true;
   return IFEM_GPU_OK;
@@ -1341,9 +1336,9 @@ int batch_upload_args(Context& c)
     a.source = batch_function(bp.itg.source, bp.d_smodes);
     a.variant = 2 * (bp.p - 1) + (B.tune[bp.p].half ? 1 : 0);
     a.ng = bp.ng;
+    a.ncomp = 1;
     const int64_t g0 = B.h_gp0.empty() ? 0 : B.h_gp0[q];
-    a.gp_kappa = tabulated(bp.itg.kappa) ? B.d_gpk + g0 : nullptr;
-    a.gp_source = tabulated(bp.itg.source) ? B.d_gps + g0 : nullptr;
+    a.gp_geo = B.d_geo + g0 * GeoRec<1>::N;
     a.kappa_c = const_value(bp.itg.kappa, 1.0);
     a.source_c = const_value(bp.itg.source, 0.0);
     a.val64 = B.d_val;
@@ -1353,7 +1348,6 @@ int batch_upload_args(Context& c)
     a.val64 = nullptr;
     a.val32 = B.d_val32;
     a.rhs = nullptr;
-    a.gp_source = nullptr; // no right-hand side in the FP32 pass
     a32[q] = a;
   }
   if (!B.d_args64) IFEM_HIP(hipMalloc(&B.d_args64, sizeof(AsmArgs) * (np > 0 ? np : 1)));
@@ -1420,21 +1414,17 @@ int batch_symbolic(Context& c)
   IFEM_HIP(hipMalloc(&B.d_rhs, sizeof(double) * (neq > 0 ? neq : 1)));
   IFEM_HIP(hipMemsetAsync(B.d_rhs, 0, sizeof(double) * (neq > 0 ? neq : 1), c.stream));
 
-  // Gauss-point slots of the tabulated fields
+  // geometry records of every patch's (element, Gauss point) pairs
   B.h_gp0.assign(np, 0);
   int64_t ngp = 0;
   for (int q = 0; q < np; q++) {
     const BatchPatch& bp = B.pp[q];
     const PatchRec& Pt = c.patches[bp.patch];
     B.h_gp0[q] = ngp;
-    if (tabulated(bp.itg.kappa) || tabulated(bp.itg.source))
-      ngp += (int64_t)Pt.nel[0] * Pt.nel[1] * Pt.nel[2] * bp.ng * bp.ng * bp.ng;
+    ngp += (int64_t)Pt.nel[0] * Pt.nel[1] * Pt.nel[2] * bp.ng * bp.ng * bp.ng;
   }
   B.ngp = ngp;
-  if (ngp > 0) {
-    IFEM_HIP(hipMalloc(&B.d_gpk, sizeof(double) * ngp));
-    IFEM_HIP(hipMalloc(&B.d_gps, sizeof(double) * ngp));
-  }
+  IFEM_HIP(hipMalloc(&B.d_geo, sizeof(double) * GeoRec<1>::N * (ngp > 0 ? ngp : 1)));
   IFEM_HIP(hipMalloc(&B.d_gp0, sizeof(int64_t) * (np > 0 ? np : 1)));
   IFEM_HIP(hipMemcpy(B.d_gp0, B.h_gp0.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice));
 
@@ -1494,9 +1484,9 @@ int batch_assemble(Context& c, int precision, double* ms_out)
   k_fill<int><<<1, 64, 0, c.stream>>>(1, B.d_queue, 0);
   IFEM_HIP(hipEventRecord(c.ev0, c.stream));
   if (B.ngp > 0)
-    k_gp_fields_batch<<<grid_for(B.ngp), 256, 0, c.stream>>>(
+    k_gp_geometry_batch<<<grid_for(B.ngp), 256, 0, c.stream>>>(
         static_cast<const AsmArgs*>(fp32 ? B.d_args32 : B.d_args64), B.d_gp0, (int)B.pp.size(),
-        B.ngp);
+        B.ngp, fp32 ? 0 : 1);
   if (B.nitems > 0) {
     if (fp32)
       k_assemble_batch<float, false><<<B.grid32, kBlock, 0, c.stream>>>(

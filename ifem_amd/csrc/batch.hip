@@ -281,8 +281,8 @@ void Batch::release()
                 (void**)&d_crow0, (void**)&d_cpatch, (void**)&d_clanes, (void**)&d_pchunk,
                 (void**)&x, (void**)&r, (void**)&z, (void**)&p, (void**)&q, (void**)&dinv,
                 (void**)&r32, (void**)&z32, (void**)&p32, (void**)&q32, (void**)&x32,
-                (void**)&dinv32, (void**)&part, (void**)&pscal, (void**)&pnorm, (void**)&d_gpk,
-                (void**)&d_gps, (void**)&d_gp0};
+                (void**)&dinv32, (void**)&part, (void**)&pscal, (void**)&pnorm, (void**)&d_geo,
+                (void**)&d_gp0};
   for (void** v : d) {
     hipFree(*v);
     *v = nullptr;

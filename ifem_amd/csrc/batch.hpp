@@ -51,9 +51,9 @@ struct Batch {
   int nitems = 0;
   int* d_queue = nullptr;
   int grid64 = 0, grid32 = 0;
-  // Fourier-series fields tabulated at the Gauss points (patch q's slots
-  // start at gp0[q]); ngp slots in all
-  double *d_gpk = nullptr, *d_gps = nullptr;
+  // geometry records per (element, Gauss point) (patch q's records start at
+  // gp0[q]); ngp records in all
+  double* d_geo = nullptr;
   int64_t* d_gp0 = nullptr;
   std::vector<int64_t> h_gp0;
   int64_t ngp = 0;

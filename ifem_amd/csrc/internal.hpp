@@ -82,12 +82,15 @@ struct Context {
   bool fp64_ready = false, fp32_ready = false;
 
   SolverWork w;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_mid = nullptr;
   double last_assemble_ms = 0.0, last_solve_ms = 0.0;
+  double last_geometry_ms = 0.0, last_lines_ms = 0.0; // parts of the last assembly
 
   // owned device buffers for integrand mode tables
   std::vector<void*> scratch_tables;
   void* d_asmargs = nullptr; // assembly kernel arguments (device copy)
+  double* d_gpgeo = nullptr; // geometry records per (element, Gauss point)
+  size_t gpgeo_cap = 0;      // doubles
   // element-chunk variant of the line kernel per (p, ng-p-1, ncomp==3, fp32):
   // -1 built-in default, 0 full, 1 half
   int asm_half[4][2][2][2] = {{{{-1, -1}, {-1, -1}}, {{-1, -1}, {-1, -1}}},

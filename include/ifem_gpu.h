@@ -202,6 +202,10 @@ int ifem_gpu_device_pointers(ifem_gpu_ctx* ctx, void** rowptr, void** colind,
 /* Device time (ms, HIP events on the ctx stream) of the last numeric
  * assembly kernel and of the last solve. */
 int ifem_gpu_last_timings(ifem_gpu_ctx* ctx, double* assemble_ms, double* solve_ms);
+/* The two parts of the last numeric assembly (device ms): the Gauss-point
+ * geometry pass (Jacobian, detJxW, material and source per element and
+ * Gauss point) and the line kernel (contraction, scatter, RHS). */
+int ifem_gpu_last_assembly_parts(ifem_gpu_ctx* ctx, double* geometry_ms, double* lines_ms);
 
 /* ---- learned preconditioner -------------------------------------------------
  * No counterpart in the reference (an addition of this library): a 3D

@@ -21,3 +21,18 @@ def gpu_session_factory():
     def make():
         return Session()
     return make
+
+
+@pytest.fixture(scope="session", autouse=True)
+def torch_runtime_first():
+    """torch ships its own HIP runtime next to the system one that
+    libifem_gpu.so links; when both are loaded in one process, torch's must
+    open the device first, or torch reports no GPU afterwards.  GPU tests that
+    hand torch tensors to the library therefore start torch's runtime at the
+    beginning of the session (counting devices does not initialise it)."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.device_count() > 0:
+        torch.cuda.init()

@@ -28,7 +28,7 @@ EXPORTS = [
     "ifem_gpu_batch_autotune", "ifem_gpu_batch_solve", "ifem_gpu_batch_results",
     "ifem_gpu_batch_patch_info", "ifem_gpu_batch_get_system", "ifem_gpu_batch_get_solution",
     "ifem_gpu_batch_timings", "ifem_gpu_autotune_assembly", "ifem_gpu_unet_backward",
-    "ifem_gpu_last_assembly_parts",
+    "ifem_gpu_last_assembly_parts", "ifem_gpu_integrate_norms",
 ]
 
 FP64, FP32 = 64, 32
@@ -108,6 +108,8 @@ def lib():
     L.ifem_gpu_device_pointers.argtypes = [vp] + [C.POINTER(vp)] * 5
     L.ifem_gpu_last_timings.argtypes = [vp, C.POINTER(dp), C.POINTER(dp)]
     L.ifem_gpu_last_assembly_parts.argtypes = [vp, C.POINTER(dp), C.POINTER(dp)]
+    L.ifem_gpu_integrate_norms.argtypes = [vp, C.POINTER(Integrand), C.POINTER(ip),
+                                           C.POINTER(Function), C.POINTER(dp)]
     L.ifem_gpu_unet_create.argtypes = [vp, C.POINTER(UNetConfig), C.POINTER(C.c_int64)]
     L.ifem_gpu_unet_get_params.argtypes = [vp, vp]
     L.ifem_gpu_unet_set_params.argtypes = [vp, vp]
@@ -251,6 +253,18 @@ class Session:
         h = C.c_int(-1)
         check(lib().ifem_gpu_autotune_assembly(self.h, C.byref(s), ngv, precision, C.byref(h)))
         return h.value
+
+    def norms(self, itg, ng, exact=None):
+        """[||u_h||_L2, ||u_h||_a, ||u||_L2, ||u||_a, ||u-u_h||_L2, ||u-u_h||_a] of the
+        current solution (exact: optional analytic scalar solution)."""
+        keep = []
+        s = self._integrand(itg, keep)
+        ngv = (C.c_int * 3)(*ng)
+        out = (C.c_double * 6)()
+        ex = self._function(exact, keep) if exact is not None else None
+        check(lib().ifem_gpu_integrate_norms(self.h, C.byref(s), ngv,
+                                             C.byref(ex) if ex is not None else None, out))
+        return np.array(list(out))
 
     def add_traction(self, face: int, t, ng: int):
         """face: 0..5 (u0,u1,v0,v1,w0,w1) -> IFEM face index face+1."""

@@ -262,6 +262,19 @@ int ifem_gpu_autotune_assembly(ifem_gpu_ctx* ctx, const ifem_gpu_integrand* itg,
   return autotune_assembly(c, *itg, ng, precision, half);
 }
 
+int ifem_gpu_integrate_norms(ifem_gpu_ctx* ctx, const ifem_gpu_integrand* itg, const int ng[3],
+                             const ifem_gpu_function* exact, double norms[6])
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (!itg || !ng || !norms) return fail(IFEM_GPU_EINVAL, "integrate_norms: null argument");
+  if (!c.pattern_ready) return fail(IFEM_GPU_EINVAL, "integrate_norms: no system");
+  const int want = itg->kind == IFEM_GPU_ELASTICITY ? 3 : (itg->kind == IFEM_GPU_POISSON ? 1 : 0);
+  if (want != c.ncomp)
+    return fail(IFEM_GPU_EINVAL, "integrate_norms: integrand component count differs from the SAM's");
+  return integrate_norms(c, *itg, ng, exact, norms);
+}
+
 int ifem_gpu_assemble_neumann(ifem_gpu_ctx* ctx, int face, const double* traction, int ng)
 {
   CHECK_CTX(ctx);

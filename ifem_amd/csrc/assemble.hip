@@ -1506,4 +1506,250 @@ int batch_assemble(Context& c, int precision, double* ms_out)
   return IFEM_GPU_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Norms of the current solution, NormBase / ElmNorm / GlbNorm with the SQRT
+// final operation (src/ASM/IntegrandBase.h:378-530, src/ASM/GlbNorm.C:33-79):
+// one thread per (element, Gauss point) evaluates u_h = sum_b N_b u_b and
+// grad u_h = J^-T sum_b u_b grad_ref N_b with the volume loop's quadrature,
+// Jacobian and singular-point rule, and accumulates
+//   [0] |u_h|^2, [1] energy density of u_h (kappa |grad u_h|^2, or
+//   eps:C:eps), and with an exact scalar solution u: [2] u^2, [3] kappa
+//   |grad u|^2, [4] (u - u_h)^2, [5] kappa |grad(u - u_h)|^2
+// times detJxW, reduced per block in fixed order.
+__device__ inline void eval_field_grad(const DevFunction& f, double x, double y, double z,
+                                       double* g)
+{
+  g[0] = g[1] = g[2] = 0.0;
+  switch (f.type) {
+  case IFEM_GPU_FN_SINPROD: {
+    const double s0 = sin(M_PI * x), s1 = sin(M_PI * y), s2 = sin(M_PI * z);
+    const double c0 = cos(M_PI * x), c1 = cos(M_PI * y), c2 = cos(M_PI * z);
+    g[0] = f.scale * M_PI * c0 * s1 * s2;
+    g[1] = f.scale * M_PI * s0 * c1 * s2;
+    g[2] = f.scale * M_PI * s0 * s1 * c2;
+    return;
+  }
+  case IFEM_GPU_FN_FOURIER:
+  case IFEM_GPU_FN_EXP_FOURIER: {
+    double s = 0.0;
+    for (int m = 0; m < f.nmodes; m++) {
+      const double* md = f.modes + 5 * m;
+      const double arg = md[0] * x + md[1] * y + md[2] * z + md[3];
+      s += md[4] * cos(arg);
+      for (int d = 0; d < 3; d++) g[d] -= f.scale * md[4] * md[d] * sin(arg);
+    }
+    if (f.type == IFEM_GPU_FN_EXP_FOURIER) {
+      const double e = exp(f.mean + f.scale * s);
+      for (int d = 0; d < 3; d++) g[d] *= e;
+    }
+    return;
+  }
+  default:
+    return;
+  }
+}
+
+struct NormArgs {
+  AsmArgs a;
+  const double* x;  // solution, equation order
+  DevFunction exact;
+  int has_exact;
+  double* part;     // 6 x gridDim.x partial sums
+};
+
+template <int P, int NC>
+__global__ __launch_bounds__(256) void k_norms(NormArgs na, int64_t total)
+{
+  constexpr int P1 = P + 1;
+  const AsmArgs& a = na.a;
+  const int ng = a.ng, gp3 = ng * ng * ng;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < total;
+       id += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = id / gp3;
+    const int q = (int)(id - e * gp3);
+    const int q1 = q % ng, q2 = (q / ng) % ng, q3 = q / (ng * ng);
+    const int e1 = (int)(e % a.nel1), e2 = (int)((e / a.nel1) % a.nel2),
+              e3 = (int)(e / ((int64_t)a.nel1 * a.nel2));
+    double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double X[3] = {0, 0, 0}, uh[NC], gr[NC][3];
+    for (int c = 0; c < NC; c++) {
+      uh[c] = 0.0;
+      gr[c][0] = gr[c][1] = gr[c][2] = 0.0;
+    }
+    for (int b3 = 0; b3 < P1; b3++)
+      for (int b2 = 0; b2 < P1; b2++)
+#pragma unroll
+        for (int b1 = 0; b1 < P1; b1++) {
+          const double n1 = a.tabN1[(e1 * ng + q1) * P1 + b1], d1 = a.tabD1[(e1 * ng + q1) * P1 + b1];
+          const double n2 = a.tabN2[(e2 * ng + q2) * P1 + b2], d2 = a.tabD2[(e2 * ng + q2) * P1 + b2];
+          const double n3 = a.tabN3[(e3 * ng + q3) * P1 + b3], d3 = a.tabD3[(e3 * ng + q3) * P1 + b3];
+          const double g0 = d1 * n2 * n3, g1 = n1 * d2 * n3, g2 = n1 * n2 * d3, nv = n1 * n2 * n3;
+          const int64_t node =
+              (int64_t)(e1 + b1) + (int64_t)a.n1 * ((e2 + b2) + (int64_t)a.n2 * (e3 + b3));
+#pragma unroll
+          for (int i = 0; i < 3; i++) {
+            const double xi = a.coefs[3 * node + i];
+            J[3 * i + 0] += xi * g0;
+            J[3 * i + 1] += xi * g1;
+            J[3 * i + 2] += xi * g2;
+            X[i] += xi * nv;
+          }
+#pragma unroll
+          for (int c = 0; c < NC; c++) {
+            const int eqn = a.eq[node * NC + c];
+            const double ub = eqn >= 0 ? na.x[eqn] : 0.0;
+            uh[c] += nv * ub;
+            gr[c][0] += g0 * ub;
+            gr[c][1] += g1 * ub;
+            gr[c][2] += g2 * ub;
+          }
+        }
+    const double det = J[0] * (J[4] * J[8] - J[7] * J[5]) - J[1] * (J[3] * J[8] - J[6] * J[5]) +
+                       J[2] * (J[3] * J[7] - J[6] * J[4]);
+    if (det <= 1e-12 && det >= -1e-12) continue;
+    double Ji[9];
+    Ji[0] = (J[4] * J[8] - J[7] * J[5]) / det;
+    Ji[3] = -(J[3] * J[8] - J[6] * J[5]) / det;
+    Ji[6] = (J[3] * J[7] - J[6] * J[4]) / det;
+    Ji[1] = -(J[1] * J[8] - J[7] * J[2]) / det;
+    Ji[4] = (J[0] * J[8] - J[6] * J[2]) / det;
+    Ji[7] = -(J[0] * J[7] - J[6] * J[1]) / det;
+    Ji[2] = (J[1] * J[5] - J[4] * J[2]) / det;
+    Ji[5] = -(J[0] * J[5] - J[3] * J[2]) / det;
+    Ji[8] = (J[0] * J[4] - J[3] * J[1]) / det;
+    const double dV = 0.125 * (a.span1[e1] * a.span2[e2] * a.span3[e3]);
+    const double w = det * (dV * a.gw[q1] * a.gw[q2] * a.gw[q3]);
+    // grad u_h: G_cj = sum_i ghat_ci Ji[i][j]
+    double G[NC][3];
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+        G[c][j] = gr[c][0] * Ji[j] + gr[c][1] * Ji[3 + j] + gr[c][2] * Ji[6 + j];
+    if (NC == 1) {
+      const double kap = field_varies(a.kappa) ? eval_field(a.kappa, X[0], X[1], X[2]) : a.kappa_c;
+      acc[0] += w * uh[0] * uh[0];
+      acc[1] += w * kap * (G[0][0] * G[0][0] + G[0][1] * G[0][1] + G[0][2] * G[0][2]);
+      if (na.has_exact) {
+        double ge[3];
+        const double ue = eval_field(na.exact, X[0], X[1], X[2]);
+        eval_field_grad(na.exact, X[0], X[1], X[2], ge);
+        const double d0 = ge[0] - G[0][0], d1 = ge[1] - G[0][1], d2 = ge[2] - G[0][2];
+        acc[2] += w * ue * ue;
+        acc[3] += w * kap * (ge[0] * ge[0] + ge[1] * ge[1] + ge[2] * ge[2]);
+        acc[4] += w * (ue - uh[0]) * (ue - uh[0]);
+        acc[5] += w * kap * (d0 * d0 + d1 * d1 + d2 * d2);
+      }
+    } else {
+      double tr = 0.0, ee = 0.0;
+#pragma unroll
+      for (int r = 0; r < 3; r++) tr += G[r][r];
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) {
+          const double eps = 0.5 * (G[r][cc] + G[cc][r]);
+          ee += eps * eps;
+        }
+      double u2 = 0.0;
+#pragma unroll
+      for (int c = 0; c < NC; c++) u2 += uh[c] * uh[c];
+      acc[0] += w * u2;
+      acc[1] += w * (a.lam * tr * tr + 2.0 * a.mu * ee);
+    }
+  }
+  __shared__ double sh[4];
+  for (int m = 0; m < 6; m++) {
+    double v = acc[m];
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) na.part[m * gridDim.x + blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+  }
+}
+
+int integrate_norms(Context& c, const ifem_gpu_integrand& itg, const int ng[3],
+                    const ifem_gpu_function* exact, double* out)
+{
+  const PatchRec& Pt = c.patches[c.patch];
+  const int p = Pt.p[0];
+  if (ng[0] != ng[1] || ng[0] != ng[2] || ng[0] < 1 || ng[0] > kMaxNG)
+    return fail(IFEM_GPU_EUNSUPPORTED, "norms: equal Gauss counts 1..5 per direction needed");
+  if (exact && c.ncomp != 1)
+    return fail(IFEM_GPU_EUNSUPPORTED, "norms: exact solutions are scalar (Poisson) only");
+  const int g = ng[0];
+  GaussArgs ga{};
+  gauss_rule(g, ga.xi, ga.w);
+  Tables tb;
+  int rc = 0;
+  switch (p) {
+  case 1: rc = make_tables<1>(c, Pt, g, tb, ga); break;
+  case 2: rc = make_tables<2>(c, Pt, g, tb, ga); break;
+  case 3: rc = make_tables<3>(c, Pt, g, tb, ga); break;
+  default: return fail(IFEM_GPU_EUNSUPPORTED, "norms: degree out of range");
+  }
+  if (rc) return rc;
+  NormArgs na{};
+  AsmArgs& a = na.a;
+  a.n1 = Pt.n[0]; a.n2 = Pt.n[1]; a.n3 = Pt.n[2];
+  a.nel1 = Pt.nel[0]; a.nel2 = Pt.nel[1]; a.nel3 = Pt.nel[2];
+  a.coefs = Pt.d_coefs;
+  a.tabN1 = tb.tabN[0]; a.tabD1 = tb.tabD[0]; a.span1 = tb.span[0];
+  a.tabN2 = tb.tabN[1]; a.tabD2 = tb.tabD[1]; a.span2 = tb.span[1];
+  a.tabN3 = tb.tabN[2]; a.tabD3 = tb.tabD[2]; a.span3 = tb.span[2];
+  for (int k = 0; k < kMaxNG; k++) a.gw[k] = ga.w[k];
+  a.eq = c.d_eq;
+  a.ng = g;
+  a.ncomp = c.ncomp;
+  for (void* q : c.scratch_tables) hipFree(q);
+  c.scratch_tables.clear();
+  rc = upload_function(c, itg.kappa, a.kappa);
+  if (rc) return rc;
+  a.kappa_c = const_value(itg.kappa, 1.0);
+  if (itg.kind == IFEM_GPU_ELASTICITY) {
+    a.lam = itg.E * itg.nu / ((1.0 + itg.nu) * (1.0 - 2.0 * itg.nu));
+    a.mu = itg.E / (2.0 * (1.0 + itg.nu));
+  }
+  if (exact) {
+    rc = upload_function(c, *exact, na.exact);
+    if (rc) return rc;
+    na.has_exact = 1;
+  }
+  na.x = c.w.x;
+  const int64_t total = (int64_t)Pt.nel[0] * Pt.nel[1] * Pt.nel[2] * g * g * g;
+  const int grid = 1024;
+  double* part = nullptr;
+  IFEM_HIP(hipMalloc(&part, sizeof(double) * 6 * grid));
+  na.part = part;
+  if (c.ncomp == 3) {
+    switch (p) {
+    case 1: k_norms<1, 3><<<grid, 256, 0, c.stream>>>(na, total); break;
+    case 2: k_norms<2, 3><<<grid, 256, 0, c.stream>>>(na, total); break;
+    default: k_norms<3, 3><<<grid, 256, 0, c.stream>>>(na, total); break;
+    }
+  } else {
+    switch (p) {
+    case 1: k_norms<1, 1><<<grid, 256, 0, c.stream>>>(na, total); break;
+    case 2: k_norms<2, 1><<<grid, 256, 0, c.stream>>>(na, total); break;
+    default: k_norms<3, 1><<<grid, 256, 0, c.stream>>>(na, total); break;
+    }
+  }
+  const hipError_t le = hipGetLastError();
+  std::vector<double> h(6 * grid);
+  const hipError_t ce = hipMemcpyAsync(h.data(), part, sizeof(double) * 6 * grid,
+                                       hipMemcpyDeviceToHost, c.stream);
+  const hipError_t se = hipStreamSynchronize(c.stream);
+  hipFree(part);
+  if (le != hipSuccess || ce != hipSuccess || se != hipSuccess)
+    return fail(IFEM_GPU_EDEVICE, "norms: device error");
+  for (int m = 0; m < 6; m++) {
+    double sum = 0.0;
+    for (int b = 0; b < grid; b++) sum += h[m * grid + b];
+    out[m] = sum < 0.0 ? -std::sqrt(-sum) : std::sqrt(sum);
+  }
+  return IFEM_GPU_OK;
+}
+
 } // namespace ifem

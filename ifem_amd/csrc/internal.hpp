@@ -124,6 +124,8 @@ int assemble_volume(Context& c, const ifem_gpu_integrand& itg, const int ng[3],
 int assemble_traction(Context& c, int face0, const double* t, int ng);
 int autotune_assembly(Context& c, const ifem_gpu_integrand& itg, const int ng[3], int precision,
                       int* half);
+int integrate_norms(Context& c, const ifem_gpu_integrand& itg, const int ng[3],
+                    const ifem_gpu_function* exact, double* out);
 int batch_symbolic(Context& c);
 int batch_upload_args(Context& c);
 int batch_assemble(Context& c, int precision, double* ms);

@@ -139,6 +139,18 @@ int ifem_gpu_autotune_assembly(ifem_gpu_ctx* ctx, const ifem_gpu_integrand* itg,
  * ASMs3D::integrate(Integrand&, int lIndex, ...) (src/ASM/ASMs3D.C:2547-2812). */
 int ifem_gpu_assemble_neumann(ifem_gpu_ctx* ctx, int face, const double* traction, int ng);
 
+/* ---- norms ------------------------------------------------------------------
+ * Norms of the current solution u_h (after ifem_gpu_solve; eliminated DOFs
+ * are 0), integrated with ng Gauss points per direction as NormBase /
+ * ElmNorm / GlbNorm do with the SQRT final operation (src/ASM/IntegrandBase.h:
+ * 378-530, src/ASM/GlbNorm.C:33-79): norms[0] = ||u_h||_L2, norms[1] =
+ * ||u_h||_a (Poisson: sqrt(int kappa |grad u_h|^2); elasticity: sqrt(int
+ * eps:C:eps)); with an exact scalar solution (Poisson only, NULL for none):
+ * norms[2] = ||u||_L2, norms[3] = ||u||_a, norms[4] = ||u - u_h||_L2,
+ * norms[5] = ||u - u_h||_a (else 0). */
+int ifem_gpu_integrate_norms(ifem_gpu_ctx* ctx, const ifem_gpu_integrand* itg, const int ng[3],
+                             const ifem_gpu_function* exact, double norms[6]);
+
 /* ---- linear solve ----------------------------------------------------------
  * LinSolParams keys (src/LinAlg/LinSolParams.C:68-77,158-195) as fields:
  * type "cg" (CG only), pc "jacobi" (Jacobi only), rtol, atol, dtol, maxits.

@@ -68,6 +68,10 @@ def lib():
         L.orc_pcg_jacobi_fixed.restype = None
         L.orc_eval_function.argtypes = [C.POINTER(_Function), _dp]
         L.orc_eval_function.restype = C.c_double
+        L.orc_eval_function_grad.argtypes = [C.POINTER(_Function), _dp, _dp]
+        L.orc_eval_function_grad.restype = None
+        L.orc_norms.argtypes = [C.POINTER(_Patch), C.POINTER(_Integrand), _ip, _dp,
+                                C.c_void_p, _dp]
         _LIB = L
     return _LIB
 
@@ -237,6 +241,28 @@ def eval_function(f, X):
     keep = _Keep()
     s = _function(f, keep)
     return lib().orc_eval_function(C.byref(s), np.asarray(X, dtype=np.float64))
+
+
+def eval_function_grad(f, X):
+    keep = _Keep()
+    s = _function(f, keep)
+    g = np.zeros(3)
+    lib().orc_eval_function_grad(C.byref(s), np.asarray(X, dtype=np.float64), g)
+    return g
+
+
+def norms(P, itg, ng, xdof, exact=None):
+    """[||u_h||_L2, ||u_h||_a, ||u||_L2, ||u||_a, ||u-u_h||_L2, ||u-u_h||_a]."""
+    keep = _Keep()
+    sp, si = _patch(P, keep), _integrand(itg, keep)
+    se = _function(exact, keep) if exact is not None else None
+    out = np.zeros(6)
+    rc = lib().orc_norms(C.byref(sp), C.byref(si), np.asarray(ng, dtype=np.int32),
+                         np.ascontiguousarray(xdof, dtype=np.float64),
+                         C.cast(C.pointer(se), C.c_void_p) if se is not None else None, out)
+    if rc:
+        raise RuntimeError("orc_norms failed")
+    return out
 
 
 def system(P, itg, ng, meqn, neq, traction=None, nthreads: int = 0):

@@ -818,3 +818,147 @@ void orc_expand_solution(int ndof, const int* meqn, const double* xeq, double* x
 {
   for (int i = 0; i < ndof; i++) xdof[i] = meqn[i] > 0 ? xeq[meqn[i] - 1] : 0.0;
 }
+
+/* Gradient of a field (analytic derivative of orc_eval_function). */
+void orc_eval_function_grad(const orc_function* f, const double X[3], double g[3])
+{
+  const double pi = 3.14159265358979323846;
+  g[0] = g[1] = g[2] = 0.0;
+  switch (f->type) {
+  case ORC_FN_SINPROD: {
+    const double s0 = sin(pi * X[0]), s1 = sin(pi * X[1]), s2 = sin(pi * X[2]);
+    const double c0 = cos(pi * X[0]), c1 = cos(pi * X[1]), c2 = cos(pi * X[2]);
+    g[0] = f->scale * pi * c0 * s1 * s2;
+    g[1] = f->scale * pi * s0 * c1 * s2;
+    g[2] = f->scale * pi * s0 * s1 * c2;
+    return;
+  }
+  case ORC_FN_FOURIER:
+  case ORC_FN_EXP_FOURIER: {
+    double s = 0.0;
+    for (int m = 0; m < f->nmodes; m++) {
+      const double* md = f->modes + 5 * m;
+      const double arg = md[0] * X[0] + md[1] * X[1] + md[2] * X[2] + md[3];
+      s += md[4] * cos(arg);
+      for (int d = 0; d < 3; d++) g[d] -= f->scale * md[4] * md[d] * sin(arg);
+    }
+    if (f->type == ORC_FN_EXP_FOURIER) {
+      const double e = exp(f->mean + f->scale * s);
+      for (int d = 0; d < 3; d++) g[d] *= e;
+    }
+    return;
+  }
+  default:
+    return;
+  }
+}
+
+/* ---- norms: NormBase/ElmNorm/GlbNorm (src/ASM/IntegrandBase.h:378-530,
+ * src/ASM/ElmNorm.h:28, src/ASM/GlbNorm.C:33-79 with the SQRT final
+ * operation).  Element sums over the Gauss points of the volume loop
+ * (same quadrature, Jacobian and singular-point rule as the assembly),
+ * summed over the elements. ---------------------------------------------- */
+int orc_norms(const orc_patch* P, const orc_integrand* itg, const int ng[3],
+              const double* xdof, const orc_function* exact, double* out)
+{
+  const int p1 = P->p[0], p2 = P->p[1], p3 = P->p[2];
+  const int nen = (p1 + 1) * (p2 + 1) * (p3 + 1);
+  const int ncomp = itg->kind == ORC_ELASTICITY ? 3 : 1;
+  int nel[3];
+  orc_num_elements(P, nel);
+  double lam = 0.0, mu = 0.0;
+  if (ncomp == 3) {
+    lam = itg->E * itg->nu / ((1.0 + itg->nu) * (1.0 - 2.0 * itg->nu));
+    mu = itg->E / (2.0 * (1.0 + itg->nu));
+  }
+  double xg[3][ORC_MAXNG], wg[3][ORC_MAXNG];
+  for (int d = 0; d < 3; d++)
+    if (orc_gauss(ng[d], xg[d], wg[d])) return -1;
+  double sum[6] = {0, 0, 0, 0, 0, 0};
+  int mnpc[ORC_MAXNEN];
+  double Xnod[3 * ORC_MAXNEN], N[ORC_MAXNEN], dNdu[3 * ORC_MAXNEN], dNdX[3 * ORC_MAXNEN], Jinv[9];
+  for (int e3 = 0; e3 < nel[2]; e3++)
+    for (int e2 = 0; e2 < nel[1]; e2++)
+      for (int e1 = 0; e1 < nel[0]; e1++) {
+        const int e[3] = {e1, e2, e3};
+        orc_element_nodes(P, e1, e2, e3, mnpc);
+        for (int a = 0; a < nen; a++)
+          for (int i = 0; i < 3; i++) Xnod[i * nen + a] = P->coefs[3 * mnpc[a] + i];
+        double span[3];
+        for (int d = 0; d < 3; d++) {
+          const int s = e[d] + P->p[d];
+          span[d] = P->knots[d][s + 1] - P->knots[d][s];
+        }
+        const double dV = 0.125 * (span[0] * span[1] * span[2]);
+        double N1[ORC_MAXNG][ORC_MAXP + 1], dN1[ORC_MAXNG][ORC_MAXP + 1];
+        double N2[ORC_MAXNG][ORC_MAXP + 1], dN2[ORC_MAXNG][ORC_MAXP + 1];
+        double N3[ORC_MAXNG][ORC_MAXP + 1], dN3[ORC_MAXNG][ORC_MAXP + 1];
+        for (int d = 0; d < 3; d++) {
+          const int s = e[d] + P->p[d];
+          const double* t = P->knots[d];
+          for (int g = 0; g < ng[d]; g++) {
+            const double u = orc_gauss_param(t[s], t[s + 1], xg[d][g]);
+            double* Nd = d == 0 ? N1[g] : (d == 1 ? N2[g] : N3[g]);
+            double* dNd = d == 0 ? dN1[g] : (d == 1 ? dN2[g] : dN3[g]);
+            orc_basis_funs(P->p[d], t, s, u, Nd, dNd);
+          }
+        }
+        double esum[6] = {0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < ng[2]; k++)
+          for (int j = 0; j < ng[1]; j++)
+            for (int i = 0; i < ng[0]; i++) {
+              int a = 0;
+              for (int a3 = 0; a3 <= p3; a3++)
+                for (int a2 = 0; a2 <= p2; a2++)
+                  for (int a1 = 0; a1 <= p1; a1++, a++) {
+                    N[a] = N1[i][a1] * N2[j][a2] * N3[k][a3];
+                    dNdu[3 * a + 0] = dN1[i][a1] * N2[j][a2] * N3[k][a3];
+                    dNdu[3 * a + 1] = N1[i][a1] * dN2[j][a2] * N3[k][a3];
+                    dNdu[3 * a + 2] = N1[i][a1] * N2[j][a2] * dN3[k][a3];
+                  }
+              double w = orc_jacobian(nen, Xnod, dNdu, Jinv, dNdX);
+              if (w == 0.0) continue;
+              w *= dV * wg[0][i] * wg[1][j] * wg[2][k];
+              double X[3], uh[3] = {0, 0, 0}, G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+              for (int d = 0; d < 3; d++) {
+                double s = 0.0;
+                for (int b = 0; b < nen; b++) s += Xnod[d * nen + b] * N[b];
+                X[d] = s;
+              }
+              for (int b = 0; b < nen; b++)
+                for (int c = 0; c < ncomp; c++) {
+                  const double xb = xdof[ncomp * mnpc[b] + c];
+                  uh[c] += N[b] * xb;
+                  for (int d = 0; d < 3; d++) G[c][d] += dNdX[3 * b + d] * xb;
+                }
+              if (ncomp == 1) {
+                const double kap =
+                    itg->kappa.type == ORC_FN_ZERO ? 1.0 : orc_eval_function(&itg->kappa, X);
+                esum[0] += w * uh[0] * uh[0];
+                esum[1] += w * kap * (G[0][0] * G[0][0] + G[0][1] * G[0][1] + G[0][2] * G[0][2]);
+                if (exact) {
+                  double ge[3];
+                  const double ue = orc_eval_function(exact, X);
+                  orc_eval_function_grad(exact, X, ge);
+                  const double d0 = ge[0] - G[0][0], d1 = ge[1] - G[0][1], d2 = ge[2] - G[0][2];
+                  esum[2] += w * ue * ue;
+                  esum[3] += w * kap * (ge[0] * ge[0] + ge[1] * ge[1] + ge[2] * ge[2]);
+                  esum[4] += w * (ue - uh[0]) * (ue - uh[0]);
+                  esum[5] += w * kap * (d0 * d0 + d1 * d1 + d2 * d2);
+                }
+              } else {
+                double eps[3][3], tr = 0.0, ee = 0.0;
+                for (int r = 0; r < 3; r++)
+                  for (int c = 0; c < 3; c++) eps[r][c] = 0.5 * (G[r][c] + G[c][r]);
+                for (int r = 0; r < 3; r++) tr += eps[r][r];
+                for (int r = 0; r < 3; r++)
+                  for (int c = 0; c < 3; c++) ee += eps[r][c] * eps[r][c];
+                esum[0] += w * (uh[0] * uh[0] + uh[1] * uh[1] + uh[2] * uh[2]);
+                esum[1] += w * (lam * tr * tr + 2.0 * mu * ee);
+              }
+            }
+        for (int m = 0; m < 6; m++) sum[m] += esum[m];
+      }
+  for (int m = 0; m < 6; m++) out[m] = sum[m] < 0.0 ? -sqrt(-sum[m]) : sqrt(sum[m]);
+  return 0;
+}

@@ -122,8 +122,18 @@ void orc_pcg_jacobi_fixed(int n, const int64_t* rowptr, const int32_t* colind,
 /* a20: equation vector -> DOF vector (fixed DOFs = 0). */
 void orc_expand_solution(int ndof, const int* meqn, const double* xeq, double* xdof);
 
-/* Field evaluation shared by all integrands. */
+/* Field evaluation shared by all integrands, and its gradient. */
 double orc_eval_function(const orc_function* f, const double X[3]);
+void orc_eval_function_grad(const orc_function* f, const double X[3], double g[3]);
+
+/* Norms of the field u_h = sum_a N_a x_a (xdof: DOF vector, eliminated DOFs
+ * included), NormBase/GlbNorm with the SQRT final operation: out[0] =
+ * ||u_h||_L2, out[1] = ||u_h||_a (Poisson: int kappa |grad u_h|^2,
+ * elasticity: int eps:C:eps); with an exact scalar solution (Poisson):
+ * out[2] = ||u||_L2, out[3] = ||u||_a, out[4] = ||u-u_h||_L2,
+ * out[5] = ||u-u_h||_a (else 0). */
+int orc_norms(const orc_patch* P, const orc_integrand* itg, const int ng[3],
+              const double* xdof, const orc_function* exact, double* out);
 
 #ifdef __cplusplus
 }

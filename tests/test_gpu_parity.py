@@ -202,3 +202,27 @@ def test_elasticity64_full_size_properties():
         del Ao
         st = s.solve(rtol=1e-8, maxits=20000)
         assert st["converged"] and st["rel_residual"] <= 1e-8
+
+
+def test_norms_match_oracle():
+    """GPU norm integration of the GPU solution against the oracle's on the same
+    coefficients: Poisson with the manufactured exact solution (config 1 at
+    16^3) and jittered elasticity."""
+    from ifem_amd import model as M
+    exact = M.Function(M.FN_SINPROD, 1.0)
+    for pb, ex in ((problems.poisson_cube(16, 2), exact),
+                   (problems.diffusion_cube(6, 3, sigma=1.0, ell=0.2, seed=4), None),
+                   (problems.elasticity_cube(6, 2, seed=9), None)):
+        _say(f"[gpu] norms {pb.name}")
+        with setup_problem(pb) as s:
+            st = s.solve(rtol=1e-12, maxits=20000)
+            assert st["converged"]
+            n = s.norms(pb.integrand, pb.ng, ex)
+            x = s.solution()
+        xd = np.zeros(pb.ndof)
+        m = pb.meqn > 0
+        xd[m] = x[pb.meqn[m] - 1]
+        no = oracle.norms(pb.patch, pb.integrand, pb.ng, xd, ex)
+        assert np.all(np.abs(n - no) <= 1e-12 * np.maximum(np.abs(no), 1e-300)), (n, no)
+        if ex is not None:
+            assert abs(n[4] - 2.26239829e-05) <= 1e-6 * n[4]  # O(h^3) error of u_h at 16^3

@@ -484,3 +484,66 @@ def test_lattice_padding_round_trip():
     # equation of node (i, j, k) = (1 + ii, jj, kk), component cc
     assert float(t[2, 3, 4, 5]) == float(((3 * 8 + 4) * 7 + 5) * 3 + 2)
     assert torch.equal(learned.from_lattice(t, pb), v)
+
+
+# ---- norms (NormBase / GlbNorm restatement) -----------------------------------
+def _expand(pb, x):
+    xd = np.zeros(pb.ndof)
+    m = pb.meqn > 0
+    xd[m] = x[pb.meqn[m] - 1]
+    return xd
+
+
+def test_norm_gradient_matches_finite_differences():
+    from ifem_amd import model as M
+    rng = np.random.default_rng(4)
+    fs = [M.Function(M.FN_SINPROD, 1.7), M.lognormal_conductivity(3, 1.0, 0.3, 8),
+          M.smooth_source(5)]
+    for f in fs:
+        for _ in range(3):
+            X = rng.random(3)
+            g = oracle.eval_function_grad(f, X)
+            h = 1e-6
+            fd = [(oracle.eval_function(f, X + h * e) - oracle.eval_function(f, X - h * e)) / (2 * h)
+                  for e in np.eye(3)]
+            assert np.allclose(g, fd, rtol=1e-6, atol=1e-7 * max(1.0, np.abs(fd).max()))
+
+
+def test_energy_norm_equals_quadratic_form_of_K():
+    """||u_h||_a^2 = x^T K x for any coefficient vector x: the norm integral
+    and the assembled matrix use the same quadrature (Poisson with log-normal
+    kappa, elasticity on a jittered patch)."""
+    rng = np.random.default_rng(8)
+    for pb in (problems.diffusion_cube(5, 2, sigma=1.0, ell=0.3, seed=2),
+               problems.elasticity_cube(4, 2, seed=6)):
+        rp, ci, A, b = oracle.system(pb.patch, pb.integrand, pb.ng, pb.meqn, pb.neq)
+        x = rng.standard_normal(pb.neq)
+        Kx = oracle.spmv(rp, ci, A, x)
+        nrm = oracle.norms(pb.patch, pb.integrand, pb.ng, _expand(pb, x))
+        assert abs(nrm[1] ** 2 - x @ Kx) <= 1e-12 * abs(x @ Kx)
+
+
+def test_manufactured_poisson_norms_closed_form_orthogonality_and_rates():
+    """-Laplace u = 3 pi^2 u, u = sin(pi x) sin(pi y) sin(pi z) (config 1):
+    ||u||_L2^2 = 1/8 and ||u||_a^2 = 3 pi^2 / 8 in closed form; Galerkin
+    orthogonality ||u - u_h||_a^2 = ||u||_a^2 - ||u_h||_a^2; and the errors
+    converge at O(h^3) in L2 and O(h^2) in energy for p = 2."""
+    from ifem_amd import model as M
+    exact = M.Function(M.FN_SINPROD, 1.0)
+    errs = []
+    for nel in (4, 8, 16):
+        pb = problems.poisson_cube(nel, 2)
+        rp, ci, A, b = oracle.system(pb.patch, pb.integrand, pb.ng, pb.meqn, pb.neq)
+        x, it, res = oracle.pcg(rp, ci, A, b, rtol=1e-13)
+        n = oracle.norms(pb.patch, pb.integrand, pb.ng, _expand(pb, x), exact)
+        if nel == 16:
+            assert abs(n[2] ** 2 - 0.125) <= 1e-8
+            assert abs(n[3] ** 2 - 3 * np.pi ** 2 / 8) <= 1e-7
+        # orthogonality holds up to the quadrature error of the load f*N,
+        # which vanishes at O(h^6): 3.6e-6, 5.5e-8, 8.5e-10 relative
+        defect = abs(n[5] ** 2 - (n[3] ** 2 - n[1] ** 2)) / n[3] ** 2
+        assert defect <= {4: 1e-5, 8: 2e-7, 16: 3e-9}[nel], (nel, defect)
+        errs.append((n[4], n[5]))
+    for (l0, e0), (l1, e1) in zip(errs, errs[1:]):
+        assert 2.7 <= np.log2(l0 / l1) <= 3.4, (l0, l1)
+        assert 1.8 <= np.log2(e0 / e1) <= 2.2, (e0, e1)

@@ -1,0 +1,158 @@
+"""Input front-end: GoTools .g2 volumes, knot insertion and degree elevation
+(geometry-preserving), and the .xinp subset of the hot path's models, checked
+against the generated benchmark problems and against evaluation of the
+spline map."""
+import math
+
+import numpy as np
+import pytest
+
+from ifem_amd import inputs, model as M, problems
+
+TRILINEAR_CUBE = """700 1 0 0
+3 0
+2 2
+0 0 1 1
+2 2
+0 0 1 1
+2 2
+0 0 1 1
+0 0 0
+1 0 0
+0 1 0
+1 1 0
+0 0 1
+1 0 1
+0 1 1
+1 1 1
+"""
+
+
+def test_g2_round_trip():
+    P = M.cube_patch(2, [3, 4, 5], jitter=0.2, seed=3)
+    Q = inputs.read_g2(inputs.write_g2(P))[0]
+    assert Q.p == P.p and Q.n == P.n
+    for d in range(3):
+        assert np.array_equal(Q.knots[d], P.knots[d])
+    assert np.array_equal(Q.coefs, P.coefs)
+
+
+def test_g2_refuses_rational_and_other_classes():
+    with pytest.raises(ValueError):
+        inputs.read_g2(TRILINEAR_CUBE.replace("3 0\n", "3 1\n", 1))
+    with pytest.raises(ValueError):
+        inputs.read_g2("200 1 0 0\n" + TRILINEAR_CUBE.split("\n", 1)[1])
+
+
+def test_raise_then_refine_gives_the_benchmark_cube():
+    """A trilinear cube raised to p=2 and refined with 15 knots per span is
+    the 16^3 p=2 cube of config 1 (identity map: control points at the
+    Greville abscissae)."""
+    P = inputs.read_g2(TRILINEAR_CUBE)[0]
+    P = inputs.raise_order(P, 1, 1, 1)
+    for d in range(3):
+        P = inputs.uniform_refine(P, d, 15)
+    R = M.cube_patch(2, 16)
+    assert P.p == R.p and P.n == R.n
+    for d in range(3):
+        assert np.max(np.abs(P.knots[d] - R.knots[d])) <= 1e-15
+    assert np.max(np.abs(P.coefs - R.coefs)) <= 1e-14
+
+
+def test_refinement_and_elevation_preserve_the_geometry():
+    rng = np.random.default_rng(5)
+    P = M.cube_patch(2, [3, 2, 4], jitter=0.25, seed=7)
+    R = P
+    for d in range(3):
+        R = inputs.uniform_refine(R, d, 2)
+    assert R.n == [P.n[0] + 2 * 3, P.n[1] + 2 * 2, P.n[2] + 2 * 4]
+    # a curved trilinear block, elevated twice
+    B = inputs.read_g2(TRILINEAR_CUBE)[0]
+    B.coefs[7] += [0.3, -0.2, 0.25]
+    B.coefs[2] += [-0.1, 0.15, 0.05]
+    E = inputs.raise_order(B, 2, 1, 2)
+    assert E.p == [3, 2, 3]
+    for _ in range(20):
+        u, v, w = rng.random(3)
+        assert np.max(np.abs(inputs.evaluate(R, u, v, w) - inputs.evaluate(P, u, v, w))) <= 1e-13
+        assert np.max(np.abs(inputs.evaluate(E, u, v, w) - inputs.evaluate(B, u, v, w))) <= 1e-13
+    with pytest.raises(ValueError):
+        inputs.raise_order(R, 1, 0, 0)  # interior knots
+
+
+POISSON_XINP = f"""<?xml version="1.0"?>
+<simulation>
+  <geometry>
+    <patch>
+{TRILINEAR_CUBE}
+    </patch>
+    <raiseorder patch="1" u="1" v="1" w="1"/>
+    <refine patch="1" u="15" v="15" w="15"/>
+    <topologysets>
+      <set name="Boundary" type="face"><item patch="1">1 2 3 4 5 6</item></set>
+    </topologysets>
+  </geometry>
+  <boundaryconditions>
+    <dirichlet set="Boundary" comp="1"/>
+  </boundaryconditions>
+  <poisson>
+    <source type="sinprod">{3 * math.pi ** 2!r}</source>
+  </poisson>
+  <discretization><nGauss>3</nGauss></discretization>
+  <linearsolver><type>cg</type><pc>jacobi</pc><rtol>1e-10</rtol><maxits>5000</maxits></linearsolver>
+</simulation>
+"""
+
+ELASTICITY_XINP = f"""<simulation>
+  <geometry>
+    <patch>{TRILINEAR_CUBE}</patch>
+    <raiseorder patch="1" u="1" v="1" w="1"/>
+    <refine patch="1" u="3" v="3" w="3"/>
+    <topologysets>
+      <set name="Left" type="face"><item patch="1">1</item></set>
+      <set name="Right" type="face"><item patch="1">2</item></set>
+    </topologysets>
+  </geometry>
+  <boundaryconditions>
+    <dirichlet set="Left" comp="123"/>
+    <neumann set="Right" direction="3">-1</neumann>
+  </boundaryconditions>
+  <elasticity><isotropic E="1" nu="0.3"/></elasticity>
+</simulation>
+"""
+
+
+def test_xinp_poisson_is_config_1():
+    pb, solver = inputs.read_xinp(POISSON_XINP)
+    ref = problems.poisson_cube(16, 2)
+    assert pb.neq == ref.neq and np.array_equal(pb.meqn, ref.meqn)
+    assert np.max(np.abs(pb.patch.coefs - ref.patch.coefs)) <= 1e-14
+    assert pb.ng == ref.ng
+    assert pb.integrand.source.type == M.FN_SINPROD
+    assert abs(pb.integrand.source.scale - ref.integrand.source.scale) <= 1e-15
+    assert inputs.solver_kwargs(solver) == {"rtol": 1e-10, "maxits": 5000}
+
+
+def test_xinp_elasticity_matches_generated_problem():
+    pb, solver = inputs.read_xinp(ELASTICITY_XINP)
+    ref = problems.elasticity_cube(4, 2, jitter=0.0)
+    assert pb.neq == ref.neq and np.array_equal(pb.meqn, ref.meqn)
+    assert pb.traction == ref.traction
+    assert pb.integrand.kind == M.ELASTICITY and pb.integrand.nu == 0.3
+    assert solver == {}
+    with pytest.raises(ValueError):
+        inputs.solver_kwargs({"type": "gmres"})
+
+
+@pytest.mark.gpu
+def test_xinp_model_solves_to_the_manufactured_error():
+    """Config 1 read from .xinp, assembled and solved on the GPU: the L2 error
+    against sin(pi x) sin(pi y) sin(pi z) equals the oracle's 2.2624e-5."""
+    from ifem_amd import setup_problem
+    pb, solver = inputs.read_xinp(POISSON_XINP)
+    print("[gpu] xinp config 1", flush=True)
+    with setup_problem(pb) as s:
+        st = s.solve(**inputs.solver_kwargs(solver))
+        assert st["converged"]
+        n = s.norms(pb.integrand, pb.ng, M.Function(M.FN_SINPROD, 1.0))
+    assert abs(n[4] - 2.26239829e-05) <= 1e-5 * n[4]

@@ -1,6 +1,7 @@
 // IFEMGpu.C -- reference-interface classes on the libifem_gpu C ABI.
 #include "IFEMGpu.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -288,6 +289,128 @@ ASMs3D::ASMs3D(const int ord[3], const int ncoef[3], const std::vector<double> k
     n[d] = ncoef[d];
     knots[d] = kn[d];
   }
+}
+
+std::unique_ptr<ASMs3D> ASMs3D::read(std::istream& is)
+{
+  int cls = 0, v1 = 0, v2 = 0, v3 = 0, dim = 0, rational = 0;
+  if (!(is >> cls >> v1 >> v2 >> v3 >> dim >> rational)) return nullptr;
+  if (cls != 700 || dim != 3) {
+    std::cerr << " *** ASMs3D::read: not a 3D spline volume (class " << cls << ")" << std::endl;
+    return nullptr;
+  }
+  if (rational) {
+    std::cerr << " *** ASMs3D::read: rational volumes are not supported on the GPU" << std::endl;
+    return nullptr;
+  }
+  int ord[3], ncf[3];
+  std::vector<double> kn[3];
+  for (int d = 0; d < 3; d++) {
+    if (!(is >> ncf[d] >> ord[d]) || ncf[d] < ord[d] || ord[d] < 1) return nullptr;
+    kn[d].resize(ncf[d] + ord[d]);
+    for (double& t : kn[d])
+      if (!(is >> t)) return nullptr;
+  }
+  std::vector<double> cf((size_t)ncf[0] * ncf[1] * ncf[2] * 3);
+  for (double& x : cf)
+    if (!(is >> x)) return nullptr;
+  return std::unique_ptr<ASMs3D>(new ASMs3D(ord, ncf, kn, cf));
+}
+
+// Control point (i, j, k) of an n[0] x n[1] x n[2] net, i fastest.
+static inline double* cpt(std::vector<double>& c, const int nn[3], int i, int j, int k)
+{
+  return &c[3 * ((size_t)i + (size_t)nn[0] * (j + (size_t)nn[1] * k))];
+}
+
+bool ASMs3D::uniformRefine(int dir, int nInsert)
+{
+  if (dir < 0 || dir > 2 || nInsert < 0) return false;
+  const std::vector<double> t0 = knots[dir];
+  for (size_t s = 0; s + 1 < t0.size(); s++) {
+    const double a = t0[s], b = t0[s + 1];
+    if (!(b > a)) continue;
+    for (int m = 1; m <= nInsert; m++) {
+      // Boehm insertion of u into direction dir
+      const double u = a + (b - a) * m / (nInsert + 1);
+      const int p = order[dir] - 1;
+      const std::vector<double>& t = knots[dir];
+      int k = (int)(std::upper_bound(t.begin(), t.end(), u) - t.begin()) - 1;
+      int nn[3] = {n[0], n[1], n[2]}, nw[3] = {n[0], n[1], n[2]};
+      nw[dir]++;
+      std::vector<double> nc((size_t)nw[0] * nw[1] * nw[2] * 3);
+      for (int kk = 0; kk < nw[2]; kk++)
+        for (int jj = 0; jj < nw[1]; jj++)
+          for (int ii = 0; ii < nw[0]; ii++) {
+            int idx[3] = {ii, jj, kk};
+            const int i = idx[dir];
+            double* q = cpt(nc, nw, ii, jj, kk);
+            auto src = [&](int ia) {
+              int id2[3] = {ii, jj, kk};
+              id2[dir] = ia;
+              return cpt(coefs, nn, id2[0], id2[1], id2[2]);
+            };
+            if (i <= k - p) {
+              const double* s0 = src(i);
+              for (int c = 0; c < 3; c++) q[c] = s0[c];
+            } else if (i <= k) {
+              const double al = (u - t[i]) / (t[i + p] - t[i]);
+              const double *s1 = src(i), *s0 = src(i - 1);
+              for (int c = 0; c < 3; c++) q[c] = al * s1[c] + (1.0 - al) * s0[c];
+            } else {
+              const double* s0 = src(i - 1);
+              for (int c = 0; c < 3; c++) q[c] = s0[c];
+            }
+          }
+      knots[dir].insert(knots[dir].begin() + k + 1, u);
+      coefs.swap(nc);
+      n[dir]++;
+    }
+  }
+  return true;
+}
+
+bool ASMs3D::raiseOrder(int ru, int rv, int rw)
+{
+  const int r[3] = {ru, rv, rw};
+  for (int d = 0; d < 3; d++)
+    for (int step = 0; step < r[d]; step++) {
+      const int p = order[d] - 1;
+      if (n[d] != p + 1) {
+        std::cerr << " *** ASMs3D::raiseOrder: interior knots in direction " << d + 1
+                  << " (raise the order before refining)" << std::endl;
+        return false;
+      }
+      int nn[3] = {n[0], n[1], n[2]}, nw[3] = {n[0], n[1], n[2]};
+      nw[d] = p + 2;
+      std::vector<double> nc((size_t)nw[0] * nw[1] * nw[2] * 3, 0.0);
+      for (int kk = 0; kk < nw[2]; kk++)
+        for (int jj = 0; jj < nw[1]; jj++)
+          for (int ii = 0; ii < nw[0]; ii++) {
+            int idx[3] = {ii, jj, kk};
+            const int i = idx[d];
+            const double a = (double)i / (p + 1);
+            double* q = cpt(nc, nw, ii, jj, kk);
+            int id2[3] = {ii, jj, kk};
+            if (i > 0) {
+              id2[d] = i - 1;
+              const double* s0 = cpt(coefs, nn, id2[0], id2[1], id2[2]);
+              for (int c = 0; c < 3; c++) q[c] += a * s0[c];
+            }
+            if (i <= p) {
+              id2[d] = i;
+              const double* s1 = cpt(coefs, nn, id2[0], id2[1], id2[2]);
+              for (int c = 0; c < 3; c++) q[c] += (1.0 - a) * s1[c];
+            }
+          }
+      const double t0 = knots[d].front(), t1 = knots[d].back();
+      knots[d].assign(p + 2, t0);
+      knots[d].insert(knots[d].end(), p + 2, t1);
+      coefs.swap(nc);
+      n[d] = p + 2;
+      order[d] = p + 2;
+    }
+  return true;
 }
 
 std::unique_ptr<ASMs3D> ASMs3D::unitCube(int p, int nel)

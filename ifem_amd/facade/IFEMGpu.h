@@ -22,6 +22,7 @@
 // integrate() return false -- there is no CPU fallback.
 #pragma once
 
+#include <istream>
 #include <map>
 #include <memory>
 #include <string>
@@ -261,6 +262,17 @@ public:
   //! Unit-cube patch of degree p with nel elements per direction
   //! (DefaultGeometry3D + refine/raiseorder, src/SIM/ModelGenerator.C:203-273).
   static std::unique_ptr<ASMs3D> unitCube(int p, int nel);
+  //! Reads one non-rational SplineVolume (GoTools .g2, class 700) as
+  //! ASMs3D::read does (src/ASM/ASMs3D.C:117-160); nullptr on failure.
+  static std::unique_ptr<ASMs3D> read(std::istream& is);
+  //! Inserts nInsert equally spaced knots in every knot span of direction
+  //! dir = 0..2 (ASM3D::uniformRefine, src/SIM/SIM3D.C:104-128), by knot
+  //! insertion, which leaves the geometry unchanged.
+  bool uniformRefine(int dir, int nInsert);
+  //! Raises the degree by (ru, rv, rw) (ASM3D::raiseOrder,
+  //! src/SIM/SIM3D.C:152-170); only for directions without interior knots
+  //! (raise before refining, as IFEM inputs do).
+  bool raiseOrder(int ru, int rv, int rw);
 
   //! Marks components cmask (bit c) of face 1..6 as homogeneous Dirichlet
   //! (ASMbase::fix/constrainFace with fixHomogeneousDirichlet).

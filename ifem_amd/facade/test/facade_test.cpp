@@ -4,6 +4,7 @@
 // case for tests/test_facade.py.
 #include <cmath>
 #include <cstdio>
+#include <sstream>
 #include <vector>
 
 #include "IFEMGpu.h"
@@ -53,6 +54,31 @@ static int poisson(int nel)
   return 0;
 }
 
+// GoTools .g2 trilinear unit cube, raised to p = 2 and refined to 16^3
+// elements: must equal the generated 16^3 p = 2 cube.
+static int g2_cube()
+{
+  std::istringstream is("700 1 0 0\n3 0\n2 2\n0 0 1 1\n2 2\n0 0 1 1\n2 2\n0 0 1 1\n"
+                        "0 0 0\n1 0 0\n0 1 0\n1 1 0\n0 0 1\n1 0 1\n0 1 1\n1 1 1\n");
+  auto pch = ASMs3D::read(is);
+  if (!pch) return 1;
+  if (!pch->raiseOrder(1, 1, 1)) return 2;
+  for (int d = 0; d < 3; d++)
+    if (!pch->uniformRefine(d, 15)) return 3;
+  auto ref = ASMs3D::unitCube(2, 16);
+  double dk = 0.0, dc = 0.0;
+  for (int d = 0; d < 3; d++) {
+    if (pch->n[d] != ref->n[d] || pch->order[d] != ref->order[d]) return 4;
+    for (size_t i = 0; i < ref->knots[d].size(); i++)
+      dk = std::fmax(dk, std::fabs(pch->knots[d][i] - ref->knots[d][i]));
+  }
+  for (size_t i = 0; i < ref->coefs.size(); i++)
+    dc = std::fmax(dc, std::fabs(pch->coefs[i] - ref->coefs[i]));
+  std::printf("{\"case\": \"g2\", \"n\": %d, \"knot_diff\": %.6g, \"coef_diff\": %.6g}\n",
+              pch->n[0], dk, dc);
+  return 0;
+}
+
 static int elasticity(int nel, const char* precision)
 {
   auto pch = ASMs3D::unitCube(2, nel);
@@ -95,7 +121,8 @@ static int elasticity(int nel, const char* precision)
 
 int main()
 {
-  int rc = poisson(16);
+  int rc = g2_cube();
+  if (!rc) rc = poisson(16);
   if (!rc) rc = elasticity(8, "fp64");
   if (!rc) rc = elasticity(8, "fp32");
   // unknown solver type is refused (bool false), as the reference does for

@@ -28,7 +28,7 @@ EXPORTS = [
     "ifem_gpu_batch_autotune", "ifem_gpu_batch_solve", "ifem_gpu_batch_results",
     "ifem_gpu_batch_patch_info", "ifem_gpu_batch_get_system", "ifem_gpu_batch_get_solution",
     "ifem_gpu_batch_timings", "ifem_gpu_autotune_assembly", "ifem_gpu_unet_backward",
-    "ifem_gpu_last_assembly_parts", "ifem_gpu_integrate_norms",
+    "ifem_gpu_last_assembly_parts", "ifem_gpu_integrate_norms", "ifem_gpu_sam_bind_constraints",
 ]
 
 FP64, FP32 = 64, 32
@@ -89,6 +89,8 @@ def lib():
                                         C.POINTER(C.POINTER(dp)), C.POINTER(dp), ip,
                                         C.POINTER(ip)]
     L.ifem_gpu_sam_bind.argtypes = [vp, ip, ip, ip, C.POINTER(ip), ip]
+    L.ifem_gpu_sam_bind_constraints.argtypes = [vp, ip, ip, ip, C.POINTER(ip), ip, ip,
+                                                C.POINTER(ip), C.POINTER(ip), C.POINTER(dp)]
     L.ifem_gpu_init_assembly.argtypes = [vp, C.POINTER(C.c_int64)]
     L.ifem_gpu_assemble.argtypes = [vp, C.POINTER(Integrand), C.POINTER(ip), ip]
     L.ifem_gpu_assemble_neumann.argtypes = [vp, ip, C.POINTER(dp), ip]
@@ -201,6 +203,22 @@ class Session:
         self.neq = neq
         self.ncomp = ncomp
 
+    def bind_constraints(self, patch_id: int, ncomp: int, sam):
+        """Binds SAM arrays with constraint equations: ``sam`` has meqn, neq,
+        mpmceq, mmceq, ttcc (1-based, SAM layout; prescribed values only)."""
+        m = np.ascontiguousarray(sam.meqn, dtype=np.int32)
+        mp = np.ascontiguousarray(sam.mpmceq, dtype=np.int32)
+        mm = np.ascontiguousarray(sam.mmceq, dtype=np.int32)
+        tt = np.ascontiguousarray(sam.ttcc, dtype=np.float64)
+        nceq = len(mp) - 1
+        ipp = C.POINTER(C.c_int)
+        check(lib().ifem_gpu_sam_bind_constraints(
+            self.h, patch_id, ncomp, len(m), m.ctypes.data_as(ipp), sam.neq, nceq,
+            mp.ctypes.data_as(ipp), mm.ctypes.data_as(ipp),
+            tt.ctypes.data_as(C.POINTER(C.c_double))))
+        self.neq = sam.neq
+        self.ncomp = ncomp
+
     def init_assembly(self) -> int:
         nnz = C.c_int64(0)
         check(lib().ifem_gpu_init_assembly(self.h, C.byref(nnz)))
@@ -303,6 +321,12 @@ class Session:
         x = np.zeros(max(self.neq, 1))
         check(lib().ifem_gpu_get_solution(self.h, _ptr(x)))
         return x[:self.neq]
+
+    def expanded_solution(self, ndof: int):
+        """Solution in DOF order: eliminated DOFs 0 or their prescribed value."""
+        x = np.zeros(max(ndof, 1))
+        check(lib().ifem_gpu_expand_solution(self.h, _ptr(x)))
+        return x[:ndof]
 
     def spmv(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

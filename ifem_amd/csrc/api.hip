@@ -26,6 +26,9 @@ static void free_system(Context& c)
 {
   solver_free(c);
   hipFree(c.d_eq);
+  hipFree(c.d_gval);
+  c.d_gval = nullptr;
+  c.h_gval.clear();
   hipFree(c.d_rowptr);
   hipFree(c.d_colind);
   hipFree(c.d_bcol);
@@ -181,11 +184,11 @@ int ifem_gpu_patch_create(ifem_gpu_ctx* ctx, const int order[3], const int ncoef
   return IFEM_GPU_OK;
 }
 
-int ifem_gpu_sam_bind(ifem_gpu_ctx* ctx, int patch_id, int ncomp, int ndof, const int* meqn,
-                      int neq)
+// Shared by both bind entry points.  gval: per DOF value of prescribed DOFs
+// (empty: none), already checked by the caller.
+static int bind_system(Context& c, int patch_id, int ncomp, int ndof, const int* meqn, int neq,
+                       std::vector<double>&& gval)
 {
-  CHECK_CTX(ctx);
-  Context& c = ctx->c;
   if (patch_id < 0 || patch_id >= (int)c.patches.size())
     return fail(IFEM_GPU_EINVAL, "sam_bind: unknown patch");
   if (ncomp != 1 && ncomp != 3) return fail(IFEM_GPU_EUNSUPPORTED, "sam_bind: ncomp must be 1 or 3");
@@ -195,15 +198,16 @@ int ifem_gpu_sam_bind(ifem_gpu_ctx* ctx, int patch_id, int ncomp, int ndof, cons
   std::vector<int> eq(ndof);
   int last = 0;
   for (int i = 0; i < ndof; i++) {
-    if (meqn[i] < 0)
-      return fail(IFEM_GPU_EUNSUPPORTED, "sam_bind: constrained (negative) equations are not supported");
+    if (meqn[i] < 0 && gval.empty())
+      return fail(IFEM_GPU_EUNSUPPORTED,
+                  "sam_bind: constrained (negative) equations need ifem_gpu_sam_bind_constraints");
     if (meqn[i] > 0) {
       if (meqn[i] != last + 1)
         return fail(IFEM_GPU_EUNSUPPORTED,
                     "sam_bind: free DOFs must be numbered 1..neq in increasing DOF order");
       last = meqn[i];
     }
-    eq[i] = meqn[i] - 1;
+    eq[i] = meqn[i] > 0 ? meqn[i] - 1 : -1;
   }
   if (last != neq) return fail(IFEM_GPU_EINVAL, "sam_bind: neq does not match meqn");
   free_system(c);
@@ -214,7 +218,53 @@ int ifem_gpu_sam_bind(ifem_gpu_ctx* ctx, int patch_id, int ncomp, int ndof, cons
   c.h_eq = eq;
   IFEM_HIP(hipMalloc(&c.d_eq, sizeof(int) * ndof));
   IFEM_HIP(hipMemcpy(c.d_eq, eq.data(), sizeof(int) * ndof, hipMemcpyHostToDevice));
+  bool any = false;
+  for (double g : gval) any = any || g != 0.0;
+  if (any) {
+    IFEM_HIP(hipMalloc(&c.d_gval, sizeof(double) * ndof));
+    IFEM_HIP(hipMemcpy(c.d_gval, gval.data(), sizeof(double) * ndof, hipMemcpyHostToDevice));
+    c.h_gval = std::move(gval);
+  }
   return IFEM_GPU_OK;
+}
+
+int ifem_gpu_sam_bind(ifem_gpu_ctx* ctx, int patch_id, int ncomp, int ndof, const int* meqn,
+                      int neq)
+{
+  CHECK_CTX(ctx);
+  return bind_system(ctx->c, patch_id, ncomp, ndof, meqn, neq, {});
+}
+
+int ifem_gpu_sam_bind_constraints(ifem_gpu_ctx* ctx, int patch_id, int ncomp, int ndof,
+                                  const int* meqn, int neq, int nceq, const int* mpmceq,
+                                  const int* mmceq, const double* ttcc)
+{
+  CHECK_CTX(ctx);
+  if (!meqn || ndof < 0 || nceq < 0 || (nceq > 0 && (!mpmceq || !mmceq || !ttcc)))
+    return fail(IFEM_GPU_EINVAL, "sam_bind_constraints: null argument");
+  // SAM layout (src/LinAlg/SAM.C:260-308): constraint i (1-based) holds the
+  // entries mpmceq[i-1] .. mpmceq[i]-1 (1-based) of mmceq/ttcc; the first is
+  // the slave DOF and the constant c0, the rest masters and coefficients.
+  if (nceq > 0 && mpmceq[0] != 1)
+    return fail(IFEM_GPU_EINVAL, "sam_bind_constraints: mpmceq[0] must be 1");
+  std::vector<double> gval(ndof > 0 ? ndof : 1, 0.0);
+  std::vector<char> seen(nceq > 0 ? nceq : 1, 0);
+  for (int i = 0; i < ndof; i++) {
+    if (meqn[i] >= 0) continue;
+    const int ceq = -meqn[i];
+    if (ceq > nceq) return fail(IFEM_GPU_EINVAL, "sam_bind_constraints: meqn names an unknown constraint");
+    const int ip = mpmceq[ceq - 1], jp = mpmceq[ceq] - 1;
+    if (jp < ip) return fail(IFEM_GPU_EINVAL, "sam_bind_constraints: mpmceq is not increasing");
+    if (mmceq[ip - 1] != i + 1)
+      return fail(IFEM_GPU_EINVAL, "sam_bind_constraints: constraint slave differs from the DOF");
+    if (jp > ip)
+      return fail(IFEM_GPU_EUNSUPPORTED,
+                  "sam_bind_constraints: constraints with master DOFs are not supported");
+    if (seen[ceq - 1]) return fail(IFEM_GPU_EINVAL, "sam_bind_constraints: constraint used twice");
+    seen[ceq - 1] = 1;
+    gval[i] = ttcc[ip - 1];
+  }
+  return bind_system(ctx->c, patch_id, ncomp, ndof, meqn, neq, std::move(gval));
 }
 
 int ifem_gpu_init_assembly(ifem_gpu_ctx* ctx, int64_t* nnz)
@@ -369,7 +419,10 @@ int ifem_gpu_expand_solution(ifem_gpu_ctx* ctx, double* xdof)
   std::vector<double> x(c.neq);
   IFEM_HIP(hipStreamSynchronize(c.stream));
   IFEM_HIP(hipMemcpy(x.data(), c.w.x, sizeof(double) * c.neq, hipMemcpyDeviceToHost));
-  for (int i = 0; i < c.ndof; i++) xdof[i] = c.h_eq[i] >= 0 ? x[c.h_eq[i]] : 0.0;
+  // eliminated DOFs: 0, or their prescribed value (src/LinAlg/SAM.C:768-810)
+  const bool g = !c.h_gval.empty();
+  for (int i = 0; i < c.ndof; i++)
+    xdof[i] = c.h_eq[i] >= 0 ? x[c.h_eq[i]] : (g ? c.h_gval[i] : 0.0);
   return IFEM_GPU_OK;
 }
 

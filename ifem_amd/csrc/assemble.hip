@@ -173,6 +173,9 @@ struct AsmArgs {
   // written by k_gp_geometry, read by the line kernels
   double* gp_geo;
   double kappa_c, source_c;            // values of constant fields
+  // per DOF prescribed value of eliminated DOFs (FP64 pass only; nullptr:
+  // homogeneous); the eliminated columns move to the RHS, b_i -= K_ij g_j
+  const double* gval;
 };
 
 // Geometry record of one Gauss point.  Vector problems: J^-1 (row-major, 9)
@@ -653,10 +656,13 @@ __device__ __forceinline__ void line_impl(
         const int req = a.eq[node * NC + ic];
         if (req >= 0) {
           const int64_t base = a.rowptr[req];
+          const double* __restrict__ gv = a.gval;
+          double corr = 0.0;
           int64_t pos = 0;
           for (int t0 = 0; t0 < WN * NC; t0 += 64) {
             const int t = t0 + lane;
             bool valid = false;
+            int64_t gdof = -1; // in-bounds eliminated column
             if (t < WN * NC) {
               const int nbr = t / NC, cc = t % NC;
               const int di = nbr % W1 - P, dj = (nbr / W1) % W1 - P, dk = nbr / (W1 * W1) - P;
@@ -664,6 +670,7 @@ __device__ __forceinline__ void line_impl(
               if (ii >= 0 && ii < a.n1 && jj >= 0 && jj < a.n2 && kk >= 0 && kk < a.n3) {
                 const int64_t nn = (int64_t)ii + (int64_t)a.n1 * (jj + (int64_t)a.n2 * kk);
                 valid = a.eq[nn * NC + cc] >= 0;
+                if (!valid) gdof = nn * NC + cc;
               }
             }
             const unsigned long long m = __ballot(valid);
@@ -672,10 +679,16 @@ __device__ __forceinline__ void line_impl(
               const T v = win[slot][wave][t / NC][t % NC];
               if (a.val32) a.val32[at] = (float)v;
               else a.val64[at] = (double)v;
+            } else if (gv && gdof >= 0) {
+              corr += (double)win[slot][wave][t / NC][t % NC] * gv[gdof];
             }
             pos += __popcll(m);
           }
-          if (HAS_RHS && a.rhs && lane == 0) a.rhs[req] = (double)rwin[slot][ic];
+          if (gv) {
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) corr += __shfl_xor(corr, off, 64);
+          }
+          if ((HAS_RHS || gv) && a.rhs && lane == 0) a.rhs[req] = (double)rwin[slot][ic] - corr;
         }
       }
       __syncthreads();
@@ -1052,6 +1065,7 @@ static int assemble_p(Context& c, const ifem_gpu_integrand& itg, int ng, int pre
     a.val64 = c.d_val;
     a.val32 = nullptr;
     a.rhs = c.d_rhs;
+    a.gval = c.d_gval;
   }
   for (void* p : c.scratch_tables) hipFree(p);
   c.scratch_tables.clear();
@@ -1598,7 +1612,7 @@ __global__ __launch_bounds__(256) void k_norms(NormArgs na, int64_t total)
 #pragma unroll
           for (int c = 0; c < NC; c++) {
             const int eqn = a.eq[node * NC + c];
-            const double ub = eqn >= 0 ? na.x[eqn] : 0.0;
+            const double ub = eqn >= 0 ? na.x[eqn] : (a.gval ? a.gval[node * NC + c] : 0.0);
             uh[c] += nv * ub;
             gr[c][0] += g0 * ub;
             gr[c][1] += g1 * ub;
@@ -1701,6 +1715,7 @@ int integrate_norms(Context& c, const ifem_gpu_integrand& itg, const int ng[3],
   a.tabN3 = tb.tabN[2]; a.tabD3 = tb.tabD[2]; a.span3 = tb.span[2];
   for (int k = 0; k < kMaxNG; k++) a.gw[k] = ga.w[k];
   a.eq = c.d_eq;
+  a.gval = c.d_gval;
   a.ng = g;
   a.ncomp = c.ncomp;
   for (void* q : c.scratch_tables) hipFree(q);

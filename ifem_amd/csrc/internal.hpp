@@ -71,6 +71,8 @@ struct Context {
   int ncomp = 0, ndof = 0, neq = 0;
   int* d_eq = nullptr;          // per DOF: 0-based equation or -1
   std::vector<int> h_eq;
+  double* d_gval = nullptr;     // per DOF prescribed value (nullptr: all 0)
+  std::vector<double> h_gval;
   int64_t nnz = 0;
   int64_t* d_rowptr = nullptr;
   int32_t* d_colind = nullptr;

@@ -151,9 +151,12 @@ void GPUMatrix::initAssembly(const SAM& sam, char)
   if (!report(ifem_gpu_patch_create(ctx, P.order, P.n, kp, P.coefs.data(), 0, &pid),
               "GPUMatrix::initAssembly"))
     return;
-  if (!report(ifem_gpu_sam_bind(ctx, pid, sam.ncomp, sam.ndof, sam.meqn.data(), sam.neq),
-              "GPUMatrix::initAssembly"))
-    return;
+  const int brc = sam.nceq > 0
+      ? ifem_gpu_sam_bind_constraints(ctx, pid, sam.ncomp, sam.ndof, sam.meqn.data(), sam.neq,
+                                      sam.nceq, sam.mpmceq.data(), sam.mmceq.data(),
+                                      sam.ttcc.data())
+      : ifem_gpu_sam_bind(ctx, pid, sam.ncomp, sam.ndof, sam.meqn.data(), sam.neq);
+  if (!report(brc, "GPUMatrix::initAssembly")) return;
   if (!report(ifem_gpu_init_assembly(ctx, &nnz), "GPUMatrix::initAssembly")) return;
   neq = sam.neq;
   ok = true;
@@ -210,7 +213,8 @@ bool SAM::expandSolution(const SystemVector& solVec, std::vector<Real>& dofVec,
   const Real* x = solVec.getRef();
   for (int i = 0; i < ndof; i++)
     if (meqn[i] > 0) dofVec[i] = x[meqn[i] - 1];
-    else dofVec[i] = 0.0 * scaleSD; // homogeneous Dirichlet value
+    else if (meqn[i] < 0) dofVec[i] = scaleSD * ttcc[mpmceq[-meqn[i] - 1] - 1]; // c0
+    else dofVec[i] = 0.0; // homogeneous Dirichlet value
   return true;
 }
 
@@ -230,23 +234,47 @@ bool SAMpatch::init(const ASMs3D& pch, int nc)
   ndof = nnod * nc;
   msc.assign(ndof, 1);
   const int n1 = pch.n[0], n2 = pch.n[1], n3 = pch.n[2];
+  auto on_face = [&](int f, int i, int j, int k) {
+    switch (f) {
+    case 1: return i == 0;
+    case 2: return i == n1 - 1;
+    case 3: return j == 0;
+    case 4: return j == n2 - 1;
+    case 5: return k == 0;
+    default: return k == n3 - 1;
+    }
+  };
+  // prescribed value per DOF (NaN: none); constraint equations are kept in
+  // slave-DOF order, as the reference's MPC set orders them
+  std::vector<double> value(ndof, std::nan(""));
   for (int k = 0; k < n3; k++)
     for (int j = 0; j < n2; j++)
       for (int i = 0; i < n1; i++) {
         int m = 0;
-        auto face = [&](int f) { auto it = pch.dirichlet.find(f); return it == pch.dirichlet.end() ? 0 : it->second; };
-        if (i == 0) m |= face(1);
-        if (i == n1 - 1) m |= face(2);
-        if (j == 0) m |= face(3);
-        if (j == n2 - 1) m |= face(4);
-        if (k == 0) m |= face(5);
-        if (k == n3 - 1) m |= face(6);
+        for (const auto& d : pch.dirichlet)
+          if (on_face(d.first, i, j, k)) m |= d.second;
         const int node = i + n1 * (j + n2 * k);
         for (int c = 0; c < nc; c++)
           if ((m >> c) & 1) msc[node * nc + c] = 0;
+        for (const auto& pr : pch.prescribed)
+          if (on_face(pr.face, i, j, k))
+            for (int c = 0; c < nc; c++)
+              if ((pr.cmask >> c) & 1 && msc[node * nc + c] == 1) value[node * nc + c] = pr.value;
       }
-  // SAM::initSystemEquations: free DOFs numbered in DOF order
+  nceq = 0;
+  mpmceq.assign(1, 1);
+  mmceq.clear();
+  ttcc.clear();
   meqn.assign(ndof, 0);
+  for (int i = 0; i < ndof; i++)
+    if (!std::isnan(value[i])) {
+      msc[i] = 0;
+      meqn[i] = -(++nceq);
+      mmceq.push_back(i + 1);
+      ttcc.push_back(value[i]);
+      mpmceq.push_back(mpmceq.back() + 1);
+    }
+  // SAM::initSystemEquations: free DOFs numbered in DOF order
   neq = 0;
   for (int i = 0; i < ndof; i++)
     if (msc[i] == 1) meqn[i] = ++neq;

@@ -211,15 +211,21 @@ public:
   int getNoDOFs() const { return ndof; }
   int getNoEquations() const { return neq; }
   const std::vector<int>& getMEQN() const { return meqn; }
-  //! Equation vector -> DOF vector, fixed DOFs = 0 (SAM::expandSolution).
+  int getNoConstraints() const { return nceq; }
+  //! Equation vector -> DOF vector, fixed DOFs = 0, prescribed DOFs
+  //! scaleSD * c0 (SAM::expandSolution, src/LinAlg/SAM.C:768-810).
   bool expandSolution(const SystemVector& solVec, std::vector<Real>& dofVec,
                       Real scaleSD = Real(1)) const;
   //! RMS norm ||x||_2 / sqrt(n) (SAM::normL2, src/LinAlg/SAM.C:860-880).
   Real normL2(const std::vector<Real>& x) const;
 
 protected:
-  int nnod = 0, ndof = 0, neq = 0, ncomp = 1;
+  int nnod = 0, ndof = 0, neq = 0, ncomp = 1, nceq = 0;
   std::vector<int> msc, meqn;
+  //! constraint equations (1-based, src/LinAlg/SAM.h:330-350): entries
+  //! mpmceq[k-1] .. mpmceq[k]-1 of mmceq / ttcc belong to constraint k
+  std::vector<int> mpmceq, mmceq;
+  std::vector<Real> ttcc;
   friend class GPUMatrix;
   const class ASMs3D* patch = nullptr;
 };
@@ -227,7 +233,9 @@ protected:
 class ASMs3D;
 
 //! SAM of one structured patch (SAMpatch::init / initNodeDofs with
-//! homogeneous Dirichlet codes, src/ASM/SAMpatch.C:21-161).
+//! homogeneous Dirichlet codes, src/ASM/SAMpatch.C:21-161; prescribed values
+//! as constraint equations without masters, SAMpatch::initConstraintEqs,
+//! src/ASM/SAMpatch.C:259-357).
 class SAMpatch : public SAM {
 public:
   bool init(const ASMs3D& pch, int ncomp);
@@ -277,6 +285,14 @@ public:
   //! Marks components cmask (bit c) of face 1..6 as homogeneous Dirichlet
   //! (ASMbase::fix/constrainFace with fixHomogeneousDirichlet).
   void constrainFace(int face, int cmask) { dirichlet[face] |= cmask; }
+  //! Prescribes the value on components cmask of face 1..6 (constrainFace
+  //! with a constant inhomogeneous Dirichlet function: one constraint
+  //! equation u = value per DOF, ASMs3D::constrainFace with code > 0,
+  //! src/ASM/ASMs3D.C:1025-1118).  A DOF already fixed homogeneously stays
+  //! fixed (ASMbase::addMPC, src/ASM/ASMbase.C:527-545); a DOF on several
+  //! prescribed faces takes the last value.
+  void prescribeFace(int face, int cmask, double value)
+  { prescribed.push_back({face, cmask, value}); }
   //! Gauss points per direction (SIMoptions nGauss, ASMbase::getNoGaussPt).
   void setGauss(int ng) { nGauss = ng; }
 
@@ -291,6 +307,8 @@ public:
   std::vector<double> knots[3];
   std::vector<double> coefs;
   std::map<int, int> dirichlet;
+  struct Prescribed { int face, cmask; double value; };
+  std::vector<Prescribed> prescribed;
   int nGauss = 0; //!< 0: p+1 per direction
 
 private:

@@ -119,12 +119,55 @@ static int elasticity(int nel, const char* precision)
   return 0;
 }
 
+// Uniaxial strain by prescribed displacements: u_x = 0 on x = 0 and d on
+// x = 1, symmetry planes y = 0 (u_y = 0) and z = 0 (u_z = 0), traction free
+// elsewhere; the exact u = (d x, -nu d y, -nu d z) is in the spline space.
+static int uniaxial(int nel)
+{
+  const double d = 0.01, nu = 0.3;
+  auto pch = ASMs3D::unitCube(2, nel);
+  pch->constrainFace(3, 2);
+  pch->constrainFace(5, 4);
+  pch->prescribeFace(1, 1, 0.0);
+  pch->prescribeFace(2, 1, d);
+  SAMpatch sam;
+  sam.init(*pch, 3);
+  LinSolParams par;
+  par.addValue("type", "cg");
+  par.addValue("pc", "jacobi");
+  par.addValue("rtol", "1e-12");
+  par.addValue("maxits", "50000");
+  AlgEqSystem eq(sam);
+  if (!eq.init(LinAlg::GPU, par)) return 21;
+  ElasticityIntegrand itg(1.0, nu);
+  TimeDomain time;
+  eq.initialize(true);
+  if (!pch->integrate(itg, eq, time)) return 22;
+  if (!eq.finalize()) return 23;
+  StdVector& b = *static_cast<StdVector*>(eq.getVector());
+  Real rc = 0.0;
+  if (!eq.getMatrix()->solve(b, &rc)) return 24;
+  std::vector<Real> sol;
+  sam.expandSolution(b, sol);
+  double err = 0.0;
+  for (int node = 0; node < pch->getNoNodes(); node++) {
+    const double* X = &pch->coefs[3 * node];
+    const double ue[3] = {d * X[0], -nu * d * X[1], -nu * d * X[2]};
+    for (int c = 0; c < 3; c++) err = std::fmax(err, std::fabs(sol[3 * node + c] - ue[c]));
+  }
+  std::printf("{\"case\": \"uniaxial\", \"nel\": %d, \"neq\": %d, \"nceq\": %d, "
+              "\"max_err\": %.6g, \"rel_residual\": %.6g}\n",
+              nel, sam.getNoEquations(), sam.getNoConstraints(), err, rc);
+  return 0;
+}
+
 int main()
 {
   int rc = g2_cube();
   if (!rc) rc = poisson(16);
   if (!rc) rc = elasticity(8, "fp64");
   if (!rc) rc = elasticity(8, "fp32");
+  if (!rc) rc = uniaxial(6);
   // unknown solver type is refused (bool false), as the reference does for
   // unavailable back-ends
   if (!rc) {

@@ -10,7 +10,9 @@ geometry generator and SAM set-up:
   seeded jitter of the interior control points.
 * ``face_status`` / ``equation_numbers``: status codes for homogeneous
   Dirichlet faces and SAM's equation numbering
-  (src/ASM/ASMbase.C:884-908, src/LinAlg/SAM.C:225-336).
+  (src/ASM/ASMbase.C:884-908, src/LinAlg/SAM.C:225-336);
+  ``prescribed_equations``: SAM arrays with prescribed (inhomogeneous
+  Dirichlet) values as constraint equations without masters.
 * ``Function`` / ``Integrand``: the recognised integrand kinds of the GPU path
   (Poisson with coefficient and source fields, isotropic elasticity).
 
@@ -155,6 +157,48 @@ def equation_numbers(msc: np.ndarray):
     meqn = np.zeros(len(msc), dtype=np.int32)
     meqn[free] = np.arange(1, int(free.sum()) + 1, dtype=np.int32)
     return meqn, int(free.sum())
+
+
+@dataclasses.dataclass
+class SamArrays:
+    """SAM's equation arrays with constraint equations (1-based, SAM layout):
+    meqn per DOF (> 0 equation, 0 fixed, -k slave of constraint k); the
+    entries of constraint k are mpmceq[k-1] .. mpmceq[k]-1 of mmceq (DOF
+    numbers, slave first) and ttcc (constant c0 first, then coefficients)."""
+    meqn: np.ndarray
+    neq: int
+    mpmceq: np.ndarray
+    mmceq: np.ndarray
+    ttcc: np.ndarray
+
+    @property
+    def nceq(self) -> int:
+        return len(self.mpmceq) - 1
+
+    def prescribed_values(self) -> np.ndarray:
+        """Per DOF c0 of its constraint (0 for free and fixed DOFs)."""
+        g = np.zeros(len(self.meqn))
+        k = -self.meqn[self.meqn < 0]
+        g[self.meqn < 0] = self.ttcc[self.mpmceq[k - 1] - 1]
+        return g
+
+
+def prescribed_equations(msc: np.ndarray, values: np.ndarray) -> SamArrays:
+    """SAM arrays for status codes 0/1 plus prescribed values: every DOF with
+    a finite ``values`` entry becomes the slave of a constraint equation
+    without masters, u_i = c0 (ASMbase::addMPC + SAMpatch::initConstraintEqs,
+    src/ASM/ASMbase.C:527-560, src/ASM/SAMpatch.C:259-357: constraints in
+    slave-DOF order, status of the slaves reset to 0, then
+    SAM::initSystemEquations numbers the free DOFs, src/LinAlg/SAM.C:225-336)."""
+    msc = np.asarray(msc, dtype=np.int32).copy()
+    values = np.asarray(values, dtype=np.float64)
+    pres = np.nonzero(np.isfinite(values))[0]
+    msc[pres] = 0
+    meqn, neq = equation_numbers(msc)
+    meqn[pres] = -np.arange(1, len(pres) + 1, dtype=np.int32)
+    mpmceq = np.arange(1, len(pres) + 2, dtype=np.int32)
+    return SamArrays(meqn, neq, mpmceq, (pres + 1).astype(np.int32),
+                     values[pres].astype(np.float64))
 
 
 # ---- integrand descriptions (same numbering as the C ABI) -----------------

@@ -65,11 +65,25 @@ int ifem_gpu_patch_create(ifem_gpu_ctx* ctx, const int order[3], const int ncoef
  * Equation numbers per DOF, node-major / component-minor (DOF = node*ncomp+c),
  * 1-based; 0 = eliminated (homogeneous Dirichlet, msc = 0).  Free DOFs must be
  * numbered in increasing DOF order, as SAM::initSystemEquations does
- * (src/LinAlg/SAM.C:319-323); constrained (negative) equations are not
- * supported.  Replaces SAM::initSystemEquations/getElmEqns
+ * (src/LinAlg/SAM.C:319-323); constrained (negative) equations need
+ * ifem_gpu_sam_bind_constraints.  Replaces SAM::initSystemEquations/getElmEqns
  * (src/LinAlg/SAM.C:225-336, 655-696) for the patch. */
 int ifem_gpu_sam_bind(ifem_gpu_ctx* ctx, int patch_id, int ncomp, int ndof,
                       const int* meqn, int neq);
+
+/* Same with constraint equations in SAM's layout (src/LinAlg/SAM.h:330-350,
+ * src/LinAlg/SAM.C:260-308): meqn[i] = -k marks DOF i as the slave of
+ * constraint k (1-based); its entries are mpmceq[k-1] .. mpmceq[k]-1 (1-based)
+ * of mmceq (DOF numbers, the slave first) and ttcc (the constant c0 first).
+ * Only prescribed values are supported (constraints without master DOFs,
+ * mpmceq[k]-mpmceq[k-1] == 1; else IFEM_GPU_EUNSUPPORTED): the assembly moves
+ * the eliminated columns to the right-hand side, b_i -= K_ij c0_j, as
+ * SparseMatrix::assemble does (src/LinAlg/SparseMatrix.C:614-688), and
+ * ifem_gpu_expand_solution / ifem_gpu_integrate_norms put c0 at those DOFs
+ * (SAM::expandSolution, src/LinAlg/SAM.C:768-810). */
+int ifem_gpu_sam_bind_constraints(ifem_gpu_ctx* ctx, int patch_id, int ncomp, int ndof,
+                                  const int* meqn, int neq, int nceq, const int* mpmceq,
+                                  const int* mmceq, const double* ttcc);
 
 /* ---- symbolic assembly -----------------------------------------------------
  * Builds the CSR sparsity pattern (symmetric, full storage, sorted columns)
@@ -193,8 +207,8 @@ int ifem_gpu_get_system(ifem_gpu_ctx* ctx, int64_t* rowptr, int32_t* colind,
 int ifem_gpu_get_values_fp32(ifem_gpu_ctx* ctx, float* values);
 /* Solution in equation order (neq). */
 int ifem_gpu_get_solution(ifem_gpu_ctx* ctx, double* x);
-/* Solution expanded to DOF order, eliminated DOFs = 0
- * (SAM::expandSolution, src/LinAlg/SAM.C:768-810). */
+/* Solution expanded to DOF order, eliminated DOFs = 0 or their prescribed
+ * value (SAM::expandSolution with scaleSD = 1, src/LinAlg/SAM.C:768-810). */
 int ifem_gpu_expand_solution(ifem_gpu_ctx* ctx, double* xdof);
 /* y = K x with host vectors (SparseMatrix::multiply,
  * src/LinAlg/SparseMatrix.C:578-605). */

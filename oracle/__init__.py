@@ -58,6 +58,11 @@ def lib():
                                    C.c_int, _lp, _ip, _dp, _dp, C.c_int]
         L.orc_assemble_range.argtypes = [C.POINTER(_Patch), C.POINTER(_Integrand), _ip, C.c_int,
                                          _ip, C.c_int, _lp, _ip, _dp, _dp, C.c_int, C.c_int, C.c_int]
+        L.orc_assemble_range_g.argtypes = [C.POINTER(_Patch), C.POINTER(_Integrand), _ip,
+                                           C.c_int, _ip, C.c_int, _lp, _ip, _dp, _dp, C.c_int,
+                                           C.c_int, C.c_int, C.c_void_p]
+        L.orc_expand_solution_g.argtypes = [C.c_int, _ip, _dp, _dp, _dp]
+        L.orc_expand_solution_g.restype = None
         L.orc_assemble_traction.argtypes = [C.POINTER(_Patch), C.c_int, _dp, C.c_int, C.c_int,
                                             _ip, _dp]
         L.orc_spmv.argtypes = [C.c_int, _lp, _ip, _dp, _dp, _dp]
@@ -191,13 +196,22 @@ def pattern(P, ncomp: int, meqn: np.ndarray, neq: int):
 
 
 def assemble(P, itg, ng, meqn, neq, rowptr, colind, nthreads: int = 0,
-             e3_range: Optional[tuple] = None):
+             e3_range: Optional[tuple] = None, gdof=None):
+    """K and b over the patch (or elements e3_range of the third direction).
+    gdof: prescribed value per DOF, read where meqn <= 0; the constrained
+    columns then move to the right-hand side (src/LinAlg/SparseMatrix.C:639-661)."""
     keep = _Keep()
     sp = _patch(P, keep); si = _integrand(itg, keep)
     meqn = np.ascontiguousarray(meqn, dtype=np.int32)
     A = np.zeros(int(rowptr[-1])); b = np.zeros(neq)
     ngv = np.asarray(ng, dtype=np.int32)
-    if e3_range is None:
+    if gdof is not None:
+        g = np.ascontiguousarray(gdof, dtype=np.float64)
+        assert g.shape == meqn.shape
+        lo, hi = e3_range if e3_range is not None else (0, 1 << 30)
+        rc = lib().orc_assemble_range_g(C.byref(sp), C.byref(si), ngv, itg.ncomp, meqn, neq,
+                                        rowptr, colind, A, b, nthreads, lo, hi, g.ctypes.data)
+    elif e3_range is None:
         rc = lib().orc_assemble(C.byref(sp), C.byref(si), ngv, itg.ncomp, meqn, neq,
                                 rowptr, colind, A, b, nthreads)
     else:
@@ -206,6 +220,17 @@ def assemble(P, itg, ng, meqn, neq, rowptr, colind, nthreads: int = 0,
     if rc:
         raise RuntimeError("orc_assemble failed")
     return A, b
+
+
+def expand_solution(meqn, xeq, gdof=None):
+    """Equation vector -> DOF vector: fixed DOFs 0, or their prescribed value
+    from gdof (SAM::expandSolution, src/LinAlg/SAM.C:768-810)."""
+    meqn = np.ascontiguousarray(meqn, dtype=np.int32)
+    g = np.zeros(len(meqn)) if gdof is None else np.ascontiguousarray(gdof, dtype=np.float64)
+    out = np.zeros(len(meqn))
+    lib().orc_expand_solution_g(len(meqn), meqn, np.ascontiguousarray(xeq, dtype=np.float64),
+                                g, out)
+    return out
 
 
 def add_traction(P, face: int, t, ng: int, ncomp: int, meqn, b):

@@ -479,10 +479,21 @@ static inline double* sm_entry(const int64_t* rowptr, const int32_t* colind,
   return NULL;
 }
 
+/* With prescribed values (constraint equations without master DOFs,
+ * meqn < 0, value c0 = ttcc of the constraint's first entry): the constrained
+ * columns move to the right-hand side, b_i -= c0_j * EK(i, j) for free rows i
+ * (src/LinAlg/SparseMatrix.C:639-661).  gval[j] holds c0 of element DOF j
+ * (0 for fixed or free DOFs); NULL means none. */
 static int assem_element(int nedof, const double* EK, const double* EV,
                          const int* meen, const int64_t* rowptr,
-                         const int32_t* colind, double* A, double* b)
+                         const int32_t* colind, double* A, double* b, const double* gval)
 {
+  if (gval)
+    for (int j = 0; j < nedof; j++) {
+      if (meen[j] >= 1 || gval[j] == 0.0) continue;
+      for (int i = 0; i < nedof; i++)
+        if (meen[i] > 0) b[meen[i] - 1] -= gval[j] * EK[i * nedof + j];
+    }
   for (int j = 0; j < nedof; j++) {
     int jeq = meen[j];
     if (jeq < 1) continue;
@@ -529,6 +540,16 @@ int orc_assemble_range(const orc_patch* P, const orc_integrand* itg,
                        const int64_t* rowptr, const int32_t* colind,
                        double* A, double* b, int nthreads,
                        int e3_begin, int e3_end)
+{
+  return orc_assemble_range_g(P, itg, ng, ncomp, meqn, neq, rowptr, colind, A, b, nthreads,
+                              e3_begin, e3_end, NULL);
+}
+
+int orc_assemble_range_g(const orc_patch* P, const orc_integrand* itg,
+                         const int ng[3], int ncomp, const int* meqn, int neq,
+                         const int64_t* rowptr, const int32_t* colind,
+                         double* A, double* b, int nthreads,
+                         int e3_begin, int e3_end, const double* gdof)
 {
   (void)neq;
   int nel[3];
@@ -582,6 +603,7 @@ int orc_assemble_range(const orc_patch* P, const orc_integrand* itg,
       double* EK = (double*)malloc(sizeof(double) * nedof * nedof);
       double* EV = (double*)malloc(sizeof(double) * nedof);
       int* meen = (int*)malloc(sizeof(int) * nedof);
+      double* gval = gdof ? (double*)malloc(sizeof(double) * nedof) : NULL;
       int mnpc[ORC_MAXNEN];
 #pragma omp for schedule(static)
       for (int t = 0; t < threads; t++) {
@@ -595,13 +617,17 @@ int orc_assemble_range(const orc_patch* P, const orc_integrand* itg,
               if (orc_element_matrices(P, itg, ng, e1, e2, e3, EK, EV)) { err = 1; continue; }
               orc_element_nodes(P, e1, e2, e3, mnpc);
               for (int a = 0; a < nen; a++)
-                for (int c = 0; c < ncomp; c++) meen[a * ncomp + c] = meqn[mnpc[a] * ncomp + c];
-              if (assem_element(nedof, EK, EV, meen, rowptr, colind, A, b)) err = 1;
+                for (int c = 0; c < ncomp; c++) {
+                  meen[a * ncomp + c] = meqn[mnpc[a] * ncomp + c];
+                  if (gval) gval[a * ncomp + c] = gdof[mnpc[a] * ncomp + c];
+                }
+              if (assem_element(nedof, EK, EV, meen, rowptr, colind, A, b, gval)) err = 1;
             }
       }
       free(EK);
       free(EV);
       free(meen);
+      free(gval);
     }
   }
   free(lo);
@@ -817,6 +843,14 @@ void orc_pcg_jacobi_fixed(int n, const int64_t* rowptr, const int32_t* colind,
 void orc_expand_solution(int ndof, const int* meqn, const double* xeq, double* xdof)
 {
   for (int i = 0; i < ndof; i++) xdof[i] = meqn[i] > 0 ? xeq[meqn[i] - 1] : 0.0;
+}
+
+/* With prescribed values: constrained DOFs take c0 (SAM::expandSolution,
+ * src/LinAlg/SAM.C:768-810, for constraints without master DOFs). */
+void orc_expand_solution_g(int ndof, const int* meqn, const double* xeq, const double* gdof,
+                           double* xdof)
+{
+  for (int i = 0; i < ndof; i++) xdof[i] = meqn[i] > 0 ? xeq[meqn[i] - 1] : gdof[i];
 }
 
 /* Gradient of a field (analytic derivative of orc_eval_function). */

@@ -99,6 +99,15 @@ int orc_assemble_range(const orc_patch* P, const orc_integrand* itg,
                        double* A, double* b, int nthreads,
                        int e3_begin, int e3_end);
 
+/* Same with prescribed values gdof (per DOF; used where meqn <= 0): the
+ * constrained columns go to the right-hand side, b_i -= g_j K_ij
+ * (src/LinAlg/SparseMatrix.C:639-661).  gdof NULL: homogeneous. */
+int orc_assemble_range_g(const orc_patch* P, const orc_integrand* itg,
+                         const int ng[3], int ncomp, const int* meqn, int neq,
+                         const int64_t* rowptr, const int32_t* colind,
+                         double* A, double* b, int nthreads,
+                         int e3_begin, int e3_end, const double* gdof);
+
 /* a21: Neumann traction t on face (0..5 as above), ng points per tangent
  * direction, added into b. */
 int orc_assemble_traction(const orc_patch* P, int face, const double t[3],
@@ -119,8 +128,11 @@ int orc_pcg_jacobi(int n, const int64_t* rowptr, const int32_t* colind,
 void orc_pcg_jacobi_fixed(int n, const int64_t* rowptr, const int32_t* colind,
                           const double* A, const double* b, double* x, int iters);
 
-/* a20: equation vector -> DOF vector (fixed DOFs = 0). */
+/* a20: equation vector -> DOF vector (fixed DOFs = 0; with gdof the
+ * prescribed values). */
 void orc_expand_solution(int ndof, const int* meqn, const double* xeq, double* xdof);
+void orc_expand_solution_g(int ndof, const int* meqn, const double* xeq, const double* gdof,
+                           double* xdof);
 
 /* Field evaluation shared by all integrands, and its gradient. */
 double orc_eval_function(const orc_function* f, const double X[3]);

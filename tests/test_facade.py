@@ -46,5 +46,9 @@ def test_facade_driver_matches_oracle():
     assert e64["rel_residual"] <= 1e-8 and e32["rel_residual"] <= 1e-8
     assert abs(e32["min_uz"] - e64["min_uz"]) <= 1e-6 * abs(e64["min_uz"])
     assert out[("gmres_refused", "")]["ok"] is True
+    ua = out[("uniaxial", "")]
+    n = 8  # control points per direction at nel = 6, p = 2
+    assert ua["nceq"] == 2 * n * n and ua["neq"] == 3 * n ** 3 - 4 * n * n
+    assert ua["max_err"] <= 1e-11 and ua["rel_residual"] <= 1e-12
     g2 = out[("g2", "")]
     assert g2["n"] == 18 and g2["knot_diff"] <= 1e-15 and g2["coef_diff"] <= 1e-14

@@ -226,3 +226,113 @@ def test_norms_match_oracle():
         assert np.all(np.abs(n - no) <= 1e-12 * np.maximum(np.abs(no), 1e-300)), (n, no)
         if ex is not None:
             assert abs(n[4] - 2.26239829e-05) <= 1e-6 * n[4]  # O(h^3) error of u_h at 16^3
+
+
+# ---- prescribed values (constraint equations without masters) -----------------
+def _prescribed_case(pb, seed, frac=0.6):
+    """Random values on a random part of the boundary, the rest fixed at 0."""
+    P, nc = pb.patch, pb.ncomp
+    n = P.n
+    k, j, i = np.meshgrid(range(n[2]), range(n[1]), range(n[0]), indexing="ij")
+    bnd = ((i == 0) | (i == n[0] - 1) | (j == 0) | (j == n[1] - 1) | (k == 0)
+           | (k == n[2] - 1)).ravel()
+    bnd = np.repeat(bnd, nc)
+    rng = np.random.default_rng(seed)
+    pick = bnd & (rng.random(P.nnod * nc) < frac)
+    vals = np.where(pick, rng.standard_normal(P.nnod * nc), np.nan)
+    msc = np.where(bnd & ~pick, 0, 1).astype(np.int32)
+    return M.prescribed_equations(msc, vals)
+
+
+def _setup_prescribed(pb, sam, precision=FP64):
+    from ifem_amd import Session
+    s = Session()
+    pid = s.add_patch(pb.patch)
+    s.bind_constraints(pid, pb.ncomp, sam)
+    s.init_assembly()
+    s.assemble(pb.integrand, pb.ng, FP64)
+    if precision == FP32:
+        s.assemble(pb.integrand, pb.ng, FP32)
+    return s
+
+
+@pytest.mark.parametrize("name,make", [
+    ("diffusion6p2", lambda: problems.diffusion_cube(6, 2, sigma=1.0, ell=0.2, seed=3)),
+    ("poisson5p3", lambda: problems.poisson_cube(5, 3, jitter=0.1, seed=2)),
+    ("elast5p2", lambda: problems.elasticity_cube(5, 2, seed=8)),
+    ("elast4p1", lambda: problems.elasticity_cube(4, 1, seed=1)),
+])
+def test_prescribed_values_match_oracle(name, make):
+    """K, b with prescribed boundary values (b_i -= K_ij g_j for the eliminated
+    columns), the FP64 solution, its expansion (g at prescribed DOFs) and the
+    norms of the expanded solution, against the oracle at 1e-12."""
+    pb = make()
+    sam = _prescribed_case(pb, seed=len(name))
+    g = sam.prescribed_values()
+    _say(f"[gpu] prescribed values {name} neq={sam.neq} nceq={sam.nceq}")
+    rp, ci = oracle.pattern(pb.patch, pb.ncomp, sam.meqn, sam.neq)
+    A, b = oracle.assemble(pb.patch, pb.integrand, pb.ng, sam.meqn, sam.neq, rp, ci, gdof=g)
+    with _setup_prescribed(pb, sam) as s:
+        grp, gci, gA, gb = s.system()
+        assert np.array_equal(grp, rp) and np.array_equal(gci, ci)
+        assert np.max(np.abs(gA - A)) <= 1e-12 * np.max(np.abs(A))
+        assert np.max(np.abs(gb - b)) <= 1e-12 * np.max(np.abs(b))
+        st = s.solve(rtol=1e-13, maxits=20000)
+        assert st["converged"]
+        xd = s.expanded_solution(pb.ndof)
+        n = s.norms(pb.integrand, pb.ng)
+    xo, it, res = oracle.pcg(rp, ci, A, b, rtol=1e-13)
+    xdo = oracle.expand_solution(sam.meqn, xo, g)
+    assert np.max(np.abs(xd - xdo)) <= 1e-10 * np.max(np.abs(xdo))
+    assert np.array_equal(xd[sam.meqn < 0], g[sam.meqn < 0])
+    no = oracle.norms(pb.patch, pb.integrand, pb.ng, xd)
+    assert np.all(np.abs(n - no) <= 1e-12 * np.maximum(np.abs(no), 1e-300)), (n, no)
+
+
+def test_prescribed_uniaxial_elasticity_closed_form_mixed_precision():
+    """u_x = 0 / d on x = 0 / 1, symmetry planes y = 0, z = 0, traction free
+    elsewhere: the exact u = (d x, -nu d y, -nu d z) is in the discrete space.
+    FP64 PCG and FP32-inner mixed precision both reproduce it at every
+    control point (16^3 p=2 jittered)."""
+    E, nu, d = 1.0, 0.3, 0.01
+    P = M.cube_patch(2, 16, jitter=0.15, seed=7)
+    itg = M.Integrand(M.ELASTICITY, E=E, nu=nu)
+    n = P.n
+    k, j, i = [a.ravel() for a in np.meshgrid(range(n[2]), range(n[1]), range(n[0]), indexing="ij")]
+    vals = np.full((P.nnod, 3), np.nan)
+    vals[i == 0, 0] = 0.0
+    vals[i == n[0] - 1, 0] = d
+    vals[j == 0, 1] = 0.0
+    vals[k == 0, 2] = 0.0
+    sam = M.prescribed_equations(np.ones(P.nnod * 3, dtype=np.int32), vals.ravel())
+    pb = problems.Problem("uniaxial16", P, itg, (3, 3, 3), sam.meqn, sam.neq)
+    X = P.coefs
+    ue = np.stack([d * X[:, 0], -nu * d * X[:, 1], -nu * d * X[:, 2]], axis=1).ravel()
+    _say(f"[gpu] uniaxial elasticity 16^3 neq={sam.neq}")
+    with _setup_prescribed(pb, sam, precision=FP32) as s:
+        st = s.solve(rtol=1e-12, maxits=20000)
+        assert st["converged"]
+        x64 = s.expanded_solution(pb.ndof)
+        st = s.solve(rtol=1e-12, maxits=20000, precision=FP32, inner_rtol=1e-4)
+        assert st["converged"]
+        x32 = s.expanded_solution(pb.ndof)
+    assert np.max(np.abs(x64 - ue)) <= 1e-10 * d
+    assert np.max(np.abs(x32 - ue)) <= 1e-10 * d
+
+
+def test_constraints_with_masters_are_rejected():
+    from ifem_amd import GpuError, Session
+    pb = problems.poisson_cube(3)
+    sam = _prescribed_case(pb, seed=5)
+    # give the first constraint a master DOF: u_slave = c0 + 1.0 * u_master
+    master = int(np.nonzero(sam.meqn > 0)[0][0]) + 1
+    bad = M.SamArrays(sam.meqn, sam.neq, np.concatenate([[1], sam.mpmceq[1:] + 1]).astype(np.int32),
+                      np.insert(sam.mmceq, 1, master).astype(np.int32),
+                      np.insert(sam.ttcc, 1, 1.0))
+    with Session() as s:
+        pid = s.add_patch(pb.patch)
+        with pytest.raises(GpuError):
+            s.bind_constraints(pid, 1, bad)
+        with pytest.raises(GpuError):
+            s.bind(pid, 1, sam.meqn, sam.neq)  # negative meqn without the arrays
+        s.bind_constraints(pid, 1, sam)

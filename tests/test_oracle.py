@@ -5,6 +5,7 @@ own code: the reference's golden values and known-answer tests, the
 reference's own compiled functions (oracle/_ref, via committed fixtures),
 library routines (scipy BSpline, spsolve) and closed forms.
 """
+import dataclasses
 import json
 import math
 import os
@@ -547,3 +548,108 @@ def test_manufactured_poisson_norms_closed_form_orthogonality_and_rates():
     for (l0, e0), (l1, e1) in zip(errs, errs[1:]):
         assert 2.7 <= np.log2(l0 / l1) <= 3.4, (l0, l1)
         assert 1.8 <= np.log2(e0 / e1) <= 2.2, (e0, e1)
+
+
+# ---- prescribed values (constraint equations without masters) ------------------
+def _boundary_mask(P):
+    n = P.n
+    k, j, i = np.meshgrid(range(n[2]), range(n[1]), range(n[0]), indexing="ij")
+    return ((i == 0) | (i == n[0] - 1) | (j == 0) | (j == n[1] - 1) | (k == 0)
+            | (k == n[2] - 1)).ravel()
+
+
+def _solve_prescribed(P, itg, ng, sam, rtol=1e-14):
+    rp, ci = oracle.pattern(P, itg.ncomp, sam.meqn, sam.neq)
+    A, b = oracle.assemble(P, itg, ng, sam.meqn, sam.neq, rp, ci,
+                           gdof=sam.prescribed_values())
+    x, it, res = oracle.pcg(rp, ci, A, b, rtol=rtol)
+    return oracle.expand_solution(sam.meqn, x, sam.prescribed_values()), (rp, ci, A, b)
+
+
+def test_sam_constraint_arrays_layout():
+    """Prescribed DOFs become slaves -1..-m in DOF order, one entry each
+    (slave DOF, c0), free DOFs numbered 1..neq in DOF order."""
+    msc = np.array([1, 0, 1, 1, 1, 0], dtype=np.int32)
+    vals = np.array([np.nan, np.nan, 2.5, np.nan, -1.0, np.nan])
+    s = M.prescribed_equations(msc, vals)
+    assert s.meqn.tolist() == [1, 0, -1, 2, -2, 0] and s.neq == 2
+    assert s.mpmceq.tolist() == [1, 2, 3] and s.mmceq.tolist() == [3, 5]
+    assert s.ttcc.tolist() == [2.5, -1.0]
+    assert s.prescribed_values().tolist() == [0, 0, 2.5, 0, -1.0, 0]
+    x = oracle.expand_solution(s.meqn, np.array([7.0, 8.0]), s.prescribed_values())
+    assert x.tolist() == [7.0, 0.0, 2.5, 8.0, -1.0, 0.0]
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_prescribed_linear_poisson_patch_test(p):
+    """-Laplace u = 0 with u = a + b.x prescribed on the whole boundary: the
+    isoparametric spline space holds u exactly, so the Galerkin solution is
+    u at every control point (jittered interior).  The residual of a test
+    function is sum_q w_q cof(J)^T grad_xi N_i . b, a polynomial of degree
+    3p-1 per direction, which p+1 Gauss points integrate exactly for p <= 2;
+    p = 3 needs 5 points."""
+    P = M.cube_patch(p, 5, jitter=0.2, seed=3)
+    ng = p + 1 if p < 3 else 5
+    itg = M.Integrand(M.POISSON)
+    a0, bv = 0.7, np.array([1.5, -2.0, 0.25])
+    u = a0 + P.coefs @ bv
+    bnd = _boundary_mask(P)
+    sam = M.prescribed_equations(np.ones(P.nnod, dtype=np.int32), np.where(bnd, u, np.nan))
+    xd, _ = _solve_prescribed(P, itg, [ng] * 3, sam)
+    assert np.max(np.abs(xd - u)) < 1e-10
+
+
+def test_prescribed_elasticity_uniaxial_closed_form():
+    """Elasticity, u_x = 0 on x = 0 and u_x = d on x = 1, symmetry planes
+    u_y = 0 on y = 0, u_z = 0 on z = 0, traction free elsewhere: the exact
+    solution u = (d x, -nu d y, -nu d z) is linear, so the discrete one
+    equals it at every control point."""
+    E, nu, d = 2.0, 0.3, 0.01
+    P = M.cube_patch(2, 4, jitter=0.2, seed=5)
+    itg = M.Integrand(M.ELASTICITY, E=E, nu=nu)
+    n = P.n
+    k, j, i = [a.ravel() for a in np.meshgrid(range(n[2]), range(n[1]), range(n[0]), indexing="ij")]
+    vals = np.full((P.nnod, 3), np.nan)
+    vals[i == 0, 0] = 0.0
+    vals[i == n[0] - 1, 0] = d
+    vals[j == 0, 1] = 0.0
+    vals[k == 0, 2] = 0.0
+    sam = M.prescribed_equations(np.ones(P.nnod * 3, dtype=np.int32), vals.ravel())
+    xd, _ = _solve_prescribed(P, itg, [3, 3, 3], sam)
+    X = P.coefs
+    ue = np.stack([d * X[:, 0], -nu * d * X[:, 1], -nu * d * X[:, 2]], axis=1).ravel()
+    assert np.max(np.abs(xd - ue)) < 1e-12
+
+
+def test_prescribed_rhs_equals_full_system_partition():
+    """b_f - K_fc g from the unconstrained system (every DOF free, the already
+    pinned homogeneous assembly) equals the right-hand side assembled with
+    the constrained columns moved over element by element; K_ff likewise."""
+    rng = np.random.default_rng(12)
+    for pb in (problems.diffusion_cube(4, 2, sigma=1.0, ell=0.3, seed=4),
+               problems.elasticity_cube(3, 3, seed=9)):
+        P, itg, nc = pb.patch, pb.integrand, pb.ncomp
+        if nc == 3:
+            itg = dataclasses.replace(itg, body=(0.1, -0.2, 0.3))
+        ndof = P.nnod * nc
+        meqn_all, neq_all = _free_all(P, nc)
+        rp, ci, A, b = oracle.system(P, itg, pb.ng, meqn_all, neq_all)
+        K = sp.csr_matrix((A, ci, rp), shape=(neq_all, neq_all)).toarray()
+        bnd = np.repeat(_boundary_mask(P), nc)
+        pick = bnd & (rng.random(ndof) < 0.6)
+        fixed = bnd & ~pick
+        vals = np.where(pick, rng.standard_normal(ndof), np.nan)
+        msc = np.where(fixed, 0, 1).astype(np.int32)
+        sam = M.prescribed_equations(msc, vals)
+        g = sam.prescribed_values()
+        rp2, ci2 = oracle.pattern(P, nc, sam.meqn, sam.neq)
+        A2, b2 = oracle.assemble(P, itg, pb.ng, sam.meqn, sam.neq, rp2, ci2, gdof=g)
+        free = sam.meqn > 0
+        bexp = b[free] - K[np.ix_(free, pick)] @ g[pick]
+        assert np.max(np.abs(b2 - bexp)) <= 1e-13 * max(1.0, np.abs(bexp).max())
+        K2 = sp.csr_matrix((A2, ci2, rp2), shape=(sam.neq, sam.neq)).toarray()
+        assert np.max(np.abs(K2 - K[np.ix_(free, free)])) <= 1e-15 * np.abs(K).max()
+        # gdof = 0 reproduces the homogeneous assembly exactly
+        A3, b3 = oracle.assemble(P, itg, pb.ng, sam.meqn, sam.neq, rp2, ci2, gdof=np.zeros(ndof))
+        A4, b4 = oracle.assemble(P, itg, pb.ng, sam.meqn, sam.neq, rp2, ci2)
+        assert np.array_equal(A3, A4) and np.array_equal(b3, b4)

@@ -21,7 +21,10 @@ def setup_problem(pb, session=None, precision=FP64):
     """Upload a Problem, build the pattern and assemble K, b (and the traction)."""
     s = session or Session()
     pid = s.add_patch(pb.patch)
-    s.bind(pid, pb.ncomp, pb.meqn, pb.neq)
+    if pb.sam is not None:
+        s.bind_constraints(pid, pb.ncomp, pb.sam)
+    else:
+        s.bind(pid, pb.ncomp, pb.meqn, pb.neq)
     s.init_assembly()
     s.assemble(pb.integrand, pb.ng, FP64)
     if pb.traction is not None:

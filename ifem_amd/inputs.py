@@ -16,7 +16,8 @@ the ``.xinp`` subset that describes the hot path's models.
   IFEM inputs do).
 * ``read_xinp``: ``<geometry>`` (``<patch>``/``<patchfile>``, ``<refine>``,
   ``<raiseorder>``, ``<topologysets>`` with face sets),
-  ``<boundaryconditions>`` (homogeneous ``<dirichlet>`` on face sets,
+  ``<boundaryconditions>`` (``<dirichlet>`` on face sets, homogeneous or
+  with a constant value,
   constant ``<neumann>`` traction), ``<discretization><nGauss>``,
   ``<linearsolver>`` keys, and the physics: ``<poisson>`` (conductivity,
   source) or ``<elasticity>`` (``<isotropic E nu>``), following the tag
@@ -254,6 +255,7 @@ def read_xinp(source: str, base_dir: Optional[str] = None) -> Tuple[Problem, Dic
     nc = itg.ncomp
     # boundary conditions: homogeneous Dirichlet on faces, constant traction
     fix = [0] * 6
+    inhom = []  # (face, mask, value) of constant inhomogeneous conditions
     traction = None
     bc = root.find("boundaryconditions")
     if bc is not None:
@@ -265,8 +267,18 @@ def read_xinp(source: str, base_dir: Optional[str] = None) -> Tuple[Problem, Dic
                 c = int(ch) - 1
                 if 0 <= c < nc:
                     mask |= 1 << c
+            # a nonzero constant value makes the condition inhomogeneous
+            # (Property::DIRICHLET_INHOM, src/SIM/SIMinput.C:576-618)
+            typ = (d.get("type") or "").lower()
+            text = (d.text or "").strip()
+            if typ in ("anasol", "expression", "field") or typ.startswith("symm"):
+                raise ValueError(f"<dirichlet type='{typ}'> is not supported")
+            value = float(text) if text else 0.0
             for f in faces:
-                fix[f - 1] |= mask
+                if value != 0.0:
+                    inhom.append((f - 1, mask, value))
+                else:
+                    fix[f - 1] |= mask
         for nm in bc.findall("neumann"):
             faces = sets.get(nm.get("set"), [])
             if len(faces) != 1 or traction is not None:
@@ -276,7 +288,16 @@ def read_xinp(source: str, base_dir: Optional[str] = None) -> Tuple[Problem, Dic
             t[direction - 1] = float((nm.text or "0").strip())
             traction = (faces[0] - 1, tuple(t), None)
     msc = M.face_status(P, nc, fix)
-    meqn, neq = M.equation_numbers(msc)
+    sam = None
+    if inhom:
+        vals = np.full(P.nnod * nc, np.nan)
+        for f, mask, value in inhom:  # later conditions win on shared DOFs
+            on = M.face_status(P, nc, [mask if g == f else 0 for g in range(6)]) == 0
+            vals[on] = value
+        sam = M.prescribed_equations(msc, vals)
+        meqn, neq = sam.meqn, sam.neq
+    else:
+        meqn, neq = M.equation_numbers(msc)
     disc = root.find("discretization")
     ng_el = disc.find("nGauss") if disc is not None else None
     ngv = int(ng_el.text) if ng_el is not None else P.p[0] + 1
@@ -288,7 +309,7 @@ def read_xinp(source: str, base_dir: Optional[str] = None) -> Tuple[Problem, Dic
     if ls is not None:
         for ch in ls:
             solver[ch.tag.lower()] = (ch.text or "").strip()
-    return Problem("xinp", P, itg, ng, meqn, neq, traction), solver
+    return Problem("xinp", P, itg, ng, meqn, neq, traction, sam), solver
 
 
 def solver_kwargs(solver: Dict[str, str]) -> dict:

@@ -184,15 +184,17 @@ class SamArrays:
 
 
 def prescribed_equations(msc: np.ndarray, values: np.ndarray) -> SamArrays:
-    """SAM arrays for status codes 0/1 plus prescribed values: every DOF with
-    a finite ``values`` entry becomes the slave of a constraint equation
+    """SAM arrays for status codes 0/1 plus prescribed values: every free DOF
+    with a finite ``values`` entry becomes the slave of a constraint equation
     without masters, u_i = c0 (ASMbase::addMPC + SAMpatch::initConstraintEqs,
     src/ASM/ASMbase.C:527-560, src/ASM/SAMpatch.C:259-357: constraints in
     slave-DOF order, status of the slaves reset to 0, then
-    SAM::initSystemEquations numbers the free DOFs, src/LinAlg/SAM.C:225-336)."""
+    SAM::initSystemEquations numbers the free DOFs, src/LinAlg/SAM.C:225-336).
+    A value on a DOF that is already fixed (msc 0) is ignored, as addMPC
+    ignores constraints on FIXED DOFs."""
     msc = np.asarray(msc, dtype=np.int32).copy()
     values = np.asarray(values, dtype=np.float64)
-    pres = np.nonzero(np.isfinite(values))[0]
+    pres = np.nonzero(np.isfinite(values) & (msc == 1))[0]
     msc[pres] = 0
     meqn, neq = equation_numbers(msc)
     meqn[pres] = -np.arange(1, len(pres) + 1, dtype=np.int32)

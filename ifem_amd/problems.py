@@ -32,6 +32,12 @@ class Problem:
     meqn: np.ndarray
     neq: int
     traction: Optional[Tuple[int, Tuple[float, float, float], int]] = None
+    # SAM arrays with prescribed values (meqn < 0 at their DOFs), or None
+    sam: Optional[M.SamArrays] = None
+
+    def prescribed_values(self) -> np.ndarray:
+        """Per DOF prescribed value (0 where none)."""
+        return self.sam.prescribed_values() if self.sam is not None else np.zeros(self.ndof)
 
     @property
     def ncomp(self) -> int:

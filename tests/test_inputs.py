@@ -7,6 +7,7 @@ import math
 import numpy as np
 import pytest
 
+import oracle
 from ifem_amd import inputs, model as M, problems
 
 TRILINEAR_CUBE = """700 1 0 0
@@ -156,3 +157,62 @@ def test_xinp_model_solves_to_the_manufactured_error():
         assert st["converged"]
         n = s.norms(pb.integrand, pb.ng, M.Function(M.FN_SINPROD, 1.0))
     assert abs(n[4] - 2.26239829e-05) <= 1e-5 * n[4]
+
+
+UNIAXIAL_XINP = f"""<simulation>
+  <geometry>
+    <patch>{TRILINEAR_CUBE}</patch>
+    <raiseorder patch="1" u="1" v="1" w="1"/>
+    <refine patch="1" u="2" v="2" w="2"/>
+    <topologysets>
+      <set name="Left" type="face"><item patch="1">1</item></set>
+      <set name="Right" type="face"><item patch="1">2</item></set>
+      <set name="Front" type="face"><item patch="1">3</item></set>
+      <set name="Bottom" type="face"><item patch="1">5</item></set>
+    </topologysets>
+  </geometry>
+  <boundaryconditions>
+    <dirichlet set="Left" comp="1"/>
+    <dirichlet set="Right" comp="1">0.01</dirichlet>
+    <dirichlet set="Front" comp="2"/>
+    <dirichlet set="Bottom" comp="3"/>
+  </boundaryconditions>
+  <elasticity><isotropic E="1" nu="0.3"/></elasticity>
+</simulation>
+"""
+
+
+def _uniaxial_exact(pb, d=0.01, nu=0.3):
+    X = pb.patch.coefs
+    return np.stack([d * X[:, 0], -nu * d * X[:, 1], -nu * d * X[:, 2]], axis=1).ravel()
+
+
+def test_xinp_inhomogeneous_dirichlet_uniaxial_strain():
+    """A nonzero constant <dirichlet> value becomes one constraint equation
+    per face DOF (u_x = 0.01 on x = 1); with the symmetry planes the exact
+    uniaxial-strain field is linear, so the oracle's solution equals it."""
+    pb, _ = inputs.read_xinp(UNIAXIAL_XINP)
+    n = pb.patch.n[0]
+    assert pb.sam is not None and pb.sam.nceq == n * n
+    assert np.all(pb.sam.ttcc == 0.01)
+    assert pb.neq == 3 * n ** 3 - 4 * n * n
+    rp, ci = oracle.pattern(pb.patch, 3, pb.meqn, pb.neq)
+    g = pb.prescribed_values()
+    A, b = oracle.assemble(pb.patch, pb.integrand, pb.ng, pb.meqn, pb.neq, rp, ci, gdof=g)
+    x, it, res = oracle.pcg(rp, ci, A, b, rtol=1e-14)
+    xd = oracle.expand_solution(pb.meqn, x, g)
+    assert np.max(np.abs(xd - _uniaxial_exact(pb))) <= 1e-13
+    with pytest.raises(ValueError):
+        inputs.read_xinp(UNIAXIAL_XINP.replace('comp="1">0.01', 'comp="1" type="expression">x'))
+
+
+@pytest.mark.gpu
+def test_xinp_inhomogeneous_dirichlet_on_gpu():
+    from ifem_amd import setup_problem
+    pb, _ = inputs.read_xinp(UNIAXIAL_XINP)
+    print("[gpu] xinp uniaxial (prescribed values)", flush=True)
+    with setup_problem(pb) as s:
+        st = s.solve(rtol=1e-13, maxits=5000)
+        assert st["converged"]
+        xd = s.expanded_solution(pb.ndof)
+    assert np.max(np.abs(xd - _uniaxial_exact(pb))) <= 1e-12

@@ -11,10 +11,10 @@ Package layout:
   build.py     hipcc / g++ build recipe
 """
 from . import model, problems
-from ._native import FP32, FP64, GpuError, Session, lib
+from ._native import FP32, FP64, PC_JACOBI, PC_MG, GpuError, Session, lib
 
-__all__ = ["model", "problems", "Session", "GpuError", "FP32", "FP64", "lib", "setup_problem",
-           "setup_batch"]
+__all__ = ["model", "problems", "Session", "GpuError", "FP32", "FP64", "PC_JACOBI", "PC_MG", "lib",
+           "setup_problem", "setup_batch"]
 
 
 def setup_problem(pb, session=None, precision=FP64):

@@ -29,9 +29,11 @@ EXPORTS = [
     "ifem_gpu_batch_patch_info", "ifem_gpu_batch_get_system", "ifem_gpu_batch_get_solution",
     "ifem_gpu_batch_timings", "ifem_gpu_autotune_assembly", "ifem_gpu_unet_backward",
     "ifem_gpu_last_assembly_parts", "ifem_gpu_integrate_norms", "ifem_gpu_sam_bind_constraints",
+    "ifem_gpu_mg_setup", "ifem_gpu_mg_level", "ifem_gpu_mg_apply",
 ]
 
 FP64, FP32 = 64, 32
+PC_JACOBI, PC_MG = 0, 1
 
 
 class GpuError(RuntimeError):
@@ -53,7 +55,8 @@ class Integrand(C.Structure):
 class SolverParams(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("dtol", C.c_double),
                 ("maxits", C.c_int), ("precision", C.c_int), ("inner_rtol", C.c_double),
-                ("check_every", C.c_int), ("verbosity", C.c_int)]
+                ("check_every", C.c_int), ("verbosity", C.c_int), ("pc", C.c_int),
+                ("mg_smooth", C.c_int)]
 
 
 class UNetConfig(C.Structure):
@@ -91,6 +94,9 @@ def lib():
     L.ifem_gpu_sam_bind.argtypes = [vp, ip, ip, ip, C.POINTER(ip), ip]
     L.ifem_gpu_sam_bind_constraints.argtypes = [vp, ip, ip, ip, C.POINTER(ip), ip, ip,
                                                 C.POINTER(ip), C.POINTER(ip), C.POINTER(dp)]
+    L.ifem_gpu_mg_setup.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(dp)]
+    L.ifem_gpu_mg_level.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(ip), C.POINTER(dp), vp]
+    L.ifem_gpu_mg_apply.argtypes = [vp, vp, vp]
     L.ifem_gpu_init_assembly.argtypes = [vp, C.POINTER(C.c_int64)]
     L.ifem_gpu_assemble.argtypes = [vp, C.POINTER(Integrand), C.POINTER(ip), ip]
     L.ifem_gpu_assemble_neumann.argtypes = [vp, ip, C.POINTER(dp), ip]
@@ -194,6 +200,7 @@ class Session:
         check(lib().ifem_gpu_patch_create(self.h, order, ncoef, kp,
                                           co.ctypes.data_as(C.POINTER(C.c_double)), 0,
                                           C.byref(pid)))
+        self._mg_p = patch.p[0]
         return pid.value
 
     def bind(self, patch_id: int, ncomp: int, meqn: np.ndarray, neq: int):
@@ -290,11 +297,12 @@ class Session:
         check(lib().ifem_gpu_assemble_neumann(self.h, face + 1, tv, ng))
 
     def solve(self, rtol=1e-8, atol=1e-300, maxits=100000, precision=FP64, inner_rtol=1e-4,
-              check_every=32, dtol=1e30):
+              check_every=32, dtol=1e30, pc=PC_JACOBI, mg_smooth=2):
         p = SolverParams()
         lib().ifem_gpu_solver_defaults(C.byref(p))
         p.rtol, p.atol, p.dtol, p.maxits = rtol, atol, dtol, maxits
         p.precision, p.inner_rtol, p.check_every = precision, inner_rtol, check_every
+        p.pc, p.mg_smooth = pc, mg_smooth
         st = Stats()
         rc = lib().ifem_gpu_solve(self.h, C.byref(p), C.byref(st))
         if rc not in (0, 4):
@@ -321,6 +329,31 @@ class Session:
         x = np.zeros(max(self.neq, 1))
         check(lib().ifem_gpu_get_solution(self.h, _ptr(x)))
         return x[:self.neq]
+
+    def mg_setup(self, smooth=2):
+        """Builds the multigrid hierarchy; returns (levels, setup ms)."""
+        nl, ms = C.c_int(0), C.c_double(0.0)
+        check(lib().ifem_gpu_mg_setup(self.h, smooth, C.byref(nl), C.byref(ms)))
+        return nl.value, ms.value
+
+    def mg_level(self, level: int, operator=False):
+        """(n[3], free DOFs, lambda_max, stencil or None) of a multigrid level."""
+        n = (C.c_int * 3)()
+        nf, lam = C.c_int(0), C.c_double(0.0)
+        check(lib().ifem_gpu_mg_level(self.h, level, n, C.byref(nf), C.byref(lam), None))
+        S = None
+        if operator:
+            nnode = n[0] * n[1] * n[2]
+            p = self._mg_p
+            S = np.zeros((2 * p + 1) ** 3 * self.ncomp ** 2 * nnode)
+            check(lib().ifem_gpu_mg_level(self.h, level, n, C.byref(nf), C.byref(lam), _ptr(S)))
+        return list(n), nf.value, lam.value, S
+
+    def mg_apply(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros(self.neq)
+        check(lib().ifem_gpu_mg_apply(self.h, _ptr(r), _ptr(z)))
+        return z
 
     def expanded_solution(self, ndof: int):
         """Solution in DOF order: eliminated DOFs 0 or their prescribed value."""

@@ -25,6 +25,7 @@ int fail(int code, const std::string& msg)
 static void free_system(Context& c)
 {
   solver_free(c);
+  c.mg.reset();
   hipFree(c.d_eq);
   hipFree(c.d_gval);
   c.d_gval = nullptr;
@@ -347,6 +348,8 @@ void ifem_gpu_solver_defaults(ifem_gpu_solver_params* p)
   p->inner_rtol = 1e-4;
   p->check_every = 32;
   p->verbosity = 0;
+  p->pc = IFEM_GPU_PC_JACOBI;
+  p->mg_smooth = 2;
 }
 
 int ifem_gpu_solve(ifem_gpu_ctx* ctx, const ifem_gpu_solver_params* params, ifem_gpu_stats* stats)
@@ -358,6 +361,52 @@ int ifem_gpu_solve(ifem_gpu_ctx* ctx, const ifem_gpu_solver_params* params, ifem
   if (params->precision == IFEM_GPU_FP32 && !c.fp32_ready)
     return fail(IFEM_GPU_EINVAL, "solve: FP32 solve needs an IFEM_GPU_FP32 assembly");
   return ifem::solve(c, *params, stats);
+}
+
+int ifem_gpu_mg_setup(ifem_gpu_ctx* ctx, int smooth, int* nlevels, double* setup_ms)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  int rc = ifem::mg_setup(c, smooth);
+  if (rc) return rc;
+  if (nlevels) *nlevels = (int)c.mg->lev.size();
+  if (setup_ms) *setup_ms = c.mg->setup_ms;
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_mg_level(ifem_gpu_ctx* ctx, int level, int n[3], int* nfree, double* lmax, double* S)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (!c.mg) return fail(IFEM_GPU_EINVAL, "mg_level: no hierarchy (ifem_gpu_mg_setup)");
+  const ifem::Multigrid& m = *c.mg;
+  if (level < 0 || level >= (int)m.lev.size()) return fail(IFEM_GPU_EINVAL, "mg_level: bad level");
+  const ifem::MgLevel& L = m.lev[level];
+  if (n)
+    for (int d = 0; d < 3; d++) n[d] = L.n[d];
+  if (nfree) *nfree = L.nfree;
+  if (lmax) *lmax = L.lmax;
+  if (S) {
+    if (level == 0) return fail(IFEM_GPU_EINVAL, "mg_level: level 0 is the CSR matrix");
+    IFEM_HIP(hipStreamSynchronize(c.stream));
+    IFEM_HIP(hipMemcpy(S, L.S64, sizeof(double) * (size_t)m.noff * m.nc * m.nc * L.nnode,
+                       hipMemcpyDeviceToHost));
+  }
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_mg_apply(ifem_gpu_ctx* ctx, const double* r, double* z)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (!r || !z) return fail(IFEM_GPU_EINVAL, "mg_apply: null vector");
+  if (!c.w.r || !c.w.z) return fail(IFEM_GPU_EINVAL, "mg_apply: no system");
+  IFEM_HIP(hipMemcpyAsync(c.w.r, r, sizeof(double) * c.neq, hipMemcpyHostToDevice, c.stream));
+  int rc = ifem::mg_apply(c, c.w.r, c.w.z, nullptr);
+  if (rc) return rc;
+  IFEM_HIP(hipMemcpyAsync(z, c.w.z, sizeof(double) * c.neq, hipMemcpyDeviceToHost, c.stream));
+  IFEM_HIP(hipStreamSynchronize(c.stream));
+  return IFEM_GPU_OK;
 }
 
 int ifem_gpu_system_info(ifem_gpu_ctx* ctx, int* neq, int64_t* nnz, int* ncomp)

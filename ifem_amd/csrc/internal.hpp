@@ -9,6 +9,7 @@
 
 #include "batch.hpp"
 #include "ifem_gpu.h"
+#include "mg.hpp"
 #include "unet.hpp"
 
 #include <memory>
@@ -117,6 +118,8 @@ struct Context {
 
   // batch of independent patches (batch.hip)
   std::unique_ptr<Batch> batch;
+  // spline multigrid hierarchy (pc = IFEM_GPU_PC_MG)
+  std::unique_ptr<Multigrid> mg;
 };
 
 // assemble.hip

@@ -31,13 +31,14 @@ namespace ifem {
 
 // ---------------------------------------------------------------------------
 // SpMV q = A p (CSR), LANES lanes per row; with DOT, per-block partial <p,q>.
-template <typename V, typename ACC, int LANES, bool DOT>
+// SKIP: return at once when scal[S_DONE] is set (work queued past convergence).
+template <typename V, typename ACC, int LANES, bool DOT, bool SKIP = DOT>
 __global__ __launch_bounds__(kThreads) void k_spmv_dot(
     int n, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
     const V* __restrict__ val, const V* __restrict__ p, V* __restrict__ q,
     double* __restrict__ part_pq, const double* __restrict__ scal)
 {
-  if (DOT && scal[S_DONE] != 0.0) return;
+  if (SKIP && scal[S_DONE] != 0.0) return;
   __shared__ double sh[kThreads / 64];
   const int groups_per_block = kThreads / LANES;
   const int g = threadIdx.x / LANES, l = threadIdx.x % LANES;
@@ -63,13 +64,13 @@ __global__ __launch_bounds__(kThreads) void k_spmv_dot(
 // gather reads one node index (bcol) per 9 values.  One wave per node; the
 // CSR values are read in place.  Blocks are mapped to the 8 XCDs in
 // contiguous node ranges so each XCD's L2 holds its slice of p.
-template <typename V, typename ACC, bool DOT>
+template <typename V, typename ACC, bool DOT, bool SKIP = DOT>
 __global__ __launch_bounds__(kThreads) void k_spmv_node3(
     int nnode, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ bcol,
     const V* __restrict__ val, const V* __restrict__ p, V* __restrict__ q,
     double* __restrict__ part_pq, const double* __restrict__ scal)
 {
-  if (DOT && scal[S_DONE] != 0.0) return;
+  if (SKIP && scal[S_DONE] != 0.0) return;
   __shared__ double sh[kThreads / 64];
   constexpr int WPB = kThreads / 64;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -275,26 +276,33 @@ static int lanes_for(const Context& c)
   return 16;
 }
 
-template <typename V, typename ACC, bool DOT>
+template <typename V, typename ACC, bool DOT, bool SKIP = DOT>
 static void launch_spmv(Context& c, const V* val, const V* p, V* q)
 {
   const int L = lanes_for(c);
   SolverWork& w = c.w;
   const int grid = kGrid;
   if (c.d_bcol) {
-    k_spmv_node3<V, ACC, DOT><<<grid, kThreads, 0, c.stream>>>(c.neq / 3, c.d_rowptr, c.d_bcol, val,
-                                                               p, q, w.partial, w.scal);
+    k_spmv_node3<V, ACC, DOT, SKIP><<<grid, kThreads, 0, c.stream>>>(
+        c.neq / 3, c.d_rowptr, c.d_bcol, val, p, q, w.partial, w.scal);
     return;
   }
   if (L == 64)
-    k_spmv_dot<V, ACC, 64, DOT><<<grid, kThreads, 0, c.stream>>>(c.neq, c.d_rowptr, c.d_colind, val,
-                                                                 p, q, w.partial, w.scal);
+    k_spmv_dot<V, ACC, 64, DOT, SKIP><<<grid, kThreads, 0, c.stream>>>(
+        c.neq, c.d_rowptr, c.d_colind, val, p, q, w.partial, w.scal);
   else if (L == 32)
-    k_spmv_dot<V, ACC, 32, DOT><<<grid, kThreads, 0, c.stream>>>(c.neq, c.d_rowptr, c.d_colind, val,
-                                                                 p, q, w.partial, w.scal);
+    k_spmv_dot<V, ACC, 32, DOT, SKIP><<<grid, kThreads, 0, c.stream>>>(
+        c.neq, c.d_rowptr, c.d_colind, val, p, q, w.partial, w.scal);
   else
-    k_spmv_dot<V, ACC, 16, DOT><<<grid, kThreads, 0, c.stream>>>(c.neq, c.d_rowptr, c.d_colind, val,
-                                                                 p, q, w.partial, w.scal);
+    k_spmv_dot<V, ACC, 16, DOT, SKIP><<<grid, kThreads, 0, c.stream>>>(
+        c.neq, c.d_rowptr, c.d_colind, val, p, q, w.partial, w.scal);
+}
+
+// FP32 SpMV for the multigrid smoother on the finest level.
+void launch_spmv32(Context& c, const float* val, const float* p, float* q, bool skip_done)
+{
+  if (skip_done) launch_spmv<float, float, false, true>(c, val, p, q);
+  else launch_spmv<float, float, false, false>(c, val, p, q);
 }
 
 int spmv_device(Context& c, const double* d_x, double* d_y)
@@ -602,8 +610,49 @@ static void apply_learned(Context& c, const V* r, const V* rold, const V* dinv, 
                                                       part_roz, c.w.scal);
 }
 
+// partials <r,z> and <r_old,z> after a preconditioner that does not form them
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_pc_dots(int n, const V* __restrict__ r,
+                                                      const V* __restrict__ rold,
+                                                      const V* __restrict__ z,
+                                                      double* __restrict__ part_rz,
+                                                      double* __restrict__ part_roz,
+                                                      const double* __restrict__ scal)
+{
+  if (scal[S_DONE] != 0.0) return;
+  __shared__ double sh[kThreads / 64];
+  double rz = 0.0, roz = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    const double zi = (double)z[i];
+    rz += (double)r[i] * zi;
+    if (rold) roz += (double)rold[i] * zi;
+  }
+  const double b1 = block_sum<double>(rz, sh);
+  const double b2 = block_sum<double>(roz, sh);
+  if (threadIdx.x == 0) {
+    part_rz[blockIdx.x] = b1;
+    part_roz[blockIdx.x] = b2;
+  }
+}
+
+enum PcKind { PC_LEARNED = 0, PC_MULTIGRID = 1 };
+
+template <typename V>
+static int apply_pc(Context& c, int kind, const V* r, const V* rold, const V* dinv, V* z,
+                    double* part_rr, double* part_rz, double* part_roz)
+{
+  if (kind == PC_LEARNED) {
+    apply_learned<V>(c, r, rold, dinv, z, part_rr, part_rz, part_roz);
+    return IFEM_GPU_OK;
+  }
+  int rc = mg_apply(c, r, z, c.w.scal);
+  if (rc) return rc;
+  k_pc_dots<V><<<kGrid, kThreads, 0, c.stream>>>(c.neq, r, rold, z, part_rz, part_roz, c.w.scal);
+  return IFEM_GPU_OK;
+}
+
 template <typename V, typename ACC>
-static int fcg_run(Context& c, const V* val, V* x, V* r, V* z, V* p, V* q, V* rold,
+static int fcg_run(Context& c, int kind, const V* val, V* x, V* r, V* z, V* p, V* q, V* rold,
                    const V* dinv, double tol2, int maxits, int check_every, int* iters,
                    double* rr_out, int* status)
 {
@@ -619,7 +668,8 @@ static int fcg_run(Context& c, const V* val, V* x, V* r, V* z, V* p, V* q, V* ro
   double* part_rr = w.partial + 2 * kGrid;
   double* part_roz = w.partial + 3 * kGrid;
   k_norm_parts<V><<<kGrid, kThreads, 0, c.stream>>>(n, r, part_rr);
-  apply_learned<V>(c, r, (const V*)nullptr, dinv, z, part_rr, part_rz, part_roz);
+  rc = apply_pc<V>(c, kind, r, (const V*)nullptr, dinv, z, part_rr, part_rz, part_roz);
+  if (rc) return rc;
   k_start_check_f<<<1, kThreads, 0, c.stream>>>(part_rz, part_rr, kGrid, w.scal);
   k_copy<V><<<(n + 255) / 256, 256, 0, c.stream>>>(n, z, p);
   IFEM_HIP(hipGetLastError());
@@ -630,7 +680,8 @@ static int fcg_run(Context& c, const V* val, V* x, V* r, V* z, V* p, V* q, V* ro
       launch_spmv<V, ACC, true>(c, val, p, q);
       k_update_f<V><<<kGrid, kThreads, 0, c.stream>>>(n, x, r, rold, p, q, part_pq, part_rr,
                                                       w.scal, parity);
-      apply_learned<V>(c, r, rold, dinv, z, part_rr, part_rz, part_roz);
+      rc = apply_pc<V>(c, kind, r, rold, dinv, z, part_rr, part_rz, part_roz);
+      if (rc) return rc;
       k_pupdate_f<V><<<kGrid, kThreads, 0, c.stream>>>(n, p, z, part_rz, part_roz, part_rr,
                                                        w.scal, parity);
     }
@@ -692,8 +743,19 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
   k_fill<double><<<nb, 256, 0, c.stream>>>(n, w.x, 0.0);
   IFEM_HIP(hipGetLastError());
 
+  const bool use_mg = prm.pc == IFEM_GPU_PC_MG;
+  if (prm.pc != IFEM_GPU_PC_JACOBI && !use_mg) return fail(IFEM_GPU_EINVAL, "solve: unknown pc");
+  int check_pc = check;
+  if (use_mg) {
+    rc = mg_setup(c, prm.mg_smooth > 0 ? prm.mg_smooth : 2);
+    if (rc) return rc;
+    if (!c.d_rold) IFEM_HIP(hipMalloc(&c.d_rold, sizeof(double) * (n > 0 ? n : 1)));
+    if (!c.d_rold32) IFEM_HIP(hipMalloc(&c.d_rold32, sizeof(float) * (n > 0 ? n : 1)));
+    check_pc = std::min(check, 8); // few, expensive iterations
+  }
+  const int pc_kind = use_mg ? PC_MULTIGRID : PC_LEARNED;
   int total_iters = 0, outer = 0;
-  bool use_learned = c.learned, fell_back = false;
+  bool use_learned = c.learned || use_mg, fell_back = false;
   double rel = bnorm > 0 ? 1.0 : 0.0;
   bool converged = bnorm == 0.0;
   bool diverged = false;
@@ -704,8 +766,9 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
       int it = 0, status = 0;
       double rr = 0.0;
       if (use_learned)
-        rc = fcg_run<double, double>(c, c.d_val, w.x, w.r, w.z, w.p, w.q, c.d_rold, w.dinv,
-                                     tol * tol, prm.maxits - total_iters, check, &it, &rr, &status);
+        rc = fcg_run<double, double>(c, pc_kind, c.d_val, w.x, w.r, w.z, w.p, w.q, c.d_rold,
+                                     w.dinv, tol * tol, prm.maxits - total_iters, check_pc, &it,
+                                     &rr, &status);
       else
         rc = pcg_run<double, double>(c, c.d_val, w.x, w.r, w.z, w.p, w.q, w.dinv, tol * tol,
                                      prm.maxits - total_iters, check, &it, &rr, &status);
@@ -744,9 +807,9 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
       // ask the inner solve for more than the outer tolerance needs
       double target = std::fmax(inner, 0.5 * tol / rn);
       if (use_learned)
-        rc = fcg_run<float, float>(c, c.d_val32, w.x32, w.r32, w.z32, w.p32, w.q32, c.d_rold32,
-                                   w.dinv32, target * target, prm.maxits - total_iters, check, &it,
-                                   &rr, &status);
+        rc = fcg_run<float, float>(c, pc_kind, c.d_val32, w.x32, w.r32, w.z32, w.p32, w.q32,
+                                   c.d_rold32, w.dinv32, target * target, prm.maxits - total_iters,
+                                   check_pc, &it, &rr, &status);
       else
         rc = pcg_run<float, float>(c, c.d_val32, w.x32, w.r32, w.z32, w.p32, w.q32, w.dinv32,
                                    target * target, prm.maxits - total_iters, check, &it, &rr,

@@ -54,7 +54,7 @@ LinSolParams::LinSolParams(LinAlg::LinearSystemType ls) : linSys(ls)
 bool LinSolParams::read(const std::string& xml)
 {
   static const char* keys[] = {"type", "pc", "rtol", "atol", "dtol", "maxits", "verbosity",
-                               "precision", "inner_rtol"};
+                               "precision", "inner_rtol", "multigrid_no_smooth"};
   for (const char* k : keys) {
     const std::string open = std::string("<") + k + ">", close = std::string("</") + k + ">";
     size_t a = xml.find(open);
@@ -179,13 +179,19 @@ bool GPUMatrix::solve(SystemVector& b, Real* rc)
     return false;
   }
   const std::string pc = par.getStringValue("pc");
-  if (pc != "jacobi" && pc != "default") {
+  // PETSc's multigrid choices map to the spline multigrid of the library
+  const bool mg = pc == "gamg" || pc == "ml" || pc == "mg" || pc == "multigrid";
+  if (pc != "jacobi" && pc != "default" && !mg) {
     std::cerr << " *** GPUMatrix::solve: preconditioner '" << pc << "' is not available"
               << std::endl;
     return false;
   }
   ifem_gpu_solver_params p;
   ifem_gpu_solver_defaults(&p);
+  if (mg) {
+    p.pc = IFEM_GPU_PC_MG;
+    if (par.hasValue("multigrid_no_smooth")) p.mg_smooth = par.getIntValue("multigrid_no_smooth");
+  }
   p.rtol = par.getDoubleValue("rtol");
   p.atol = par.getDoubleValue("atol");
   p.dtol = par.getDoubleValue("dtol");

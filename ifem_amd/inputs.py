@@ -317,9 +317,15 @@ def solver_kwargs(solver: Dict[str, str]) -> dict:
     (LinSolParams, src/LinAlg/LinSolParams.C:68-77,158-195)."""
     if solver.get("type", "cg").lower() != "cg":
         raise ValueError("the GPU path solves with CG only")
-    if solver.get("pc", "jacobi").lower() not in ("jacobi", "default"):
-        raise ValueError("the GPU path preconditions with Jacobi")
+    pc = solver.get("pc", "jacobi").lower()
     kw = {}
+    if pc in ("gamg", "ml", "mg", "multigrid"):
+        from ._native import PC_MG
+        kw["pc"] = PC_MG
+        if "multigrid_no_smooth" in solver:
+            kw["mg_smooth"] = int(solver["multigrid_no_smooth"])
+    elif pc not in ("jacobi", "default"):
+        raise ValueError("the GPU path preconditions with Jacobi or multigrid")
     for key, name, conv in (("rtol", "rtol", float), ("atol", "atol", float),
                             ("dtol", "dtol", float), ("maxits", "maxits", int)):
         if key in solver:

@@ -167,7 +167,12 @@ int ifem_gpu_integrate_norms(ifem_gpu_ctx* ctx, const ifem_gpu_integrand* itg, c
 
 /* ---- linear solve ----------------------------------------------------------
  * LinSolParams keys (src/LinAlg/LinSolParams.C:68-77,158-195) as fields:
- * type "cg" (CG only), pc "jacobi" (Jacobi only), rtol, atol, dtol, maxits.
+ * type "cg" (CG only), pc "jacobi" or a multigrid ("gamg", "ml": PETSc's
+ * algebraic multigrids, src/LinAlg/LinSolParams.C:60-155, here a spline
+ * multigrid: nested knot vectors, knot-insertion prolongation, Galerkin
+ * coarse operators, Chebyshev-Jacobi smoothing of mg_smooth steps
+ * (multigrid_no_smooth), dense coarsest solve; applied in flexible CG, FP32
+ * inside), rtol, atol, dtol, maxits.
  * Extensions: precision (FP64: plain FP64 PCG; FP32: FP32 inner PCG inside
  * FP64 iterative refinement), inner_rtol (FP32 inner residual reduction),
  * check_every (iterations between host convergence checks).
@@ -175,6 +180,7 @@ int ifem_gpu_integrate_norms(ifem_gpu_ctx* ctx, const ifem_gpu_integrand* itg, c
  * residual b - A x.  Replaces SystemMatrix::solve with PETSc KSPCG+PCJACOBI /
  * dune CGSolver+SeqJac (src/LinAlg/PETScMatrix.C:778-832,980-990;
  * src/LinAlg/ISTLMatrix.C:169-196). */
+enum { IFEM_GPU_PC_JACOBI = 0, IFEM_GPU_PC_MG = 1 };
 typedef struct {
   double rtol, atol, dtol;
   int maxits;
@@ -182,6 +188,8 @@ typedef struct {
   double inner_rtol;
   int check_every;
   int verbosity;
+  int pc;          /* IFEM_GPU_PC_JACOBI, or IFEM_GPU_PC_MG: spline multigrid */
+  int mg_smooth;   /* Chebyshev steps per pre/post smoothing (multigrid) */
 } ifem_gpu_solver_params;
 
 typedef struct {
@@ -282,6 +290,23 @@ int ifem_gpu_unet_backward(ifem_gpu_ctx* ctx, const float* x, const float* dy, i
  * keeps a linear network linear in the residual. */
 int ifem_gpu_set_learned_preconditioner(ifem_gpu_ctx* ctx, int enable, int D, int H, int W,
                                         const float* aux, int naux, int features, double scale);
+
+/* ---- multigrid inspection -----------------------------------------------
+ * No counterpart in the reference (PETSc keeps its multigrid internal); for
+ * tests and tuning.  ifem_gpu_mg_setup builds the spline multigrid hierarchy
+ * of the bound system from the current FP64 matrix (ifem_gpu_solve with pc =
+ * IFEM_GPU_PC_MG does this itself): level l+1 drops every second distinct
+ * interior knot of level l per direction.  ifem_gpu_mg_level: control points
+ * per direction, free DOFs and lambda_max(D^-1 A) of level l (0 on the
+ * coarsest); with S non-NULL (levels >= 1) the Galerkin operator P^T A P in
+ * stencil form, S[((off*nc + ci)*nc + cj)*nnode + node], node = i + n0*(j +
+ * n1*k), off = ((e3+p)*(2p+1) + e2+p)*(2p+1) + e1+p for the neighbour
+ * (i+e1, j+e2, k+e3).  ifem_gpu_mg_apply: z = M^-1 r, one V-cycle, host
+ * vectors in equation order. */
+int ifem_gpu_mg_setup(ifem_gpu_ctx* ctx, int smooth, int* nlevels, double* setup_ms);
+int ifem_gpu_mg_level(ifem_gpu_ctx* ctx, int level, int n[3], int* nfree, double* lmax,
+                      double* S);
+int ifem_gpu_mg_apply(ifem_gpu_ctx* ctx, const double* r, double* z);
 
 /* ---- batch of independent patches ------------------------------------------
  * Many scalar (Poisson / diffusion) patches, each with its own SAM numbering

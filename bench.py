@@ -13,7 +13,8 @@ assembly device time).  Inputs (patch, SAM, sparsity pattern) are resident in
 HBM before the timed region; the symbolic pattern build is reported apart.
 
 Also reported: the FP32-inner / FP64-refinement solve (time, iterations,
-energy-norm and max-nodal error against the FP64 solve), the roofline of the
+energy-norm and max-nodal error against the FP64 solve), flexible CG with the
+FP32 spline multigrid preconditioner (setup included in its time), the roofline of the
 assembly kernel and of the SpMV, the oracle (CPU restatement) timed on a
 bounded sample on the host cores, and the batch of 256 uneven diffusion
 patches (one persistent assembly launch per precision, per-patch PCG).
@@ -59,6 +60,7 @@ def parse():
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-mg", action="store_true", help="skip the multigrid-preconditioned leg")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched uneven-patch leg (256 diffusion patches)")
     ap.add_argument("--cpu-layers", type=int, default=16,
@@ -189,6 +191,30 @@ def main():
         }
         log(f"[bench] mixed: {mixed}")
 
+    # ---- spline multigrid (FP32) inside FP64 flexible CG ----------------------
+    mg = None
+    if not args.no_mg:
+        from ifem_amd import PC_MG
+        def err_vs_fp64(x):
+            e = x - x64
+            return (float(np.sqrt(abs(e @ s.spmv(e))) / np.sqrt(x64 @ s.spmv(x64))),
+                    float(np.max(np.abs(e)) / np.max(np.abs(x64))))
+        legs = {}
+        for name, prec in (("fp64_krylov", FP64), ("fp32_krylov", FP32)):
+            if prec == FP32 and mixed is None:
+                continue
+            runs = [s.solve(rtol=args.rtol, maxits=5000, pc=PC_MG, precision=prec, mg_smooth=2,
+                            check_every=8) for _ in range(3)]
+            en, mn = err_vs_fp64(s.solution())
+            legs[name] = {"solve_ms": round(float(np.median([r["solve_ms"] for r in runs])), 3),
+                          "iterations": runs[-1]["iterations"], "outer": runs[-1]["outer"],
+                          "rel_residual": runs[-1]["rel_residual"],
+                          "converged": all(r["converged"] for r in runs),
+                          "energy_norm_err_vs_fp64": en, "max_nodal_err_vs_fp64": mn}
+        nlev, setup_ms = s.mg_setup(2)
+        mg = {"levels": nlev, "setup_ms_included": round(setup_ms, 3), **legs}
+        log(f"[bench] multigrid: {mg}")
+
     # ---- batched uneven patches: one persistent assembly launch ----------------
     batch = None
     if not args.no_batch:
@@ -310,6 +336,10 @@ def main():
         "fp64_time_to_tol_ms": round(solve64_ms, 2), "fp64_iterations": iters64,
         "mixed": mixed,
         "speedup_mixed_vs_fp64": (round(solve64_ms / mixed["solve_ms"], 3) if mixed else None),
+        "multigrid": mg,
+        "speedup_mg_vs_fp64": (round(solve64_ms / min(v["solve_ms"] for k, v in mg.items()
+                                                      if isinstance(v, dict)), 3)
+                               if mg else None),
         "roofline": roofline, "roofline_spmv": spmv_roof, "cpu_baseline": cpu,
         "batch": batch,
     }

@@ -7,12 +7,12 @@
 //     its B-splines are exact combinations of level l's (knot insertion,
 //     Boehm's algorithm, gives the prolongation P = P3 x P2 x P1);
 //   * A_{l+1} = P^T A_l P (Galerkin), formed one direction at a time on a
-//     stencil layout [offset][ci][cj][node] (node fastest, so threads over
-//     nodes read and write contiguous memory);
+//     node-major stencil layout [node][offset][ci][cj] (threads over
+//     offsets read and write contiguous memory);
 //   * a coarse DOF is kept iff all fine DOFs it is made of are free, so
 //     homogeneous and prescribed Dirichlet faces carry over;
-//   * smoothing: Chebyshev iteration on D^-1 A over [0.1, 1.1] x lambda_max
-//     (lambda_max by power iteration), symmetric pre/post; the coarsest
+//   * smoothing: Chebyshev iteration on D^-1 A over [0.1, 1.05] x lambda_max
+//     (lambda_max from 10 Lanczos steps), symmetric pre/post; the coarsest
 //     level is solved with a dense inverse (Cholesky on the host, small);
 //   * level 0 smooths with the FP32 CSR matrix in equation order; coarse
 //     levels use FP32 stencils in DOF order with zeros at eliminated DOFs.
@@ -41,7 +41,11 @@ Multigrid::~Multigrid()
   }
   hipFree(S0); hipFree(T1); hipFree(T2);
   hipFree(val32); hipFree(cdofs); hipFree(cinv); hipFree(part); hipFree(lam); hipFree(err);
-  hipFree(dof_of_eq);
+  hipFree(dof_of_eq); hipFree(part2); hipFree(lzh);
+  if (h_lzh) hipHostFree(h_lzh);
+  if (h_coarse) hipHostFree(h_coarse);
+  if (h_err) hipHostFree(h_err);
+  if (e_copy) hipEventDestroy(e_copy);
   if (e0) hipEventDestroy(e0);
   if (e1) hipEventDestroy(e1);
 }
@@ -158,12 +162,14 @@ struct Grid3 {
   int n0, n1, n2;
 };
 
-__device__ inline void off3(int off, int p, int e[3])
+// x / n and x % n for 0 <= x < 2^24 through a float reciprocal, corrected
+__device__ inline int divmod_small(int x, int n, float inv, int& r)
 {
-  const int W1 = 2 * p + 1;
-  e[0] = off % W1 - p;
-  e[1] = (off / W1) % W1 - p;
-  e[2] = off / (W1 * W1) - p;
+  int q = (int)((float)x * inv);
+  r = x - q * n;
+  if (r < 0) { q--; r += n; }
+  else if (r >= n) { q++; r -= n; }
+  return q;
 }
 
 // Fine CSR (equation order) -> node-major stencil S0[node][off][ci][cj]
@@ -185,10 +191,14 @@ __global__ __launch_bounds__(256) void k_mg_csr_scatter(
   const int i = node % g.n0, j = (node / g.n0) % g.n1, k = node / (g.n0 * g.n1);
   double* out = S + (int64_t)node * NOFF * NCC + ci * NC;
   const int64_t b = rowptr[wave], e = rowptr[wave + 1];
+  const float inv0 = 1.0f / (float)g.n0, inv1 = 1.0f / (float)g.n1;
   for (int64_t q = b + lane; q < e; q += 64) {
     const int cd = dof_of_eq[colind[q]];
     const int cn = cd / NC, cj = cd - cn * NC;
-    const int e0 = cn % g.n0 - i, e1 = (cn / g.n0) % g.n1 - j, e2 = cn / (g.n0 * g.n1) - k;
+    int r0, r1;
+    const int q0 = divmod_small(cn, g.n0, inv0, r0);
+    const int q1 = divmod_small(q0, g.n1, inv1, r1);
+    const int e0 = r0 - i, e1 = r1 - j, e2 = q1 - k;
     if (e0 < -P || e0 > P || e1 < -P || e1 > P || e2 < -P || e2 > P) {
       err[0] = 1;
       continue;
@@ -199,22 +209,27 @@ __global__ __launch_bounds__(256) void k_mg_csr_scatter(
 }
 
 // One direction of the Galerkin product: Sout = P_d^T Sin P_d, where
-// direction d of the grid goes from gin.n_d fine to ncd coarse points.
-// AOS_IN: Sin is node-major, Sin[node][off][ci][cj] (the fine level).
-template <int P, int NC, bool AOS_IN>
+// direction d of the grid goes from gin.n_d fine to ncd coarse points.  All
+// stencils are node-major, S[node][off][ci][cj]; one thread per (node,
+// offset) with the offset fastest, so reads and writes are contiguous.
+// With freem (the last pass of a level): rows and columns of eliminated
+// coarse DOFs are zeroed, the FP32 copy S32 and the inverse diagonal dinv
+// are written as well.
+template <int P, int NC>
 __global__ __launch_bounds__(256) void k_mg_galerkin(
     const double* __restrict__ Sin, double* __restrict__ Sout, Grid3 gin, int d, int ncd,
-    const int* __restrict__ clo, const int* __restrict__ ccnt, const double* __restrict__ cw)
+    const int* __restrict__ clo, const int* __restrict__ ccnt, const double* __restrict__ cw,
+    const unsigned char* __restrict__ freem, float* __restrict__ S32, float* __restrict__ dinv)
 {
-  constexpr int W1 = 2 * P + 1, NCC = NC * NC;
+  constexpr int W1 = 2 * P + 1, NOFF = W1 * W1 * W1, NCC = NC * NC;
   int nin[3] = {gin.n0, gin.n1, gin.n2};
   int nout[3] = {gin.n0, gin.n1, gin.n2};
   nout[d] = ncd;
   const int64_t nnout = (int64_t)nout[0] * nout[1] * nout[2];
-  const int64_t nnin = (int64_t)nin[0] * nin[1] * nin[2];
-  const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int off = blockIdx.y;
-  if (node >= nnout) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nnout * NOFF) return;
+  const int64_t node = t / NOFF;
+  const int off = (int)(t - node * NOFF);
   int o[3];
   o[0] = (int)(node % nout[0]);
   o[1] = (int)((node / nout[0]) % nout[1]);
@@ -248,103 +263,73 @@ __global__ __launch_bounds__(256) void k_mg_galerkin(
         const int db = b0 + kb - a;
         if (db < -P || db > P) continue;
         const double w = wa * cw[(int64_t)J * kMgKC + kb];
-        const int offi = obase + (db + P) * so;
-        if (AOS_IN) {
-          const double* src = Sin + (ni * (W1 * W1 * W1) + offi) * NCC;
+        const double* src = Sin + (ni * NOFF + obase + (db + P) * so) * NCC;
 #pragma unroll
-          for (int m = 0; m < NCC; m++) acc[m] += w * src[m];
-        } else {
-          const double* src = Sin + (int64_t)offi * NCC * nnin + ni;
-#pragma unroll
-          for (int m = 0; m < NCC; m++) acc[m] += w * src[(int64_t)m * nnin];
-        }
+        for (int m = 0; m < NCC; m++) acc[m] += w * src[m];
       }
     }
   }
+  double* dst = Sout + t * NCC;
+  if (freem) {
+    constexpr int center = (P * W1 + P) * W1 + P;
+    const int64_t nb = ok ? node + e[0] + (int64_t)nout[0] * (e[1] + (int64_t)nout[1] * e[2]) : 0;
 #pragma unroll
-  for (int m = 0; m < NCC; m++) Sout[((int64_t)off * NCC + m) * nnout + node] = acc[m];
+    for (int ci = 0; ci < NC; ci++)
+#pragma unroll
+      for (int cj = 0; cj < NC; cj++) {
+        double v = acc[ci * NC + cj];
+        if (!ok || !freem[node * NC + ci] || !freem[nb * NC + cj]) v = 0.0;
+        dst[ci * NC + cj] = v;
+        S32[t * NCC + ci * NC + cj] = (float)v;
+        if (off == center && ci == cj)
+          dinv[node * NC + ci] = (freem[node * NC + ci] && v > 0.0) ? (float)(1.0 / v) : 0.0f;
+      }
+    return;
+  }
+#pragma unroll
+  for (int m = 0; m < NCC; m++) dst[m] = acc[m];
 }
 
-// Zero rows/columns of eliminated DOFs, FP32 copy, inverse diagonal.
-__global__ __launch_bounds__(256) void k_mg_finish(double* __restrict__ S64,
-                                                   float* __restrict__ S32,
-                                                   float* __restrict__ dinv,
-                                                   const unsigned char* __restrict__ freem,
-                                                   Grid3 g, int p, int nc)
-{
-  const int64_t nnode = (int64_t)g.n0 * g.n1 * g.n2;
-  const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int off = blockIdx.y;
-  if (node >= nnode) return;
-  const int W1 = 2 * p + 1, ncc = nc * nc, center = (p * W1 + p) * W1 + p;
-  const int i = (int)(node % g.n0), j = (int)((node / g.n0) % g.n1), k = (int)(node / ((int64_t)g.n0 * g.n1));
-  int e[3];
-  off3(off, p, e);
-  const int ii = i + e[0], jj = j + e[1], kk = k + e[2];
-  const bool inr = ii >= 0 && ii < g.n0 && jj >= 0 && jj < g.n1 && kk >= 0 && kk < g.n2;
-  const int64_t nb = inr ? (int64_t)ii + (int64_t)g.n0 * (jj + (int64_t)g.n1 * kk) : 0;
-  for (int ci = 0; ci < nc; ci++)
-    for (int cj = 0; cj < nc; cj++) {
-      const int64_t at = ((int64_t)off * ncc + ci * nc + cj) * nnode + node;
-      double v = S64[at];
-      if (!inr || !freem[node * nc + ci] || !freem[nb * nc + cj]) v = 0.0;
-      S64[at] = v;
-      S32[at] = (float)v;
-      if (off == center && ci == cj)
-        dinv[node * nc + ci] = (freem[node * nc + ci] && v > 0.0) ? (float)(1.0 / v) : 0.0f;
-    }
-}
-
-// y = A x on a coarse level (stencil, DOF order).  A block holds 64 DOFs
-// times the 2p+1 offset slices in direction 3; each thread sums one slice,
-// the slices are added in order through LDS.
+// y = A x on a coarse level (node-major stencil, DOF order): one wave per
+// node, lanes over the offsets (contiguous 4*NC*NC-byte blocks), then a
+// fixed-order wave reduction.
 template <int P, int NC>
-__global__ __launch_bounds__(64 * (2 * P + 1)) void k_mg_spmv(const float* __restrict__ S,
-                                                            const float* __restrict__ x,
-                                                            float* __restrict__ y, Grid3 g,
-                                                            const double* __restrict__ scal)
+__global__ __launch_bounds__(256) void k_mg_spmv(const float* __restrict__ S,
+                                                 const float* __restrict__ x,
+                                                 float* __restrict__ y, Grid3 g,
+                                                 const double* __restrict__ scal)
 {
-  constexpr int W1 = 2 * P + 1, NCC = NC * NC;
+  constexpr int W1 = 2 * P + 1, NOFF = W1 * W1 * W1, NCC = NC * NC;
   if (scal && scal[S_DONE] != 0.0) return;
-  __shared__ float part[W1][64];
   const int64_t nnode = (int64_t)g.n0 * g.n1 * g.n2;
-  const int slice = threadIdx.x >> 6, dl = threadIdx.x & 63;
-  const int64_t t = (int64_t)blockIdx.x * 64 + dl;
-  float acc = 0.f;
-  if (t < nnode * NC) {
-    const int64_t node = t / NC;
-    const int ci = (int)(t - node * NC);
-    const int i = (int)(node % g.n0), j = (int)((node / g.n0) % g.n1);
-    const int k = (int)(node / ((int64_t)g.n0 * g.n1));
-    const int e2 = slice - P, kk = k + e2;
-    if (kk >= 0 && kk < g.n2) {
-      const float* s0 = S + (int64_t)ci * NC * nnode + node;
+  const int64_t node = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (node >= nnode) return;
+  const int i = (int)(node % g.n0), j = (int)((node / g.n0) % g.n1), k = (int)(node / ((int64_t)g.n0 * g.n1));
+  const float* sn = S + node * NOFF * NCC;
+  float acc[NC];
 #pragma unroll
-      for (int e1 = -P; e1 <= P; e1++) {
-        const int jj = j + e1;
-        if (jj < 0 || jj >= g.n1) continue;
-        const int obase = ((e2 + P) * W1 + (e1 + P)) * W1 + P;
-        const float* xr = x + (int64_t)g.n0 * (jj + (int64_t)g.n1 * kk) * NC;
+  for (int c = 0; c < NC; c++) acc[c] = 0.f;
+  for (int off = lane; off < NOFF; off += 64) {
+    const int ii = i + off % W1 - P, jj = j + (off / W1) % W1 - P, kk = k + off / (W1 * W1) - P;
+    if (ii < 0 || ii >= g.n0 || jj < 0 || jj >= g.n1 || kk < 0 || kk >= g.n2) continue;
+    const float* xv = x + ((int64_t)ii + (int64_t)g.n0 * (jj + (int64_t)g.n1 * kk)) * NC;
+    const float* sv = sn + off * NCC;
+    float xs[NC];
 #pragma unroll
-        for (int e0 = -P; e0 <= P; e0++) {
-          const int ii = i + e0;
-          if (ii >= 0 && ii < g.n0) {
-            const float* sp = s0 + (int64_t)(obase + e0) * NCC * nnode;
+    for (int cj = 0; cj < NC; cj++) xs[cj] = xv[cj];
 #pragma unroll
-            for (int cj = 0; cj < NC; cj++) acc += sp[(int64_t)cj * nnode] * xr[ii * NC + cj];
-          }
-        }
-      }
-    }
+    for (int ci = 0; ci < NC; ci++)
+#pragma unroll
+      for (int cj = 0; cj < NC; cj++) acc[ci] += sv[ci * NC + cj] * xs[cj];
   }
-  part[slice][dl] = acc;
-  __syncthreads();
-  if (slice == 0 && t < nnode * NC) {
-    float sum = part[0][dl];
 #pragma unroll
-    for (int q = 1; q < W1; q++) sum += part[q][dl];
-    y[t] = sum;
-  }
+  for (int c = 0; c < NC; c++)
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) acc[c] += __shfl_xor(acc[c], o, 64);
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < NC; c++) y[node * NC + c] = acc[c];
 }
 
 // Chebyshev step.  mode 0: r = b, d = c2 D^-1 r, x = d;  mode 1: r -= q,
@@ -469,7 +454,7 @@ __global__ __launch_bounds__(256) void k_mg_prolong(float* __restrict__ xf,
   }
 }
 
-// Coarsest level: x[dof_i] = sum_j Ainv(i, j) b[dof_j].
+// Coarsest level: x[dof_i] = sum_j Ainv(i, j) b[dof_j]; one wave per row.
 __global__ __launch_bounds__(256) void k_mg_coarse(int ncf, const int* __restrict__ dofs,
                                                    const float* __restrict__ inv,
                                                    const float* __restrict__ b,
@@ -477,53 +462,110 @@ __global__ __launch_bounds__(256) void k_mg_coarse(int ncf, const int* __restric
                                                    const double* __restrict__ scal)
 {
   if (scal && scal[S_DONE] != 0.0) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
   if (i >= ncf) return;
   float s = 0.f;
-  for (int j = 0; j < ncf; j++) s += inv[(int64_t)i * ncf + j] * b[dofs[j]];
-  x[dofs[i]] = s;
+  for (int j = lane; j < ncf; j += 64) s += inv[(int64_t)i * ncf + j] * b[dofs[j]];
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) x[dofs[i]] = s;
 }
 
-// Power iteration for lambda_max(D^-1 A): start vector, then v = D^-1 q with
-// partial sums of |v|^2, then v /= |v| (lam = |v| before scaling).
-__global__ void k_mg_power_init(int n, float* __restrict__ v, const float* __restrict__ dinv)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t h = (uint32_t)i * 2654435761u + 12345u;
-  h ^= h >> 15;
-  h *= 2246822519u;
-  h ^= h >> 13;
-  const float u = (float)(h & 0xFFFFFF) / 16777216.0f;
-  v[i] = dinv[i] != 0.f ? (0.5f + u) : 0.f;
-}
-
-__global__ __launch_bounds__(kThreads) void k_mg_power_scale(int n, float* __restrict__ v,
-                                                             const float* __restrict__ q,
-                                                             const float* __restrict__ dinv,
-                                                             double* __restrict__ part)
+// lambda_max(D^-1 A) by Lanczos: a few Jacobi-preconditioned CG steps on a
+// fixed pseudo-random right-hand side; the CG coefficients give the Lanczos
+// tridiagonal matrix (diagonal 1/alpha_j + beta_{j-1}/alpha_{j-1},
+// off-diagonal sqrt(beta_j)/alpha_j), whose largest eigenvalue (host) is
+// the estimate.  h[0..] holds rz_0, then per step alpha_j, beta_j, rz_{j+1}.
+__global__ __launch_bounds__(kThreads) void k_lz_start(int n, float* __restrict__ r,
+                                                       float* __restrict__ z,
+                                                       float* __restrict__ p,
+                                                       const float* __restrict__ dinv,
+                                                       double* __restrict__ part)
 {
   __shared__ double sh[kThreads / 64];
   double s = 0.0;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
-    const float w = dinv[i] * q[i];
-    v[i] = w;
-    s += (double)w * (double)w;
+    uint32_t h = (uint32_t)i * 2654435761u + 12345u;
+    h ^= h >> 15;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    const float u = (float)(h & 0xFFFFFF) / 16777216.0f - 0.5f;
+    const float ri = dinv[i] != 0.f ? u : 0.f;
+    const float zi = dinv[i] * ri;
+    r[i] = ri;
+    z[i] = zi;
+    p[i] = zi;
+    s += (double)ri * (double)zi;
   }
   const double b = block_sum<double>(s, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = b;
 }
 
-__global__ __launch_bounds__(kThreads) void k_mg_power_norm(int n, float* __restrict__ v,
-                                                            const double* __restrict__ part,
-                                                            int nparts, double* __restrict__ lam)
+__global__ __launch_bounds__(kThreads) void k_lz_sum(const double* __restrict__ part, int nparts,
+                                                     double* __restrict__ out)
 {
   __shared__ double sh[kThreads / 64];
-  const double ss = sum_parts(part, nparts, sh);
-  const double nv = sqrt(ss);
-  if (blockIdx.x == 0 && threadIdx.x == 0) *lam = nv;
-  const float inv = nv > 0.0 ? (float)(1.0 / nv) : 0.f;
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) v[i] *= inv;
+  const double s = sum_parts(part, nparts, sh);
+  if (threadIdx.x == 0) *out = s;
+}
+
+__global__ __launch_bounds__(kThreads) void k_lz_dot(int n, const float* __restrict__ a,
+                                                     const float* __restrict__ b,
+                                                     double* __restrict__ part)
+{
+  __shared__ double sh[kThreads / 64];
+  double s = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
+    s += (double)a[i] * (double)b[i];
+  const double bs = block_sum<double>(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// alpha = rz_j / <p,q>; r -= alpha q; z = D^-1 r; partials of <r,z>
+__global__ __launch_bounds__(kThreads) void k_lz_update(int n, float* __restrict__ r,
+                                                        float* __restrict__ z,
+                                                        const float* __restrict__ q,
+                                                        const float* __restrict__ dinv,
+                                                        const double* __restrict__ part_pq,
+                                                        int nparts, double* __restrict__ h,
+                                                        int j, double* __restrict__ part_rz)
+{
+  __shared__ double sh[kThreads / 64];
+  const double pq = sum_parts(part_pq, nparts, sh);
+  const double rz = h[3 * j];
+  const double alpha = pq > 0.0 ? rz / pq : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) h[3 * j + 1] = alpha;
+  const float a = (float)alpha;
+  double s = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    const float ri = r[i] - a * q[i];
+    const float zi = dinv[i] * ri;
+    r[i] = ri;
+    z[i] = zi;
+    s += (double)ri * (double)zi;
+  }
+  const double b = block_sum<double>(s, sh);
+  if (threadIdx.x == 0) part_rz[blockIdx.x] = b;
+}
+
+// beta = rz_{j+1} / rz_j; p = z + beta p
+__global__ __launch_bounds__(kThreads) void k_lz_pupdate(int n, float* __restrict__ p,
+                                                         const float* __restrict__ z,
+                                                         const double* __restrict__ part_rz,
+                                                         int nparts, double* __restrict__ h, int j)
+{
+  __shared__ double sh[kThreads / 64];
+  const double rzn = sum_parts(part_rz, nparts, sh);
+  const double rz = h[3 * j];
+  const double beta = rz > 0.0 ? rzn / rz : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    h[3 * j + 2] = beta;
+    h[3 * j + 3] = rzn;
+  }
+  const float bf = (float)beta;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
+    p[i] = z[i] + bf * p[i];
 }
 
 template <typename V>
@@ -552,10 +594,50 @@ __global__ void k_mg_demote(int64_t n, const double* __restrict__ a, float* __re
 
 static constexpr int kMgCoarseMax = 400;   // stop coarsening at this many free DOFs
 static constexpr int kMgDenseMax = 3000;   // largest coarsest level solved densely
-static constexpr int kMgPowerIts = 10;
+static constexpr int kMgLanczos = 10;      // CG steps of the eigenvalue estimate
 static constexpr int kMgParts = 1024;
 
 static inline unsigned grid1(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+// Largest eigenvalue of the Lanczos tridiagonal matrix built from k CG steps
+// (h: rz_0, then alpha_j, beta_j, rz_{j+1} per step), by bisection on the
+// Sturm sequence count.  Steps after a breakdown (alpha <= 0) are dropped.
+static double lanczos_max(const double* h, int k)
+{
+  std::vector<double> dg, od;
+  for (int j = 0; j < k; j++) {
+    const double a = h[3 * j + 1];
+    if (!(a > 0.0) || !std::isfinite(a)) break;
+    double dj = 1.0 / a;
+    if (j > 0) dj += h[3 * j - 1] / h[3 * j - 2];
+    dg.push_back(dj);
+    if (j > 0) od.push_back(std::sqrt(std::max(h[3 * j - 1], 0.0)) / h[3 * j - 2]);
+  }
+  const int n = (int)dg.size();
+  if (n == 0) return 0.0;
+  double lo = 0.0, hi = 0.0;
+  for (int i = 0; i < n; i++) {
+    const double r = (i > 0 ? std::fabs(od[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(od[i]) : 0.0);
+    hi = std::max(hi, dg[i] + r);
+  }
+  auto count_below = [&](double x) { // eigenvalues < x
+    int cnt = 0;
+    double q = dg[0] - x;
+    if (q < 0) cnt++;
+    for (int i = 1; i < n; i++) {
+      if (q == 0.0) q = 1e-300;
+      q = dg[i] - x - od[i - 1] * od[i - 1] / q;
+      if (q < 0) cnt++;
+    }
+    return cnt;
+  };
+  for (int it = 0; it < 200 && hi - lo > 1e-12 * hi; it++) {
+    const double mid = 0.5 * (lo + hi);
+    if (count_below(mid) >= n) hi = mid;
+    else lo = mid;
+  }
+  return hi;
+}
 
 static DirDev dirdev(const MgDir& D)
 {
@@ -687,7 +769,14 @@ static int mg_build(Context& c, Multigrid& m)
   if (rc) return rc;
   IFEM_HIP(hipMalloc(&m.cinv, sizeof(float) * (size_t)std::max(1, m.ncf * m.ncf)));
   IFEM_HIP(hipMalloc(&m.part, sizeof(double) * kMgParts));
-  IFEM_HIP(hipMalloc(&m.lam, sizeof(double) * m.lev.size()));
+  IFEM_HIP(hipMalloc(&m.part2, sizeof(double) * kMgParts));
+  const size_t nlz = m.lev.size() * (3 * kMgLanczos + 1);
+  IFEM_HIP(hipMalloc(&m.lzh, sizeof(double) * nlz));
+  IFEM_HIP(hipHostMalloc(&m.h_lzh, sizeof(double) * nlz, hipHostMallocDefault));
+  IFEM_HIP(hipHostMalloc(&m.h_coarse, sizeof(double) * (size_t)m.noff * ncc * Lc.nnode,
+                         hipHostMallocDefault));
+  IFEM_HIP(hipHostMalloc(&m.h_err, sizeof(int), hipHostMallocDefault));
+  IFEM_HIP(hipEventCreate(&m.e_copy));
   IFEM_HIP(hipMalloc(&m.err, sizeof(int)));
   IFEM_HIP(hipEventCreate(&m.e0));
   IFEM_HIP(hipEventCreate(&m.e1));
@@ -719,10 +808,9 @@ static void level_spmv(Context& c, Multigrid& m, int l, const float* x, float* y
     return;
   }
   const MgLevel& L = m.lev[l];
-  const unsigned gb = (unsigned)(((int64_t)L.nnode * m.nc + 63) / 64);
+  const unsigned gb = (unsigned)(((int64_t)L.nnode + 3) / 4); // one wave per node
   MG_DISPATCH(m.p, m.nc,
-              (k_mg_spmv<KP, KN><<<gb, 64 * (2 * KP + 1), 0, c.stream>>>(L.S32, x, y, grid_of(L),
-                                                                          scal)));
+              (k_mg_spmv<KP, KN><<<gb, 256, 0, c.stream>>>(L.S32, x, y, grid_of(L), scal)));
 }
 
 // Dense inverse of the coarsest operator (host Cholesky; the level is small).
@@ -730,10 +818,7 @@ static int coarse_inverse(Context& c, Multigrid& m)
 {
   const MgLevel& L = m.lev.back();
   const int ncc = m.nc * m.nc, nc = m.nc;
-  std::vector<double> S((size_t)m.noff * ncc * L.nnode);
-  IFEM_HIP(hipMemcpyAsync(S.data(), L.S64, sizeof(double) * S.size(), hipMemcpyDeviceToHost,
-                          c.stream));
-  IFEM_HIP(hipStreamSynchronize(c.stream));
+  const double* S = m.h_coarse; // node-major, copied by mg_setup
   const int n = m.ncf;
   std::vector<int> pos(L.ndof, -1);
   int k = 0;
@@ -752,7 +837,7 @@ static int coarse_inverse(Context& c, Multigrid& m)
         for (int cj = 0; cj < nc; cj++) {
           const int r = pos[node * nc + ci], s = pos[nb * nc + cj];
           if (r < 0 || s < 0) continue;
-          A[(size_t)r * n + s] = S[((size_t)off * ncc + ci * nc + cj) * L.nnode + node];
+          A[(size_t)r * n + s] = S[((size_t)node * m.noff + off) * ncc + ci * nc + cj];
         }
     }
   }
@@ -790,7 +875,9 @@ static int coarse_inverse(Context& c, Multigrid& m)
     }
     for (int r = 0; r < n; r++) inv[(size_t)r * n + col] = (float)y[r];
   }
-  IFEM_HIP(hipMemcpy(m.cinv, inv.data(), sizeof(float) * inv.size(), hipMemcpyHostToDevice));
+  IFEM_HIP(hipMemcpyAsync(m.cinv, inv.data(), sizeof(float) * inv.size(), hipMemcpyHostToDevice,
+                          c.stream));
+  IFEM_HIP(hipStreamSynchronize(c.stream)); // inv is a host temporary
   return IFEM_GPU_OK;
 }
 
@@ -839,51 +926,60 @@ int mg_setup(Context& c, int degree)
     const int64_t n1 = (int64_t)g1.n0 * g1.n1 * g1.n2, n2 = (int64_t)g2.n0 * g2.n1 * g2.n2;
     if ((size_t)n1 * m.noff * ncc > m.t1cap || (size_t)n2 * m.noff * ncc > m.t2cap)
       return fail(IFEM_GPU_EDEVICE, "mg: work buffer too small");
-    if (l == 1)
-      MG_DISPATCH(m.p, m.nc,
-                  (k_mg_galerkin<KP, KN, true><<<dim3(grid1(n1), m.noff), 256, 0, c.stream>>>(
-                      Sin, m.T1, g0, 0, C.n[0], C.dir[0].clo, C.dir[0].ccnt, C.dir[0].cw)));
-    else
-      MG_DISPATCH(m.p, m.nc,
-                  (k_mg_galerkin<KP, KN, false><<<dim3(grid1(n1), m.noff), 256, 0, c.stream>>>(
-                      Sin, m.T1, g0, 0, C.n[0], C.dir[0].clo, C.dir[0].ccnt, C.dir[0].cw)));
     MG_DISPATCH(m.p, m.nc,
-                (k_mg_galerkin<KP, KN, false><<<dim3(grid1(n2), m.noff), 256, 0, c.stream>>>(
-                    m.T1, m.T2, g1, 1, C.n[1], C.dir[1].clo, C.dir[1].ccnt, C.dir[1].cw)));
+                (k_mg_galerkin<KP, KN><<<grid1(n1 * m.noff), 256, 0, c.stream>>>(
+                    Sin, m.T1, g0, 0, C.n[0], C.dir[0].clo, C.dir[0].ccnt, C.dir[0].cw, nullptr,
+                    nullptr, nullptr)));
     MG_DISPATCH(m.p, m.nc,
-                (k_mg_galerkin<KP, KN, false><<<dim3(grid1(C.nnode), m.noff), 256, 0, c.stream>>>(
-                    m.T2, C.S64, g2, 2, C.n[2], C.dir[2].clo, C.dir[2].ccnt, C.dir[2].cw)));
-    k_mg_finish<<<dim3(grid1(C.nnode), m.noff), 256, 0, c.stream>>>(C.S64, C.S32, C.dinv,
-                                                                   C.freem, grid_of(C), m.p, m.nc);
+                (k_mg_galerkin<KP, KN><<<grid1(n2 * m.noff), 256, 0, c.stream>>>(
+                    m.T1, m.T2, g1, 1, C.n[1], C.dir[1].clo, C.dir[1].ccnt, C.dir[1].cw, nullptr,
+                    nullptr, nullptr)));
+    MG_DISPATCH(m.p, m.nc,
+                (k_mg_galerkin<KP, KN><<<grid1((int64_t)C.nnode * m.noff), 256, 0, c.stream>>>(
+                    m.T2, C.S64, g2, 2, C.n[2], C.dir[2].clo, C.dir[2].ccnt, C.dir[2].cw, C.freem,
+                    C.S32, C.dinv)));
     IFEM_HIP(hipGetLastError());
   }
-  int herr = 0;
-  IFEM_HIP(hipMemcpyAsync(&herr, m.err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-  IFEM_HIP(hipStreamSynchronize(c.stream));
-  if (herr) return fail(IFEM_GPU_EDEVICE, "mg: matrix pattern differs from the stencil layout");
-  int rc = coarse_inverse(c, m);
-  if (rc) return rc;
-  // largest eigenvalue of D^-1 A per smoothed level (power iteration)
-  for (size_t l = 0; l + 1 < m.lev.size(); l++) {
+  // coarsest operator to the host (pinned, asynchronous) while the GPU runs
+  // the eigenvalue estimates
+  const MgLevel& Lc = m.lev.back();
+  IFEM_HIP(hipMemcpyAsync(m.h_coarse, Lc.S64, sizeof(double) * (size_t)m.noff * ncc * Lc.nnode,
+                          hipMemcpyDeviceToHost, c.stream));
+  IFEM_HIP(hipMemcpyAsync(m.h_err, m.err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  IFEM_HIP(hipEventRecord(m.e_copy, c.stream));
+  // largest eigenvalue of D^-1 A per smoothed level: Lanczos from CG steps
+  const int nl = (int)m.lev.size() - 1;
+  for (int l = 0; l < nl; l++) {
     MgLevel& L = m.lev[l];
     const int n = L.nvec;
-    k_mg_power_init<<<grid1(n), 256, 0, c.stream>>>(n, L.d, L.dinv);
-    for (int it = 0; it < kMgPowerIts; it++) {
-      level_spmv(c, m, (int)l, L.d, L.q, nullptr);
-      k_mg_power_scale<<<kMgParts, kThreads, 0, c.stream>>>(n, L.d, L.q, L.dinv, m.part);
-      k_mg_power_norm<<<kMgParts, kThreads, 0, c.stream>>>(n, L.d, m.part, kMgParts, m.lam + l);
+    double* h = m.lzh + (size_t)l * (3 * kMgLanczos + 1);
+    k_lz_start<<<kMgParts, kThreads, 0, c.stream>>>(n, L.r, L.d, L.x, L.dinv, m.part);
+    k_lz_sum<<<1, kThreads, 0, c.stream>>>(m.part, kMgParts, h);
+    for (int j = 0; j < kMgLanczos; j++) {
+      level_spmv(c, m, l, L.x, L.q, nullptr);
+      k_lz_dot<<<kMgParts, kThreads, 0, c.stream>>>(n, L.x, L.q, m.part);
+      k_lz_update<<<kMgParts, kThreads, 0, c.stream>>>(n, L.r, L.d, L.q, L.dinv, m.part, kMgParts,
+                                                       h, j, m.part2);
+      k_lz_pupdate<<<kMgParts, kThreads, 0, c.stream>>>(n, L.x, L.d, m.part2, kMgParts, h, j);
     }
     IFEM_HIP(hipGetLastError());
   }
-  std::vector<double> lam(m.lev.size(), 0.0);
-  IFEM_HIP(hipMemcpyAsync(lam.data(), m.lam, sizeof(double) * (m.lev.size() - 1),
+  IFEM_HIP(hipMemcpyAsync(m.h_lzh, m.lzh, sizeof(double) * (size_t)nl * (3 * kMgLanczos + 1),
                           hipMemcpyDeviceToHost, c.stream));
+  // the iteration vectors must start from zero again
+  for (int l = 0; l < (int)m.lev.size(); l++)
+    k_fill<float><<<grid1(m.lev[l].nvec), 256, 0, c.stream>>>(m.lev[l].nvec, m.lev[l].x, 0.f);
+  IFEM_HIP(hipEventSynchronize(m.e_copy));
+  if (*m.h_err) return fail(IFEM_GPU_EDEVICE, "mg: matrix pattern differs from the stencil layout");
+  int rc = coarse_inverse(c, m);
+  if (rc) return rc;
   IFEM_HIP(hipEventRecord(m.e1, c.stream));
   IFEM_HIP(hipEventSynchronize(m.e1));
-  for (size_t l = 0; l + 1 < m.lev.size(); l++) {
-    if (!(lam[l] > 0.0) || !std::isfinite(lam[l]))
+  for (int l = 0; l < nl; l++) {
+    const double lam = lanczos_max(m.h_lzh + (size_t)l * (3 * kMgLanczos + 1), kMgLanczos);
+    if (!(lam > 0.0) || !std::isfinite(lam))
       return fail(IFEM_GPU_EDEVICE, "mg: eigenvalue estimate failed");
-    m.lev[l].lmax = lam[l];
+    m.lev[l].lmax = lam;
   }
   float ms = 0.f;
   IFEM_HIP(hipEventElapsedTime(&ms, m.e0, m.e1));
@@ -904,11 +1000,12 @@ static void vcycle(Context& c, Multigrid& m, int l, const double* scal)
   const int n = L.nvec;
   const unsigned gb = std::min<unsigned>(grid1(n), 4096);
   if (l == (int)m.lev.size() - 1) {
-    k_mg_coarse<<<grid1(m.ncf), 256, 0, c.stream>>>(m.ncf, m.cdofs, m.cinv, L.b, L.x, scal);
+    k_mg_coarse<<<(unsigned)((m.ncf + 3) / 4), 256, 0, c.stream>>>(m.ncf, m.cdofs, m.cinv, L.b,
+                                                                   L.x, scal);
     return;
   }
   // Chebyshev coefficients on [0.1, 1.1] x lambda_max
-  const double hi = 1.1 * L.lmax, lo = 0.1 * L.lmax;
+  const double hi = 1.05 * L.lmax, lo = 0.1 * L.lmax;
   const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
   std::vector<float> c1(m.degree), c2(m.degree);
   double rho = 1.0 / sigma;

@@ -38,7 +38,7 @@ struct MgLevel {
   int nvec = 0;             // vector length: neq on level 0, ndof below
   int nfree = 0;
   MgDir dir[3];             // from the finer level (unused on level 0)
-  double* S64 = nullptr;    // stencil [noff][nc][nc][nnode] (coarse levels)
+  double* S64 = nullptr;    // stencil [nnode][noff][nc][nc] (coarse levels)
   float* S32 = nullptr;
   unsigned char* freem = nullptr; // per DOF, coarse levels
   float* dinv = nullptr;    // per vector entry, 0 at eliminated DOFs
@@ -62,10 +62,14 @@ struct Multigrid {
   int ncf = 0;              // free DOFs of the coarsest level
   int* cdofs = nullptr;
   float* cinv = nullptr;    // dense inverse of the coarsest operator
-  double* part = nullptr;   // partial sums (power iteration)
-  double* lam = nullptr;    // per level eigenvalue estimates (device)
+  double *part = nullptr, *part2 = nullptr; // partial sums (eigenvalue estimate)
+  double* lam = nullptr;
+  double* lzh = nullptr;    // Lanczos coefficients per level (device)
+  double* h_lzh = nullptr;  // pinned host copies
+  double* h_coarse = nullptr;
+  int* h_err = nullptr;
   int* err = nullptr;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr, e_copy = nullptr;
   double setup_ms = 0.0;
   ~Multigrid();
 };

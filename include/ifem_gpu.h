@@ -299,9 +299,9 @@ int ifem_gpu_set_learned_preconditioner(ifem_gpu_ctx* ctx, int enable, int D, in
  * interior knot of level l per direction.  ifem_gpu_mg_level: control points
  * per direction, free DOFs and lambda_max(D^-1 A) of level l (0 on the
  * coarsest); with S non-NULL (levels >= 1) the Galerkin operator P^T A P in
- * stencil form, S[((off*nc + ci)*nc + cj)*nnode + node], node = i + n0*(j +
- * n1*k), off = ((e3+p)*(2p+1) + e2+p)*(2p+1) + e1+p for the neighbour
- * (i+e1, j+e2, k+e3).  ifem_gpu_mg_apply: z = M^-1 r, one V-cycle, host
+ * stencil form, S[((node*noff + off)*nc + ci)*nc + cj], node = i + n0*(j +
+ * n1*k), noff = (2p+1)^3, off = ((e3+p)*(2p+1) + e2+p)*(2p+1) + e1+p for the
+ * neighbour (i+e1, j+e2, k+e3).  ifem_gpu_mg_apply: z = M^-1 r, one V-cycle, host
  * vectors in equation order. */
 int ifem_gpu_mg_setup(ifem_gpu_ctx* ctx, int smooth, int* nlevels, double* setup_ms);
 int ifem_gpu_mg_level(ifem_gpu_ctx* ctx, int level, int n[3], int* nfree, double* lmax,

@@ -72,7 +72,7 @@ def _dof_prolongation(patch, ncomp):
 def _stencil_to_dense(S, n, p, nc):
     nnode = n[0] * n[1] * n[2]
     W1 = 2 * p + 1
-    S = S.reshape(W1 ** 3, nc, nc, nnode)
+    S = S.reshape(nnode, W1 ** 3, nc, nc).transpose(1, 2, 3, 0)
     A = np.zeros((nnode * nc, nnode * nc))
     k, j, i = [a.ravel() for a in np.meshgrid(range(n[2]), range(n[1]), range(n[0]), indexing="ij")]
     node = np.arange(nnode)

@@ -14,8 +14,11 @@
 //   * smoothing: Chebyshev iteration on D^-1 A over [0.1, 1.05] x lambda_max
 //     (lambda_max from 10 Lanczos steps), symmetric pre/post; the coarsest
 //     level is solved with a dense inverse (Cholesky on the host, small);
-//   * level 0 smooths with the FP32 CSR matrix in equation order; coarse
-//     levels use FP32 stencils in DOF order with zeros at eliminated DOFs.
+//   * level 0 smooths with an FP16 copy of the CSR matrix (scaled by its
+//     largest diagonal entry, FP32 sums) in equation order; coarse levels
+//     use FP32 stencils in DOF order with zeros at eliminated DOFs.
+#include <hip/hip_fp16.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -26,7 +29,8 @@
 
 namespace ifem {
 
-void launch_spmv32(Context& c, const float* val, const float* p, float* q, bool skip_done);
+void launch_spmv16(Context& c, const __half* val, float scale, const float* p, float* q,
+                   bool skip_done, hipStream_t st);
 
 Multigrid::~Multigrid()
 {
@@ -40,7 +44,7 @@ Multigrid::~Multigrid()
     hipFree(L.x); hipFree(L.b); hipFree(L.r); hipFree(L.d); hipFree(L.q);
   }
   hipFree(S0); hipFree(T1); hipFree(T2);
-  hipFree(val32); hipFree(cdofs); hipFree(cinv); hipFree(part); hipFree(lam); hipFree(err);
+  hipFree(val16); hipFree(cdofs); hipFree(cinv); hipFree(part); hipFree(lam); hipFree(err);
   hipFree(dof_of_eq); hipFree(part2); hipFree(lzh);
   if (h_lzh) hipHostFree(h_lzh);
   if (h_coarse) hipHostFree(h_coarse);
@@ -582,11 +586,56 @@ __global__ void k_mg_cast_out(int n, const float* __restrict__ a, V* __restrict_
   if (i < n) b[i] = (V)a[i];
 }
 
-__global__ void k_mg_demote(int64_t n, const double* __restrict__ a, float* __restrict__ b)
+// FP16 copy of K / scale for the fine-level smoother (|K_ij| <= max diag for
+// an SPD matrix, so with scale = max diag every value lies in [-1, 1]).
+__global__ void k_mg_demote16(int64_t n, const double* __restrict__ a, __half* __restrict__ b,
+                              double inv_scale)
 {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = (float)a[i];
+    b[i] = __float2half((float)(a[i] * inv_scale));
+}
+
+// FP32 inverse diagonal of the FP64 CSR matrix, and per-block maxima of
+// the diagonal.
+__global__ __launch_bounds__(kThreads) void k_mg_diag(int n, const int64_t* __restrict__ rowptr,
+                                                      const int32_t* __restrict__ colind,
+                                                      const double* __restrict__ val,
+                                                      float* __restrict__ dinv,
+                                                      double* __restrict__ part)
+{
+  __shared__ double sh[kThreads / 64];
+  double mx = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    int64_t lo = rowptr[i], hi = rowptr[i + 1] - 1;
+    double d = 0.0;
+    while (lo <= hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      const int cc = colind[mid];
+      if (cc == i) { d = val[mid]; break; }
+      if (cc < i) lo = mid + 1; else hi = mid - 1;
+    }
+    dinv[i] = d > 0.0 ? (float)(1.0 / d) : 0.f;
+    mx = fmax(mx, fabs(d));
+  }
+  for (int o = 32; o >= 1; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) sh[wave] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < kThreads / 64; w++) m = fmax(m, sh[w]);
+    part[blockIdx.x] = m;
+  }
+}
+
+__global__ void k_mg_max(const double* __restrict__ part, int nparts, double* __restrict__ out)
+{
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double m = 0.0;
+    for (int i = 0; i < nparts; i++) m = fmax(m, part[i]);
+    *out = m;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -801,16 +850,35 @@ static int mg_build(Context& c, Multigrid& m)
   } while (0)
 
 static void level_spmv(Context& c, Multigrid& m, int l, const float* x, float* y,
-                       const double* scal)
+                       const double* scal, hipStream_t st = nullptr)
 {
+  if (!st) st = c.stream;
   if (l == 0) {
-    launch_spmv32(c, m.val32 ? m.val32 : c.d_val32, x, y, scal != nullptr);
+    launch_spmv16(c, m.val16, (float)m.kscale, x, y, scal != nullptr, st);
     return;
   }
   const MgLevel& L = m.lev[l];
   const unsigned gb = (unsigned)(((int64_t)L.nnode + 3) / 4); // one wave per node
-  MG_DISPATCH(m.p, m.nc,
-              (k_mg_spmv<KP, KN><<<gb, 256, 0, c.stream>>>(L.S32, x, y, grid_of(L), scal)));
+  MG_DISPATCH(m.p, m.nc, (k_mg_spmv<KP, KN><<<gb, 256, 0, st>>>(L.S32, x, y, grid_of(L), scal)));
+}
+
+// Lanczos eigenvalue estimate of level l on stream st, partial buffers
+// part/part2 (kMgParts each).
+static void lanczos_level(Context& c, Multigrid& m, int l, hipStream_t st, double* part,
+                          double* part2)
+{
+  MgLevel& L = m.lev[l];
+  const int n = L.nvec;
+  double* h = m.lzh + (size_t)l * (3 * kMgLanczos + 1);
+  k_lz_start<<<kMgParts, kThreads, 0, st>>>(n, L.r, L.d, L.x, L.dinv, part);
+  k_lz_sum<<<1, kThreads, 0, st>>>(part, kMgParts, h);
+  for (int j = 0; j < kMgLanczos; j++) {
+    level_spmv(c, m, l, L.x, L.q, nullptr, st);
+    k_lz_dot<<<kMgParts, kThreads, 0, st>>>(n, L.x, L.q, part);
+    k_lz_update<<<kMgParts, kThreads, 0, st>>>(n, L.r, L.d, L.q, L.dinv, part, kMgParts, h, j,
+                                               part2);
+    k_lz_pupdate<<<kMgParts, kThreads, 0, st>>>(n, L.x, L.d, part2, kMgParts, h, j);
+  }
 }
 
 // Dense inverse of the coarsest operator (host Cholesky; the level is small).
@@ -896,17 +964,16 @@ int mg_setup(Context& c, int degree)
   Multigrid& m = *c.mg;
   m.degree = degree;
   IFEM_HIP(hipEventRecord(m.e0, c.stream));
-  // FP32 fine values: the FP32 assembly, else a copy of the FP64 one
-  if (!c.fp32_ready) {
-    if (!m.val32) IFEM_HIP(hipMalloc(&m.val32, sizeof(float) * (c.nnz > 0 ? c.nnz : 1)));
-    k_mg_demote<<<4096, 256, 0, c.stream>>>(c.nnz, c.d_val, m.val32);
-  } else if (m.val32) {
-    hipFree(m.val32);
-    m.val32 = nullptr;
-  }
-  const float* v32 = m.val32 ? m.val32 : c.d_val32;
+  // fine level: inverse diagonal, then K / max diag in FP16 for the smoother
   MgLevel& F = m.lev[0];
-  k_diag_inv<float><<<grid1(c.neq), 256, 0, c.stream>>>(c.neq, c.d_rowptr, c.d_colind, v32, F.dinv);
+  k_mg_diag<<<kMgParts, kThreads, 0, c.stream>>>(c.neq, c.d_rowptr, c.d_colind, c.d_val, F.dinv,
+                                                 m.part);
+  k_mg_max<<<1, 64, 0, c.stream>>>(m.part, kMgParts, m.lzh);
+  IFEM_HIP(hipMemcpyAsync(m.h_lzh, m.lzh, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  IFEM_HIP(hipStreamSynchronize(c.stream));
+  m.kscale = m.h_lzh[0] > 0.0 ? m.h_lzh[0] : 1.0;
+  if (!m.val16) IFEM_HIP(hipMalloc(&m.val16, sizeof(__half) * (c.nnz > 0 ? c.nnz : 1)));
+  k_mg_demote16<<<4096, 256, 0, c.stream>>>(c.nnz, c.d_val, m.val16, 1.0 / m.kscale);
   // fine operator as a stencil, then Galerkin products level by level
   IFEM_HIP(hipMemsetAsync(m.err, 0, sizeof(int), c.stream));
   const int ncc = m.nc * m.nc;
@@ -949,21 +1016,8 @@ int mg_setup(Context& c, int degree)
   IFEM_HIP(hipEventRecord(m.e_copy, c.stream));
   // largest eigenvalue of D^-1 A per smoothed level: Lanczos from CG steps
   const int nl = (int)m.lev.size() - 1;
-  for (int l = 0; l < nl; l++) {
-    MgLevel& L = m.lev[l];
-    const int n = L.nvec;
-    double* h = m.lzh + (size_t)l * (3 * kMgLanczos + 1);
-    k_lz_start<<<kMgParts, kThreads, 0, c.stream>>>(n, L.r, L.d, L.x, L.dinv, m.part);
-    k_lz_sum<<<1, kThreads, 0, c.stream>>>(m.part, kMgParts, h);
-    for (int j = 0; j < kMgLanczos; j++) {
-      level_spmv(c, m, l, L.x, L.q, nullptr);
-      k_lz_dot<<<kMgParts, kThreads, 0, c.stream>>>(n, L.x, L.q, m.part);
-      k_lz_update<<<kMgParts, kThreads, 0, c.stream>>>(n, L.r, L.d, L.q, L.dinv, m.part, kMgParts,
-                                                       h, j, m.part2);
-      k_lz_pupdate<<<kMgParts, kThreads, 0, c.stream>>>(n, L.x, L.d, m.part2, kMgParts, h, j);
-    }
-    IFEM_HIP(hipGetLastError());
-  }
+  for (int l = 0; l < nl; l++) lanczos_level(c, m, l, c.stream, m.part, m.part2);
+  IFEM_HIP(hipGetLastError());
   IFEM_HIP(hipMemcpyAsync(m.h_lzh, m.lzh, sizeof(double) * (size_t)nl * (3 * kMgLanczos + 1),
                           hipMemcpyDeviceToHost, c.stream));
   // the iteration vectors must start from zero again

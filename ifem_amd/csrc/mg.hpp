@@ -9,6 +9,7 @@
 // a dense inverse.
 #pragma once
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -58,7 +59,8 @@ struct Multigrid {
   int* dof_of_eq = nullptr; // fine equation -> DOF
   double *T1 = nullptr, *T2 = nullptr;
   size_t s0cap = 0, t1cap = 0, t2cap = 0;
-  float* val32 = nullptr;   // FP32 copy of K when no FP32 assembly exists
+  __half* val16 = nullptr;  // K / kscale in FP16 (fine-level smoother)
+  double kscale = 1.0;      // largest diagonal entry of K
   int ncf = 0;              // free DOFs of the coarsest level
   int* cdofs = nullptr;
   float* cinv = nullptr;    // dense inverse of the coarsest operator

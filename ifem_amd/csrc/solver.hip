@@ -24,6 +24,10 @@
 #include <cstring>
 #include <string>
 
+#include <hip/hip_fp16.h>
+
+#include <type_traits>
+
 #include "devutil.hpp"
 #include "internal.hpp"
 
@@ -32,11 +36,12 @@ namespace ifem {
 // ---------------------------------------------------------------------------
 // SpMV q = A p (CSR), LANES lanes per row; with DOT, per-block partial <p,q>.
 // SKIP: return at once when scal[S_DONE] is set (work queued past convergence).
-template <typename V, typename ACC, int LANES, bool DOT, bool SKIP = DOT>
+// VV: type of the stored values (V, or FP16 values scaled by 1/scale).
+template <typename V, typename ACC, int LANES, bool DOT, bool SKIP = DOT, typename VV = V>
 __global__ __launch_bounds__(kThreads) void k_spmv_dot(
     int n, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
-    const V* __restrict__ val, const V* __restrict__ p, V* __restrict__ q,
-    double* __restrict__ part_pq, const double* __restrict__ scal)
+    const VV* __restrict__ val, const V* __restrict__ p, V* __restrict__ q,
+    double* __restrict__ part_pq, const double* __restrict__ scal, ACC scale)
 {
   if (SKIP && scal[S_DONE] != 0.0) return;
   __shared__ double sh[kThreads / 64];
@@ -49,6 +54,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv_dot(
     for (int64_t k = b + l; k < e; k += LANES) s += (ACC)val[k] * (ACC)p[colind[k]];
 #pragma unroll
     for (int off = LANES / 2; off >= 1; off >>= 1) s += __shfl_xor(s, off, LANES);
+    if (!std::is_same<VV, V>::value) s *= scale;
     if (l == 0) {
       q[row] = (V)s;
       if (DOT) pq += (double)((ACC)p[row] * s);
@@ -64,11 +70,11 @@ __global__ __launch_bounds__(kThreads) void k_spmv_dot(
 // gather reads one node index (bcol) per 9 values.  One wave per node; the
 // CSR values are read in place.  Blocks are mapped to the 8 XCDs in
 // contiguous node ranges so each XCD's L2 holds its slice of p.
-template <typename V, typename ACC, bool DOT, bool SKIP = DOT>
+template <typename V, typename ACC, bool DOT, bool SKIP = DOT, typename VV = V>
 __global__ __launch_bounds__(kThreads) void k_spmv_node3(
     int nnode, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ bcol,
-    const V* __restrict__ val, const V* __restrict__ p, V* __restrict__ q,
-    double* __restrict__ part_pq, const double* __restrict__ scal)
+    const VV* __restrict__ val, const V* __restrict__ p, V* __restrict__ q,
+    double* __restrict__ part_pq, const double* __restrict__ scal, ACC scale)
 {
   if (SKIP && scal[S_DONE] != 0.0) return;
   __shared__ double sh[kThreads / 64];
@@ -82,9 +88,9 @@ __global__ __launch_bounds__(kThreads) void k_spmv_node3(
     const int64_t r0 = rowptr[3 * nd];
     const int cnt = (int)((rowptr[3 * nd + 1] - r0) / 3);
     const int64_t nb = r0 / 9;
-    const V* v0 = val + r0;
-    const V* v1 = v0 + 3 * cnt;
-    const V* v2 = v1 + 3 * cnt;
+    const VV* v0 = val + r0;
+    const VV* v1 = v0 + 3 * cnt;
+    const VV* v2 = v1 + 3 * cnt;
     ACC s0 = 0, s1 = 0, s2 = 0;
     for (int k = lane; k < cnt; k += 64) {
       const int64_t c = 3 * (int64_t)bcol[nb + k];
@@ -98,6 +104,11 @@ __global__ __launch_bounds__(kThreads) void k_spmv_node3(
       s0 += __shfl_xor(s0, off, 64);
       s1 += __shfl_xor(s1, off, 64);
       s2 += __shfl_xor(s2, off, 64);
+    }
+    if (!std::is_same<VV, V>::value) {
+      s0 *= scale;
+      s1 *= scale;
+      s2 *= scale;
     }
     if (lane == 0) {
       q[3 * nd] = (V)s0;
@@ -276,26 +287,28 @@ static int lanes_for(const Context& c)
   return 16;
 }
 
-template <typename V, typename ACC, bool DOT, bool SKIP = DOT>
-static void launch_spmv(Context& c, const V* val, const V* p, V* q)
+template <typename V, typename ACC, bool DOT, bool SKIP = DOT, typename VV = V>
+static void launch_spmv(Context& c, const VV* val, const V* p, V* q, ACC scale = ACC(1),
+                        hipStream_t st = nullptr)
 {
   const int L = lanes_for(c);
   SolverWork& w = c.w;
   const int grid = kGrid;
+  if (!st) st = c.stream;
   if (c.d_bcol) {
-    k_spmv_node3<V, ACC, DOT, SKIP><<<grid, kThreads, 0, c.stream>>>(
-        c.neq / 3, c.d_rowptr, c.d_bcol, val, p, q, w.partial, w.scal);
+    k_spmv_node3<V, ACC, DOT, SKIP, VV><<<grid, kThreads, 0, st>>>(
+        c.neq / 3, c.d_rowptr, c.d_bcol, val, p, q, w.partial, w.scal, scale);
     return;
   }
   if (L == 64)
-    k_spmv_dot<V, ACC, 64, DOT, SKIP><<<grid, kThreads, 0, c.stream>>>(
-        c.neq, c.d_rowptr, c.d_colind, val, p, q, w.partial, w.scal);
+    k_spmv_dot<V, ACC, 64, DOT, SKIP, VV><<<grid, kThreads, 0, st>>>(
+        c.neq, c.d_rowptr, c.d_colind, val, p, q, w.partial, w.scal, scale);
   else if (L == 32)
-    k_spmv_dot<V, ACC, 32, DOT, SKIP><<<grid, kThreads, 0, c.stream>>>(
-        c.neq, c.d_rowptr, c.d_colind, val, p, q, w.partial, w.scal);
+    k_spmv_dot<V, ACC, 32, DOT, SKIP, VV><<<grid, kThreads, 0, st>>>(
+        c.neq, c.d_rowptr, c.d_colind, val, p, q, w.partial, w.scal, scale);
   else
-    k_spmv_dot<V, ACC, 16, DOT, SKIP><<<grid, kThreads, 0, c.stream>>>(
-        c.neq, c.d_rowptr, c.d_colind, val, p, q, w.partial, w.scal);
+    k_spmv_dot<V, ACC, 16, DOT, SKIP, VV><<<grid, kThreads, 0, st>>>(
+        c.neq, c.d_rowptr, c.d_colind, val, p, q, w.partial, w.scal, scale);
 }
 
 // FP32 SpMV for the multigrid smoother on the finest level.
@@ -303,6 +316,14 @@ void launch_spmv32(Context& c, const float* val, const float* p, float* q, bool 
 {
   if (skip_done) launch_spmv<float, float, false, true>(c, val, p, q);
   else launch_spmv<float, float, false, false>(c, val, p, q);
+}
+
+// Same with FP16 values holding K / scale (y = scale * (val x), FP32 sums).
+void launch_spmv16(Context& c, const __half* val, float scale, const float* p, float* q,
+                   bool skip_done, hipStream_t st)
+{
+  if (skip_done) launch_spmv<float, float, false, true, __half>(c, val, p, q, scale, st);
+  else launch_spmv<float, float, false, false, __half>(c, val, p, q, scale, st);
 }
 
 int spmv_device(Context& c, const double* d_x, double* d_y)

@@ -126,6 +126,7 @@ int ifem_gpu_ctx_destroy(ifem_gpu_ctx* ctx)
   ctx->c.batch.reset();
   hipFree(ctx->c.d_asmargs);
   hipFree(ctx->c.d_gpgeo);
+  hipFree(ctx->c.d_facew);
   hipEventDestroy(ctx->c.ev0);
   hipEventDestroy(ctx->c.ev1);
   hipEventDestroy(ctx->c.ev_mid);

@@ -1170,12 +1170,14 @@ int assemble_volume(Context& c, const ifem_gpu_integrand& itg, const int ng[3], 
 // fixed order (src/ASM/ASMs3D.C:2547-2812; face utl::Jacobian,
 // src/Utility/CoordinateMapping.C:90-133: dS = |J_t1 x J_t2|, zero if J is
 // singular; detJxW = dS * dA * w_i * w_j with dA = 0.25 * area).
+// Pass 1: one thread per (face element, Gauss point): w = dS * dA * w_i * w_j
+// (0 where J is singular), face element (ea, eb) along the tangent
+// directions, w[((eb*nel_ta + ea)*ng + gb)*ng + ga].
 template <int P>
-__global__ void k_traction(int n1, int n2, int n3, int nel1, int nel2, int nel3,
-                           const double* __restrict__ coefs, const double* k1,
-                           const double* k2, const double* k3, int face, int ng, GaussArgs ga,
-                           double t0, double t1, double t2, int nc, const int* __restrict__ eq,
-                           double* __restrict__ rhs)
+__global__ void k_traction_w(int n1, int n2, int n3, int nel1, int nel2, int nel3,
+                             const double* __restrict__ coefs, const double* k1,
+                             const double* k2, const double* k3, int face, int ng, GaussArgs ga,
+                             double* __restrict__ wout)
 {
   constexpr int P1 = P + 1;
   const int dn = face / 2, side = face % 2;
@@ -1183,78 +1185,116 @@ __global__ void k_traction(int n1, int n2, int n3, int nel1, int nel2, int nel3,
   const double* kn[3] = {k1, k2, k3};
   const int ta = dn == 0 ? 1 : 0, tb = dn == 2 ? 1 : 2; // tangent directions, ascending
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nf = n[ta] * n[tb];
-  if (id >= nf * nc) return;
-  const int comp = id % nc, fnode = id / nc;
-  const int ia = fnode % n[ta], ib = fnode / n[ta];
+  const int ntot = nel[ta] * nel[tb] * ng * ng;
+  if (id >= ntot) return;
+  const int gA = id % ng, gb = (id / ng) % ng, fe = id / (ng * ng);
+  const int ea = fe % nel[ta], eb = fe / nel[ta];
+  // faceDir = +-(dn+1): t1 = 1 + |faceDir| % 3, t2 = 1 + t1 % 3 (1-based)
+  const int T1 = (dn + 1) % 3, T2 = (T1 + 1) % 3;
+  int e[3];
+  e[dn] = side ? nel[dn] - 1 : 0;
+  e[ta] = ea;
+  e[tb] = eb;
+  double spn[3];
+  for (int d = 0; d < 3; d++) spn[d] = kn[d][e[d] + P + 1] - kn[d][e[d] + P];
+  const double dA = 0.25 * (spn[ta] * spn[tb]);
+  double Nd[3][P1], dNd[3][P1];
+  for (int d = 0; d < 3; d++) {
+    const int s = e[d] + P;
+    const double* kt = kn[d];
+    double u;
+    if (d == dn) u = side ? kt[n[d]] : kt[P];
+    else {
+      const double xi = ga.xi[d == ta ? gA : gb];
+      u = 0.5 * ((kt[s + 1] - kt[s]) * xi + kt[s + 1] + kt[s]);
+    }
+    bspline_1d<P>(kt, s, u, Nd[d], dNd[d]);
+  }
+  double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int b3 = 0; b3 < P1; b3++)
+    for (int b2 = 0; b2 < P1; b2++)
+      for (int b1 = 0; b1 < P1; b1++) {
+        const int64_t nb = (int64_t)(e[0] + b1) + (int64_t)n1 * ((e[1] + b2) + (int64_t)n2 * (e[2] + b3));
+        const double g0 = dNd[0][b1] * Nd[1][b2] * Nd[2][b3];
+        const double g1 = Nd[0][b1] * dNd[1][b2] * Nd[2][b3];
+        const double g2 = Nd[0][b1] * Nd[1][b2] * dNd[2][b3];
+        for (int i = 0; i < 3; i++) {
+          const double x = coefs[3 * nb + i];
+          J[3 * i + 0] += x * g0;
+          J[3 * i + 1] += x * g1;
+          J[3 * i + 2] += x * g2;
+        }
+      }
+  const double v1[3] = {J[T1], J[3 + T1], J[6 + T1]};
+  const double v2[3] = {J[T2], J[3 + T2], J[6 + T2]};
+  const double nx = v1[1] * v2[2] - v1[2] * v2[1];
+  const double ny = v1[2] * v2[0] - v1[0] * v2[2];
+  const double nz = v1[0] * v2[1] - v1[1] * v2[0];
+  double dS = sqrt(nx * nx + ny * ny + nz * nz);
+  const double det = J[0] * (J[4] * J[8] - J[7] * J[5]) - J[1] * (J[3] * J[8] - J[6] * J[5]) +
+                     J[2] * (J[3] * J[7] - J[6] * J[4]);
+  if (det <= 1e-12 && det >= -1e-12) dS = 0.0;
+  wout[id] = dS == 0.0 ? 0.0 : dS * (dA * ga.w[gA] * ga.w[gb]);
+}
+
+// Pass 2: one thread per face node sums w * N_a over the face elements and
+// Gauss points holding it (fixed order), then adds t_c times the sum to the
+// node's free components.
+template <int P>
+__global__ void k_traction_sum(int n1, int n2, int n3, int nel1, int nel2, int nel3,
+                               const double* k1, const double* k2, const double* k3, int face,
+                               int ng, GaussArgs ga, const double* __restrict__ wf, double t0,
+                               double t1, double t2, int nc, const int* __restrict__ eq,
+                               double* __restrict__ rhs)
+{
+  constexpr int P1 = P + 1;
+  const int dn = face / 2, side = face % 2;
+  const int n[3] = {n1, n2, n3}, nel[3] = {nel1, nel2, nel3};
+  const double* kn[3] = {k1, k2, k3};
+  const int ta = dn == 0 ? 1 : 0, tb = dn == 2 ? 1 : 2;
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n[ta] * n[tb]) return;
+  const int ia = id % n[ta], ib = id / n[ta];
   int idx[3];
   idx[dn] = side ? n[dn] - 1 : 0;
   idx[ta] = ia;
   idx[tb] = ib;
   const int64_t node = (int64_t)idx[0] + (int64_t)n1 * (idx[1] + (int64_t)n2 * idx[2]);
-  const int r = eq[node * nc + comp];
-  if (r < 0) return;
-  const double tv = comp == 0 ? t0 : (comp == 1 ? t1 : t2);
-  // faceDir = +-(dn+1): t1 = 1 + |faceDir| % 3, t2 = 1 + t1 % 3 (1-based)
-  const int T1 = (dn + 1) % 3, T2 = (T1 + 1) % 3;
+  // basis value of the node in the normal direction on the face
   const int en = side ? nel[dn] - 1 : 0;
+  double Nn[P1], dNn[P1];
+  {
+    const double* kt = kn[dn];
+    bspline_1d<P>(kt, en + P, side ? kt[n[dn]] : kt[P], Nn, dNn);
+  }
+  const double nface = Nn[idx[dn] - en];
   double sum = 0.0;
   for (int eb = max(0, ib - P); eb <= min(nel[tb] - 1, ib); eb++)
     for (int ea = max(0, ia - P); ea <= min(nel[ta] - 1, ia); ea++) {
-      int e[3];
-      e[dn] = en;
-      e[ta] = ea;
-      e[tb] = eb;
-      double spn[3];
-      for (int d = 0; d < 3; d++) spn[d] = kn[d][e[d] + P + 1] - kn[d][e[d] + P];
-      const double dA = 0.25 * (spn[ta] * spn[tb]);
-      int loc[3];
-      for (int d = 0; d < 3; d++) loc[d] = idx[d] - e[d];
-      for (int gb = 0; gb < ng; gb++)
+      const double* w = wf + (int64_t)(eb * nel[ta] + ea) * ng * ng;
+      for (int gb = 0; gb < ng; gb++) {
+        double Nb[P1], dNb[P1];
+        const double* kb = kn[tb];
+        const int sb = eb + P;
+        bspline_1d<P>(kb, sb, 0.5 * ((kb[sb + 1] - kb[sb]) * ga.xi[gb] + kb[sb + 1] + kb[sb]), Nb,
+                      dNb);
         for (int gA = 0; gA < ng; gA++) {
-          double Nd[3][P1], dNd[3][P1];
-          for (int d = 0; d < 3; d++) {
-            const int s = e[d] + P;
-            const double* kt = kn[d];
-            double u;
-            if (d == dn) u = side ? kt[n[d]] : kt[P];
-            else {
-              const double xi = ga.xi[d == ta ? gA : gb];
-              u = 0.5 * ((kt[s + 1] - kt[s]) * xi + kt[s + 1] + kt[s]);
-            }
-            bspline_1d<P>(kt, s, u, Nd[d], dNd[d]);
-          }
-          double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-          for (int b3 = 0; b3 < P1; b3++)
-            for (int b2 = 0; b2 < P1; b2++)
-              for (int b1 = 0; b1 < P1; b1++) {
-                const int64_t nb = (int64_t)(e[0] + b1) + (int64_t)n1 * ((e[1] + b2) + (int64_t)n2 * (e[2] + b3));
-                const double g0 = dNd[0][b1] * Nd[1][b2] * Nd[2][b3];
-                const double g1 = Nd[0][b1] * dNd[1][b2] * Nd[2][b3];
-                const double g2 = Nd[0][b1] * Nd[1][b2] * dNd[2][b3];
-                for (int i = 0; i < 3; i++) {
-                  const double x = coefs[3 * nb + i];
-                  J[3 * i + 0] += x * g0;
-                  J[3 * i + 1] += x * g1;
-                  J[3 * i + 2] += x * g2;
-                }
-              }
-          const double v1[3] = {J[T1], J[3 + T1], J[6 + T1]};
-          const double v2[3] = {J[T2], J[3 + T2], J[6 + T2]};
-          const double nx = v1[1] * v2[2] - v1[2] * v2[1];
-          const double ny = v1[2] * v2[0] - v1[0] * v2[2];
-          const double nz = v1[0] * v2[1] - v1[1] * v2[0];
-          double dS = sqrt(nx * nx + ny * ny + nz * nz);
-          const double det = J[0] * (J[4] * J[8] - J[7] * J[5]) - J[1] * (J[3] * J[8] - J[6] * J[5]) +
-                             J[2] * (J[3] * J[7] - J[6] * J[4]);
-          if (det <= 1e-12 && det >= -1e-12) dS = 0.0;
-          if (dS == 0.0) continue;
-          const double w = dS * (dA * ga.w[gA] * ga.w[gb]);
-          const double Na = Nd[0][loc[0]] * Nd[1][loc[1]] * Nd[2][loc[2]];
-          sum += w * tv * Na;
+          const double wg = w[gb * ng + gA];
+          if (wg == 0.0) continue;
+          double Na[P1], dNa[P1];
+          const double* ka = kn[ta];
+          const int sa = ea + P;
+          bspline_1d<P>(ka, sa, 0.5 * ((ka[sa + 1] - ka[sa]) * ga.xi[gA] + ka[sa + 1] + ka[sa]),
+                        Na, dNa);
+          sum += wg * (Na[ia - ea] * Nb[ib - eb] * nface);
         }
+      }
     }
-  rhs[r] += sum;
+  const double tv[3] = {t0, t1, t2};
+  for (int c = 0; c < nc; c++) {
+    const int r = eq[node * nc + c];
+    if (r >= 0) rhs[r] += tv[c] * sum;
+  }
 }
 
 int assemble_traction(Context& c, int face0, const double* t, int ng)
@@ -1264,12 +1304,23 @@ int assemble_traction(Context& c, int face0, const double* t, int ng)
   gauss_rule(ng, ga.xi, ga.w);
   const int dn = face0 / 2;
   const int ta = dn == 0 ? 1 : 0, tb = dn == 2 ? 1 : 2;
-  const int tot = Pt.n[ta] * Pt.n[tb] * c.ncomp;
+  const int nfn = Pt.n[ta] * Pt.n[tb];
+  const int nw = Pt.nel[ta] * Pt.nel[tb] * ng * ng;
   const double t1 = c.ncomp > 1 ? t[1] : 0.0, t2 = c.ncomp > 2 ? t[2] : 0.0;
+  if (c.facew_cap < (size_t)nw) {
+    hipFree(c.d_facew);
+    c.d_facew = nullptr;
+    c.facew_cap = 0;
+    IFEM_HIP(hipMalloc(&c.d_facew, sizeof(double) * (nw > 0 ? nw : 1)));
+    c.facew_cap = nw;
+  }
 #define LAUNCH_TR(PP)                                                                          \
-  k_traction<PP><<<(tot + 127) / 128, 128, 0, c.stream>>>(                                      \
+  k_traction_w<PP><<<(nw + 255) / 256, 256, 0, c.stream>>>(                                     \
       Pt.n[0], Pt.n[1], Pt.n[2], Pt.nel[0], Pt.nel[1], Pt.nel[2], Pt.d_coefs, Pt.d_knots[0],    \
-      Pt.d_knots[1], Pt.d_knots[2], face0, ng, ga, t[0], t1, t2, c.ncomp, c.d_eq, c.d_rhs)
+      Pt.d_knots[1], Pt.d_knots[2], face0, ng, ga, c.d_facew);                                  \
+  k_traction_sum<PP><<<(nfn + 127) / 128, 128, 0, c.stream>>>(                                  \
+      Pt.n[0], Pt.n[1], Pt.n[2], Pt.nel[0], Pt.nel[1], Pt.nel[2], Pt.d_knots[0], Pt.d_knots[1], \
+      Pt.d_knots[2], face0, ng, ga, c.d_facew, t[0], t1, t2, c.ncomp, c.d_eq, c.d_rhs)
   switch (Pt.p[0]) {
   case 1: LAUNCH_TR(1); break;
   case 2: LAUNCH_TR(2); break;

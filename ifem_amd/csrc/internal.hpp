@@ -120,6 +120,9 @@ struct Context {
   std::unique_ptr<Batch> batch;
   // spline multigrid hierarchy (pc = IFEM_GPU_PC_MG)
   std::unique_ptr<Multigrid> mg;
+  // face Gauss-point weights of the traction pass
+  double* d_facew = nullptr;
+  size_t facew_cap = 0;
 };
 
 // assemble.hip

@@ -34,6 +34,7 @@ void launch_spmv16(Context& c, const __half* val, float scale, const float* p, f
 
 Multigrid::~Multigrid()
 {
+  if (side) hipStreamSynchronize(side);
   for (MgLevel& L : lev) {
     for (int d = 0; d < 3; d++) {
       MgDir& D = L.dir[d];
@@ -50,6 +51,10 @@ Multigrid::~Multigrid()
   if (h_coarse) hipHostFree(h_coarse);
   if (h_err) hipHostFree(h_err);
   if (e_copy) hipEventDestroy(e_copy);
+  if (e_gal) hipEventDestroy(e_gal);
+  if (e_side) hipEventDestroy(e_side);
+  if (side) hipStreamDestroy(side);
+  hipFree(part_s); hipFree(part_s2);
   if (e0) hipEventDestroy(e0);
   if (e1) hipEventDestroy(e1);
 }
@@ -826,6 +831,11 @@ static int mg_build(Context& c, Multigrid& m)
                          hipHostMallocDefault));
   IFEM_HIP(hipHostMalloc(&m.h_err, sizeof(int), hipHostMallocDefault));
   IFEM_HIP(hipEventCreate(&m.e_copy));
+  IFEM_HIP(hipEventCreateWithFlags(&m.e_gal, hipEventDisableTiming));
+  IFEM_HIP(hipEventCreateWithFlags(&m.e_side, hipEventDisableTiming));
+  IFEM_HIP(hipStreamCreateWithFlags(&m.side, hipStreamNonBlocking)); // default priority
+  IFEM_HIP(hipMalloc(&m.part_s, sizeof(double) * kMgParts));
+  IFEM_HIP(hipMalloc(&m.part_s2, sizeof(double) * kMgParts));
   IFEM_HIP(hipMalloc(&m.err, sizeof(int)));
   IFEM_HIP(hipEventCreate(&m.e0));
   IFEM_HIP(hipEventCreate(&m.e1));
@@ -1014,9 +1024,16 @@ int mg_setup(Context& c, int degree)
                           hipMemcpyDeviceToHost, c.stream));
   IFEM_HIP(hipMemcpyAsync(m.h_err, m.err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
   IFEM_HIP(hipEventRecord(m.e_copy, c.stream));
-  // largest eigenvalue of D^-1 A per smoothed level: Lanczos from CG steps
+  // largest eigenvalue of D^-1 A per smoothed level: Lanczos from CG steps.
+  // The coarse levels (small, launch-latency bound) run on a side stream
+  // beside the fine level (bandwidth bound); they share no data.
   const int nl = (int)m.lev.size() - 1;
-  for (int l = 0; l < nl; l++) lanczos_level(c, m, l, c.stream, m.part, m.part2);
+  IFEM_HIP(hipEventRecord(m.e_gal, c.stream));
+  IFEM_HIP(hipStreamWaitEvent(m.side, m.e_gal, 0));
+  for (int l = 1; l < nl; l++) lanczos_level(c, m, l, m.side, m.part_s, m.part_s2);
+  IFEM_HIP(hipEventRecord(m.e_side, m.side));
+  if (nl > 0) lanczos_level(c, m, 0, c.stream, m.part, m.part2);
+  IFEM_HIP(hipStreamWaitEvent(c.stream, m.e_side, 0));
   IFEM_HIP(hipGetLastError());
   IFEM_HIP(hipMemcpyAsync(m.h_lzh, m.lzh, sizeof(double) * (size_t)nl * (3 * kMgLanczos + 1),
                           hipMemcpyDeviceToHost, c.stream));

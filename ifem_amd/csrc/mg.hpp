@@ -71,7 +71,9 @@ struct Multigrid {
   double* h_coarse = nullptr;
   int* h_err = nullptr;
   int* err = nullptr;
-  hipEvent_t e0 = nullptr, e1 = nullptr, e_copy = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr, e_copy = nullptr, e_gal = nullptr, e_side = nullptr;
+  hipStream_t side = nullptr; // coarse-level eigenvalue estimates beside the fine one
+  double *part_s = nullptr, *part_s2 = nullptr;
   double setup_ms = 0.0;
   ~Multigrid();
 };

@@ -29,7 +29,8 @@ EXPORTS = [
     "ifem_gpu_batch_patch_info", "ifem_gpu_batch_get_system", "ifem_gpu_batch_get_solution",
     "ifem_gpu_batch_timings", "ifem_gpu_autotune_assembly", "ifem_gpu_unet_backward",
     "ifem_gpu_last_assembly_parts", "ifem_gpu_integrate_norms", "ifem_gpu_sam_bind_constraints",
-    "ifem_gpu_mg_setup", "ifem_gpu_mg_level", "ifem_gpu_mg_apply",
+    "ifem_gpu_mg_setup", "ifem_gpu_mg_level", "ifem_gpu_mg_apply", "ifem_gpu_sam_bind_patches",
+    "ifem_gpu_assemble_neumann_patch",
 ]
 
 FP64, FP32 = 64, 32
@@ -97,6 +98,10 @@ def lib():
     L.ifem_gpu_mg_setup.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(dp)]
     L.ifem_gpu_mg_level.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(ip), C.POINTER(dp), vp]
     L.ifem_gpu_mg_apply.argtypes = [vp, vp, vp]
+    L.ifem_gpu_sam_bind_patches.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(C.POINTER(ip)), ip,
+                                            ip, C.POINTER(ip), ip, ip, C.POINTER(ip),
+                                            C.POINTER(ip), C.POINTER(dp)]
+    L.ifem_gpu_assemble_neumann_patch.argtypes = [vp, ip, ip, C.POINTER(dp), ip]
     L.ifem_gpu_init_assembly.argtypes = [vp, C.POINTER(C.c_int64)]
     L.ifem_gpu_assemble.argtypes = [vp, C.POINTER(Integrand), C.POINTER(ip), ip]
     L.ifem_gpu_assemble_neumann.argtypes = [vp, ip, C.POINTER(dp), ip]
@@ -225,6 +230,36 @@ class Session:
             tt.ctypes.data_as(C.POINTER(C.c_double))))
         self.neq = sam.neq
         self.ncomp = ncomp
+
+    def bind_patches(self, patch_ids, mlgns, ncomp: int, meqn, neq: int, sam=None):
+        """Several patches sharing nodes: mlgns[k] maps patch k's local nodes
+        to global nodes (1-based); meqn/neq per global DOF, or sam (SAM arrays
+        with prescribed values)."""
+        ipp = C.POINTER(C.c_int)
+        ids = (C.c_int * len(patch_ids))(*patch_ids)
+        keep = [np.ascontiguousarray(m, dtype=np.int32) for m in mlgns]
+        arr = (ipp * len(keep))(*[k.ctypes.data_as(ipp) for k in keep])
+        if sam is not None:
+            meqn, neq = sam.meqn, sam.neq
+        m = np.ascontiguousarray(meqn, dtype=np.int32)
+        nnod = len(m) // ncomp
+        if sam is not None:
+            mp = np.ascontiguousarray(sam.mpmceq, dtype=np.int32)
+            mm = np.ascontiguousarray(sam.mmceq, dtype=np.int32)
+            tt = np.ascontiguousarray(sam.ttcc, dtype=np.float64)
+            args = (len(mp) - 1, mp.ctypes.data_as(ipp), mm.ctypes.data_as(ipp),
+                    tt.ctypes.data_as(C.POINTER(C.c_double)))
+        else:
+            args = (0, None, None, None)
+        check(lib().ifem_gpu_sam_bind_patches(self.h, len(patch_ids), ids, arr, nnod, ncomp,
+                                              m.ctypes.data_as(ipp), neq, *args))
+        self.neq = neq
+        self.ncomp = ncomp
+
+    def add_traction_patch(self, index: int, face: int, t, ng: int):
+        """Traction on face 0..5 of patch `index` of a multi-patch system."""
+        tv = (C.c_double * 3)(*[float(x) for x in t] + [0.0] * (3 - len(t)))
+        check(lib().ifem_gpu_assemble_neumann_patch(self.h, index, face + 1, tv, ng))
 
     def init_assembly(self) -> int:
         nnz = C.c_int64(0)

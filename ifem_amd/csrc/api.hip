@@ -26,6 +26,11 @@ static void free_system(Context& c)
 {
   solver_free(c);
   c.mg.reset();
+  for (SubPatch& sp : c.multi) {
+    hipFree(sp.d_leq);
+    hipFree(sp.d_lgval);
+  }
+  c.multi.clear();
   hipFree(c.d_eq);
   hipFree(c.d_gval);
   c.d_gval = nullptr;
@@ -188,15 +193,17 @@ int ifem_gpu_patch_create(ifem_gpu_ctx* ctx, const int order[3], const int ncoef
 
 // Shared by both bind entry points.  gval: per DOF value of prescribed DOFs
 // (empty: none), already checked by the caller.
+// global_ndof: the DOF count of a multi-patch system (0: the patch's own).
 static int bind_system(Context& c, int patch_id, int ncomp, int ndof, const int* meqn, int neq,
-                       std::vector<double>&& gval)
+                       std::vector<double>&& gval, int global_ndof = 0)
 {
   if (patch_id < 0 || patch_id >= (int)c.patches.size())
     return fail(IFEM_GPU_EINVAL, "sam_bind: unknown patch");
   if (ncomp != 1 && ncomp != 3) return fail(IFEM_GPU_EUNSUPPORTED, "sam_bind: ncomp must be 1 or 3");
   const PatchRec& P = c.patches[patch_id];
   const int64_t nnod = (int64_t)P.n[0] * P.n[1] * P.n[2];
-  if (!meqn || ndof != nnod * ncomp) return fail(IFEM_GPU_EINVAL, "sam_bind: ndof != nnod*ncomp");
+  const int64_t want = global_ndof > 0 ? global_ndof : nnod * ncomp;
+  if (!meqn || ndof != want) return fail(IFEM_GPU_EINVAL, "sam_bind: ndof != nnod*ncomp");
   std::vector<int> eq(ndof);
   int last = 0;
   for (int i = 0; i < ndof; i++) {
@@ -237,11 +244,11 @@ int ifem_gpu_sam_bind(ifem_gpu_ctx* ctx, int patch_id, int ncomp, int ndof, cons
   return bind_system(ctx->c, patch_id, ncomp, ndof, meqn, neq, {});
 }
 
-int ifem_gpu_sam_bind_constraints(ifem_gpu_ctx* ctx, int patch_id, int ncomp, int ndof,
-                                  const int* meqn, int neq, int nceq, const int* mpmceq,
-                                  const int* mmceq, const double* ttcc)
+// Prescribed value per DOF from SAM's constraint arrays (constraints
+// without masters only).
+static int constraint_values(int ndof, const int* meqn, int nceq, const int* mpmceq,
+                             const int* mmceq, const double* ttcc, std::vector<double>& gval)
 {
-  CHECK_CTX(ctx);
   if (!meqn || ndof < 0 || nceq < 0 || (nceq > 0 && (!mpmceq || !mmceq || !ttcc)))
     return fail(IFEM_GPU_EINVAL, "sam_bind_constraints: null argument");
   // SAM layout (src/LinAlg/SAM.C:260-308): constraint i (1-based) holds the
@@ -249,7 +256,7 @@ int ifem_gpu_sam_bind_constraints(ifem_gpu_ctx* ctx, int patch_id, int ncomp, in
   // the slave DOF and the constant c0, the rest masters and coefficients.
   if (nceq > 0 && mpmceq[0] != 1)
     return fail(IFEM_GPU_EINVAL, "sam_bind_constraints: mpmceq[0] must be 1");
-  std::vector<double> gval(ndof > 0 ? ndof : 1, 0.0);
+  gval.assign(ndof > 0 ? ndof : 1, 0.0);
   std::vector<char> seen(nceq > 0 ? nceq : 1, 0);
   for (int i = 0; i < ndof; i++) {
     if (meqn[i] >= 0) continue;
@@ -266,7 +273,78 @@ int ifem_gpu_sam_bind_constraints(ifem_gpu_ctx* ctx, int patch_id, int ncomp, in
     seen[ceq - 1] = 1;
     gval[i] = ttcc[ip - 1];
   }
+  return IFEM_GPU_OK;
+}
+
+int ifem_gpu_sam_bind_constraints(ifem_gpu_ctx* ctx, int patch_id, int ncomp, int ndof,
+                                  const int* meqn, int neq, int nceq, const int* mpmceq,
+                                  const int* mmceq, const double* ttcc)
+{
+  CHECK_CTX(ctx);
+  std::vector<double> gval;
+  int rc = constraint_values(ndof, meqn, nceq, mpmceq, mmceq, ttcc, gval);
+  if (rc) return rc;
   return bind_system(ctx->c, patch_id, ncomp, ndof, meqn, neq, std::move(gval));
+}
+
+int ifem_gpu_sam_bind_patches(ifem_gpu_ctx* ctx, int npatch, const int* patch_ids,
+                              const int* const* mlgn, int nnod, int ncomp, const int* meqn,
+                              int neq, int nceq, const int* mpmceq, const int* mmceq,
+                              const double* ttcc)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (npatch < 1 || !patch_ids || !mlgn || nnod < 1)
+    return fail(IFEM_GPU_EINVAL, "sam_bind_patches: null or empty argument");
+  const int ndof = nnod * ncomp;
+  for (int k = 0; k < npatch; k++) {
+    if (patch_ids[k] < 0 || patch_ids[k] >= (int)c.patches.size() || !mlgn[k])
+      return fail(IFEM_GPU_EINVAL, "sam_bind_patches: unknown patch");
+    const PatchRec& P = c.patches[patch_ids[k]];
+    if (P.p[0] != P.p[1] || P.p[0] != P.p[2])
+      return fail(IFEM_GPU_EUNSUPPORTED, "sam_bind_patches: equal degrees per direction needed");
+    // each patch maps its nodes to distinct global nodes 1..nnod
+    const int ln = P.n[0] * P.n[1] * P.n[2];
+    std::vector<char> used(nnod, 0);
+    for (int i = 0; i < ln; i++) {
+      const int g = mlgn[k][i];
+      if (g < 1 || g > nnod) return fail(IFEM_GPU_EINVAL, "sam_bind_patches: mlgn out of range");
+      if (used[g - 1]) return fail(IFEM_GPU_EUNSUPPORTED, "sam_bind_patches: a patch maps two nodes to one");
+      used[g - 1] = 1;
+    }
+  }
+  std::vector<double> gval;
+  if (nceq > 0) {
+    int rc = constraint_values(ndof, meqn, nceq, mpmceq, mmceq, ttcc, gval);
+    if (rc) return rc;
+  }
+  std::vector<double> gkeep = gval;
+  int rc = bind_system(c, patch_ids[0], ncomp, ndof, meqn, neq, std::move(gval), ndof);
+  if (rc) return rc;
+  const bool anyg = !c.h_gval.empty();
+  for (int k = 0; k < npatch; k++) {
+    const PatchRec& P = c.patches[patch_ids[k]];
+    const int ln = P.n[0] * P.n[1] * P.n[2];
+    SubPatch sp;
+    sp.patch = patch_ids[k];
+    sp.h_leq.resize((size_t)ln * ncomp);
+    std::vector<double> lg(anyg ? (size_t)ln * ncomp : 0);
+    for (int i = 0; i < ln; i++)
+      for (int cc = 0; cc < ncomp; cc++) {
+        const int gd = (mlgn[k][i] - 1) * ncomp + cc;
+        sp.h_leq[(size_t)i * ncomp + cc] = c.h_eq[gd];
+        if (anyg) lg[(size_t)i * ncomp + cc] = gkeep[gd];
+      }
+    c.multi.push_back(sp);
+    SubPatch& s = c.multi.back();
+    IFEM_HIP(hipMalloc(&s.d_leq, sizeof(int) * s.h_leq.size()));
+    IFEM_HIP(hipMemcpy(s.d_leq, s.h_leq.data(), sizeof(int) * s.h_leq.size(), hipMemcpyHostToDevice));
+    if (anyg) {
+      IFEM_HIP(hipMalloc(&s.d_lgval, sizeof(double) * lg.size()));
+      IFEM_HIP(hipMemcpy(s.d_lgval, lg.data(), sizeof(double) * lg.size(), hipMemcpyHostToDevice));
+    }
+  }
+  return IFEM_GPU_OK;
 }
 
 int ifem_gpu_init_assembly(ifem_gpu_ctx* ctx, int64_t* nnz)
@@ -335,7 +413,19 @@ int ifem_gpu_assemble_neumann(ifem_gpu_ctx* ctx, int face, const double* tractio
   if (face < 1 || face > 6) return fail(IFEM_GPU_EINVAL, "assemble_neumann: face must be 1..6");
   if (ng < 1 || ng > kMaxNG) return fail(IFEM_GPU_EUNSUPPORTED, "assemble_neumann: ng out of range");
   if (!c.d_rhs) return fail(IFEM_GPU_EINVAL, "assemble_neumann: assemble the volume terms first");
-  return assemble_traction(c, face - 1, traction, ng);
+  return assemble_traction(c, 0, face - 1, traction, ng);
+}
+
+int ifem_gpu_assemble_neumann_patch(ifem_gpu_ctx* ctx, int index, int face,
+                                    const double* traction, int ng)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (!traction) return fail(IFEM_GPU_EINVAL, "assemble_neumann: null traction");
+  if (face < 1 || face > 6) return fail(IFEM_GPU_EINVAL, "assemble_neumann: face must be 1..6");
+  if (ng < 1 || ng > kMaxNG) return fail(IFEM_GPU_EUNSUPPORTED, "assemble_neumann: ng out of range");
+  if (!c.d_rhs) return fail(IFEM_GPU_EINVAL, "assemble_neumann: assemble the volume terms first");
+  return assemble_traction(c, index, face - 1, traction, ng);
 }
 
 void ifem_gpu_solver_defaults(ifem_gpu_solver_params* p)

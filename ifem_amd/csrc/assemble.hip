@@ -176,6 +176,10 @@ struct AsmArgs {
   // per DOF prescribed value of eliminated DOFs (FP64 pass only; nullptr:
   // homogeneous); the eliminated columns move to the RHS, b_i -= K_ij g_j
   const double* gval;
+  // several patches in one system: rows are shared across patches, so
+  // values are added at the column found by binary search in the sorted row
+  const int32_t* colind;
+  int accum;
 };
 
 // Geometry record of one Gauss point.  Vector problems: J^-1 (row-major, 9)
@@ -674,7 +678,24 @@ __device__ __forceinline__ void line_impl(
               }
             }
             const unsigned long long m = __ballot(valid);
-            if (valid) {
+            if (valid && a.accum) {
+              const T v = win[slot][wave][t / NC][t % NC];
+              const int nbr = t / NC;
+              const int ii = i + nbr % W1 - P, jj = jc + (nbr / W1) % W1 - P, kk = kc + nbr / (W1 * W1) - P;
+              const int col = a.eq[((int64_t)ii + (int64_t)a.n1 * (jj + (int64_t)a.n2 * kk)) * NC + t % NC];
+              int64_t lo = base, hi = a.rowptr[req + 1] - 1;
+              while (lo <= hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                const int cm = a.colind[mid];
+                if (cm == col) {
+                  if (a.val32) a.val32[mid] += (float)v;
+                  else a.val64[mid] += (double)v;
+                  break;
+                }
+                if (cm < col) lo = mid + 1;
+                else hi = mid - 1;
+              }
+            } else if (valid) {
               const int64_t at = base + pos + __popcll(m & ((1ull << lane) - 1ull));
               const T v = win[slot][wave][t / NC][t % NC];
               if (a.val32) a.val32[at] = (float)v;
@@ -688,7 +709,10 @@ __device__ __forceinline__ void line_impl(
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) corr += __shfl_xor(corr, off, 64);
           }
-          if ((HAS_RHS || gv) && a.rhs && lane == 0) a.rhs[req] = (double)rwin[slot][ic] - corr;
+          if ((HAS_RHS || gv) && a.rhs && lane == 0) {
+            if (a.accum) a.rhs[req] += (double)rwin[slot][ic] - corr;
+            else a.rhs[req] = (double)rwin[slot][ic] - corr;
+          }
         }
       }
       __syncthreads();
@@ -870,8 +894,75 @@ __global__ void k_scan_add(int64_t* __restrict__ out, const int64_t* __restrict_
   if (i == 0) out[0] = 0;
 }
 
+// Several patches: the global pattern is the union of the patches' node
+// couplings (nodes within +-p in every direction of one patch), built on the
+// host (SAM::getDofCouplings over all patches, src/LinAlg/SAM.C:377-427).
+static int build_pattern_multi(Context& c)
+{
+  const int nc = c.ncomp, neq = c.neq;
+  std::vector<std::vector<int>> cols(neq > 0 ? neq : 1);
+  for (const SubPatch& sp : c.multi) {
+    const PatchRec& P = c.patches[sp.patch];
+    const int p = P.p[0], n1 = P.n[0], n2 = P.n[1], n3 = P.n[2];
+    for (int k = 0; k < n3; k++)
+      for (int j = 0; j < n2; j++)
+        for (int i = 0; i < n1; i++) {
+          const int node = i + n1 * (j + n2 * k);
+          for (int ci = 0; ci < nc; ci++) {
+            const int r = sp.h_leq[node * nc + ci];
+            if (r < 0) continue;
+            std::vector<int>& row = cols[r];
+            for (int kk = std::max(0, k - p); kk <= std::min(n3 - 1, k + p); kk++)
+              for (int jj = std::max(0, j - p); jj <= std::min(n2 - 1, j + p); jj++)
+                for (int ii = std::max(0, i - p); ii <= std::min(n1 - 1, i + p); ii++) {
+                  const int nb = ii + n1 * (jj + n2 * kk);
+                  for (int cj = 0; cj < nc; cj++) {
+                    const int e = sp.h_leq[nb * nc + cj];
+                    if (e >= 0) row.push_back(e);
+                  }
+                }
+          }
+        }
+  }
+  std::vector<int64_t> rowptr(neq + 1, 0);
+  for (int r = 0; r < neq; r++) {
+    std::vector<int>& row = cols[r];
+    std::sort(row.begin(), row.end());
+    row.erase(std::unique(row.begin(), row.end()), row.end());
+    rowptr[r + 1] = rowptr[r] + (int64_t)row.size();
+  }
+  const int64_t nnz = rowptr[neq];
+  std::vector<int32_t> colind(nnz > 0 ? nnz : 1);
+  for (int r = 0; r < neq; r++)
+    std::copy(cols[r].begin(), cols[r].end(), colind.begin() + rowptr[r]);
+  hipFree(c.d_rowptr);
+  hipFree(c.d_colind);
+  hipFree(c.d_val);
+  hipFree(c.d_val32);
+  hipFree(c.d_rhs);
+  c.d_rowptr = nullptr;
+  c.d_colind = nullptr;
+  c.d_val = nullptr;
+  c.d_val32 = nullptr;
+  c.d_rhs = nullptr;
+  c.fp64_ready = c.fp32_ready = false;
+  c.nnz = nnz;
+  IFEM_HIP(hipMalloc(&c.d_rowptr, sizeof(int64_t) * (neq + 1)));
+  IFEM_HIP(hipMemcpy(c.d_rowptr, rowptr.data(), sizeof(int64_t) * (neq + 1), hipMemcpyHostToDevice));
+  IFEM_HIP(hipMalloc(&c.d_colind, sizeof(int32_t) * colind.size()));
+  IFEM_HIP(hipMemcpy(c.d_colind, colind.data(), sizeof(int32_t) * colind.size(),
+                     hipMemcpyHostToDevice));
+  IFEM_HIP(hipMalloc(&c.d_val, sizeof(double) * (nnz > 0 ? nnz : 1)));
+  IFEM_HIP(hipMalloc(&c.d_rhs, sizeof(double) * (neq > 0 ? neq : 1)));
+  IFEM_HIP(hipMemsetAsync(c.d_rhs, 0, sizeof(double) * (neq > 0 ? neq : 1), c.stream));
+  IFEM_HIP(hipStreamSynchronize(c.stream));
+  c.pattern_ready = true;
+  return solver_setup(c);
+}
+
 int build_pattern(Context& c)
 {
+  if (!c.multi.empty()) return build_pattern_multi(c);
   const PatchRec& P = c.patches[c.patch];
   if (P.p[0] != P.p[1] || P.p[0] != P.p[2])
     return fail(IFEM_GPU_EUNSUPPORTED, "init_assembly: the GPU path needs equal degrees in all directions");
@@ -1032,10 +1123,13 @@ static int dispatch_ng(Context& c, const AsmArgs& args, int ng, int nc, bool fp3
   return fail(IFEM_GPU_EUNSUPPORTED, "assemble: the GPU path supports ng = p+1 or p+2");
 }
 
+// One patch: Pt with eq (local DOF -> equation or -1) and gval (local DOF
+// -> prescribed value, or nullptr).  accum: the system holds several
+// patches; values and right-hand side are added (zeroed by the caller).
 template <int P>
-static int assemble_p(Context& c, const ifem_gpu_integrand& itg, int ng, int precision)
+static int assemble_p(Context& c, const PatchRec& Pt, const int* eq, const double* gval,
+                      int accum, const ifem_gpu_integrand& itg, int ng, int precision)
 {
-  const PatchRec& Pt = c.patches[c.patch];
   GaussArgs ga{};
   gauss_rule(ng, ga.xi, ga.w);
   Tables tb;
@@ -1053,8 +1147,10 @@ static int assemble_p(Context& c, const ifem_gpu_integrand& itg, int ng, int pre
   a.tabN2 = tb.tabN[1]; a.tabD2 = tb.tabD[1]; a.span2 = tb.span[1];
   a.tabN3 = tb.tabN[2]; a.tabD3 = tb.tabD[2]; a.span3 = tb.span[2];
   for (int g = 0; g < kMaxNG; g++) a.gw[g] = ga.w[g];
-  a.eq = c.d_eq;
+  a.eq = eq;
   a.rowptr = c.d_rowptr;
+  a.colind = c.d_colind;
+  a.accum = accum;
   const bool fp32 = precision == IFEM_GPU_FP32;
   if (fp32) {
     if (!c.d_val32) IFEM_HIP(hipMalloc(&c.d_val32, sizeof(float) * (c.nnz > 0 ? c.nnz : 1)));
@@ -1065,7 +1161,7 @@ static int assemble_p(Context& c, const ifem_gpu_integrand& itg, int ng, int pre
     a.val64 = c.d_val;
     a.val32 = nullptr;
     a.rhs = c.d_rhs;
-    a.gval = c.d_gval;
+    a.gval = gval;
   }
   for (void* p : c.scratch_tables) hipFree(p);
   c.scratch_tables.clear();
@@ -1081,7 +1177,8 @@ static int assemble_p(Context& c, const ifem_gpu_integrand& itg, int ng, int pre
   if (!fp32) {
     if (c.ncomp == 1) has_rhs = itg.source.type != IFEM_GPU_FN_ZERO;
     else has_rhs = itg.body[0] != 0.0 || itg.body[1] != 0.0 || itg.body[2] != 0.0;
-    IFEM_HIP(hipMemsetAsync(c.d_rhs, 0, sizeof(double) * (c.neq > 0 ? c.neq : 1), c.stream));
+    if (!accum)
+      IFEM_HIP(hipMemsetAsync(c.d_rhs, 0, sizeof(double) * (c.neq > 0 ? c.neq : 1), c.stream));
   }
   // geometry records of all (element, Gauss point) pairs, then the lines
   a.ng = ng;
@@ -1150,18 +1247,45 @@ int autotune_assembly(Context& c, const ifem_gpu_integrand& itg, const int ng[3]
   return assemble_volume(c, itg, ng, precision);
 }
 
+static int assemble_one(Context& c, const PatchRec& Pt, const int* eq, const double* gval,
+                        int accum, const ifem_gpu_integrand& itg, int ng, int precision)
+{
+  switch (Pt.p[0]) {
+  case 1: return assemble_p<1>(c, Pt, eq, gval, accum, itg, ng, precision);
+  case 2: return assemble_p<2>(c, Pt, eq, gval, accum, itg, ng, precision);
+  case 3: return assemble_p<3>(c, Pt, eq, gval, accum, itg, ng, precision);
+  default: return fail(IFEM_GPU_EUNSUPPORTED, "assemble: degree out of range");
+  }
+}
+
 int assemble_volume(Context& c, const ifem_gpu_integrand& itg, const int ng[3], int precision)
 {
-  const PatchRec& Pt = c.patches[c.patch];
   if (ng[0] != ng[1] || ng[0] != ng[2])
     return fail(IFEM_GPU_EUNSUPPORTED, "assemble: the GPU path needs equal Gauss counts per direction");
   if (ng[0] < 1 || ng[0] > kMaxNG) return fail(IFEM_GPU_EUNSUPPORTED, "assemble: ng out of range");
-  switch (Pt.p[0]) {
-  case 1: return assemble_p<1>(c, itg, ng[0], precision);
-  case 2: return assemble_p<2>(c, itg, ng[0], precision);
-  case 3: return assemble_p<3>(c, itg, ng[0], precision);
-  default: return fail(IFEM_GPU_EUNSUPPORTED, "assemble: degree out of range");
+  if (c.multi.empty())
+    return assemble_one(c, c.patches[c.patch], c.d_eq, c.d_gval, 0, itg, ng[0], precision);
+  // several patches: zero the system, then add patch by patch in a fixed order
+  const bool fp32 = precision == IFEM_GPU_FP32;
+  const size_t nz = (size_t)(c.nnz > 0 ? c.nnz : 1);
+  if (fp32) {
+    if (!c.d_val32) IFEM_HIP(hipMalloc(&c.d_val32, sizeof(float) * nz));
+    IFEM_HIP(hipMemsetAsync(c.d_val32, 0, sizeof(float) * nz, c.stream));
+  } else {
+    IFEM_HIP(hipMemsetAsync(c.d_val, 0, sizeof(double) * nz, c.stream));
+    IFEM_HIP(hipMemsetAsync(c.d_rhs, 0, sizeof(double) * (c.neq > 0 ? c.neq : 1), c.stream));
   }
+  double tot = 0.0, geo = 0.0;
+  for (const SubPatch& sp : c.multi) {
+    int rc = assemble_one(c, c.patches[sp.patch], sp.d_leq, sp.d_lgval, 1, itg, ng[0], precision);
+    if (rc) return rc;
+    tot += c.last_assemble_ms;
+    geo += c.last_geometry_ms;
+  }
+  c.last_assemble_ms = tot;
+  c.last_geometry_ms = geo;
+  c.last_lines_ms = tot - geo;
+  return IFEM_GPU_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1297,9 +1421,12 @@ __global__ void k_traction_sum(int n1, int n2, int n3, int nel1, int nel2, int n
   }
 }
 
-int assemble_traction(Context& c, int face0, const double* t, int ng)
+int assemble_traction(Context& c, int index, int face0, const double* t, int ng)
 {
-  const PatchRec& Pt = c.patches[c.patch];
+  if (index < 0 || index >= (c.multi.empty() ? 1 : (int)c.multi.size()))
+    return fail(IFEM_GPU_EINVAL, "traction: patch index out of range");
+  const PatchRec& Pt = c.patches[c.multi.empty() ? c.patch : c.multi[index].patch];
+  const int* eqmap = c.multi.empty() ? c.d_eq : c.multi[index].d_leq;
   GaussArgs ga{};
   gauss_rule(ng, ga.xi, ga.w);
   const int dn = face0 / 2;
@@ -1320,7 +1447,7 @@ int assemble_traction(Context& c, int face0, const double* t, int ng)
       Pt.d_knots[1], Pt.d_knots[2], face0, ng, ga, c.d_facew);                                  \
   k_traction_sum<PP><<<(nfn + 127) / 128, 128, 0, c.stream>>>(                                  \
       Pt.n[0], Pt.n[1], Pt.n[2], Pt.nel[0], Pt.nel[1], Pt.nel[2], Pt.d_knots[0], Pt.d_knots[1], \
-      Pt.d_knots[2], face0, ng, ga, c.d_facew, t[0], t1, t2, c.ncomp, c.d_eq, c.d_rhs)
+      Pt.d_knots[2], face0, ng, ga, c.d_facew, t[0], t1, t2, c.ncomp, eqmap, c.d_rhs)
   switch (Pt.p[0]) {
   case 1: LAUNCH_TR(1); break;
   case 2: LAUNCH_TR(2); break;
@@ -1735,10 +1862,32 @@ __global__ __launch_bounds__(256) void k_norms(NormArgs na, int64_t total)
   }
 }
 
+// Squared norm integrals of one patch added to sq[6].
+static int norms_patch(Context& c, const PatchRec& Pt, const int* eqmap, const double* gval,
+                       const ifem_gpu_integrand& itg, const int ng[3],
+                       const ifem_gpu_function* exact, double* sq);
+
 int integrate_norms(Context& c, const ifem_gpu_integrand& itg, const int ng[3],
                     const ifem_gpu_function* exact, double* out)
 {
-  const PatchRec& Pt = c.patches[c.patch];
+  double sq[6] = {0, 0, 0, 0, 0, 0};
+  if (c.multi.empty()) {
+    int rc = norms_patch(c, c.patches[c.patch], c.d_eq, c.d_gval, itg, ng, exact, sq);
+    if (rc) return rc;
+  } else {
+    for (const SubPatch& sp : c.multi) {
+      int rc = norms_patch(c, c.patches[sp.patch], sp.d_leq, sp.d_lgval, itg, ng, exact, sq);
+      if (rc) return rc;
+    }
+  }
+  for (int m = 0; m < 6; m++) out[m] = sq[m] < 0.0 ? -std::sqrt(-sq[m]) : std::sqrt(sq[m]);
+  return IFEM_GPU_OK;
+}
+
+static int norms_patch(Context& c, const PatchRec& Pt, const int* eqmap, const double* gval,
+                       const ifem_gpu_integrand& itg, const int ng[3],
+                       const ifem_gpu_function* exact, double* sq)
+{
   const int p = Pt.p[0];
   if (ng[0] != ng[1] || ng[0] != ng[2] || ng[0] < 1 || ng[0] > kMaxNG)
     return fail(IFEM_GPU_EUNSUPPORTED, "norms: equal Gauss counts 1..5 per direction needed");
@@ -1765,8 +1914,8 @@ int integrate_norms(Context& c, const ifem_gpu_integrand& itg, const int ng[3],
   a.tabN2 = tb.tabN[1]; a.tabD2 = tb.tabD[1]; a.span2 = tb.span[1];
   a.tabN3 = tb.tabN[2]; a.tabD3 = tb.tabD[2]; a.span3 = tb.span[2];
   for (int k = 0; k < kMaxNG; k++) a.gw[k] = ga.w[k];
-  a.eq = c.d_eq;
-  a.gval = c.d_gval;
+  a.eq = eqmap;
+  a.gval = gval;
   a.ng = g;
   a.ncomp = c.ncomp;
   for (void* q : c.scratch_tables) hipFree(q);
@@ -1813,7 +1962,7 @@ int integrate_norms(Context& c, const ifem_gpu_integrand& itg, const int ng[3],
   for (int m = 0; m < 6; m++) {
     double sum = 0.0;
     for (int b = 0; b < grid; b++) sum += h[m * grid + b];
-    out[m] = sum < 0.0 ? -std::sqrt(-sum) : std::sqrt(sum);
+    sq[m] += sum;
   }
   return IFEM_GPU_OK;
 }

@@ -61,6 +61,14 @@ struct SolverWork {
   int nblocks = 0;
 };
 
+// One patch of a system with several patches sharing nodes.
+struct SubPatch {
+  int patch = -1;
+  int* d_leq = nullptr;       // local DOF -> global equation (0-based) or -1
+  double* d_lgval = nullptr;  // local DOF -> prescribed value (nullptr: none)
+  std::vector<int> h_leq;
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -74,6 +82,7 @@ struct Context {
   std::vector<int> h_eq;
   double* d_gval = nullptr;     // per DOF prescribed value (nullptr: all 0)
   std::vector<double> h_gval;
+  std::vector<SubPatch> multi;  // non-empty: the system holds several patches
   int64_t nnz = 0;
   int64_t* d_rowptr = nullptr;
   int32_t* d_colind = nullptr;
@@ -129,7 +138,7 @@ struct Context {
 int build_pattern(Context& c);
 int assemble_volume(Context& c, const ifem_gpu_integrand& itg, const int ng[3],
                     int precision);
-int assemble_traction(Context& c, int face0, const double* t, int ng);
+int assemble_traction(Context& c, int index, int face0, const double* t, int ng);
 int autotune_assembly(Context& c, const ifem_gpu_integrand& itg, const int ng[3], int precision,
                       int* half);
 int integrate_norms(Context& c, const ifem_gpu_integrand& itg, const int ng[3],

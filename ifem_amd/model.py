@@ -203,6 +203,45 @@ def prescribed_equations(msc: np.ndarray, values: np.ndarray) -> SamArrays:
                      values[pres].astype(np.float64))
 
 
+@dataclasses.dataclass
+class MultiPatch:
+    """Patches side by side along x that share their common faces: patch q
+    covers [x_q, x_{q+1}] x [0,1]^2; global nodes form one (gx, ny, nz) grid
+    (i fastest) in which the shared faces appear once (SAM's MLGN)."""
+    patches: List[Patch]
+    mlgn: List[np.ndarray]     # per patch: local node -> global node (1-based)
+    dims: List[int]            # global node grid (gx, ny, nz)
+
+    @property
+    def nnod(self) -> int:
+        return self.dims[0] * self.dims[1] * self.dims[2]
+
+
+def x_split_patches(p: int, nels: Sequence[int], nyz: int, jitter: float = 0.0,
+                    seed: int = 0) -> MultiPatch:
+    """len(nels) degree-p patches along x with nels[q] x nyz x nyz elements,
+    widths proportional to nels[q]; interior control points of every patch
+    jittered as in cube_patch (seed + q), patch faces left exact so the shared
+    faces coincide."""
+    tot = float(sum(nels))
+    x0 = 0.0
+    patches, mlgn = [], []
+    off = 0
+    ny = nyz + p
+    n_tot = sum(ne + p for ne in nels) - (len(nels) - 1)
+    for q, ne in enumerate(nels):
+        P = cube_patch(p, [ne, nyz, nyz], jitter, seed + q)
+        w = ne / tot
+        P.coefs[:, 0] = x0 + w * P.coefs[:, 0]
+        x0 += w
+        n1 = P.n[0]
+        k, j, i = np.meshgrid(np.arange(ny), np.arange(ny), np.arange(n1), indexing="ij")
+        mlgn.append((off + i.ravel() + n_tot * (j.ravel() + ny * k.ravel()) + 1).astype(np.int32))
+        off += n1 - 1
+        patches.append(P)
+    return MultiPatch(patches, mlgn, [n_tot, ny, ny])
+
+
 # ---- integrand descriptions (same numbering as the C ABI) -----------------
 FN_ZERO, FN_CONST, FN_SINPROD, FN_FOURIER, FN_EXP_FOURIER = range(5)
 POISSON, ELASTICITY = 1, 2

@@ -85,6 +85,21 @@ int ifem_gpu_sam_bind_constraints(ifem_gpu_ctx* ctx, int patch_id, int ncomp, in
                                   const int* meqn, int neq, int nceq, const int* mpmceq,
                                   const int* mmceq, const double* ttcc);
 
+/* Several patches in one system, sharing the nodes of their common faces
+ * (ASMs3D::connectPatch / SAM's global node numbering MLGN,
+ * src/ASM/ASMs3D.C:702-815, src/ASM/SAMpatch.C:21-161): mlgn[k] maps local
+ * node i of patch patch_ids[k] (GoTools order) to its global node (1-based,
+ * distinct within the patch); meqn holds ncomp equation numbers per global
+ * node, as in ifem_gpu_sam_bind; nceq > 0 adds prescribed values as in
+ * ifem_gpu_sam_bind_constraints.  ifem_gpu_init_assembly then builds the
+ * union pattern on the host, and ifem_gpu_assemble adds the patches in
+ * order into the shared rows (the same integrand on every patch).  The
+ * multigrid and learned preconditioners need a single patch. */
+int ifem_gpu_sam_bind_patches(ifem_gpu_ctx* ctx, int npatch, const int* patch_ids,
+                              const int* const* mlgn, int nnod, int ncomp, const int* meqn,
+                              int neq, int nceq, const int* mpmceq, const int* mmceq,
+                              const double* ttcc);
+
 /* ---- symbolic assembly -----------------------------------------------------
  * Builds the CSR sparsity pattern (symmetric, full storage, sorted columns)
  * on the device.  Replaces SystemMatrix::initAssembly -> SparseMatrix::
@@ -152,6 +167,10 @@ int ifem_gpu_autotune_assembly(ifem_gpu_ctx* ctx, const ifem_gpu_integrand* itg,
  * ng Gauss points per tangent direction, added into b.  Replaces
  * ASMs3D::integrate(Integrand&, int lIndex, ...) (src/ASM/ASMs3D.C:2547-2812). */
 int ifem_gpu_assemble_neumann(ifem_gpu_ctx* ctx, int face, const double* traction, int ng);
+/* Same on patch `index` (0-based, in the order given to
+ * ifem_gpu_sam_bind_patches; 0 for a single-patch system). */
+int ifem_gpu_assemble_neumann_patch(ifem_gpu_ctx* ctx, int index, int face,
+                                    const double* traction, int ng);
 
 /* ---- norms ------------------------------------------------------------------
  * Norms of the current solution u_h (after ifem_gpu_solve; eliminated DOFs

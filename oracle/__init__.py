@@ -222,6 +222,21 @@ def assemble(P, itg, ng, meqn, neq, rowptr, colind, nthreads: int = 0,
     return A, b
 
 
+def assemble_into(P, itg, ng, meqn, neq, rowptr, colind, A, b, gdof=None):
+    """Adds one patch's element matrices and vectors into A, b (a system of
+    several patches: meqn holds the global equation numbers of the patch's
+    local DOFs, gdof their prescribed values)."""
+    keep = _Keep()
+    sp = _patch(P, keep); si = _integrand(itg, keep)
+    meqn = np.ascontiguousarray(meqn, dtype=np.int32)
+    g = None if gdof is None else np.ascontiguousarray(gdof, dtype=np.float64)
+    rc = lib().orc_assemble_range_g(C.byref(sp), C.byref(si), np.asarray(ng, dtype=np.int32),
+                                    itg.ncomp, meqn, neq, rowptr, colind, A, b, 0, 0, 1 << 30,
+                                    None if g is None else g.ctypes.data)
+    if rc:
+        raise RuntimeError("orc_assemble_range_g failed")
+
+
 def expand_solution(meqn, xeq, gdof=None):
     """Equation vector -> DOF vector: fixed DOFs 0, or their prescribed value
     from gdof (SAM::expandSolution, src/LinAlg/SAM.C:768-810)."""

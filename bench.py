@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-mg", action="store_true", help="skip the multigrid-preconditioned leg")
+    ap.add_argument("--no-multipatch", action="store_true",
+                    help="skip the two-patch assembly leg")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched uneven-patch leg (256 diffusion patches)")
     ap.add_argument("--cpu-layers", type=int, default=16,
@@ -218,6 +220,48 @@ def main():
         mg = {"levels": nlev, "setup_ms_included": round(setup_ms, 3), **legs}
         log(f"[bench] multigrid: {mg}")
 
+    # ---- several patches sharing faces: assembly into shared rows -------------
+    multipatch = None
+    if not args.no_multipatch:
+        import types
+        from ifem_amd import model as M
+        mp = M.x_split_patches(2, [16, 16], 32, jitter=0.15, seed=5)
+        mmsc = M.face_status(types.SimpleNamespace(n=mp.dims), 3, [7, 0, 0, 0, 0, 0])
+        mmeqn, mneq = M.equation_numbers(mmsc)
+        itg_e = pb.integrand
+        ms = Session(stream=stream.cuda_stream)
+        pids = [ms.add_patch(P) for P in mp.patches]
+        ms.bind_patches(pids, mp.mlgn, 3, mmeqn, mneq)
+        t0 = time.perf_counter()
+        mnnz = ms.init_assembly()
+        msym = (time.perf_counter() - t0) * 1e3
+        mt = []
+        for _ in range(4):
+            ms.assemble(itg_e, pb.ng, FP64)
+            mt.append(ms.timings()[0])
+        sp_ms = float(np.median(mt[1:]))
+        # one patch of 32 x 32 x 32 elements for comparison
+        one = problems.elasticity_cube(32, 2)
+        os_ = Session(stream=stream.cuda_stream)
+        opid = os_.add_patch(one.patch)
+        os_.bind(opid, 3, one.meqn, one.neq)
+        os_.init_assembly()
+        ot = []
+        for _ in range(4):
+            os_.assemble(one.integrand, one.ng, FP64)
+            ot.append(os_.timings()[0])
+        multipatch = {
+            "workload": "elasticity p=2, two 16x32x32-element patches sharing the x=0.5 face, "
+                        "clamped x=0",
+            "dofs": int(len(mmeqn)), "neq": int(mneq), "nnz": int(mnnz),
+            "host_pattern_ms": round(msym, 1), "assembly_kernels_ms": round(sp_ms, 3),
+            "assembled_dof_per_s": round(len(mmeqn) / (sp_ms * 1e-3), 1),
+            "one_patch_32cubed_assembly_kernels_ms": round(float(np.median(ot[1:])), 3),
+        }
+        ms.close()
+        os_.close()
+        log(f"[bench] multipatch: {multipatch}")
+
     # ---- batched uneven patches: one persistent assembly launch ----------------
     batch = None
     if not args.no_batch:
@@ -345,6 +389,7 @@ def main():
                                if mg else None),
         "roofline": roofline, "roofline_spmv": spmv_roof, "cpu_baseline": cpu,
         "batch": batch,
+        "multipatch": multipatch,
     }
     s.close()
     print(json.dumps(out), flush=True)

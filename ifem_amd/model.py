@@ -242,6 +242,51 @@ def x_split_patches(p: int, nels: Sequence[int], nyz: int, jitter: float = 0.0,
     return MultiPatch(patches, mlgn, [n_tot, ny, ny])
 
 
+def constraint_equations(msc: np.ndarray, constraints) -> SamArrays:
+    """SAM arrays for status codes 0/1 plus linear constraint equations
+    u_s = c0 + sum_m c_m u_m, given as (slave DOF, c0, [(master DOF, c_m)...])
+    with 0-based DOFs (SAMpatch::initConstraintEqs, src/ASM/SAMpatch.C:259-357:
+    constraints in slave-DOF order; a constraint on a fixed DOF is ignored; a
+    fixed master is dropped, leaving its constant; the slaves' status is reset
+    to 0 before SAM::initSystemEquations numbers the free DOFs)."""
+    msc = np.asarray(msc, dtype=np.int32).copy()
+    fixed0 = msc == 0
+    cons = sorted([c for c in constraints if not fixed0[c[0]]], key=lambda c: c[0])
+    slaves = [c[0] for c in cons]
+    if len(set(slaves)) != len(slaves):
+        raise ValueError("one constraint per slave DOF")
+    msc[slaves] = 0
+    meqn, neq = equation_numbers(msc)
+    mpmceq, mmceq, ttcc = [1], [], []
+    for k, (sd, c0, masters) in enumerate(cons):
+        meqn[sd] = -(k + 1)
+        mmceq.append(sd + 1)
+        ttcc.append(float(c0))
+        for md, cm in masters:
+            if fixed0[md]:
+                continue
+            if md in slaves:
+                raise ValueError("master DOFs must be free (resolve chains first)")
+            mmceq.append(md + 1)
+            ttcc.append(float(cm))
+        mpmceq.append(len(mmceq) + 1)
+    return SamArrays(meqn, neq, np.array(mpmceq, dtype=np.int32),
+                     np.array(mmceq, dtype=np.int32), np.array(ttcc, dtype=np.float64))
+
+
+def periodic_constraints(patch: Patch, ncomp: int, direction: int = 0):
+    """Constraints tying each DOF on the face at the end of `direction` to
+    the DOF of the node at its start (u_end = u_start), as periodicity
+    conditions are given to the reference (ASMs3D::addPeriodicity)."""
+    n = patch.n
+    k, j, i = [a.ravel() for a in np.meshgrid(range(n[2]), range(n[1]), range(n[0]), indexing="ij")]
+    idx = (i, j, k)[direction]
+    end = np.nonzero(idx == n[direction] - 1)[0]
+    shift = [1, n[0], n[0] * n[1]][direction] * (n[direction] - 1)
+    return [(int(nd) * ncomp + c, 0.0, [((int(nd) - shift) * ncomp + c, 1.0)])
+            for nd in end for c in range(ncomp)]
+
+
 # ---- integrand descriptions (same numbering as the C ABI) -----------------
 FN_ZERO, FN_CONST, FN_SINPROD, FN_FOURIER, FN_EXP_FOURIER = range(5)
 POISSON, ELASTICITY = 1, 2

@@ -37,6 +37,22 @@ class _Patch(C.Structure):
                 ("knots", C.POINTER(C.c_double) * 3), ("coefs", C.POINTER(C.c_double))]
 
 
+class _Sam(C.Structure):
+    _fields_ = [("meqn", C.POINTER(C.c_int)), ("ndof", C.c_int), ("nceq", C.c_int),
+                ("mpmceq", C.POINTER(C.c_int)), ("mmceq", C.POINTER(C.c_int)),
+                ("ttcc", C.POINTER(C.c_double))]
+
+
+def _sam(s, keep):
+    arrs = [np.ascontiguousarray(a, dtype=np.int32) for a in (s.meqn, s.mpmceq, s.mmceq)]
+    tt = np.ascontiguousarray(s.ttcc, dtype=np.float64)
+    keep.bufs.extend(arrs + [tt])
+    ip = C.POINTER(C.c_int)
+    return _Sam(arrs[0].ctypes.data_as(ip), len(arrs[0]), len(arrs[1]) - 1,
+                arrs[1].ctypes.data_as(ip), arrs[2].ctypes.data_as(ip),
+                tt.ctypes.data_as(C.POINTER(C.c_double)))
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -63,6 +79,13 @@ def lib():
                                            C.c_int, C.c_int, C.c_void_p]
         L.orc_expand_solution_g.argtypes = [C.c_int, _ip, _dp, _dp, _dp]
         L.orc_expand_solution_g.restype = None
+        L.orc_pattern_sam.argtypes = [C.POINTER(_Patch), C.c_int, C.POINTER(_Sam), C.c_int, _lp,
+                                      C.c_void_p]
+        L.orc_pattern_sam.restype = C.c_int64
+        L.orc_assemble_sam.argtypes = [C.POINTER(_Patch), C.POINTER(_Integrand), _ip, C.c_int,
+                                       C.POINTER(_Sam), C.c_int, _lp, _ip, _dp, _dp]
+        L.orc_expand_sam.argtypes = [C.POINTER(_Sam), _dp, _dp]
+        L.orc_expand_sam.restype = None
         L.orc_assemble_traction.argtypes = [C.POINTER(_Patch), C.c_int, _dp, C.c_int, C.c_int,
                                             _ip, _dp]
         L.orc_spmv.argtypes = [C.c_int, _lp, _ip, _dp, _dp, _dp]
@@ -96,6 +119,10 @@ def ref_lib():
                                         C.POINTER(C.c_int), _lp, _ip, C.c_int64]
         L.ref_sam_couplings.restype = C.c_int64
         L.ref_sam_elmeqns.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, C.c_int, _ip]
+        L.ref_sam_couplings_mpc.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, C.c_int,
+                                            _ip, _ip, _dp, _ip, C.POINTER(C.c_int), _lp, _ip,
+                                            C.c_int64, _dp, _dp]
+        L.ref_sam_couplings_mpc.restype = C.c_int64
         _REF = L
     return _REF
 
@@ -235,6 +262,40 @@ def assemble_into(P, itg, ng, meqn, neq, rowptr, colind, A, b, gdof=None):
                                     None if g is None else g.ctypes.data)
     if rc:
         raise RuntimeError("orc_assemble_range_g failed")
+
+
+def pattern_sam(P, ncomp, sam):
+    """CSR pattern with constraint equations (masters included)."""
+    keep = _Keep()
+    sp, ss = _patch(P, keep), _sam(sam, keep)
+    rowptr = np.zeros(sam.neq + 1, dtype=np.int64)
+    nnz = lib().orc_pattern_sam(C.byref(sp), ncomp, C.byref(ss), sam.neq, rowptr, None)
+    if nnz < 0:
+        raise RuntimeError("orc_pattern_sam failed")
+    colind = np.zeros(max(nnz, 1), dtype=np.int32)
+    lib().orc_pattern_sam(C.byref(sp), ncomp, C.byref(ss), sam.neq, rowptr, colind.ctypes.data)
+    return rowptr, colind[:nnz]
+
+
+def assemble_sam(P, itg, ng, sam, rowptr, colind, A=None, b=None):
+    """K and b with constraint equations (masters included), added into A, b
+    (fresh zeros when not given)."""
+    keep = _Keep()
+    sp, si, ss = _patch(P, keep), _integrand(itg, keep), _sam(sam, keep)
+    A = np.zeros(int(rowptr[-1])) if A is None else A
+    b = np.zeros(sam.neq) if b is None else b
+    if lib().orc_assemble_sam(C.byref(sp), C.byref(si), np.asarray(ng, dtype=np.int32), itg.ncomp,
+                              C.byref(ss), sam.neq, rowptr, colind, A, b):
+        raise RuntimeError("orc_assemble_sam failed")
+    return A, b
+
+
+def expand_sam(sam, xeq):
+    keep = _Keep()
+    ss = _sam(sam, keep)
+    out = np.zeros(len(sam.meqn))
+    lib().orc_expand_sam(C.byref(ss), np.ascontiguousarray(xeq, dtype=np.float64), out)
+    return out
 
 
 def expand_solution(meqn, xeq, gdof=None):

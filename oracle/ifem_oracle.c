@@ -996,3 +996,214 @@ int orc_norms(const orc_patch* P, const orc_integrand* itg, const int ng[3],
   for (int m = 0; m < 6; m++) out[m] = sum[m] < 0.0 ? -sqrt(-sum[m]) : sqrt(sum[m]);
   return 0;
 }
+
+/* ---- constraint equations with master DOFs ---------------------------------
+ * SAM's layout (src/LinAlg/SAM.h:330-350): meqn < 0 marks the slave of
+ * constraint k = -meqn; its entries are the 1-based positions
+ * mpmceq[k-1] .. mpmceq[k]-1 of mmceq/ttcc, the first the slave itself with
+ * the constant c0, the others its masters (free DOFs) and coefficients. */
+
+/* Element equation numbers meen from the DOF numbers (SAM::getElmEqns,
+ * src/LinAlg/SAM.C:655-696: meqn of each element DOF). */
+static void elm_eqns(const orc_patch* P, const orc_sam* s, int ncomp, int e1, int e2, int e3,
+                     int* meen, int* nedof)
+{
+  int mnpc[ORC_MAXNEN];
+  orc_element_nodes(P, e1, e2, e3, mnpc);
+  const int nen = (P->p[0] + 1) * (P->p[1] + 1) * (P->p[2] + 1);
+  for (int a = 0; a < nen; a++)
+    for (int c = 0; c < ncomp; c++) meen[a * ncomp + c] = s->meqn[mnpc[a] * ncomp + c];
+  *nedof = nen * ncomp;
+}
+
+typedef struct { int* v; int n, cap; } ivec;
+static void iv_push(ivec* a, int x)
+{
+  if (a->n == a->cap) {
+    a->cap = a->cap ? 2 * a->cap : 64;
+    a->v = (int*)realloc(a->v, sizeof(int) * a->cap);
+  }
+  a->v[a->n++] = x;
+}
+
+/* SAM::getDofCouplings with constraint equations (src/LinAlg/SAM.C:377-427),
+ * 0-based columns, sorted and unique per row. */
+int64_t orc_pattern_sam(const orc_patch* P, int ncomp, const orc_sam* s, int neq,
+                        int64_t* rowptr, int32_t* colind)
+{
+  int nel[3];
+  if (orc_num_elements(P, nel)) return -1;
+  ivec* rows = (ivec*)calloc(neq > 0 ? neq : 1, sizeof(ivec));
+  int meen[3 * ORC_MAXNEN], nedof = 0;
+  for (int e3 = 0; e3 < nel[2]; e3++)
+    for (int e2 = 0; e2 < nel[1]; e2++)
+      for (int e1 = 0; e1 < nel[0]; e1++) {
+        elm_eqns(P, s, ncomp, e1, e2, e3, meen, &nedof);
+        for (int j = 0; j < nedof; j++) {
+          const int jeq = meen[j];
+          if (jeq > 0) {
+            iv_push(&rows[jeq - 1], jeq);
+            for (int i = 0; i < j; i++)
+              if (meen[i] > 0) {
+                iv_push(&rows[meen[i] - 1], jeq);
+                iv_push(&rows[jeq - 1], meen[i]);
+              }
+          } else if (jeq < 0) {
+            const int jc = -jeq;
+            for (int jp = s->mpmceq[jc - 1]; jp < s->mpmceq[jc] - 1; jp++) {
+              if (s->mmceq[jp] <= 0) continue;
+              const int mj = s->meqn[s->mmceq[jp] - 1];
+              for (int i = 0; i < nedof; i++) {
+                const int ieq = meen[i];
+                if (ieq > 0) {
+                  iv_push(&rows[ieq - 1], mj);
+                  iv_push(&rows[mj - 1], ieq);
+                } else if (ieq < 0) {
+                  const int ic = -ieq;
+                  for (int ip = s->mpmceq[ic - 1]; ip < s->mpmceq[ic] - 1; ip++)
+                    if (s->mmceq[ip] > 0) iv_push(&rows[s->meqn[s->mmceq[ip] - 1] - 1], mj);
+                }
+              }
+            }
+          }
+        }
+      }
+  int64_t nnz = 0;
+  rowptr[0] = 0;
+  for (int r = 0; r < neq; r++) {
+    ivec* a = &rows[r];
+    qsort(a->v, a->n, sizeof(int), cmp_int);
+    int u = 0;
+    for (int q = 0; q < a->n; q++)
+      if (u == 0 || a->v[q] != a->v[u - 1]) a->v[u++] = a->v[q];
+    if (colind)
+      for (int q = 0; q < u; q++) colind[nnz + q] = a->v[q] - 1;
+    nnz += u;
+    rowptr[r + 1] = nnz;
+    free(a->v);
+  }
+  free(rows);
+  return nnz;
+}
+
+static int add_entry(const int64_t* rowptr, const int32_t* colind, double* A, int r, int c,
+                     double v)
+{
+  double* d = sm_entry(rowptr, colind, A, r, c);
+  if (!d) return -1;
+  *d += v;
+  return 0;
+}
+
+/* SparseMatrix assemSparse with constraint equations
+ * (src/LinAlg/SparseMatrix.C:612-688) and SAM::assembleRHS for the element
+ * vector (src/LinAlg/SAM.C:605-617), one element. */
+static int assem_element_sam(int nedof, const double* EK, const double* EV, const int* meen,
+                             const orc_sam* s, const int64_t* rowptr, const int32_t* colind,
+                             double* A, double* b)
+{
+  int err = 0;
+  /* free-free block */
+  for (int j = 0; j < nedof; j++) {
+    const int jeq = meen[j];
+    if (jeq < 1) continue;
+    err |= add_entry(rowptr, colind, A, jeq - 1, jeq - 1, EK[j * nedof + j]);
+    for (int i = 0; i < j; i++) {
+      const int ieq = meen[i];
+      if (ieq < 1) continue;
+      err |= add_entry(rowptr, colind, A, ieq - 1, jeq - 1, EK[i * nedof + j]);
+      err |= add_entry(rowptr, colind, A, jeq - 1, ieq - 1, EK[j * nedof + i]);
+    }
+  }
+  /* constrained (prescribed and dependent) columns */
+  for (int j = 0; j < nedof; j++) {
+    const int jc = -meen[j];
+    if (jc < 1) continue;
+    const int jp0 = s->mpmceq[jc - 1];
+    const double c0 = s->ttcc[jp0 - 1];
+    for (int i = 0; i < nedof; i++) {
+      const int ieq = meen[i];
+      if (ieq > 0) b[ieq - 1] -= c0 * EK[i * nedof + j];
+      else if (ieq < 0) {
+        const int ic = -ieq;
+        for (int ip = s->mpmceq[ic - 1]; ip < s->mpmceq[ic] - 1; ip++)
+          if (s->mmceq[ip] > 0)
+            b[s->meqn[s->mmceq[ip] - 1] - 1] -= c0 * s->ttcc[ip] * EK[i * nedof + j];
+      }
+    }
+    for (int jp = s->mpmceq[jc - 1]; jp < s->mpmceq[jc] - 1; jp++) {
+      if (s->mmceq[jp] <= 0) continue;
+      const int jeq = s->meqn[s->mmceq[jp] - 1];
+      for (int i = 0; i < nedof; i++) {
+        const int ieq = meen[i];
+        if (ieq > 0) {
+          err |= add_entry(rowptr, colind, A, ieq - 1, jeq - 1, s->ttcc[jp] * EK[i * nedof + j]);
+          err |= add_entry(rowptr, colind, A, jeq - 1, ieq - 1, s->ttcc[jp] * EK[j * nedof + i]);
+        } else if (ieq < 0) {
+          const int ic = -ieq;
+          for (int ip = s->mpmceq[ic - 1]; ip < s->mpmceq[ic] - 1; ip++)
+            if (s->mmceq[ip] > 0)
+              err |= add_entry(rowptr, colind, A, s->meqn[s->mmceq[ip] - 1] - 1, jeq - 1,
+                               s->ttcc[ip] * s->ttcc[jp] * EK[i * nedof + j]);
+        }
+      }
+    }
+  }
+  /* element vector */
+  for (int i = 0; i < nedof; i++) {
+    const int ieq = meen[i];
+    if (ieq > 0) b[ieq - 1] += EV[i];
+    else if (ieq < 0) {
+      const int ic = -ieq;
+      for (int ip = s->mpmceq[ic - 1]; ip < s->mpmceq[ic] - 1; ip++)
+        if (s->mmceq[ip] > 0) b[s->meqn[s->mmceq[ip] - 1] - 1] += s->ttcc[ip] * EV[i];
+    }
+  }
+  return err;
+}
+
+/* Element loop (serial) adding into A, b (not zeroed: several patches can
+ * be added with their own local meqn). */
+int orc_assemble_sam(const orc_patch* P, const orc_integrand* itg, const int ng[3], int ncomp,
+                     const orc_sam* s, int neq, const int64_t* rowptr, const int32_t* colind,
+                     double* A, double* b)
+{
+  (void)neq;
+  int nel[3];
+  if (orc_num_elements(P, nel)) return -1;
+  const int nen = (P->p[0] + 1) * (P->p[1] + 1) * (P->p[2] + 1);
+  const int nedof = nen * ncomp;
+  double* EK = (double*)malloc(sizeof(double) * nedof * nedof);
+  double* EV = (double*)malloc(sizeof(double) * nedof);
+  int meen[3 * ORC_MAXNEN], nd = 0, err = 0;
+  for (int e3 = 0; e3 < nel[2] && !err; e3++)
+    for (int e2 = 0; e2 < nel[1] && !err; e2++)
+      for (int e1 = 0; e1 < nel[0] && !err; e1++) {
+        if (orc_element_matrices(P, itg, ng, e1, e2, e3, EK, EV)) { err = 1; break; }
+        elm_eqns(P, s, ncomp, e1, e2, e3, meen, &nd);
+        if (assem_element_sam(nedof, EK, EV, meen, s, rowptr, colind, A, b)) err = 1;
+      }
+  free(EK);
+  free(EV);
+  return err;
+}
+
+/* SAM::expandVector with scaleSD = 1 (src/LinAlg/SAM.C:783-807). */
+void orc_expand_sam(const orc_sam* s, const double* xeq, double* xdof)
+{
+  for (int i = 0; i < s->ndof; i++) {
+    xdof[i] = 0.0;
+    const int ieq = s->meqn[i];
+    if (ieq > 0) xdof[i] += xeq[ieq - 1];
+    else if (ieq < 0) {
+      const int ic = -ieq;
+      int ip = s->mpmceq[ic - 1];
+      xdof[i] += s->ttcc[ip - 1];
+      for (; ip < s->mpmceq[ic] - 1; ip++)
+        if (s->mmceq[ip] > 0) {
+          const int me = s->meqn[s->mmceq[ip] - 1];
+          if (me > 0) xdof[i] += s->ttcc[ip] * xeq[me - 1];
+        }
+    }
+  }
+}

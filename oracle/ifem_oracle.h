@@ -108,6 +108,32 @@ int orc_assemble_range_g(const orc_patch* P, const orc_integrand* itg,
                          double* A, double* b, int nthreads,
                          int e3_begin, int e3_end, const double* gdof);
 
+/* SAM arrays with constraint equations (1-based, SAM layout): meqn per DOF
+ * (> 0 equation, 0 fixed, -k slave of constraint k); constraint k holds the
+ * entries mpmceq[k-1] .. mpmceq[k]-1 of mmceq (DOF numbers) and ttcc (first
+ * the slave and c0, then masters and coefficients). */
+typedef struct {
+  const int* meqn;
+  int ndof;
+  int nceq;
+  const int* mpmceq;
+  const int* mmceq;
+  const double* ttcc;
+} orc_sam;
+
+/* SAM::getDofCouplings with constraints (src/LinAlg/SAM.C:377-427): nnz, or
+ * -1; colind may be NULL (count only). */
+int64_t orc_pattern_sam(const orc_patch* P, int ncomp, const orc_sam* s, int neq,
+                        int64_t* rowptr, int32_t* colind);
+/* assemSparse + SAM::assembleRHS with constraint equations, masters included
+ * (src/LinAlg/SparseMatrix.C:612-688, src/LinAlg/SAM.C:605-617); adds into
+ * A and b. */
+int orc_assemble_sam(const orc_patch* P, const orc_integrand* itg, const int ng[3], int ncomp,
+                     const orc_sam* s, int neq, const int64_t* rowptr, const int32_t* colind,
+                     double* A, double* b);
+/* SAM::expandVector, scaleSD = 1 (src/LinAlg/SAM.C:783-807). */
+void orc_expand_sam(const orc_sam* s, const double* xeq, double* xdof);
+
 /* a21: Neumann traction t on face (0..5 as above), ng points per tangent
  * direction, added into b. */
 int orc_assemble_traction(const orc_patch* P, int face, const double t[3],

@@ -13,6 +13,7 @@
 //   SAM::initSystemEquations              src/LinAlg/SAM.C:225-336
 //   SAM::getDofCouplings                  src/LinAlg/SAM.C:377-427
 //   SAM::getElmEqns                       src/LinAlg/SAM.C:655-696
+//   SAM::expandVector                     src/LinAlg/SAM.C:783-807
 #include "CoordinateMapping.h"
 #include "GaussQuadrature.h"
 #include "SAM.h"
@@ -31,8 +32,11 @@ namespace {
 class ShimSAM : public SAM
 {
 public:
+  // With nceq > 0, the constraint equations in SAM's own layout (the slave
+  // DOFs carry status 0, as SAMpatch::initConstraintEqs leaves them).
   ShimSAM(int nnod_, int ncomp, int nel_, const int* mpmnpc_, const int* mmnpc_,
-          const int* msc_)
+          const int* msc_, int nceq_ = 0, const int* mpmceq_ = nullptr,
+          const int* mmceq_ = nullptr, const double* ttcc_ = nullptr)
   {
     nnod = nnod_;
     nel = nel_;
@@ -46,10 +50,23 @@ public:
     for (int i = 0; i <= nnod_; i++) madof[i] = 1 + i * ncomp;
     msc = new int[ndof];
     std::memcpy(msc, msc_, sizeof(int) * ndof);
+    if (nceq_ > 0) {
+      nceq = nceq_;
+      nmmceq = mpmceq_[nceq_] - 1;
+      mpmceq = new int[nceq_ + 1];
+      std::memcpy(mpmceq, mpmceq_, sizeof(int) * (nceq_ + 1));
+      mmceq = new int[nmmceq];
+      std::memcpy(mmceq, mmceq_, sizeof(int) * nmmceq);
+      ttcc = new Real[nmmceq];
+      std::memcpy(ttcc, ttcc_, sizeof(Real) * nmmceq);
+    }
     ok = this->initSystemEquations();
   }
   bool ok = false;
   const int* eqns() const { return meqn; }
+  // SAM::expandVector with scaleSD = 1 (what SAM::expandSolution uses by
+  // default)
+  bool expand(const double* x, Vector& dofs) const { return expandVector(x, dofs, 1.0); }
 };
 
 } // namespace
@@ -122,6 +139,39 @@ int ref_sam_elmeqns(int nnod, int ncomp, int nel, const int* mpmnpc,
   if (!sam.getElmEqns(v, iel)) return -1;
   for (size_t i = 0; i < v.size(); i++) meen[i] = v[i];
   return (int)v.size();
+}
+
+// Couplings, equation numbers and solution expansion with constraint
+// equations (masters included): SAM::initSystemEquations,
+// SAM::getDofCouplings (src/LinAlg/SAM.C:377-427) and SAM::expandVector
+// (src/LinAlg/SAM.C:783-807, scaleSD = 1) on xeq (neq entries) -> xdof.
+int64_t ref_sam_couplings_mpc(int nnod, int ncomp, int nel, const int* mpmnpc,
+                              const int* mmnpc, const int* msc, int nceq, const int* mpmceq,
+                              const int* mmceq, const double* ttcc, int* meqn, int* neq_out,
+                              int64_t* rowptr, int32_t* colind, int64_t cap, const double* xeq,
+                              double* xdof)
+{
+  ShimSAM sam(nnod, ncomp, nel, mpmnpc, mmnpc, msc, nceq, mpmceq, mmceq, ttcc);
+  if (!sam.ok) return -1;
+  const int ndof = nnod * ncomp;
+  std::memcpy(meqn, sam.eqns(), sizeof(int) * ndof);
+  const int neq = sam.getNoEquations();
+  *neq_out = neq;
+  std::vector<std::set<int>> dofc;
+  sam.getDofCouplings(dofc);
+  int64_t nnz = 0;
+  rowptr[0] = 0;
+  for (int r = 0; r < neq; r++) {
+    if (nnz + (int64_t)dofc[r].size() > cap) return -2;
+    for (int c : dofc[r]) colind[nnz++] = c - 1;
+    rowptr[r + 1] = nnz;
+  }
+  if (xeq && xdof) {
+    Vector dofs;
+    if (!sam.expand(xeq, dofs)) return -3;
+    for (int i = 0; i < ndof; i++) xdof[i] = dofs[i];
+  }
+  return nnz;
 }
 
 } // extern "C"

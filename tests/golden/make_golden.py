@@ -10,8 +10,13 @@
   (src/LinAlg/SAM.C:225-427) for small 3D patches with Dirichlet faces.
 * scipy_poisson16.npz: scipy.sparse.linalg.spsolve solution of the 16^3 p=2
   Poisson system assembled by the oracle (a library solve, to pin the PCG).
+* reference_sam_mpc.json: SAM equation numbers, DOF couplings and solution
+  expansion with constraint equations that have master DOFs
+  (src/LinAlg/SAM.C:225-427, 783-807): periodicity in x plus one general
+  constraint with two masters and a constant, through oracle/_ref.
 
 Run from the repository root:  python tests/golden/make_golden.py
+(python tests/golden/make_golden.py mpc writes only reference_sam_mpc.json)
 """
 import json
 import os
@@ -106,5 +111,65 @@ def main():
     print("golden fixtures written")
 
 
+def mpc_cases():
+    """Small patches with periodicity in x and one extra constraint with two
+    masters and a constant c0 (the SAM arrays come from
+    model.constraint_equations)."""
+    out = []
+    for (p, nel, ncomp, fix) in [(2, (3, 2, 2), 1, [0, 0, 1, 1, 0, 0]),
+                                 (1, (3, 2, 2), 3, [0, 0, 7, 0, 0, 0]),
+                                 (2, (2, 2, 3), 3, [0, 0, 0, 0, 7, 0])]:
+        P = M.cube_patch(p, list(nel))
+        msc = M.face_status(P, ncomp, fix)
+        cons = M.periodic_constraints(P, ncomp, 0)
+        # an interior DOF tied to two others with a constant
+        interior = [d for d in range(P.nnod * ncomp) if msc[d] == 1
+                    and d not in {c[0] for c in cons} and d not in
+                    {m for c in cons for m, _ in c[2]}]
+        s_, m1, m2 = interior[len(interior) // 2], interior[0], interior[-1]
+        cons.append((s_, 0.25, [(m1, 0.5), (m2, -1.5)]))
+        out.append((P, ncomp, msc, M.constraint_equations(msc, cons)))
+    return out
+
+
+def write_mpc():
+    import ctypes as C
+    R = oracle.ref_lib()
+    if R is None:
+        raise SystemExit("oracle/_ref/libifemref.so not built (make -C oracle)")
+    cases = []
+    for P, ncomp, msc, sam in mpc_cases():
+        mpmnpc, mmnpc = element_connectivity(P)
+        ndof = P.nnod * ncomp
+        msc_ref = np.where(sam.meqn < 0, 0, msc).astype(np.int32)
+        meqn = np.zeros(ndof, dtype=np.int32)
+        nq = C.c_int(0)
+        cap = ndof * ndof
+        rowptr = np.zeros(ndof + 1, dtype=np.int64)
+        colind = np.zeros(cap, dtype=np.int32)
+        xeq = 1.0 + 0.1 * np.arange(ndof, dtype=np.float64)
+        xdof = np.zeros(ndof)
+        nnz = R.ref_sam_couplings_mpc(P.nnod, ncomp, int(np.prod(P.nel)), mpmnpc, mmnpc, msc_ref,
+                                      sam.nceq, np.ascontiguousarray(sam.mpmceq, dtype=np.int32),
+                                      np.ascontiguousarray(sam.mmceq, dtype=np.int32),
+                                      np.ascontiguousarray(sam.ttcc, dtype=np.float64), meqn,
+                                      C.byref(nq), rowptr, colind, cap, xeq, xdof)
+        assert nnz >= 0, nnz
+        cases.append({"p": P.p[0], "nel": P.nel, "ncomp": ncomp, "msc": [int(v) for v in msc],
+                      "mpmceq": sam.mpmceq.tolist(), "mmceq": sam.mmceq.tolist(),
+                      "ttcc": [float(v).hex() for v in sam.ttcc], "neq": nq.value,
+                      "meqn": meqn.tolist(), "rowptr": rowptr[:nq.value + 1].tolist(),
+                      "colind": colind[:nnz].tolist(),
+                      "xeq": [float(v).hex() for v in xeq[:nq.value]],
+                      "xdof": [float(v).hex() for v in xdof]})
+    with open(os.path.join(HERE, "reference_sam_mpc.json"), "w") as f:
+        json.dump(cases, f)
+    print("reference_sam_mpc.json written")
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["mpc"]:
+        write_mpc()
+    else:
+        main()
+        write_mpc()

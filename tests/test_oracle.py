@@ -653,3 +653,102 @@ def test_prescribed_rhs_equals_full_system_partition():
         A3, b3 = oracle.assemble(P, itg, pb.ng, sam.meqn, sam.neq, rp2, ci2, gdof=np.zeros(ndof))
         A4, b4 = oracle.assemble(P, itg, pb.ng, sam.meqn, sam.neq, rp2, ci2)
         assert np.array_equal(A3, A4) and np.array_equal(b3, b4)
+
+
+# ---- constraint equations with master DOFs ---------------------------------------
+def _mpc_fixture():
+    with open(os.path.join(GOLD, "reference_sam_mpc.json")) as f:
+        return json.load(f)
+
+
+def test_constraint_numbering_couplings_and_expansion_match_reference():
+    """Equation numbers, DOF couplings and solution expansion with periodic
+    and general constraints (masters, constants) equal the reference's own
+    SAM (oracle/_ref fixtures, src/LinAlg/SAM.C:225-427, 783-807)."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    from make_golden import mpc_cases
+    for (P, ncomp, msc, sam), c in zip(mpc_cases(), _mpc_fixture()):
+        assert msc.tolist() == c["msc"]
+        assert sam.meqn.tolist() == c["meqn"] and sam.neq == c["neq"]
+        assert sam.mpmceq.tolist() == c["mpmceq"] and sam.mmceq.tolist() == c["mmceq"]
+        rp, ci = oracle.pattern_sam(P, ncomp, sam)
+        assert rp.tolist() == c["rowptr"]
+        assert ci.tolist() == c["colind"]
+        xd = oracle.expand_sam(sam, _hex(c["xeq"]))
+        assert np.array_equal(xd, _hex(c["xdof"]))
+
+
+def _condensed_reference(P, itg, ng, ncomp, msc, sam):
+    """T^T K T and T^T (b - K t0) from the system with every non-fixed DOF
+    free (scipy), T mapping the constrained unknowns to all DOFs."""
+    meq_all, neq_all = M.equation_numbers(np.asarray(msc))
+    rp, ci, A, b = oracle.system(P, itg, ng, meq_all, neq_all)
+    K = sp.csr_matrix((A, ci, rp), shape=(neq_all, neq_all))
+    T = sp.lil_matrix((neq_all, sam.neq))
+    t0 = np.zeros(neq_all)
+    for d in range(len(msc)):
+        ea = meq_all[d] - 1
+        if ea < 0:
+            continue
+        if sam.meqn[d] > 0:
+            T[ea, sam.meqn[d] - 1] = 1.0
+        elif sam.meqn[d] < 0:
+            k = -sam.meqn[d]
+            lo, hi = sam.mpmceq[k - 1] - 1, sam.mpmceq[k] - 1
+            t0[ea] = sam.ttcc[lo]
+            for q in range(lo + 1, hi):
+                T[ea, sam.meqn[sam.mmceq[q] - 1] - 1] += sam.ttcc[q]
+    T = T.tocsr()
+    return (T.T @ K @ T).toarray(), T.T @ (b - K @ t0)
+
+
+def test_constrained_assembly_equals_condensation_of_the_free_system():
+    """assemSparse with masters (element by element) equals T^T K T and
+    T^T (b - K t0) formed with scipy from the all-free system, for Poisson
+    (with a source) and elasticity (with a body force)."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    from make_golden import mpc_cases
+    for P, ncomp, msc, sam in mpc_cases():
+        P = M.cube_patch(P.p[0], P.nel, jitter=0.1, seed=3)
+        itg = (M.Integrand(M.POISSON, source=M.smooth_source(2)) if ncomp == 1 else
+               M.Integrand(M.ELASTICITY, body=(0.2, -0.1, 0.3)))
+        ng = [P.p[0] + 1] * 3
+        rp, ci = oracle.pattern_sam(P, ncomp, sam)
+        A, b = oracle.assemble_sam(P, itg, ng, sam, rp, ci)
+        Kc, bc = _condensed_reference(P, itg, ng, ncomp, msc, sam)
+        K = sp.csr_matrix((A, ci, rp), shape=(sam.neq, sam.neq)).toarray()
+        assert np.max(np.abs(K - Kc)) <= 1e-13 * np.abs(Kc).max()
+        assert np.max(np.abs(b - bc)) <= 1e-13 * np.abs(bc).max()
+
+
+def _periodic_poisson(nel, p=2):
+    """-Laplace u = f with u = cos(2 pi x) sin(pi y) sin(pi z): periodic in x,
+    u = 0 on the y and z faces."""
+    P = M.cube_patch(p, nel)
+    modes = np.array([[2 * np.pi, np.pi, -np.pi, 0.0, 0.25], [2 * np.pi, -np.pi, np.pi, 0.0, 0.25],
+                      [2 * np.pi, np.pi, np.pi, 0.0, -0.25], [2 * np.pi, -np.pi, -np.pi, 0.0, -0.25]])
+    exact = M.Function(M.FN_FOURIER, 1.0, 0.0, modes)
+    itg = M.Integrand(M.POISSON, source=M.Function(M.FN_FOURIER, 6 * np.pi ** 2, 0.0, modes))
+    msc = M.face_status(P, 1, [0, 0, 1, 1, 1, 1])
+    sam = M.constraint_equations(msc, M.periodic_constraints(P, 1, 0))
+    return P, itg, exact, sam
+
+
+def test_periodic_poisson_converges_to_the_periodic_solution():
+    """The constrained system's solution converges to the periodic exact
+    solution at O(h^3) in L2 (p = 2), and the slave DOFs equal their
+    masters after expansion."""
+    errs = []
+    for nel in (8, 16):
+        P, itg, exact, sam = _periodic_poisson(nel)
+        rp, ci = oracle.pattern_sam(P, 1, sam)
+        A, b = oracle.assemble_sam(P, itg, [3, 3, 3], sam, rp, ci)
+        x, it, res = oracle.pcg(rp, ci, A, b, rtol=1e-13)
+        xd = oracle.expand_sam(sam, x)
+        n = P.n
+        xd3 = xd.reshape(n[2], n[1], n[0])
+        assert np.array_equal(xd3[:, :, 0], xd3[:, :, -1])
+        errs.append(oracle.norms(P, itg, [3, 3, 3], xd, exact)[4])
+    assert 6.0 < errs[0] / errs[1] < 10.0, errs

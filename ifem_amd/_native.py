@@ -217,7 +217,8 @@ class Session:
 
     def bind_constraints(self, patch_id: int, ncomp: int, sam):
         """Binds SAM arrays with constraint equations: ``sam`` has meqn, neq,
-        mpmceq, mmceq, ttcc (1-based, SAM layout; prescribed values only)."""
+        mpmceq, mmceq, ttcc (1-based, SAM layout; prescribed values and
+        constraints u_s = c0 + sum c_m u_m with free masters)."""
         m = np.ascontiguousarray(sam.meqn, dtype=np.int32)
         mp = np.ascontiguousarray(sam.mpmceq, dtype=np.int32)
         mm = np.ascontiguousarray(sam.mmceq, dtype=np.int32)

@@ -26,6 +26,7 @@ static void free_system(Context& c)
 {
   solver_free(c);
   c.mg.reset();
+  mpc_free(c);
   for (SubPatch& sp : c.multi) {
     hipFree(sp.d_leq);
     hipFree(sp.d_lgval);
@@ -247,8 +248,10 @@ int ifem_gpu_sam_bind(ifem_gpu_ctx* ctx, int patch_id, int ncomp, int ndof, cons
 // Prescribed value per DOF from SAM's constraint arrays (constraints
 // without masters only).
 static int constraint_values(int ndof, const int* meqn, int nceq, const int* mpmceq,
-                             const int* mmceq, const double* ttcc, std::vector<double>& gval)
+                             const int* mmceq, const double* ttcc, std::vector<double>& gval,
+                             bool* masters = nullptr)
 {
+  if (masters) *masters = false;
   if (!meqn || ndof < 0 || nceq < 0 || (nceq > 0 && (!mpmceq || !mmceq || !ttcc)))
     return fail(IFEM_GPU_EINVAL, "sam_bind_constraints: null argument");
   // SAM layout (src/LinAlg/SAM.C:260-308): constraint i (1-based) holds the
@@ -266,9 +269,12 @@ static int constraint_values(int ndof, const int* meqn, int nceq, const int* mpm
     if (jp < ip) return fail(IFEM_GPU_EINVAL, "sam_bind_constraints: mpmceq is not increasing");
     if (mmceq[ip - 1] != i + 1)
       return fail(IFEM_GPU_EINVAL, "sam_bind_constraints: constraint slave differs from the DOF");
-    if (jp > ip)
-      return fail(IFEM_GPU_EUNSUPPORTED,
-                  "sam_bind_constraints: constraints with master DOFs are not supported");
+    if (jp > ip) {
+      if (!masters)
+        return fail(IFEM_GPU_EUNSUPPORTED,
+                    "sam_bind_patches: constraints with master DOFs need a single patch");
+      *masters = true;
+    }
     if (seen[ceq - 1]) return fail(IFEM_GPU_EINVAL, "sam_bind_constraints: constraint used twice");
     seen[ceq - 1] = 1;
     gval[i] = ttcc[ip - 1];
@@ -282,9 +288,13 @@ int ifem_gpu_sam_bind_constraints(ifem_gpu_ctx* ctx, int patch_id, int ncomp, in
 {
   CHECK_CTX(ctx);
   std::vector<double> gval;
-  int rc = constraint_values(ndof, meqn, nceq, mpmceq, mmceq, ttcc, gval);
+  bool masters = false;
+  int rc = constraint_values(ndof, meqn, nceq, mpmceq, mmceq, ttcc, gval, &masters);
   if (rc) return rc;
-  return bind_system(ctx->c, patch_id, ncomp, ndof, meqn, neq, std::move(gval));
+  rc = bind_system(ctx->c, patch_id, ncomp, ndof, meqn, neq, std::move(gval));
+  if (rc || !masters) return rc;
+  // slaves with masters: extended system + condensation
+  return mpc_bind(ctx->c, nceq, mpmceq, mmceq, ttcc);
 }
 
 int ifem_gpu_sam_bind_patches(ifem_gpu_ctx* ctx, int npatch, const int* patch_ids,
@@ -559,6 +569,10 @@ int ifem_gpu_expand_solution(ifem_gpu_ctx* ctx, double* xdof)
   std::vector<double> x(c.neq);
   IFEM_HIP(hipStreamSynchronize(c.stream));
   IFEM_HIP(hipMemcpy(x.data(), c.w.x, sizeof(double) * c.neq, hipMemcpyDeviceToHost));
+  if (c.mpc) { // slaves with masters: c0 + sum c_m x_m
+    mpc_expand_host(c, x, xdof);
+    return IFEM_GPU_OK;
+  }
   // eliminated DOFs: 0, or their prescribed value (src/LinAlg/SAM.C:768-810)
   const bool g = !c.h_gval.empty();
   for (int i = 0; i < c.ndof; i++)

@@ -897,11 +897,12 @@ __global__ void k_scan_add(int64_t* __restrict__ out, const int64_t* __restrict_
 // Several patches: the global pattern is the union of the patches' node
 // couplings (nodes within +-p in every direction of one patch), built on the
 // host (SAM::getDofCouplings over all patches, src/LinAlg/SAM.C:377-427).
-static int build_pattern_multi(Context& c)
+int host_union_pattern(const Context& c, const std::vector<SubPatch>& subs, int neq,
+                       std::vector<int64_t>& rowptr, std::vector<int32_t>& colind)
 {
-  const int nc = c.ncomp, neq = c.neq;
+  const int nc = c.ncomp;
   std::vector<std::vector<int>> cols(neq > 0 ? neq : 1);
-  for (const SubPatch& sp : c.multi) {
+  for (const SubPatch& sp : subs) {
     const PatchRec& P = c.patches[sp.patch];
     const int p = P.p[0], n1 = P.n[0], n2 = P.n[1], n3 = P.n[2];
     for (int k = 0; k < n3; k++)
@@ -924,7 +925,7 @@ static int build_pattern_multi(Context& c)
           }
         }
   }
-  std::vector<int64_t> rowptr(neq + 1, 0);
+  rowptr.assign(neq + 1, 0);
   for (int r = 0; r < neq; r++) {
     std::vector<int>& row = cols[r];
     std::sort(row.begin(), row.end());
@@ -932,9 +933,20 @@ static int build_pattern_multi(Context& c)
     rowptr[r + 1] = rowptr[r] + (int64_t)row.size();
   }
   const int64_t nnz = rowptr[neq];
-  std::vector<int32_t> colind(nnz > 0 ? nnz : 1);
+  colind.assign(nnz > 0 ? nnz : 1, 0);
   for (int r = 0; r < neq; r++)
     std::copy(cols[r].begin(), cols[r].end(), colind.begin() + rowptr[r]);
+  return IFEM_GPU_OK;
+}
+
+static int build_pattern_multi(Context& c)
+{
+  const int neq = c.neq;
+  std::vector<int64_t> rowptr;
+  std::vector<int32_t> colind;
+  int rc0 = host_union_pattern(c, c.multi, neq, rowptr, colind);
+  if (rc0) return rc0;
+  const int64_t nnz = rowptr[neq];
   hipFree(c.d_rowptr);
   hipFree(c.d_colind);
   hipFree(c.d_val);
@@ -962,6 +974,7 @@ static int build_pattern_multi(Context& c)
 
 int build_pattern(Context& c)
 {
+  if (c.mpc) return mpc_build_pattern(c);
   if (!c.multi.empty()) return build_pattern_multi(c);
   const PatchRec& P = c.patches[c.patch];
   if (P.p[0] != P.p[1] || P.p[0] != P.p[2])
@@ -1127,8 +1140,9 @@ static int dispatch_ng(Context& c, const AsmArgs& args, int ng, int nc, bool fp3
 // -> prescribed value, or nullptr).  accum: the system holds several
 // patches; values and right-hand side are added (zeroed by the caller).
 template <int P>
-static int assemble_p(Context& c, const PatchRec& Pt, const int* eq, const double* gval,
-                      int accum, const ifem_gpu_integrand& itg, int ng, int precision)
+static int assemble_p(Context& c, const SysView& sv, const PatchRec& Pt, const int* eq,
+                      const double* gval, int accum, const ifem_gpu_integrand& itg, int ng,
+                      int precision)
 {
   GaussArgs ga{};
   gauss_rule(ng, ga.xi, ga.w);
@@ -1148,19 +1162,18 @@ static int assemble_p(Context& c, const PatchRec& Pt, const int* eq, const doubl
   a.tabN3 = tb.tabN[2]; a.tabD3 = tb.tabD[2]; a.span3 = tb.span[2];
   for (int g = 0; g < kMaxNG; g++) a.gw[g] = ga.w[g];
   a.eq = eq;
-  a.rowptr = c.d_rowptr;
-  a.colind = c.d_colind;
+  a.rowptr = sv.rowptr;
+  a.colind = sv.colind;
   a.accum = accum;
   const bool fp32 = precision == IFEM_GPU_FP32;
   if (fp32) {
-    if (!c.d_val32) IFEM_HIP(hipMalloc(&c.d_val32, sizeof(float) * (c.nnz > 0 ? c.nnz : 1)));
-    a.val32 = c.d_val32;
+    a.val32 = sv.val32;
     a.val64 = nullptr;
     a.rhs = nullptr;
   } else {
-    a.val64 = c.d_val;
+    a.val64 = sv.val;
     a.val32 = nullptr;
-    a.rhs = c.d_rhs;
+    a.rhs = sv.rhs;
     a.gval = gval;
   }
   for (void* p : c.scratch_tables) hipFree(p);
@@ -1178,7 +1191,7 @@ static int assemble_p(Context& c, const PatchRec& Pt, const int* eq, const doubl
     if (c.ncomp == 1) has_rhs = itg.source.type != IFEM_GPU_FN_ZERO;
     else has_rhs = itg.body[0] != 0.0 || itg.body[1] != 0.0 || itg.body[2] != 0.0;
     if (!accum)
-      IFEM_HIP(hipMemsetAsync(c.d_rhs, 0, sizeof(double) * (c.neq > 0 ? c.neq : 1), c.stream));
+      IFEM_HIP(hipMemsetAsync(sv.rhs, 0, sizeof(double) * (sv.neq > 0 ? sv.neq : 1), c.stream));
   }
   // geometry records of all (element, Gauss point) pairs, then the lines
   a.ng = ng;
@@ -1212,8 +1225,6 @@ static int assemble_p(Context& c, const PatchRec& Pt, const int* eq, const doubl
   c.last_assemble_ms = ms;
   c.last_geometry_ms = ms_geo;
   c.last_lines_ms = ms - ms_geo;
-  if (fp32) c.fp32_ready = true;
-  else c.fp64_ready = true;
   return IFEM_GPU_OK;
 }
 
@@ -1247,37 +1258,33 @@ int autotune_assembly(Context& c, const ifem_gpu_integrand& itg, const int ng[3]
   return assemble_volume(c, itg, ng, precision);
 }
 
-static int assemble_one(Context& c, const PatchRec& Pt, const int* eq, const double* gval,
-                        int accum, const ifem_gpu_integrand& itg, int ng, int precision)
+static int assemble_one(Context& c, const SysView& sv, const PatchRec& Pt, const int* eq,
+                        const double* gval, int accum, const ifem_gpu_integrand& itg, int ng,
+                        int precision)
 {
   switch (Pt.p[0]) {
-  case 1: return assemble_p<1>(c, Pt, eq, gval, accum, itg, ng, precision);
-  case 2: return assemble_p<2>(c, Pt, eq, gval, accum, itg, ng, precision);
-  case 3: return assemble_p<3>(c, Pt, eq, gval, accum, itg, ng, precision);
+  case 1: return assemble_p<1>(c, sv, Pt, eq, gval, accum, itg, ng, precision);
+  case 2: return assemble_p<2>(c, sv, Pt, eq, gval, accum, itg, ng, precision);
+  case 3: return assemble_p<3>(c, sv, Pt, eq, gval, accum, itg, ng, precision);
   default: return fail(IFEM_GPU_EUNSUPPORTED, "assemble: degree out of range");
   }
 }
 
-int assemble_volume(Context& c, const ifem_gpu_integrand& itg, const int ng[3], int precision)
+// Patches added one after the other into the zeroed system sv.
+static int assemble_sum(Context& c, SysView& sv, const std::vector<SubPatch>& subs,
+                        const ifem_gpu_integrand& itg, int ng, int precision)
 {
-  if (ng[0] != ng[1] || ng[0] != ng[2])
-    return fail(IFEM_GPU_EUNSUPPORTED, "assemble: the GPU path needs equal Gauss counts per direction");
-  if (ng[0] < 1 || ng[0] > kMaxNG) return fail(IFEM_GPU_EUNSUPPORTED, "assemble: ng out of range");
-  if (c.multi.empty())
-    return assemble_one(c, c.patches[c.patch], c.d_eq, c.d_gval, 0, itg, ng[0], precision);
-  // several patches: zero the system, then add patch by patch in a fixed order
   const bool fp32 = precision == IFEM_GPU_FP32;
-  const size_t nz = (size_t)(c.nnz > 0 ? c.nnz : 1);
+  const size_t nz = (size_t)(sv.nnz > 0 ? sv.nnz : 1);
   if (fp32) {
-    if (!c.d_val32) IFEM_HIP(hipMalloc(&c.d_val32, sizeof(float) * nz));
-    IFEM_HIP(hipMemsetAsync(c.d_val32, 0, sizeof(float) * nz, c.stream));
+    IFEM_HIP(hipMemsetAsync(sv.val32, 0, sizeof(float) * nz, c.stream));
   } else {
-    IFEM_HIP(hipMemsetAsync(c.d_val, 0, sizeof(double) * nz, c.stream));
-    IFEM_HIP(hipMemsetAsync(c.d_rhs, 0, sizeof(double) * (c.neq > 0 ? c.neq : 1), c.stream));
+    IFEM_HIP(hipMemsetAsync(sv.val, 0, sizeof(double) * nz, c.stream));
+    IFEM_HIP(hipMemsetAsync(sv.rhs, 0, sizeof(double) * (sv.neq > 0 ? sv.neq : 1), c.stream));
   }
   double tot = 0.0, geo = 0.0;
-  for (const SubPatch& sp : c.multi) {
-    int rc = assemble_one(c, c.patches[sp.patch], sp.d_leq, sp.d_lgval, 1, itg, ng[0], precision);
+  for (const SubPatch& sp : subs) {
+    int rc = assemble_one(c, sv, c.patches[sp.patch], sp.d_leq, sp.d_lgval, 1, itg, ng, precision);
     if (rc) return rc;
     tot += c.last_assemble_ms;
     geo += c.last_geometry_ms;
@@ -1285,6 +1292,41 @@ int assemble_volume(Context& c, const ifem_gpu_integrand& itg, const int ng[3], 
   c.last_assemble_ms = tot;
   c.last_geometry_ms = geo;
   c.last_lines_ms = tot - geo;
+  return IFEM_GPU_OK;
+}
+
+int assemble_volume(Context& c, const ifem_gpu_integrand& itg, const int ng[3], int precision)
+{
+  if (ng[0] != ng[1] || ng[0] != ng[2])
+    return fail(IFEM_GPU_EUNSUPPORTED, "assemble: the GPU path needs equal Gauss counts per direction");
+  if (ng[0] < 1 || ng[0] > kMaxNG) return fail(IFEM_GPU_EUNSUPPORTED, "assemble: ng out of range");
+  const bool fp32 = precision == IFEM_GPU_FP32;
+  if (fp32 && !c.d_val32)
+    IFEM_HIP(hipMalloc(&c.d_val32, sizeof(float) * (c.nnz > 0 ? c.nnz : 1)));
+  SysView sv{c.d_rowptr, c.d_colind, c.d_val, c.d_val32, c.d_rhs, c.nnz, c.neq};
+  if (c.mpc) {
+    // constraint equations with masters: the extended system (slaves as
+    // unknowns) is assembled, then condensed to the free equations
+    MpcSystem& x = *c.mpc;
+    if (fp32 && !x.d_val32)
+      IFEM_HIP(hipMalloc(&x.d_val32, sizeof(float) * (x.nnz > 0 ? x.nnz : 1)));
+    SysView xv{x.d_rowptr, x.d_colind, x.d_val, x.d_val32, x.d_rhs, x.nnz, x.ne};
+    std::vector<SubPatch> subs(1, x.ext);
+    int rc = assemble_sum(c, xv, subs, itg, ng[0], precision);
+    subs[0].d_leq = nullptr; // owned by x.ext
+    subs[0].d_lgval = nullptr;
+    if (rc) return rc;
+    rc = mpc_condense(c, fp32, false);
+    if (rc) return rc;
+  } else if (c.multi.empty()) {
+    int rc = assemble_one(c, sv, c.patches[c.patch], c.d_eq, c.d_gval, 0, itg, ng[0], precision);
+    if (rc) return rc;
+  } else {
+    int rc = assemble_sum(c, sv, c.multi, itg, ng[0], precision);
+    if (rc) return rc;
+  }
+  if (fp32) c.fp32_ready = true;
+  else c.fp64_ready = true;
   return IFEM_GPU_OK;
 }
 
@@ -1427,6 +1469,14 @@ int assemble_traction(Context& c, int index, int face0, const double* t, int ng)
     return fail(IFEM_GPU_EINVAL, "traction: patch index out of range");
   const PatchRec& Pt = c.patches[c.multi.empty() ? c.patch : c.multi[index].patch];
   const int* eqmap = c.multi.empty() ? c.d_eq : c.multi[index].d_leq;
+  double* rhs = c.d_rhs;
+  if (c.mpc) { // into the extended right-hand side, condensed below
+    if (!c.fp64_ready)
+      return fail(IFEM_GPU_EINVAL,
+                  "assemble_neumann: constraints with masters need the FP64 system first");
+    eqmap = c.mpc->ext.d_leq;
+    rhs = c.mpc->d_rhs;
+  }
   GaussArgs ga{};
   gauss_rule(ng, ga.xi, ga.w);
   const int dn = face0 / 2;
@@ -1447,7 +1497,7 @@ int assemble_traction(Context& c, int index, int face0, const double* t, int ng)
       Pt.d_knots[1], Pt.d_knots[2], face0, ng, ga, c.d_facew);                                  \
   k_traction_sum<PP><<<(nfn + 127) / 128, 128, 0, c.stream>>>(                                  \
       Pt.n[0], Pt.n[1], Pt.n[2], Pt.nel[0], Pt.nel[1], Pt.nel[2], Pt.d_knots[0], Pt.d_knots[1], \
-      Pt.d_knots[2], face0, ng, ga, c.d_facew, t[0], t1, t2, c.ncomp, eqmap, c.d_rhs)
+      Pt.d_knots[2], face0, ng, ga, c.d_facew, t[0], t1, t2, c.ncomp, eqmap, rhs)
   switch (Pt.p[0]) {
   case 1: LAUNCH_TR(1); break;
   case 2: LAUNCH_TR(2); break;
@@ -1456,6 +1506,10 @@ int assemble_traction(Context& c, int index, int face0, const double* t, int ng)
   }
 #undef LAUNCH_TR
   IFEM_HIP(hipGetLastError());
+  if (c.mpc) {
+    int rc = mpc_condense(c, false, true);
+    if (rc) return rc;
+  }
   IFEM_HIP(hipStreamSynchronize(c.stream));
   return IFEM_GPU_OK;
 }
@@ -1743,7 +1797,8 @@ __device__ inline void eval_field_grad(const DevFunction& f, double x, double y,
 
 struct NormArgs {
   AsmArgs a;
-  const double* x;  // solution, equation order
+  const double* x;     // solution, equation order
+  const double* xdof;  // or the expanded solution per DOF (nullptr: use x)
   DevFunction exact;
   int has_exact;
   double* part;     // 6 x gridDim.x partial sums
@@ -1790,7 +1845,8 @@ __global__ __launch_bounds__(256) void k_norms(NormArgs na, int64_t total)
 #pragma unroll
           for (int c = 0; c < NC; c++) {
             const int eqn = a.eq[node * NC + c];
-            const double ub = eqn >= 0 ? na.x[eqn] : (a.gval ? a.gval[node * NC + c] : 0.0);
+            const double ub = na.xdof ? na.xdof[node * NC + c]
+                                      : (eqn >= 0 ? na.x[eqn] : (a.gval ? a.gval[node * NC + c] : 0.0));
             uh[c] += nv * ub;
             gr[c][0] += g0 * ub;
             gr[c][1] += g1 * ub;
@@ -1864,19 +1920,25 @@ __global__ __launch_bounds__(256) void k_norms(NormArgs na, int64_t total)
 
 // Squared norm integrals of one patch added to sq[6].
 static int norms_patch(Context& c, const PatchRec& Pt, const int* eqmap, const double* gval,
-                       const ifem_gpu_integrand& itg, const int ng[3],
+                       const double* xdof, const ifem_gpu_integrand& itg, const int ng[3],
                        const ifem_gpu_function* exact, double* sq);
 
 int integrate_norms(Context& c, const ifem_gpu_integrand& itg, const int ng[3],
                     const ifem_gpu_function* exact, double* out)
 {
   double sq[6] = {0, 0, 0, 0, 0, 0};
-  if (c.multi.empty()) {
-    int rc = norms_patch(c, c.patches[c.patch], c.d_eq, c.d_gval, itg, ng, exact, sq);
+  if (c.mpc) { // slaves with masters: integrate the expanded solution
+    int rc = mpc_expand_device(c);
+    if (!rc) rc = norms_patch(c, c.patches[c.patch], c.d_eq, c.d_gval, c.mpc->d_xdof, itg, ng,
+                              exact, sq);
+    if (rc) return rc;
+  } else if (c.multi.empty()) {
+    int rc = norms_patch(c, c.patches[c.patch], c.d_eq, c.d_gval, nullptr, itg, ng, exact, sq);
     if (rc) return rc;
   } else {
     for (const SubPatch& sp : c.multi) {
-      int rc = norms_patch(c, c.patches[sp.patch], sp.d_leq, sp.d_lgval, itg, ng, exact, sq);
+      int rc = norms_patch(c, c.patches[sp.patch], sp.d_leq, sp.d_lgval, nullptr, itg, ng, exact,
+                           sq);
       if (rc) return rc;
     }
   }
@@ -1885,7 +1947,7 @@ int integrate_norms(Context& c, const ifem_gpu_integrand& itg, const int ng[3],
 }
 
 static int norms_patch(Context& c, const PatchRec& Pt, const int* eqmap, const double* gval,
-                       const ifem_gpu_integrand& itg, const int ng[3],
+                       const double* xdof, const ifem_gpu_integrand& itg, const int ng[3],
                        const ifem_gpu_function* exact, double* sq)
 {
   const int p = Pt.p[0];
@@ -1933,6 +1995,7 @@ static int norms_patch(Context& c, const PatchRec& Pt, const int* eqmap, const d
     na.has_exact = 1;
   }
   na.x = c.w.x;
+  na.xdof = xdof;
   const int64_t total = (int64_t)Pt.nel[0] * Pt.nel[1] * Pt.nel[2] * g * g * g;
   const int grid = 1024;
   double* part = nullptr;

@@ -69,6 +69,46 @@ struct SubPatch {
   std::vector<int> h_leq;
 };
 
+// Target system of an assembly pass.
+struct SysView {
+  int64_t* rowptr;
+  int32_t* colind;
+  double* val;
+  float* val32;
+  double* rhs;
+  int64_t nnz;
+  int neq;
+};
+
+// Constraint equations with master DOFs: the slaves with masters become
+// extra unknowns ne = neq .. neq+ns-1 of an extended system that the line
+// kernels assemble; it is condensed to the free equations, K = T^T K_x T,
+// b = T^T (b_x - K_x t0) with u_x = T u + t0.
+struct MpcSystem {
+  int ne = 0, ns = 0;
+  int64_t nnz = 0;
+  int64_t* d_rowptr = nullptr;
+  int32_t* d_colind = nullptr;
+  double* d_val = nullptr;
+  float* d_val32 = nullptr;
+  double* d_rhs = nullptr;
+  SubPatch ext;                 // local DOF -> extended equation; prescribed values
+  // per reduced row: contributing extended rows and weights
+  int* d_cptr = nullptr;
+  int* d_cext = nullptr;
+  double* d_cw = nullptr;
+  // per slave k (extended equation neq+k): masters (reduced equations),
+  // coefficients and constant
+  int* d_sptr = nullptr;
+  int* d_smaster = nullptr;
+  double* d_scoef = nullptr;
+  double* d_sc0 = nullptr;
+  int* d_slave_of_dof = nullptr; // per DOF: k or -1
+  double* d_xdof = nullptr;      // expanded solution (norm integration)
+  std::vector<int> h_slave_of_dof, h_sptr, h_smaster;
+  std::vector<double> h_scoef, h_sc0;
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -83,6 +123,7 @@ struct Context {
   double* d_gval = nullptr;     // per DOF prescribed value (nullptr: all 0)
   std::vector<double> h_gval;
   std::vector<SubPatch> multi;  // non-empty: the system holds several patches
+  std::unique_ptr<MpcSystem> mpc; // constraint equations with masters
   int64_t nnz = 0;
   int64_t* d_rowptr = nullptr;
   int32_t* d_colind = nullptr;
@@ -139,6 +180,13 @@ int build_pattern(Context& c);
 int assemble_volume(Context& c, const ifem_gpu_integrand& itg, const int ng[3],
                     int precision);
 int assemble_traction(Context& c, int index, int face0, const double* t, int ng);
+// constraint equations with masters (mpc.hip)
+int mpc_bind(Context& c, int nceq, const int* mpmceq, const int* mmceq, const double* ttcc);
+int mpc_build_pattern(Context& c);
+int mpc_condense(Context& c, bool fp32, bool rhs_only);
+int mpc_expand_device(Context& c);
+void mpc_free(Context& c);
+void mpc_expand_host(const Context& c, const std::vector<double>& x, double* xdof);
 int autotune_assembly(Context& c, const ifem_gpu_integrand& itg, const int ng[3], int precision,
                       int* half);
 int integrate_norms(Context& c, const ifem_gpu_integrand& itg, const int ng[3],

@@ -963,6 +963,7 @@ int mg_setup(Context& c, int degree)
 {
   if (!c.fp64_ready) return fail(IFEM_GPU_EINVAL, "mg: no FP64 matrix assembled");
   if (!c.multi.empty()) return fail(IFEM_GPU_EUNSUPPORTED, "mg: one patch per system only");
+  if (c.mpc) return fail(IFEM_GPU_EUNSUPPORTED, "mg: constraints with master DOFs");
   if (degree < 1 || degree > 8) return fail(IFEM_GPU_EINVAL, "mg: smoothing degree must be 1..8");
   if (!c.mg || c.mg->patch != c.patch || c.mg->neq != c.neq || c.mg->nnz != c.nnz) {
     c.mg.reset(new Multigrid());

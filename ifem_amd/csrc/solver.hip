@@ -878,8 +878,9 @@ int set_learned(Context& c, int enable, int D, int H, int W, const float* aux, i
     c.learned = false;
     return IFEM_GPU_OK;
   }
-  if (!c.multi.empty())
-    return fail(IFEM_GPU_EUNSUPPORTED, "set_learned_preconditioner: one patch per system only");
+  if (!c.multi.empty() || c.mpc)
+    return fail(IFEM_GPU_EUNSUPPORTED,
+                "set_learned_preconditioner: one patch without master constraints only");
   if (!c.net) return fail(IFEM_GPU_EINVAL, "set_learned_preconditioner: create the network first");
   if (!c.pattern_ready) return fail(IFEM_GPU_EINVAL, "set_learned_preconditioner: no system");
   // box of the nodes with free DOFs; all components free inside, none outside

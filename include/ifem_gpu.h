@@ -75,12 +75,23 @@ int ifem_gpu_sam_bind(ifem_gpu_ctx* ctx, int patch_id, int ncomp, int ndof,
  * src/LinAlg/SAM.C:260-308): meqn[i] = -k marks DOF i as the slave of
  * constraint k (1-based); its entries are mpmceq[k-1] .. mpmceq[k]-1 (1-based)
  * of mmceq (DOF numbers, the slave first) and ttcc (the constant c0 first).
- * Only prescribed values are supported (constraints without master DOFs,
- * mpmceq[k]-mpmceq[k-1] == 1; else IFEM_GPU_EUNSUPPORTED): the assembly moves
- * the eliminated columns to the right-hand side, b_i -= K_ij c0_j, as
- * SparseMatrix::assemble does (src/LinAlg/SparseMatrix.C:614-688), and
- * ifem_gpu_expand_solution / ifem_gpu_integrate_norms put c0 at those DOFs
- * (SAM::expandSolution, src/LinAlg/SAM.C:768-810). */
+ * Constraints without masters (mpmceq[k]-mpmceq[k-1] == 1) are prescribed
+ * values: the assembly moves the eliminated columns to the right-hand side,
+ * b_i -= K_ij c0_j, as SparseMatrix::assemble does
+ * (src/LinAlg/SparseMatrix.C:614-688).  A constraint with masters,
+ * u_s = c0 + sum_m c_m u_m (periodicity, couplings), is eliminated as
+ * assemSparse/SAM::assembleRHS do (src/LinAlg/SparseMatrix.C:612-688,
+ * src/LinAlg/SAM.C:605-617): the GPU assembles an extended system with the
+ * slaves as extra unknowns and condenses it to K = T^T K_x T,
+ * b = T^T (b_x - K_x t0); the masters must be free DOFs (chains resolved, as
+ * SAMpatch::initConstraintEqs leaves them, else IFEM_GPU_EINVAL), and the
+ * pattern then holds the master-master couplings of SAM::getDofCouplings
+ * (src/LinAlg/SAM.C:377-427).  ifem_gpu_expand_solution and
+ * ifem_gpu_integrate_norms use u_s = c0 + sum c_m u_m at the slaves
+ * (SAM::expandSolution, src/LinAlg/SAM.C:768-810).  The multigrid and learned
+ * preconditioners refuse systems with master constraints
+ * (IFEM_GPU_EUNSUPPORTED); a traction needs the FP64 system assembled
+ * first. */
 int ifem_gpu_sam_bind_constraints(ifem_gpu_ctx* ctx, int patch_id, int ncomp, int ndof,
                                   const int* meqn, int neq, int nceq, const int* mpmceq,
                                   const int* mmceq, const double* ttcc);
@@ -91,7 +102,8 @@ int ifem_gpu_sam_bind_constraints(ifem_gpu_ctx* ctx, int patch_id, int ncomp, in
  * node i of patch patch_ids[k] (GoTools order) to its global node (1-based,
  * distinct within the patch); meqn holds ncomp equation numbers per global
  * node, as in ifem_gpu_sam_bind; nceq > 0 adds prescribed values as in
- * ifem_gpu_sam_bind_constraints.  ifem_gpu_init_assembly then builds the
+ * ifem_gpu_sam_bind_constraints (constraints with masters:
+ * IFEM_GPU_EUNSUPPORTED).  ifem_gpu_init_assembly then builds the
  * union pattern on the host, and ifem_gpu_assemble adds the patches in
  * order into the shared rows (the same integrand on every patch).  The
  * multigrid and learned preconditioners need a single patch. */

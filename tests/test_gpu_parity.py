@@ -320,12 +320,14 @@ def test_prescribed_uniaxial_elasticity_closed_form_mixed_precision():
     assert np.max(np.abs(x32 - ue)) <= 1e-10 * d
 
 
-def test_constraints_with_masters_are_rejected():
+def test_constraint_chains_are_rejected():
+    """A master that is itself a slave (a chain the SAM set-up resolves
+    first) is an error; negative equation numbers need the arrays."""
     from ifem_amd import GpuError, Session
     pb = problems.poisson_cube(3)
     sam = _prescribed_case(pb, seed=5)
-    # give the first constraint a master DOF: u_slave = c0 + 1.0 * u_master
-    master = int(np.nonzero(sam.meqn > 0)[0][0]) + 1
+    # the first constraint's master is the slave of the last one
+    master = int(sam.mmceq[sam.mpmceq[-2] - 1])
     bad = M.SamArrays(sam.meqn, sam.neq, np.concatenate([[1], sam.mpmceq[1:] + 1]).astype(np.int32),
                       np.insert(sam.mmceq, 1, master).astype(np.int32),
                       np.insert(sam.ttcc, 1, 1.0))

@@ -372,6 +372,19 @@ constexpr int kBlock = 256;
 // (double: FP64 path, float: FP32 path).  NC: components (1 Poisson,
 // 3 elasticity).  SPLIT: number of blocks per line sharing the row
 // components (NC/SPLIT rows each).  HALF: half-size element chunks.
+// Equation number of neighbour DOF t (node offset t / NC in the (2p+1)^3
+// window around node (i, jc, kc), component t % NC); -2 outside the patch.
+template <int P, int NC>
+__device__ __forceinline__ int window_eq(ConstArgs& a, int i, int jc, int kc, int t)
+{
+  constexpr int W1 = 2 * P + 1;
+  const int nbr = t / NC, cc = t % NC;
+  const int ii = i + nbr % W1 - P, jj = jc + (nbr / W1) % W1 - P, kk = kc + nbr / (W1 * W1) - P;
+  if (t >= W1 * W1 * W1 * NC || ii < 0 || ii >= a.n1 || jj < 0 || jj >= a.n2 || kk < 0 || kk >= a.n3)
+    return -2;
+  return a.eq[((int64_t)ii + (int64_t)a.n1 * (jj + (int64_t)a.n2 * kk)) * NC + cc];
+}
+
 template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF>
 struct LineLds {
   static constexpr int P1 = P + 1;
@@ -401,13 +414,17 @@ struct LineLds {
   alignas(16) T tD3[P1][NG][P1];
   alignas(16) int be2[NB];
   alignas(16) int be3[NB];
+  static constexpr int NT = (WN * NC + 63) / 64;
+  alignas(16) int eqw[NT * 64];
+  alignas(16) int64_t pbase[NCR];
+  int prow[NCR];
   int nbundle;
 };
 
 // The LDS arrays come in as separate __restrict__ pointers so that, after
 // inlining, the compiler knows they do not alias (fields of one struct with
 // run-time indices would otherwise be assumed to overlap).
-template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF>
+template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF, bool PIPE>
 __device__ __forceinline__ void line_impl(
     ConstArgs& a, const int line, const int split,
     T (*__restrict__ win)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NCR]
@@ -420,7 +437,9 @@ __device__ __forceinline__ void line_impl(
     T (*__restrict__ tN1)[P + 1], T (*__restrict__ tD1)[P + 1],
     T (*__restrict__ tN2)[NG][P + 1], T (*__restrict__ tD2)[NG][P + 1],
     T (*__restrict__ tN3)[NG][P + 1], T (*__restrict__ tD3)[NG][P + 1],
-    int* __restrict__ be2, int* __restrict__ be3, int* __restrict__ nbundlep)
+    int* __restrict__ be2, int* __restrict__ be3,
+    int* __restrict__ eqw,
+    int64_t* __restrict__ pbase, int* __restrict__ prow, int* __restrict__ nbundlep)
 {
   using LL = LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>;
   constexpr int P1 = LL::P1;
@@ -477,13 +496,24 @@ __device__ __forceinline__ void line_impl(
 
   const T lam = (T)a.lam, mu = (T)a.mu;
 
+  constexpr int NT = LL::NT; // phase D: 64-column steps per row
   for (int e1 = 0; e1 < a.nel1; e1++) {
     // ---- direction-1 tables of this layer ----------------------------------
+    // with the equation numbers of row e1 and of its neighbour columns for
+    // phase D (their loads overlap those of the tables)
     for (int t = tid; t < NG * P1; t += kBlock) {
       (&tN1[0][0])[t] = (T)a.tabN1[e1 * NG * P1 + t];
       (&tD1[0][0])[t] = (T)a.tabD1[e1 * NG * P1 + t];
     }
+    for (int t = tid; t < NT * 64; t += kBlock) eqw[t] = window_eq<P, NC>(a, e1, jc, kc, t);
+    if (tid < NCR) {
+      const int64_t node = (int64_t)e1 + (int64_t)a.n1 * (jc + (int64_t)a.n2 * kc);
+      prow[tid] = a.eq[node * NC + split * NCR + tid];
+    }
     __syncthreads();
+    // row start of row e1, read while phase A runs
+    int64_t pf_base = 0;
+    if (tid < NCR && prow[tid] >= 0) pf_base = a.rowptr[prow[tid]];
 
     for (int c0 = 0; c0 < nb; c0 += CH) {
       const int nch = (nb - c0) < CH ? (nb - c0) : CH;
@@ -536,6 +566,7 @@ __device__ __forceinline__ void line_impl(
           }
         }
       }
+      if (c0 == 0 && tid < NCR) pbase[tid] = pf_base;
       __syncthreads();
 
       // ---- RHS: fixed-order reduction over (element, GP) -------------------
@@ -553,6 +584,8 @@ __device__ __forceinline__ void line_impl(
       }
 
       // ---- phase B: element-row x column-node products ----------------------
+      if constexpr (!PIPE) {
+      // one thread per (element, column node b), Gauss points in fixed order
       for (int t = tid; t < nch * NEN; t += kBlock) {
         const int le = t / NEN, b = t % NEN;
         const int b1 = b % P1, b2 = (b / P1) % P1, b3 = b / (P1 * P1);
@@ -608,6 +641,80 @@ __device__ __forceinline__ void line_impl(
 #pragma unroll
           for (int s = 0; s < NS; s++) stage[le][b][a1][s] = acc[a1][s];
       }
+      } else {
+      // One thread per (element, column node b); the Gauss points are fully
+      // unrolled with the next point's geometry row loaded into registers
+      // while the current one is used, and the column node's 1D basis
+      // values held in registers.
+      for (int t = tid; t < nch * NEN; t += kBlock) {
+        const int le = t / NEN, b = t % NEN;
+        const int b1 = b % P1, b2 = (b / P1) % P1, b3 = b / (P1 * P1);
+        const int e2 = be2[c0 + le], e3 = be3[c0 + le];
+        const int m2 = e2 - jc + P, m3 = e3 - kc + P;
+        T vn1[NG], vd1[NG], vn2[NG], vd2[NG], vn3[NG], vd3[NG];
+#pragma unroll
+        for (int q = 0; q < NG; q++) {
+          vn1[q] = tN1[q][b1]; vd1[q] = tD1[q][b1];
+          vn2[q] = tN2[m2][q][b2]; vd2[q] = tD2[m2][q][b2];
+          vn3[q] = tN3[m3][q][b3]; vd3[q] = tD3[m3][q][b3];
+        }
+        T acc[P1][NS];
+#pragma unroll
+        for (int a1 = 0; a1 < P1; a1++)
+#pragma unroll
+          for (int s = 0; s < NS; s++) acc[a1][s] = 0;
+        const T* __restrict__ Gbase = geo[le * GP3];
+        T Gc[LL::GEO];
+#pragma unroll
+        for (int m = 0; m < LL::GEO; m++) Gc[m] = Gbase[m];
+#pragma unroll
+        for (int q = 0; q < GP3; q++) {
+          T Gn[LL::GEO];
+          if (q + 1 < GP3) {
+#pragma unroll
+            for (int m = 0; m < LL::GEO; m++) Gn[m] = Gbase[(q + 1) * LL::GEO + m];
+          }
+          const int q1 = q % NG, q2 = (q / NG) % NG, q3 = q / (NG * NG);
+          const T n23 = vn2[q2] * vn3[q3], d23 = vd2[q2] * vn3[q3], n2d3 = vn2[q2] * vd3[q3];
+          const T r0 = vd1[q1] * n23, r1 = vn1[q1] * d23, r2 = vn1[q1] * n2d3;
+          const T* G = Gc;
+          if (NC == 1) {
+#pragma unroll
+            for (int a1 = 0; a1 < P1; a1++)
+              acc[a1][0] += r0 * G[3 * a1] + r1 * G[3 * a1 + 1] + r2 * G[3 * a1 + 2];
+          } else {
+            const T gb0 = r0 * G[0] + r1 * G[3] + r2 * G[6];
+            const T gb1 = r0 * G[1] + r1 * G[4] + r2 * G[7];
+            const T gb2 = r0 * G[2] + r1 * G[5] + r2 * G[8];
+#pragma unroll
+            for (int a1 = 0; a1 < P1; a1++) {
+              const T wa0 = G[9 + 3 * a1], wa1 = G[10 + 3 * a1], wa2 = G[11 + 3 * a1];
+              if (SACC) {
+                acc[a1][0] += wa0 * gb0; acc[a1][1] += wa0 * gb1; acc[a1][2] += wa0 * gb2;
+                acc[a1][3] += wa1 * gb0; acc[a1][4] += wa1 * gb1; acc[a1][5] += wa1 * gb2;
+                acc[a1][6] += wa2 * gb0; acc[a1][7] += wa2 * gb1; acc[a1][8] += wa2 * gb2;
+              } else {
+                const T wa[3] = {wa0, wa1, wa2}, gb[3] = {gb0, gb1, gb2};
+                const int ic = split * NCR;
+                const T dot = wa0 * gb0 + wa1 * gb1 + wa2 * gb2;
+#pragma unroll
+                for (int jj = 0; jj < 3; jj++)
+                  acc[a1][jj] += lam * wa[ic] * gb[jj] + mu * wa[jj] * gb[ic] +
+                                 (jj == ic ? mu * dot : T(0));
+              }
+            }
+          }
+          if (q + 1 < GP3) {
+#pragma unroll
+            for (int m = 0; m < LL::GEO; m++) Gc[m] = Gn[m];
+          }
+        }
+#pragma unroll
+        for (int a1 = 0; a1 < P1; a1++)
+#pragma unroll
+          for (int s = 0; s < NS; s++) stage[le][b][a1][s] = acc[a1][s];
+      }
+      }
       __syncthreads();
 
       // ---- phase C: ordered reduction of the chunk into the row window ------
@@ -657,42 +764,31 @@ __device__ __forceinline__ void line_impl(
       const int wave = tid >> 6, lane = tid & 63;
       if (wave < NCR) {
         const int ic = split * NCR + wave;
-        const int req = a.eq[node * NC + ic];
+        const bool pf = i == e1;
+        const int req = pf ? prow[wave] : a.eq[node * NC + ic];
         if (req >= 0) {
-          const int64_t base = a.rowptr[req];
+          const int64_t base = pf ? pbase[wave] : a.rowptr[req];
           const double* __restrict__ gv = a.gval;
           double corr = 0.0;
           int64_t pos = 0;
-          for (int t0 = 0; t0 < WN * NC; t0 += 64) {
-            const int t = t0 + lane;
-            bool valid = false;
-            int64_t gdof = -1; // in-bounds eliminated column
-            if (t < WN * NC) {
-              const int nbr = t / NC, cc = t % NC;
-              const int di = nbr % W1 - P, dj = (nbr / W1) % W1 - P, dk = nbr / (W1 * W1) - P;
-              const int ii = i + di, jj = jc + dj, kk = kc + dk;
-              if (ii >= 0 && ii < a.n1 && jj >= 0 && jj < a.n2 && kk >= 0 && kk < a.n3) {
-                const int64_t nn = (int64_t)ii + (int64_t)a.n1 * (jj + (int64_t)a.n2 * kk);
-                valid = a.eq[nn * NC + cc] >= 0;
-                if (!valid) gdof = nn * NC + cc;
-              }
-            }
+#pragma unroll 1
+          for (int k = 0; k < NT; k++) {
+            const int t = k * 64 + lane;
+            const int ev = pf ? eqw[t] : window_eq<P, NC>(a, i, jc, kc, t);
+            const bool valid = ev >= 0;
             const unsigned long long m = __ballot(valid);
             if (valid && a.accum) {
               const T v = win[slot][wave][t / NC][t % NC];
-              const int nbr = t / NC;
-              const int ii = i + nbr % W1 - P, jj = jc + (nbr / W1) % W1 - P, kk = kc + nbr / (W1 * W1) - P;
-              const int col = a.eq[((int64_t)ii + (int64_t)a.n1 * (jj + (int64_t)a.n2 * kk)) * NC + t % NC];
               int64_t lo = base, hi = a.rowptr[req + 1] - 1;
               while (lo <= hi) {
                 const int64_t mid = (lo + hi) >> 1;
                 const int cm = a.colind[mid];
-                if (cm == col) {
+                if (cm == ev) {
                   if (a.val32) a.val32[mid] += (float)v;
                   else a.val64[mid] += (double)v;
                   break;
                 }
-                if (cm < col) lo = mid + 1;
+                if (cm < ev) lo = mid + 1;
                 else hi = mid - 1;
               }
             } else if (valid) {
@@ -700,7 +796,10 @@ __device__ __forceinline__ void line_impl(
               const T v = win[slot][wave][t / NC][t % NC];
               if (a.val32) a.val32[at] = (float)v;
               else a.val64[at] = (double)v;
-            } else if (gv && gdof >= 0) {
+            } else if (gv && ev == -1) { // eliminated column inside the patch
+              const int nbr = t / NC;
+              const int ii = i + nbr % W1 - P, jj = jc + (nbr / W1) % W1 - P, kk = kc + nbr / (W1 * W1) - P;
+              const int64_t gdof = ((int64_t)ii + (int64_t)a.n1 * (jj + (int64_t)a.n2 * kk)) * NC + t % NC;
               corr += (double)win[slot][wave][t / NC][t % NC] * gv[gdof];
             }
             pos += __popcll(m);
@@ -727,9 +826,9 @@ template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF>
 __device__ __forceinline__ void line_body(ConstArgs& a, const int line, const int split,
                                           LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>& L)
 {
-  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF>(a, line, split, L.win, L.geo, L.stage, L.rstage,
-                                                L.rwin, L.tN1, L.tD1, L.tN2, L.tD2, L.tN3, L.tD3,
-                                                L.be2, L.be3, &L.nbundle);
+  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF, false>(
+      a, line, split, L.win, L.geo, L.stage, L.rstage, L.rwin, L.tN1, L.tD1, L.tN2, L.tD2, L.tN3,
+      L.tD3, L.be2, L.be3, L.eqw, L.pbase, L.prow, &L.nbundle);
 }
 
 // Single patch: the LDS arrays are separate __shared__ variables.
@@ -748,10 +847,15 @@ __global__ __launch_bounds__(kBlock) void k_assemble_lines(const AsmArgs* __rest
   __shared__ T tN2[P1][NG][P1], tD2[P1][NG][P1];
   __shared__ T tN3[P1][NG][P1], tD3[P1][NG][P1];
   __shared__ int be2[LL::NB], be3[LL::NB];
+  __shared__ int eqw[LL::NT * 64];
+  __shared__ int64_t pbase[LL::NCR];
+  __shared__ int prow[LL::NCR];
   __shared__ int nbundle;
-  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF>(a, blockIdx.x, blockIdx.y, win, geo, stage, rstage,
-                                             rwin, tN1, tD1, tN2, tD2, tN3, tD3, be2, be3,
-                                             &nbundle);
+  // register-pipelined phase B where the unrolled Gauss loop stays small
+  constexpr bool PIPE = LL::GP3 <= 27;
+  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF, PIPE>(a, blockIdx.x, blockIdx.y, win, geo, stage,
+                                                      rstage, rwin, tN1, tD1, tN2, tD2, tN3, tD3,
+                                                      be2, be3, eqw, pbase, prow, &nbundle);
 }
 
 // Persistent batch assembly over many independent patches (degree p = 1..3,

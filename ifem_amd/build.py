@@ -17,6 +17,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "ifem_amd")
 LIB = os.path.join(PKG, "libifem_gpu.so")
 FACADE_LIB = os.path.join(PKG, "libifem_facade.so")
+HDF5_LIB = os.path.join(PKG, "libifem_hdf5.so")
+# the image's HDF5 (C library and headers)
+HDF5_ROOT = os.environ.get("IFEM_HDF5_ROOT", "/opt/conda")
 FACADE_TEST = os.path.join(PKG, "facade_test")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -45,6 +48,24 @@ def build_library() -> str:
     return LIB
 
 
+def build_hdf5() -> str:
+    """libifem_hdf5.so (include/ifem_hdf5.h): plain C over the HDF5 C
+    library, found through its own run path so that nothing else is taken
+    from that prefix."""
+    inc = os.path.join(HDF5_ROOT, "include")
+    lib = os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        sys.stderr.write(f"HDF5 headers not found under {inc}: export library not built\n")
+        return ""
+    tmp = HDF5_LIB + ".tmp"
+    _run(["gcc", "-O2", "-std=c11", "-Wall", "-fPIC", "-shared",
+          "-I" + os.path.join(ROOT, "include"), "-I" + inc,
+          os.path.join(PKG, "facade", "hdf5_writer.c"), "-o", tmp,
+          "-L" + lib, "-l:libhdf5.so.103", "-Wl,-rpath," + lib, "-Wl,--enable-new-dtags"])
+    os.replace(tmp, HDF5_LIB)
+    return HDF5_LIB
+
+
 def build_facade() -> str:
     fdir = os.path.join(PKG, "facade")
     srcs = sorted(glob.glob(os.path.join(fdir, "*.C")))
@@ -53,13 +74,13 @@ def build_facade() -> str:
     inc = ["-I" + os.path.join(ROOT, "include"), "-I" + fdir]
     tmp = FACADE_LIB + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", *inc, "-o", tmp, *srcs,
-          "-L" + PKG, "-lifem_gpu", "-Wl,-rpath,$ORIGIN"])
+          "-L" + PKG, "-lifem_gpu", "-lifem_hdf5", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, FACADE_LIB)
     test_src = os.path.join(fdir, "test", "facade_test.cpp")
     if os.path.exists(test_src):
         tmp = FACADE_TEST + ".tmp"
         _run(["g++", "-O2", "-std=c++17", *inc, "-o", tmp, test_src,
-              "-L" + PKG, "-lifem_facade", "-lifem_gpu", "-Wl,-rpath,$ORIGIN"])
+              "-L" + PKG, "-lifem_facade", "-lifem_gpu", "-lifem_hdf5", "-Wl,-rpath,$ORIGIN"])
         os.replace(tmp, FACADE_TEST)
     return FACADE_LIB
 
@@ -70,6 +91,7 @@ def build_oracle() -> None:
 
 if __name__ == "__main__":
     build_library()
+    build_hdf5()
     build_facade()
     build_oracle()
     print("built", LIB)

@@ -1,5 +1,6 @@
 // IFEMGpu.C -- reference-interface classes on the libifem_gpu C ABI.
 #include "IFEMGpu.h"
+#include "ifem_hdf5.h"
 
 #include <algorithm>
 #include <cmath>
@@ -323,6 +324,35 @@ ASMs3D::ASMs3D(const int ord[3], const int ncoef[3], const std::vector<double> k
     n[d] = ncoef[d];
     knots[d] = kn[d];
   }
+}
+
+bool ASMs3D::write(std::ostream& os, int basis) const
+{
+  if (basis != 1) return false; // one basis per patch here
+  const double* kn[3] = {knots[0].data(), knots[1].data(), knots[2].data()};
+  size_t len = 0;
+  if (ifem_g2_format(order, n, kn, coefs.data(), nullptr, 0, &len)) return false;
+  std::string text(len, '\0');
+  if (ifem_g2_format(order, n, kn, coefs.data(), &text[0], len, &len)) return false;
+  os << text;
+  return os.good();
+}
+
+std::string Integrand::getField1Name(size_t idx, const char* prefix) const
+{
+  std::string name = "primary solution " + std::to_string(1 + idx);
+  return prefix ? prefix + std::string(" ") + name : name;
+}
+
+std::string PoissonIntegrand::getField1Name(size_t, const char* prefix) const
+{
+  return prefix ? prefix + std::string(" u") : std::string("u");
+}
+
+std::string ElasticityIntegrand::getField1Name(size_t idx, const char* prefix) const
+{
+  std::string name = idx < 3 ? std::string("u_") + char('x' + idx) : std::string("u");
+  return prefix ? prefix + std::string(" ") + name : name;
 }
 
 std::unique_ptr<ASMs3D> ASMs3D::read(std::istream& is)

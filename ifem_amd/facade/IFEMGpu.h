@@ -24,6 +24,7 @@
 
 #include <istream>
 #include <map>
+#include <ostream>
 #include <memory>
 #include <string>
 #include <vector>
@@ -99,6 +100,10 @@ public:
   virtual bool gpuDescription(ifem_gpu_integrand& d) const = 0;
   //! Neumann traction of face lIndex (false if none).
   virtual bool gpuTraction(int lIndex, double t[3]) const { (void)lIndex; (void)t; return false; }
+  //! Name of primary solution field idx ("primary solution <idx+1>",
+  //! IntegrandBase::getField1Name, src/ASM/IntegrandBase.C:398-403); the
+  //! result export names the whole primary solution with idx = 11.
+  virtual std::string getField1Name(size_t idx, const char* prefix = nullptr) const;
 };
 
 //! -div(kappa grad u) = f (Weak::Laplacian/LaplacianCoeff + Weak::Source,
@@ -108,6 +113,8 @@ public:
   PoissonIntegrand() { kappa = {}; source = {}; }
   int getNoFields() const override { return 1; }
   bool gpuDescription(ifem_gpu_integrand& d) const override;
+  //! "u" (the scalar solution).
+  std::string getField1Name(size_t idx, const char* prefix = nullptr) const override;
   ifem_gpu_function kappa;   //!< conductivity (ZERO: 1)
   ifem_gpu_function source;  //!< source f
   std::vector<double> kappaModes, sourceModes; //!< storage for mode tables
@@ -120,6 +127,8 @@ public:
   int getNoFields() const override { return 3; }
   bool gpuDescription(ifem_gpu_integrand& d) const override;
   bool gpuTraction(int lIndex, double t[3]) const override;
+  //! "u_x", "u_y", "u_z" for idx 0..2, "u" for the displacement vector.
+  std::string getField1Name(size_t idx, const char* prefix = nullptr) const override;
   void setTraction(int lIndex, double tx, double ty, double tz);
   double E, nu;
   double body[3] = {0.0, 0.0, 0.0};
@@ -273,6 +282,9 @@ public:
   //! Reads one non-rational SplineVolume (GoTools .g2, class 700) as
   //! ASMs3D::read does (src/ASM/ASMs3D.C:117-160); nullptr on failure.
   static std::unique_ptr<ASMs3D> read(std::istream& is);
+  //! Writes the patch as GoTools text (ASMs3D::write, src/ASM/ASMs3D.C:163-173):
+  //! "700 1 0 0", the spline volume, numbers with 15 significant digits.
+  bool write(std::ostream& os, int basis = 1) const;
   //! Inserts nInsert equally spaced knots in every knot span of direction
   //! dir = 0..2 (ASM3D::uniformRefine, src/SIM/SIM3D.C:104-128), by knot
   //! insertion, which leaves the geometry unchanged.

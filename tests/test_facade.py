@@ -4,6 +4,7 @@ AlgEqSystem, GPUMatrix::solve and SAM::expandSolution; its results are
 checked here against the oracle."""
 import json
 import os
+import shutil
 import subprocess
 
 import numpy as np
@@ -21,9 +22,11 @@ def test_facade_built():
 
 
 @pytest.mark.gpu
-def test_facade_driver_matches_oracle():
+def test_facade_driver_matches_oracle(tmp_path):
     print("[gpu] facade driver", flush=True)
-    r = subprocess.run([EXE], capture_output=True, text=True, timeout=300)
+    h5name = str(tmp_path / "facade_poisson")
+    env = dict(os.environ, IFEM_FACADE_HDF5=h5name)
+    r = subprocess.run([EXE], capture_output=True, text=True, timeout=300, env=env)
     assert r.returncode == 0, r.stdout + r.stderr
     out = {}
     for line in r.stdout.splitlines():
@@ -41,6 +44,18 @@ def test_facade_driver_matches_oracle():
     rms = np.sqrt(np.mean(xd ** 2))  # SAM::normL2 is an RMS
     assert abs(pz["normL2"] - rms) <= 1e-9 * rms
     assert pz["max_nodal_err"] < 0.02
+    # result export through the facade's HDF5Writer: the expanded solution
+    # of the one patch, and its basis
+    assert pz["hdf5"] == 1
+    h5dump = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
+    if os.path.exists(h5dump):
+        raw = str(tmp_path / "u.bin")
+        subprocess.run([h5dump, "-d", "/0/Poisson-1/fields/u/1", "-b", "LE", "-o", raw,
+                        h5name + ".hdf5"], capture_output=True, check=True)
+        u = np.fromfile(raw, dtype="<f8")
+        assert len(u) == pb.ndof
+        assert abs(np.sqrt(np.mean(u ** 2)) - pz["normL2"]) <= 1e-13 * pz["normL2"]
+        assert np.max(np.abs(u - xd)) <= 1e-8 * np.max(np.abs(xd))
     e64, e32 = out[("elasticity", "fp64")], out[("elasticity", "fp32")]
     assert abs(e64["force_z"] + 1.0) < 1e-12
     assert e64["rel_residual"] <= 1e-8 and e32["rel_residual"] <= 1e-8

@@ -424,7 +424,7 @@ struct LineLds {
 // The LDS arrays come in as separate __restrict__ pointers so that, after
 // inlining, the compiler knows they do not alias (fields of one struct with
 // run-time indices would otherwise be assumed to overlap).
-template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF, bool PIPE>
+template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF, bool PIPE, bool ORD>
 __device__ __forceinline__ void line_impl(
     ConstArgs& a, const int line, const int split,
     T (*__restrict__ win)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NCR]
@@ -432,7 +432,7 @@ __device__ __forceinline__ void line_impl(
     T (*__restrict__ geo)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::GEO],
     T (*__restrict__ stage)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NEN][P + 1]
                            [LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NS],
-    T (*__restrict__ rstage)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NRHS],
+    T* __restrict__ rstage,
     T (*__restrict__ rwin)[NC],
     T (*__restrict__ tN1)[P + 1], T (*__restrict__ tD1)[P + 1],
     T (*__restrict__ tN2)[NG][P + 1], T (*__restrict__ tD2)[NG][P + 1],
@@ -451,6 +451,11 @@ __device__ __forceinline__ void line_impl(
   constexpr bool SACC = LL::SACC;
   constexpr int NS = LL::NS;
   constexpr int CH = LL::CH;
+  // ORD: the whole bundle in one chunk, column sums added to the window
+  // element by element (no stage); the RHS stage then holds w*detJ*N_a and
+  // the body force is applied after the sum
+  static_assert(!ORD || (CH == LL::NB && PIPE), "ordered updates need one chunk per layer");
+  constexpr int RS = ORD ? (HAS_RHS ? P1 : 1) : LL::NRHS; // RHS stage row length
 
   const int tid = threadIdx.x;
   const int jc = line % a.n2, kc = line / a.n2;
@@ -558,10 +563,12 @@ __device__ __forceinline__ void line_impl(
           if (HAS_RHS) {
             const T Na = n1 * n2a * n3a;
             if (NC == 1) {
-              rstage[t][a1] = R[6] * Na;
+              rstage[t * RS + a1] = R[6] * Na;
+            } else if (ORD) {
+              rstage[t * RS + a1] = R[9] * Na;
             } else {
 #pragma unroll
-              for (int c = 0; c < NC; c++) rstage[t][a1 * NC + c] = R[9] * (T)a.body[c] * Na;
+              for (int c = 0; c < NC; c++) rstage[t * RS + a1 * NC + c] = R[9] * (T)a.body[c] * Na;
             }
           }
         }
@@ -573,18 +580,118 @@ __device__ __forceinline__ void line_impl(
       if (HAS_RHS) {
         const int v = tid >> 4, l = tid & 15;
         T s = T(0);
+        const int vs = (ORD && NC == 3) ? v / NC : v;
         if (v < P1 * NC)
-          for (int pr = l; pr < nch * GP3; pr += 16) s += rstage[pr][v];
+          for (int pr = l; pr < nch * GP3; pr += 16) s += rstage[pr * RS + vs];
 #pragma unroll
         for (int off = 8; off >= 1; off >>= 1) s += __shfl_xor(s, off, 16);
         if (v < P1 * NC && l == 0) {
           const int a1 = v / NC, c = v % NC;
-          rwin[(e1 + a1) % P1][c] += s;
+          rwin[(e1 + a1) % P1][c] += (ORD && NC == 3) ? s * (T)a.body[c] : s;
         }
       }
 
       // ---- phase B: element-row x column-node products ----------------------
-      if constexpr (!PIPE) {
+      if constexpr (ORD) {
+      // One (element, column node b) item per thread for the whole bundle,
+      // Gauss loop as in the pipelined variant below; the sums stay in
+      // registers and go into the row window one element after the other
+      // (element order fixed, so every window entry is summed in that order).
+      const bool has = tid < nch * NEN;
+      const int le = has ? tid / NEN : 0, b = has ? tid % NEN : 0;
+      const int b1 = b % P1, b2 = (b / P1) % P1, b3 = b / (P1 * P1);
+      const int e2 = be2[c0 + le], e3 = be3[c0 + le];
+      T acc[P1][NS];
+#pragma unroll
+      for (int a1 = 0; a1 < P1; a1++)
+#pragma unroll
+        for (int s = 0; s < NS; s++) acc[a1][s] = 0;
+      if (has) {
+        const int m2 = e2 - jc + P, m3 = e3 - kc + P;
+        T vn1[NG], vd1[NG], vn2[NG], vd2[NG], vn3[NG], vd3[NG];
+#pragma unroll
+        for (int q = 0; q < NG; q++) {
+          vn1[q] = tN1[q][b1]; vd1[q] = tD1[q][b1];
+          vn2[q] = tN2[m2][q][b2]; vd2[q] = tD2[m2][q][b2];
+          vn3[q] = tN3[m3][q][b3]; vd3[q] = tD3[m3][q][b3];
+        }
+        const T* __restrict__ Gbase = geo[le * GP3];
+        T Gc[LL::GEO];
+#pragma unroll
+        for (int m = 0; m < LL::GEO; m++) Gc[m] = Gbase[m];
+#pragma unroll
+        for (int q = 0; q < GP3; q++) {
+          T Gn[LL::GEO];
+          if (q + 1 < GP3) {
+#pragma unroll
+            for (int m = 0; m < LL::GEO; m++) Gn[m] = Gbase[(q + 1) * LL::GEO + m];
+          }
+          const int q1 = q % NG, q2 = (q / NG) % NG, q3 = q / (NG * NG);
+          const T n23 = vn2[q2] * vn3[q3], d23 = vd2[q2] * vn3[q3], n2d3 = vn2[q2] * vd3[q3];
+          const T r0 = vd1[q1] * n23, r1 = vn1[q1] * d23, r2 = vn1[q1] * n2d3;
+          const T* G = Gc;
+          if (NC == 1) {
+#pragma unroll
+            for (int a1 = 0; a1 < P1; a1++)
+              acc[a1][0] += r0 * G[3 * a1] + r1 * G[3 * a1 + 1] + r2 * G[3 * a1 + 2];
+          } else {
+            const T gb0 = r0 * G[0] + r1 * G[3] + r2 * G[6];
+            const T gb1 = r0 * G[1] + r1 * G[4] + r2 * G[7];
+            const T gb2 = r0 * G[2] + r1 * G[5] + r2 * G[8];
+#pragma unroll
+            for (int a1 = 0; a1 < P1; a1++) {
+              const T wa0 = G[9 + 3 * a1], wa1 = G[10 + 3 * a1], wa2 = G[11 + 3 * a1];
+              if (SACC) {
+                acc[a1][0] += wa0 * gb0; acc[a1][1] += wa0 * gb1; acc[a1][2] += wa0 * gb2;
+                acc[a1][3] += wa1 * gb0; acc[a1][4] += wa1 * gb1; acc[a1][5] += wa1 * gb2;
+                acc[a1][6] += wa2 * gb0; acc[a1][7] += wa2 * gb1; acc[a1][8] += wa2 * gb2;
+              } else {
+                const T wa[3] = {wa0, wa1, wa2}, gb[3] = {gb0, gb1, gb2};
+                const int ic = split * NCR;
+                const T dot = wa0 * gb0 + wa1 * gb1 + wa2 * gb2;
+#pragma unroll
+                for (int jj = 0; jj < 3; jj++)
+                  acc[a1][jj] += lam * wa[ic] * gb[jj] + mu * wa[jj] * gb[ic] +
+                                 (jj == ic ? mu * dot : T(0));
+              }
+            }
+          }
+          if (q + 1 < GP3) {
+#pragma unroll
+            for (int m = 0; m < LL::GEO; m++) Gc[m] = Gn[m];
+          }
+        }
+      }
+      // rounds over the bundle elements: in round r the items of element r
+      // add their column sums (distinct window entries within a round)
+      const int cj = e2 + b2, ck = e3 + b3;
+      for (int r = 0; r < nch; r++) {
+        __syncthreads();
+        if (has && le == r) {
+#pragma unroll
+          for (int a1 = 0; a1 < P1; a1++) {
+            const int slot = (e1 + a1) % P1;
+            const int di = b1 - a1, dj = cj - jc, dk = ck - kc;
+            const int nbr = ((dk + P) * W1 + (dj + P)) * W1 + (di + P);
+            const T* sv = acc[a1];
+            if (NC == 1) {
+              win[slot][0][nbr][0] += sv[0];
+            } else if (SACC) {
+              const T tr = sv[0] + sv[4] + sv[8];
+#pragma unroll
+              for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                  win[slot][i][nbr][j] += lam * sv[3 * i + j] + mu * sv[3 * j + i] +
+                                          (i == j ? mu * tr : T(0));
+            } else {
+#pragma unroll
+              for (int j = 0; j < 3; j++) win[slot][0][nbr][j] += sv[j];
+            }
+          }
+        }
+      }
+      } else if constexpr (!PIPE) {
       // one thread per (element, column node b), Gauss points in fixed order
       for (int t = tid; t < nch * NEN; t += kBlock) {
         const int le = t / NEN, b = t % NEN;
@@ -718,6 +825,7 @@ __device__ __forceinline__ void line_impl(
       __syncthreads();
 
       // ---- phase C: ordered reduction of the chunk into the row window ------
+      if (!ORD)
       for (int t = tid; t < P1 * P1 * W1 * W1; t += kBlock) {
         const int a1 = t % P1, r = t / P1;
         const int b1 = r % P1, r2 = r / P1;
@@ -826,8 +934,8 @@ template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF>
 __device__ __forceinline__ void line_body(ConstArgs& a, const int line, const int split,
                                           LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>& L)
 {
-  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF, false>(
-      a, line, split, L.win, L.geo, L.stage, L.rstage, L.rwin, L.tN1, L.tD1, L.tN2, L.tD2, L.tN3,
+  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF, false, false>(
+      a, line, split, L.win, L.geo, L.stage, &L.rstage[0][0], L.rwin, L.tN1, L.tD1, L.tN2, L.tD2, L.tN3,
       L.tD3, L.be2, L.be3, L.eqw, L.pbase, L.prow, &L.nbundle);
 }
 
@@ -838,10 +946,14 @@ __global__ __launch_bounds__(kBlock) void k_assemble_lines(const AsmArgs* __rest
   ConstArgs& a = *const_args(args);
   using LL = LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>;
   constexpr int P1 = P + 1;
+  // register-pipelined phase B where the unrolled Gauss loop stays small;
+  // ordered window updates (no stage) where the whole bundle is one chunk
+  constexpr bool PIPE = LL::GP3 <= 27;
+  constexpr bool ORD = PIPE && HALF == 0 && LL::CH == LL::NB && LL::NB * LL::NEN <= kBlock;
   __shared__ T win[P1][LL::NCR][LL::WN][NC];
   __shared__ T geo[LL::CH * LL::GP3][LL::GEO];
-  __shared__ T stage[LL::CH][LL::NEN][P1][LL::NS];
-  __shared__ T rstage[HAS_RHS ? LL::CH * LL::GP3 : 1][LL::NRHS];
+  __shared__ T stage[ORD ? 1 : LL::CH][LL::NEN][P1][LL::NS];
+  __shared__ T rstage[HAS_RHS ? LL::CH * LL::GP3 * (ORD ? P1 : LL::NRHS) : 1];
   __shared__ T rwin[P1][NC];
   __shared__ T tN1[NG][P1], tD1[NG][P1];
   __shared__ T tN2[P1][NG][P1], tD2[P1][NG][P1];
@@ -851,9 +963,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_lines(const AsmArgs* __rest
   __shared__ int64_t pbase[LL::NCR];
   __shared__ int prow[LL::NCR];
   __shared__ int nbundle;
-  // register-pipelined phase B where the unrolled Gauss loop stays small
-  constexpr bool PIPE = LL::GP3 <= 27;
-  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF, PIPE>(a, blockIdx.x, blockIdx.y, win, geo, stage,
+  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF, PIPE, ORD>(a, blockIdx.x, blockIdx.y, win, geo, stage,
                                                       rstage, rwin, tN1, tD1, tN2, tD2, tN3, tD3,
                                                       be2, be3, eqw, pbase, prow, &nbundle);
 }
@@ -1197,7 +1307,11 @@ static int asm_half(Context& c, int p, int ng, int nc, bool fp32)
 {
   const int h = asm_half_slot(c, p, ng, nc, fp32);
   if (h >= 0) return h;
-  return (!fp32 && nc == 3) ? 1 : 0;
+  // full chunks take the ordered-update path (no stage) for p <= 2 with
+  // ng <= 3; elsewhere FP64 elasticity needs half chunks to keep two
+  // workgroups per CU
+  const bool ordered = p <= 2 && ng <= 3;
+  return (!fp32 && nc == 3 && !ordered) ? 1 : 0;
 }
 
 template <int P, int NG, int NC, typename T, int HALF>

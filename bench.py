@@ -350,9 +350,9 @@ def main():
     flops = (36 * nen * nen + 6 * nen) * ngp * nel_tot
     asm_bytes = nnz * 8 + neq * 8 + pb.patch.nnod * 3 * 8
     achieved = flops / (lines_ms * 1e-3)
-    traffic = measured_traffic("ifem::k_assemble_lines<2, 3, 3, double, 1, false, 1>") \
+    traffic = measured_traffic("ifem::k_assemble_lines<2, 3, 3, double, 1, false, 0>") \
         if args.nel == 64 else None
-    roofline = {"kernel": "k_assemble_lines<2,3,3,double,1,false,1>", "bound": "mfma",
+    roofline = {"kernel": "k_assemble_lines<2,3,3,double,1,false,0>", "bound": "mfma",
                 "achieved": round(achieved / 1e12, 3), "peak": PEAK_FP64 / 1e12,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64, 4), "traffic": traffic,
                 "algorithmic_flops": flops, "algorithmic_bytes": asm_bytes,

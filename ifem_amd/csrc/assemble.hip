@@ -664,31 +664,54 @@ __device__ __forceinline__ void line_impl(
       }
       // rounds over the bundle elements: in round r the items of element r
       // add their column sums (distinct window entries within a round)
+      // Each thread's window entries are distinct, so all of them are read
+      // first and written after (one LDS round trip per round, not one per
+      // entry).
       const int cj = e2 + b2, ck = e3 + b3;
+      constexpr int NV = NC == 1 ? 1 : (SACC ? 9 : 3); // window values per row function
+      T* wp[P1];
+#pragma unroll
+      for (int a1 = 0; a1 < P1; a1++) {
+        const int slot = (e1 + a1) % P1;
+        const int di = b1 - a1, dj = cj - jc, dk = ck - kc;
+        const int nbr = ((dk + P) * W1 + (dj + P)) * W1 + (di + P);
+        wp[a1] = &win[slot][0][nbr][0];
+      }
+      T add[P1][NV];
+#pragma unroll
+      for (int a1 = 0; a1 < P1; a1++) {
+        const T* sv = acc[a1];
+        if (NC == 1) {
+          add[a1][0] = sv[0];
+        } else if (SACC) {
+          const T tr = sv[0] + sv[4] + sv[8];
+#pragma unroll
+          for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+              add[a1][3 * i + j] = lam * sv[3 * i + j] + mu * sv[3 * j + i] + (i == j ? mu * tr : T(0));
+        } else {
+#pragma unroll
+          for (int j = 0; j < 3; j++) add[a1][j] = sv[j];
+        }
+      }
+      // offset of value v of a row function inside win[slot]: (i, j) of the
+      // 3x3 block -> [i][nbr][j]
+      constexpr int ISTR = LL::WN * NC;
       for (int r = 0; r < nch; r++) {
         __syncthreads();
         if (has && le == r) {
+          T cur[P1][NV];
 #pragma unroll
-          for (int a1 = 0; a1 < P1; a1++) {
-            const int slot = (e1 + a1) % P1;
-            const int di = b1 - a1, dj = cj - jc, dk = ck - kc;
-            const int nbr = ((dk + P) * W1 + (dj + P)) * W1 + (di + P);
-            const T* sv = acc[a1];
-            if (NC == 1) {
-              win[slot][0][nbr][0] += sv[0];
-            } else if (SACC) {
-              const T tr = sv[0] + sv[4] + sv[8];
+          for (int a1 = 0; a1 < P1; a1++)
 #pragma unroll
-              for (int i = 0; i < 3; i++)
+            for (int v = 0; v < NV; v++)
+              cur[a1][v] = wp[a1][(NV == 9 ? v / 3 : 0) * ISTR + (NV == 9 ? v % 3 : v)];
 #pragma unroll
-                for (int j = 0; j < 3; j++)
-                  win[slot][i][nbr][j] += lam * sv[3 * i + j] + mu * sv[3 * j + i] +
-                                          (i == j ? mu * tr : T(0));
-            } else {
+          for (int a1 = 0; a1 < P1; a1++)
 #pragma unroll
-              for (int j = 0; j < 3; j++) win[slot][0][nbr][j] += sv[j];
-            }
-          }
+            for (int v = 0; v < NV; v++)
+              wp[a1][(NV == 9 ? v / 3 : 0) * ISTR + (NV == 9 ? v % 3 : v)] = cur[a1][v] + add[a1][v];
         }
       }
       } else if constexpr (!PIPE) {

@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--no-mg", action="store_true", help="skip the multigrid-preconditioned leg")
     ap.add_argument("--no-multipatch", action="store_true",
                     help="skip the two-patch assembly leg")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the 96^3 p=3 elasticity leg (largest single-GPU case)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched uneven-patch leg (256 diffusion patches)")
     ap.add_argument("--cpu-layers", type=int, default=16,
@@ -262,6 +264,73 @@ def main():
         os_.close()
         log(f"[bench] multipatch: {multipatch}")
 
+    # ---- largest single-GPU case: 96^3 p=3 elasticity, 4x4x4 Gauss points -----
+    large = None
+    if not args.no_large:
+        from ifem_amd import PC_MG
+        pbL = problems.elasticity_cube(96, 3)
+        log(f"[bench] large: elasticity 96^3 p=3, ndof={pbL.ndof}")
+        with Session(stream=stream.cuda_stream) as sL:
+            sL.bind(sL.add_patch(pbL.patch), 3, pbL.meqn, pbL.neq)
+            t0 = time.perf_counter()
+            nnzL = sL.init_assembly()
+            symL = (time.perf_counter() - t0) * 1e3
+            faceL, tracL, ngL = pbL.traction
+            runs = []
+            for _ in range(3):
+                with torch.cuda.stream(stream):
+                    sL.assemble(pbL.integrand, pbL.ng, FP64)
+                    sL.add_traction(faceL, tracL, ngL)
+                stream.synchronize()
+                runs.append((sL.timings()[0], *sL.assembly_parts()))
+            kL = float(np.median([r[0] for r in runs]))
+            gL = float(np.median([r[1] for r in runs]))
+            lL = float(np.median([r[2] for r in runs]))
+            nenL = pbL.patch.nen
+            flopsL = (36 * nenL * nenL + 6 * nenL) * int(np.prod(pbL.ng)) * int(np.prod(pbL.patch.nel))
+            st64 = sL.solve(rtol=args.rtol, maxits=200000, check_every=64)
+            xL = sL.solution()
+            KxL = sL.spmv(xL)
+
+            def errL(x):
+                e = x - xL
+                return (float(np.sqrt(abs(e @ sL.spmv(e))) / np.sqrt(xL @ KxL)),
+                        float(np.max(np.abs(e)) / np.max(np.abs(xL))))
+            with torch.cuda.stream(stream):
+                sL.assemble(pbL.integrand, pbL.ng, FP32)
+            stream.synchronize()
+            a32 = sL.timings()[0]
+            stm = sL.solve(rtol=args.rtol, maxits=200000, precision=FP32, inner_rtol=1e-4,
+                           check_every=64)
+            em = errL(sL.solution())
+            stg = [sL.solve(rtol=args.rtol, maxits=5000, pc=PC_MG, check_every=8) for _ in range(2)]
+            eg = errL(sL.solution())
+            large = {
+                "workload": "elasticity 96^3 p=3 jittered cube, 4x4x4 Gauss, clamped x=0, "
+                            "traction (0,0,-1) on x=1",
+                "dofs": pbL.ndof, "neq": pbL.neq, "nnz": nnzL, "symbolic_ms": round(symL, 1),
+                "assembly_kernels_ms": round(kL, 3),
+                "assembled_dof_per_s": round(pbL.ndof / (kL * 1e-3), 1),
+                "line_kernel_ms": round(lL, 3), "geometry_pass_ms": round(gL, 3),
+                "line_kernel_tflops": round(flopsL / (lL * 1e-3) / 1e12, 3),
+                "line_kernel_frac_fp64_peak": round(flopsL / (lL * 1e-3) / PEAK_FP64, 4),
+                "fp64": {"solve_ms": round(st64["solve_ms"], 3), "iterations": st64["iterations"],
+                         "rel_residual": st64["rel_residual"], "converged": st64["converged"]},
+                "mixed": {"fp32_assembly_kernels_ms": round(a32, 3),
+                          "solve_ms": round(stm["solve_ms"], 3), "iterations": stm["iterations"],
+                          "outer": stm["outer"], "rel_residual": stm["rel_residual"],
+                          "converged": stm["converged"], "energy_norm_err_vs_fp64": em[0],
+                          "max_nodal_err_vs_fp64": em[1]},
+                "multigrid": {"solve_ms": round(stg[1]["solve_ms"], 3),
+                              "first_solve_ms": round(stg[0]["solve_ms"], 3),
+                              "iterations": stg[1]["iterations"],
+                              "rel_residual": stg[1]["rel_residual"],
+                              "converged": all(r["converged"] for r in stg),
+                              "energy_norm_err_vs_fp64": eg[0], "max_nodal_err_vs_fp64": eg[1]},
+                "speedup_mg_vs_fp64": round(st64["solve_ms"] / stg[1]["solve_ms"], 3),
+            }
+        log(f"[bench] large: {large}")
+
     # ---- batched uneven patches: one persistent assembly launch ----------------
     batch = None
     if not args.no_batch:
@@ -390,6 +459,7 @@ def main():
         "roofline": roofline, "roofline_spmv": spmv_roof, "cpu_baseline": cpu,
         "batch": batch,
         "multipatch": multipatch,
+        "large": large,
     }
     s.close()
     print(json.dumps(out), flush=True)

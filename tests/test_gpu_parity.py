@@ -12,7 +12,7 @@ import pytest
 import scipy.sparse as sp
 
 import oracle
-from ifem_amd import FP32, FP64, model as M, problems, setup_problem
+from ifem_amd import FP32, FP64, PC_MG, model as M, problems, setup_problem
 
 pytestmark = pytest.mark.gpu
 
@@ -202,6 +202,59 @@ def test_elasticity64_full_size_properties():
         del Ao
         st = s.solve(rtol=1e-8, maxits=20000)
         assert st["converged"] and st["rel_residual"] <= 1e-8
+
+
+def _free_count(n, lo, p):
+    """Per index i >= lo of 0..n-1: free neighbours within +-p (indices >= lo)."""
+    i = np.arange(n)
+    return np.where(i >= lo, np.minimum(n - 1, i + p) - np.maximum(lo, i - p) + 1, 0)
+
+
+def test_elasticity96_p3_full_size_properties():
+    """Largest single-GPU case: 96^3 p=3 jittered elasticity, 4x4x4 Gauss
+    points (2,910,897 DOFs, nnz 2.81e9 > 2^31): closed-form nnz, traction
+    resultant, the rows of the bottom three node layers against the oracle,
+    FP64 Jacobi-PCG to 1e-8 on the true residual, FP32 assembly with the
+    mixed-precision solve (energy-norm error vs FP64 within 1e-6) and the
+    multigrid-preconditioned solve."""
+    pb = problems.elasticity_cube(96, 3)
+    n, p = pb.patch.n[0], 3
+    nnz = 9 * int(_free_count(n, 1, p).sum()) * int(_free_count(n, 0, p).sum()) ** 2
+    assert nnz == 2813174226  # 312,574,914 3x3 blocks
+    _say("[gpu] full-size elasticity 96^3 p=3: set-up and FP64 assembly")
+    with setup_problem(pb) as s:
+        assert s.nnz == nnz and s.neq == pb.neq == 2881494
+        _say("[gpu]   system to host")
+        rp, ci, A, b = s.system()
+        assert abs(b[2::3].sum() + 1.0) < 1e-12
+        _say("[gpu]   oracle rows of node layers k < 3")
+        Ao, bo = oracle.assemble(pb.patch, pb.integrand, pb.ng, pb.meqn, pb.neq, rp, ci,
+                                 e3_range=(0, 3))
+        n1, n2 = pb.patch.n[0], pb.patch.n[1]
+        last_row = int(pb.meqn[: 3 * n1 * n2 * 3].max())
+        hi = rp[last_row]
+        err = np.max(np.abs(A[:hi] - Ao[:hi])) / np.max(np.abs(A[:hi]))
+        assert err <= 1e-12, err
+        del Ao, A, ci
+        _say("[gpu]   FP64 Jacobi-PCG")
+        st = s.solve(rtol=1e-8, maxits=50000)
+        assert st["converged"] and st["rel_residual"] <= 1e-8
+        x64 = s.solution()
+        xk = s.spmv(x64)
+        _say("[gpu]   FP32 assembly and mixed-precision solve")
+        s.assemble(pb.integrand, pb.ng, FP32)
+        stm = s.solve(rtol=1e-8, maxits=50000, precision=FP32)
+        assert stm["converged"] and stm["rel_residual"] <= 1e-8
+        xm = s.solution()
+        d = xm - x64
+        assert np.sqrt(d @ s.spmv(d)) <= 1e-6 * np.sqrt(x64 @ xk)
+        _say("[gpu]   multigrid-preconditioned solve")
+        stg = s.solve(rtol=1e-8, maxits=2000, pc=PC_MG)
+        assert stg["converged"] and stg["rel_residual"] <= 1e-8
+        d = s.solution() - x64
+        assert np.sqrt(d @ s.spmv(d)) <= 1e-6 * np.sqrt(x64 @ xk)
+        _say(f"[gpu]   iterations FP64 {st['iterations']} mixed {stm['iterations']} "
+             f"multigrid {stg['iterations']}")
 
 
 def test_norms_match_oracle():

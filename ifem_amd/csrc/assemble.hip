@@ -424,7 +424,8 @@ struct LineLds {
 // The LDS arrays come in as separate __restrict__ pointers so that, after
 // inlining, the compiler knows they do not alias (fields of one struct with
 // run-time indices would otherwise be assumed to overlap).
-template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF, bool PIPE, bool ORD>
+template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF, bool PIPE, bool ORD,
+          bool SEMI = false>
 __device__ __forceinline__ void line_impl(
     ConstArgs& a, const int line, const int split,
     T (*__restrict__ win)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NCR]
@@ -714,6 +715,80 @@ __device__ __forceinline__ void line_impl(
               wp[a1][(NV == 9 ? v / 3 : 0) * ISTR + (NV == 9 ? v % 3 : v)] = cur[a1][v] + add[a1][v];
         }
       }
+      } else if constexpr (SEMI) {
+      // Larger Gauss rules: loop over the (q2, q3) rows with the q1 row
+      // unrolled; the next point's geometry row is loaded into registers while
+      // the current one is used (across rows too), the column node's
+      // direction-1 basis values are held in registers.
+      for (int t = tid; t < nch * NEN; t += kBlock) {
+        const int le = t / NEN, b = t % NEN;
+        const int b1 = b % P1, b2 = (b / P1) % P1, b3 = b / (P1 * P1);
+        const int e2 = be2[c0 + le], e3 = be3[c0 + le];
+        const int m2 = e2 - jc + P, m3 = e3 - kc + P;
+        T vn1[NG], vd1[NG];
+#pragma unroll
+        for (int q = 0; q < NG; q++) {
+          vn1[q] = tN1[q][b1];
+          vd1[q] = tD1[q][b1];
+        }
+        T acc[P1][NS];
+#pragma unroll
+        for (int a1 = 0; a1 < P1; a1++)
+#pragma unroll
+          for (int s = 0; s < NS; s++) acc[a1][s] = 0;
+        const T* __restrict__ Gbase = geo[le * GP3];
+        T Gc[LL::GEO];
+#pragma unroll
+        for (int m = 0; m < LL::GEO; m++) Gc[m] = Gbase[m];
+        for (int g23 = 0; g23 < NG * NG; g23++) {
+          const int q2 = g23 % NG, q3 = g23 / NG;
+          const T n3 = tN3[m3][q3][b3], d3 = tD3[m3][q3][b3];
+          const T n2 = tN2[m2][q2][b2], d2 = tD2[m2][q2][b2];
+          const T n23 = n2 * n3, d23 = d2 * n3, n2d3 = n2 * d3;
+#pragma unroll
+          for (int q1 = 0; q1 < NG; q1++) {
+            // next row (the last point reloads itself: in bounds, unused)
+            const int qn = g23 * NG + q1 + 1 < GP3 ? g23 * NG + q1 + 1 : GP3 - 1;
+            T Gn[LL::GEO];
+#pragma unroll
+            for (int m = 0; m < LL::GEO; m++) Gn[m] = Gbase[qn * LL::GEO + m];
+            const T r0 = vd1[q1] * n23, r1 = vn1[q1] * d23, r2 = vn1[q1] * n2d3;
+            const T* G = Gc;
+            if (NC == 1) {
+#pragma unroll
+              for (int a1 = 0; a1 < P1; a1++)
+                acc[a1][0] += r0 * G[3 * a1] + r1 * G[3 * a1 + 1] + r2 * G[3 * a1 + 2];
+            } else {
+              const T gb0 = r0 * G[0] + r1 * G[3] + r2 * G[6];
+              const T gb1 = r0 * G[1] + r1 * G[4] + r2 * G[7];
+              const T gb2 = r0 * G[2] + r1 * G[5] + r2 * G[8];
+#pragma unroll
+              for (int a1 = 0; a1 < P1; a1++) {
+                const T wa0 = G[9 + 3 * a1], wa1 = G[10 + 3 * a1], wa2 = G[11 + 3 * a1];
+                if (SACC) {
+                  acc[a1][0] += wa0 * gb0; acc[a1][1] += wa0 * gb1; acc[a1][2] += wa0 * gb2;
+                  acc[a1][3] += wa1 * gb0; acc[a1][4] += wa1 * gb1; acc[a1][5] += wa1 * gb2;
+                  acc[a1][6] += wa2 * gb0; acc[a1][7] += wa2 * gb1; acc[a1][8] += wa2 * gb2;
+                } else {
+                  const T wa[3] = {wa0, wa1, wa2}, gb[3] = {gb0, gb1, gb2};
+                  const int ic = split * NCR;
+                  const T dot = wa0 * gb0 + wa1 * gb1 + wa2 * gb2;
+#pragma unroll
+                  for (int jj = 0; jj < 3; jj++)
+                    acc[a1][jj] += lam * wa[ic] * gb[jj] + mu * wa[jj] * gb[ic] +
+                                   (jj == ic ? mu * dot : T(0));
+                }
+              }
+            }
+#pragma unroll
+            for (int m = 0; m < LL::GEO; m++) Gc[m] = Gn[m];
+          }
+        }
+#pragma unroll
+        for (int a1 = 0; a1 < P1; a1++)
+#pragma unroll
+          for (int s = 0; s < NS; s++) stage[le][b][a1][s] = acc[a1][s];
+      }
       } else if constexpr (!PIPE) {
       // one thread per (element, column node b), Gauss points in fixed order
       for (int t = tid; t < nch * NEN; t += kBlock) {
@@ -986,7 +1061,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_lines(const AsmArgs* __rest
   __shared__ int64_t pbase[LL::NCR];
   __shared__ int prow[LL::NCR];
   __shared__ int nbundle;
-  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF, PIPE, ORD>(a, blockIdx.x, blockIdx.y, win, geo, stage,
+  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF, PIPE, ORD, !PIPE>(a, blockIdx.x, blockIdx.y, win, geo, stage,
                                                       rstage, rwin, tN1, tD1, tN2, tD2, tN3, tD3,
                                                       be2, be3, eqw, pbase, prow, &nbundle);
 }
@@ -1334,7 +1409,7 @@ static int asm_half(Context& c, int p, int ng, int nc, bool fp32)
   // ng <= 3; elsewhere FP64 elasticity needs half chunks to keep two
   // workgroups per CU
   const bool ordered = p <= 2 && ng <= 3;
-  return (!fp32 && nc == 3 && !ordered) ? 1 : 0;
+  return (!fp32 && nc == 3 && !ordered && p < 3) ? 1 : 0;
 }
 
 template <int P, int NG, int NC, typename T, int HALF>

@@ -1032,7 +1032,7 @@ template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF>
 __device__ __forceinline__ void line_body(ConstArgs& a, const int line, const int split,
                                           LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>& L)
 {
-  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF, false, false>(
+  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF, false, false, true>(
       a, line, split, L.win, L.geo, L.stage, &L.rstage[0][0], L.rwin, L.tN1, L.tD1, L.tN2, L.tD2, L.tN3,
       L.tD3, L.be2, L.be3, L.eqw, L.pbase, L.prow, &L.nbundle);
 }

@@ -30,6 +30,8 @@ static void free_system(Context& c)
   for (SubPatch& sp : c.multi) {
     hipFree(sp.d_leq);
     hipFree(sp.d_lgval);
+    hipFree(sp.d_pmap);
+    hipFree(sp.d_pptr);
   }
   c.multi.clear();
   hipFree(c.d_eq);

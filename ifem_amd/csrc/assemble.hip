@@ -180,6 +180,9 @@ struct AsmArgs {
   // values are added at the column found by binary search in the sorted row
   const int32_t* colind;
   int accum;
+  // accumulate mode with precomputed positions (SubPatch::d_pmap / d_pptr)
+  const int64_t* pmap;
+  const int64_t* pptr;
 };
 
 // Geometry record of one Gauss point.  Vector problems: J^-1 (row-major, 9)
@@ -983,7 +986,12 @@ __device__ __forceinline__ void line_impl(
             const int ev = pf ? eqw[t] : window_eq<P, NC>(a, i, jc, kc, t);
             const bool valid = ev >= 0;
             const unsigned long long m = __ballot(valid);
-            if (valid && a.accum) {
+            if (valid && a.accum && a.pmap) {
+              const T v = win[slot][wave][t / NC][t % NC];
+              const int64_t at = a.pmap[a.pptr[node * NC + ic] + pos + __popcll(m & ((1ull << lane) - 1ull))];
+              if (a.val32) a.val32[at] += (float)v;
+              else a.val64[at] += (double)v;
+            } else if (valid && a.accum) {
               const T v = win[slot][wave][t / NC][t % NC];
               int64_t lo = base, hi = a.rowptr[req + 1] - 1;
               while (lo <= hi) {
@@ -1251,12 +1259,65 @@ int host_union_pattern(const Context& c, const std::vector<SubPatch>& subs, int 
   return IFEM_GPU_OK;
 }
 
+// Positions of every (local row, valid window column) of each patch in the
+// shared CSR, in the order the line kernels visit them (window offsets
+// (dk, dj, di) ascending, components minor), found once by binary search.
+int sub_position_maps(Context& c, std::vector<SubPatch>& subs, const std::vector<int64_t>& rowptr,
+                      const std::vector<int32_t>& colind)
+{
+  const int nc = c.ncomp;
+  for (SubPatch& sp : subs) {
+    const PatchRec& P = c.patches[sp.patch];
+    const int p = P.p[0], n1 = P.n[0], n2 = P.n[1], n3 = P.n[2];
+    const int64_t nd = (int64_t)n1 * n2 * n3 * nc;
+    std::vector<int64_t> pptr(nd + 1, 0), pmap;
+    for (int k = 0; k < n3; k++)
+      for (int j = 0; j < n2; j++)
+        for (int i = 0; i < n1; i++) {
+          const int node = i + n1 * (j + n2 * k);
+          for (int ci = 0; ci < nc; ci++) {
+            const int64_t d = (int64_t)node * nc + ci;
+            const int r = sp.h_leq[d];
+            if (r >= 0) {
+              const int32_t* rb = colind.data() + rowptr[r];
+              const int32_t* re = colind.data() + rowptr[r + 1];
+              for (int kk = k - p; kk <= k + p; kk++)
+                for (int jj = j - p; jj <= j + p; jj++)
+                  for (int ii = i - p; ii <= i + p; ii++) {
+                    if (ii < 0 || ii >= n1 || jj < 0 || jj >= n2 || kk < 0 || kk >= n3) continue;
+                    const int nb = ii + n1 * (jj + n2 * kk);
+                    for (int cj = 0; cj < nc; cj++) {
+                      const int e = sp.h_leq[(int64_t)nb * nc + cj];
+                      if (e < 0) continue;
+                      const int32_t* f = std::lower_bound(rb, re, e);
+                      if (f == re || *f != e)
+                        return fail(IFEM_GPU_EINVAL, "pattern: column missing from the union row");
+                      pmap.push_back(rowptr[r] + (f - rb));
+                    }
+                  }
+            }
+            pptr[d + 1] = (int64_t)pmap.size();
+          }
+        }
+    hipFree(sp.d_pmap);
+    hipFree(sp.d_pptr);
+    sp.d_pmap = sp.d_pptr = nullptr;
+    IFEM_HIP(hipMalloc(&sp.d_pptr, sizeof(int64_t) * pptr.size()));
+    IFEM_HIP(hipMemcpy(sp.d_pptr, pptr.data(), sizeof(int64_t) * pptr.size(), hipMemcpyHostToDevice));
+    IFEM_HIP(hipMalloc(&sp.d_pmap, sizeof(int64_t) * (pmap.empty() ? 1 : pmap.size())));
+    if (!pmap.empty())
+      IFEM_HIP(hipMemcpy(sp.d_pmap, pmap.data(), sizeof(int64_t) * pmap.size(), hipMemcpyHostToDevice));
+  }
+  return IFEM_GPU_OK;
+}
+
 static int build_pattern_multi(Context& c)
 {
   const int neq = c.neq;
   std::vector<int64_t> rowptr;
   std::vector<int32_t> colind;
   int rc0 = host_union_pattern(c, c.multi, neq, rowptr, colind);
+  if (!rc0) rc0 = sub_position_maps(c, c.multi, rowptr, colind);
   if (rc0) return rc0;
   const int64_t nnz = rowptr[neq];
   hipFree(c.d_rowptr);
@@ -1457,8 +1518,8 @@ static int dispatch_ng(Context& c, const AsmArgs& args, int ng, int nc, bool fp3
 // patches; values and right-hand side are added (zeroed by the caller).
 template <int P>
 static int assemble_p(Context& c, const SysView& sv, const PatchRec& Pt, const int* eq,
-                      const double* gval, int accum, const ifem_gpu_integrand& itg, int ng,
-                      int precision)
+                      const double* gval, int accum, const int64_t* pmap, const int64_t* pptr,
+                      const ifem_gpu_integrand& itg, int ng, int precision)
 {
   GaussArgs ga{};
   gauss_rule(ng, ga.xi, ga.w);
@@ -1481,6 +1542,8 @@ static int assemble_p(Context& c, const SysView& sv, const PatchRec& Pt, const i
   a.rowptr = sv.rowptr;
   a.colind = sv.colind;
   a.accum = accum;
+  a.pmap = accum ? pmap : nullptr;
+  a.pptr = accum ? pptr : nullptr;
   const bool fp32 = precision == IFEM_GPU_FP32;
   if (fp32) {
     a.val32 = sv.val32;
@@ -1576,12 +1639,13 @@ int autotune_assembly(Context& c, const ifem_gpu_integrand& itg, const int ng[3]
 
 static int assemble_one(Context& c, const SysView& sv, const PatchRec& Pt, const int* eq,
                         const double* gval, int accum, const ifem_gpu_integrand& itg, int ng,
-                        int precision)
+                        int precision, const int64_t* pmap = nullptr,
+                        const int64_t* pptr = nullptr)
 {
   switch (Pt.p[0]) {
-  case 1: return assemble_p<1>(c, sv, Pt, eq, gval, accum, itg, ng, precision);
-  case 2: return assemble_p<2>(c, sv, Pt, eq, gval, accum, itg, ng, precision);
-  case 3: return assemble_p<3>(c, sv, Pt, eq, gval, accum, itg, ng, precision);
+  case 1: return assemble_p<1>(c, sv, Pt, eq, gval, accum, pmap, pptr, itg, ng, precision);
+  case 2: return assemble_p<2>(c, sv, Pt, eq, gval, accum, pmap, pptr, itg, ng, precision);
+  case 3: return assemble_p<3>(c, sv, Pt, eq, gval, accum, pmap, pptr, itg, ng, precision);
   default: return fail(IFEM_GPU_EUNSUPPORTED, "assemble: degree out of range");
   }
 }
@@ -1600,7 +1664,8 @@ static int assemble_sum(Context& c, SysView& sv, const std::vector<SubPatch>& su
   }
   double tot = 0.0, geo = 0.0;
   for (const SubPatch& sp : subs) {
-    int rc = assemble_one(c, sv, c.patches[sp.patch], sp.d_leq, sp.d_lgval, 1, itg, ng, precision);
+    int rc = assemble_one(c, sv, c.patches[sp.patch], sp.d_leq, sp.d_lgval, 1, itg, ng, precision,
+                          sp.d_pmap, sp.d_pptr);
     if (rc) return rc;
     tot += c.last_assemble_ms;
     geo += c.last_geometry_ms;

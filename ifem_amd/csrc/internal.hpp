@@ -66,6 +66,11 @@ struct SubPatch {
   int patch = -1;
   int* d_leq = nullptr;       // local DOF -> global equation (0-based) or -1
   double* d_lgval = nullptr;  // local DOF -> prescribed value (nullptr: none)
+  // position of each (local row, valid window column) in the shared CSR:
+  // d_pmap[d_pptr[d] + k] for the k-th valid column of local DOF d in the
+  // (2p+1)^3 window order (nullptr: search the row instead)
+  int64_t* d_pmap = nullptr;
+  int64_t* d_pptr = nullptr;
   std::vector<int> h_leq;
 };
 

@@ -22,6 +22,8 @@ namespace ifem {
 
 int host_union_pattern(const Context& c, const std::vector<SubPatch>& subs, int neq,
                        std::vector<int64_t>& rowptr, std::vector<int32_t>& colind);
+int sub_position_maps(Context& c, std::vector<SubPatch>& subs, const std::vector<int64_t>& rowptr,
+                      const std::vector<int32_t>& colind);
 int solver_setup(Context& c);
 
 void mpc_free(Context& c)
@@ -29,7 +31,7 @@ void mpc_free(Context& c)
   if (!c.mpc) return;
   MpcSystem& x = *c.mpc;
   hipFree(x.d_rowptr); hipFree(x.d_colind); hipFree(x.d_val); hipFree(x.d_val32); hipFree(x.d_rhs);
-  hipFree(x.ext.d_leq); hipFree(x.ext.d_lgval);
+  hipFree(x.ext.d_leq); hipFree(x.ext.d_lgval); hipFree(x.ext.d_pmap); hipFree(x.ext.d_pptr);
   hipFree(x.d_cptr); hipFree(x.d_cext); hipFree(x.d_cw);
   hipFree(x.d_sptr); hipFree(x.d_smaster); hipFree(x.d_scoef); hipFree(x.d_sc0);
   hipFree(x.d_slave_of_dof); hipFree(x.d_xdof);
@@ -109,7 +111,10 @@ int mpc_build_pattern(Context& c)
   std::vector<int32_t> xci;
   std::vector<SubPatch> subs(1, x.ext);
   int rc = host_union_pattern(c, subs, x.ne, xrp, xci);
+  if (!rc) rc = sub_position_maps(c, subs, xrp, xci);
   if (rc) return rc;
+  x.ext.d_pmap = subs[0].d_pmap; // owned by x.ext
+  x.ext.d_pptr = subs[0].d_pptr;
   x.nnz = xrp[x.ne];
   // contributions per reduced row
   std::vector<std::vector<std::pair<int, double>>> contrib(neq > 0 ? neq : 1);

@@ -1,9 +1,10 @@
 """GPU parity: the HIP path (through the C ABI) against the oracle on the same
 seeded inputs, plus size-independent properties at the full benchmark sizes.
 
-Tolerances: FP64 assembly and solution 1e-12 relative to max|.| (the north-star
-parity bar); FP32 assembly 1e-5 relative; mixed-precision solution: energy
-norm error vs the FP64 solution <= 1e-6.
+Tolerances: FP64 assembly and solution 1e-12 relative to max|.| (the parity
+bar of the FP64 path against the reference's assembly and solution); FP32
+assembly 1e-5 relative; mixed-precision solution: energy norm error vs the
+FP64 solution <= 1e-6.
 """
 import sys
 

@@ -418,7 +418,10 @@ struct LineLds {
   alignas(16) int be2[NB];
   alignas(16) int be3[NB];
   static constexpr int NT = (WN * NC + 63) / 64;
-  alignas(16) int eqw[NT * 64];
+  // neighbour equation numbers kept in LDS for phase D (not for p = 3,
+  // whose LDS budget goes to the element chunk)
+  static constexpr bool PFD = P < 3;
+  alignas(16) int eqw[PFD ? NT * 64 : 1];
   alignas(16) int64_t pbase[NCR];
   int prow[NCR];
   int nbundle;
@@ -458,7 +461,7 @@ __device__ __forceinline__ void line_impl(
   // ORD: the whole bundle in one chunk, column sums added to the window
   // element by element (no stage); the RHS stage then holds w*detJ*N_a and
   // the body force is applied after the sum
-  static_assert(!ORD || (CH == LL::NB && PIPE), "ordered updates need one chunk per layer");
+  static_assert(!ORD || CH * LL::NEN <= kBlock, "ordered updates need one item per thread");
   constexpr int RS = ORD ? (HAS_RHS ? P1 : 1) : LL::NRHS; // RHS stage row length
 
   const int tid = threadIdx.x;
@@ -514,7 +517,8 @@ __device__ __forceinline__ void line_impl(
       (&tN1[0][0])[t] = (T)a.tabN1[e1 * NG * P1 + t];
       (&tD1[0][0])[t] = (T)a.tabD1[e1 * NG * P1 + t];
     }
-    for (int t = tid; t < NT * 64; t += kBlock) eqw[t] = window_eq<P, NC>(a, e1, jc, kc, t);
+    if (LL::PFD)
+      for (int t = tid; t < NT * 64; t += kBlock) eqw[t] = window_eq<P, NC>(a, e1, jc, kc, t);
     if (tid < NCR) {
       const int64_t node = (int64_t)e1 + (int64_t)a.n1 * (jc + (int64_t)a.n2 * kc);
       prow[tid] = a.eq[node * NC + split * NCR + tid];
@@ -597,8 +601,9 @@ __device__ __forceinline__ void line_impl(
 
       // ---- phase B: element-row x column-node products ----------------------
       if constexpr (ORD) {
-      // One (element, column node b) item per thread for the whole bundle,
-      // Gauss loop as in the pipelined variant below; the sums stay in
+      // One (element, column node b) item per thread for the chunk (the
+      // whole bundle when it fits), Gauss loop as in the pipelined or the
+      // row-prefetching variant below; the sums stay in
       // registers and go into the row window one element after the other
       // (element order fixed, so every window entry is summed in that order).
       const bool has = tid < nch * NEN;
@@ -611,6 +616,7 @@ __device__ __forceinline__ void line_impl(
 #pragma unroll
         for (int s = 0; s < NS; s++) acc[a1][s] = 0;
       if (has) {
+        if constexpr (PIPE) {
         const int m2 = e2 - jc + P, m3 = e3 - kc + P;
         T vn1[NG], vd1[NG], vn2[NG], vd2[NG], vn3[NG], vd3[NG];
 #pragma unroll
@@ -664,6 +670,62 @@ __device__ __forceinline__ void line_impl(
 #pragma unroll
             for (int m = 0; m < LL::GEO; m++) Gc[m] = Gn[m];
           }
+        }
+              } else {
+        const int m2 = e2 - jc + P, m3 = e3 - kc + P;
+        T vn1[NG], vd1[NG];
+#pragma unroll
+        for (int q = 0; q < NG; q++) {
+          vn1[q] = tN1[q][b1];
+          vd1[q] = tD1[q][b1];
+        }
+        const T* __restrict__ Gbase = geo[le * GP3];
+        T Gc[LL::GEO];
+#pragma unroll
+        for (int m = 0; m < LL::GEO; m++) Gc[m] = Gbase[m];
+        for (int g23 = 0; g23 < NG * NG; g23++) {
+          const int q2 = g23 % NG, q3 = g23 / NG;
+          const T n3 = tN3[m3][q3][b3], d3 = tD3[m3][q3][b3];
+          const T n2 = tN2[m2][q2][b2], d2 = tD2[m2][q2][b2];
+          const T n23 = n2 * n3, d23 = d2 * n3, n2d3 = n2 * d3;
+#pragma unroll
+          for (int q1 = 0; q1 < NG; q1++) {
+            const int qn = g23 * NG + q1 + 1 < GP3 ? g23 * NG + q1 + 1 : GP3 - 1;
+            T Gn[LL::GEO];
+#pragma unroll
+            for (int m = 0; m < LL::GEO; m++) Gn[m] = Gbase[qn * LL::GEO + m];
+            const T r0 = vd1[q1] * n23, r1 = vn1[q1] * d23, r2 = vn1[q1] * n2d3;
+            const T* G = Gc;
+            if (NC == 1) {
+#pragma unroll
+              for (int a1 = 0; a1 < P1; a1++)
+                acc[a1][0] += r0 * G[3 * a1] + r1 * G[3 * a1 + 1] + r2 * G[3 * a1 + 2];
+            } else {
+              const T gb0 = r0 * G[0] + r1 * G[3] + r2 * G[6];
+              const T gb1 = r0 * G[1] + r1 * G[4] + r2 * G[7];
+              const T gb2 = r0 * G[2] + r1 * G[5] + r2 * G[8];
+#pragma unroll
+              for (int a1 = 0; a1 < P1; a1++) {
+                const T wa0 = G[9 + 3 * a1], wa1 = G[10 + 3 * a1], wa2 = G[11 + 3 * a1];
+                if (SACC) {
+                  acc[a1][0] += wa0 * gb0; acc[a1][1] += wa0 * gb1; acc[a1][2] += wa0 * gb2;
+                  acc[a1][3] += wa1 * gb0; acc[a1][4] += wa1 * gb1; acc[a1][5] += wa1 * gb2;
+                  acc[a1][6] += wa2 * gb0; acc[a1][7] += wa2 * gb1; acc[a1][8] += wa2 * gb2;
+                } else {
+                  const T wa[3] = {wa0, wa1, wa2}, gb[3] = {gb0, gb1, gb2};
+                  const int ic = split * NCR;
+                  const T dot = wa0 * gb0 + wa1 * gb1 + wa2 * gb2;
+#pragma unroll
+                  for (int jj = 0; jj < 3; jj++)
+                    acc[a1][jj] += lam * wa[ic] * gb[jj] + mu * wa[jj] * gb[ic] +
+                                   (jj == ic ? mu * dot : T(0));
+                }
+              }
+            }
+#pragma unroll
+            for (int m = 0; m < LL::GEO; m++) Gc[m] = Gn[m];
+          }
+        }
         }
       }
       // rounds over the bundle elements: in round r the items of element r
@@ -975,6 +1037,7 @@ __device__ __forceinline__ void line_impl(
         const int ic = split * NCR + wave;
         const bool pf = i == e1;
         const int req = pf ? prow[wave] : a.eq[node * NC + ic];
+        const bool pfe = LL::PFD && pf;
         if (req >= 0) {
           const int64_t base = pf ? pbase[wave] : a.rowptr[req];
           const double* __restrict__ gv = a.gval;
@@ -983,7 +1046,7 @@ __device__ __forceinline__ void line_impl(
 #pragma unroll 1
           for (int k = 0; k < NT; k++) {
             const int t = k * 64 + lane;
-            const int ev = pf ? eqw[t] : window_eq<P, NC>(a, i, jc, kc, t);
+            const int ev = pfe ? eqw[t] : window_eq<P, NC>(a, i, jc, kc, t);
             const bool valid = ev >= 0;
             const unsigned long long m = __ballot(valid);
             if (valid && a.accum && a.pmap) {
@@ -1055,7 +1118,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_lines(const AsmArgs* __rest
   // register-pipelined phase B where the unrolled Gauss loop stays small;
   // ordered window updates (no stage) where the whole bundle is one chunk
   constexpr bool PIPE = LL::GP3 <= 27;
-  constexpr bool ORD = PIPE && HALF == 0 && LL::CH == LL::NB && LL::NB * LL::NEN <= kBlock;
+  constexpr bool ORD = HALF == 0 && LL::CH * LL::NEN <= kBlock;
   __shared__ T win[P1][LL::NCR][LL::WN][NC];
   __shared__ T geo[LL::CH * LL::GP3][LL::GEO];
   __shared__ T stage[ORD ? 1 : LL::CH][LL::NEN][P1][LL::NS];
@@ -1065,7 +1128,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_lines(const AsmArgs* __rest
   __shared__ T tN2[P1][NG][P1], tD2[P1][NG][P1];
   __shared__ T tN3[P1][NG][P1], tD3[P1][NG][P1];
   __shared__ int be2[LL::NB], be3[LL::NB];
-  __shared__ int eqw[LL::NT * 64];
+  __shared__ int eqw[LL::PFD ? LL::NT * 64 : 1];
   __shared__ int64_t pbase[LL::NCR];
   __shared__ int prow[LL::NCR];
   __shared__ int nbundle;

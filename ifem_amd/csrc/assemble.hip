@@ -388,7 +388,7 @@ __device__ __forceinline__ int window_eq(ConstArgs& a, int i, int jc, int kc, in
   return a.eq[((int64_t)ii + (int64_t)a.n1 * (jj + (int64_t)a.n2 * kk)) * NC + cc];
 }
 
-template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF>
+template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF, bool ORDL = false>
 struct LineLds {
   static constexpr int P1 = P + 1;
   static constexpr int NEN = P1 * P1 * P1;
@@ -406,8 +406,9 @@ struct LineLds {
   // every member 16-byte aligned: lets the compiler use 128-bit LDS loads
   alignas(16) T win[P1][NCR][WN][NC];
   alignas(16) T geo[CH * GP3][GEO];
-  alignas(16) T stage[CH][NEN][P1][NS];
-  alignas(16) T rstage[HAS_RHS ? CH * GP3 : 1][NRHS];
+  // ORDL (ordered window updates): no stage; RHS stage of w*detJ*N_a only
+  alignas(16) T stage[ORDL ? 1 : CH][NEN][P1][NS];
+  alignas(16) T rstage[HAS_RHS ? CH * GP3 : 1][ORDL ? (HAS_RHS ? P1 : 1) : NRHS];
   alignas(16) T rwin[P1][NC];
   alignas(16) T tN1[NG][P1];
   alignas(16) T tD1[NG][P1];
@@ -1099,11 +1100,19 @@ __device__ __forceinline__ void line_impl(
   }
 }
 
+// Batch variants with full chunks of at most one item per thread take the
+// ordered window updates (no stage).
+template <int P, int NG, int HALF>
+struct BatchOrd {
+  static constexpr bool v = HALF == 0 && Chunk<P, NG, 0>::v * (P + 1) * (P + 1) * (P + 1) <= 256;
+};
+
 template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF>
-__device__ __forceinline__ void line_body(ConstArgs& a, const int line, const int split,
-                                          LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>& L)
+__device__ __forceinline__ void line_body(
+    ConstArgs& a, const int line, const int split,
+    LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF, BatchOrd<P, NG, HALF>::v>& L)
 {
-  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF, false, false, true>(
+  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF, false, BatchOrd<P, NG, HALF>::v, true>(
       a, line, split, L.win, L.geo, L.stage, &L.rstage[0][0], L.rwin, L.tN1, L.tD1, L.tN2, L.tD2, L.tN3,
       L.tD3, L.be2, L.be3, L.eqw, L.pbase, L.prow, &L.nbundle);
 }
@@ -1146,12 +1155,12 @@ __global__ __launch_bounds__(kBlock) void k_assemble_lines(const AsmArgs* __rest
 // (AsmArgs::variant = 2*(p-1) + half); the LDS is one union of the variants.
 template <typename T, bool HAS_RHS>
 union BatchLds {
-  LineLds<1, 2, 1, T, 1, HAS_RHS, 0> v0;
-  LineLds<1, 2, 1, T, 1, HAS_RHS, 1> v1;
-  LineLds<2, 3, 1, T, 1, HAS_RHS, 0> v2;
-  LineLds<2, 3, 1, T, 1, HAS_RHS, 1> v3;
-  LineLds<3, 4, 1, T, 1, HAS_RHS, 0> v4;
-  LineLds<3, 4, 1, T, 1, HAS_RHS, 1> v5;
+  LineLds<1, 2, 1, T, 1, HAS_RHS, 0, BatchOrd<1, 2, 0>::v> v0;
+  LineLds<1, 2, 1, T, 1, HAS_RHS, 1, BatchOrd<1, 2, 1>::v> v1;
+  LineLds<2, 3, 1, T, 1, HAS_RHS, 0, BatchOrd<2, 3, 0>::v> v2;
+  LineLds<2, 3, 1, T, 1, HAS_RHS, 1, BatchOrd<2, 3, 1>::v> v3;
+  LineLds<3, 4, 1, T, 1, HAS_RHS, 0, BatchOrd<3, 4, 0>::v> v4;
+  LineLds<3, 4, 1, T, 1, HAS_RHS, 1, BatchOrd<3, 4, 1>::v> v5;
 };
 
 template <typename T, bool HAS_RHS>

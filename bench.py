@@ -63,6 +63,13 @@ def parse():
     ap.add_argument("--no-mg", action="store_true", help="skip the multigrid-preconditioned leg")
     ap.add_argument("--no-multipatch", action="store_true",
                     help="skip the two-patch assembly leg")
+    ap.add_argument("--learned", action="store_true",
+                    help="run the learned-preconditioner leg (48^3 heterogeneous diffusion: "
+                         "train on PCG pairs, compare with Jacobi-PCG on held-out seeds)")
+    ap.add_argument("--learned-train-seeds", type=int, default=20)
+    ap.add_argument("--learned-pairs-per-seed", type=int, default=100)
+    ap.add_argument("--learned-eval-seeds", type=int, default=20)
+    ap.add_argument("--learned-epochs", type=int, default=4)
     ap.add_argument("--no-large", action="store_true",
                     help="skip the 96^3 p=3 elasticity leg (largest single-GPU case)")
     ap.add_argument("--no-batch", action="store_true",
@@ -264,6 +271,65 @@ def main():
         os_.close()
         log(f"[bench] multipatch: {multipatch}")
 
+    # ---- learned preconditioner: 48^3 p=2 heterogeneous diffusion -------------
+    learned_leg = None
+    if args.learned:
+        from ifem_amd import learned as LN, setup_problem
+        mkd = lambda seed: problems.diffusion_cube(48, 2, sigma=1.5, ell=0.1, seed=seed)
+        spec = LN.NetSpec(levels=3, channels=(8, 16, 48), seed=5)
+        t0 = time.perf_counter()
+        X, T = [], []
+        for seed in range(1000, 1000 + args.learned_train_seeds):
+            pbd = mkd(seed)
+            with setup_problem(pbd) as sd:
+                ins, tgs = LN.pcg_pairs(sd, pbd, LN.default_iters(args.learned_pairs_per_seed, 300))
+            X += ins
+            T += tgs
+        X = torch.stack(X).contiguous()
+        T = torch.stack(T).contiguous()
+        pairs_s = time.perf_counter() - t0
+        npairs, cin, cout = int(X.shape[0]), int(X.shape[1]), int(T.shape[1])
+        tscale = LN.target_scale(T)
+        T = (T / tscale).contiguous()
+        tr = Session()
+        tr.unet_create(spec.levels, spec.channels, X.shape[1], T.shape[1], seed=spec.seed)
+        t0 = time.perf_counter()
+        losses = LN.train(tr, X, T, epochs=args.learned_epochs, batch=8, lr=5e-4,
+                          weights=LN.sample_weights(T))
+        train_s = time.perf_counter() - t0
+        params = tr.unet_params()
+        nparam = int(params.size)
+        tr.close()
+        del X, T
+        evals = []
+        for seed in range(5000, 5000 + args.learned_eval_seeds):
+            pbd = mkd(seed)
+            with setup_problem(pbd) as sd:
+                sj = sd.solve(rtol=args.rtol, maxits=200000, check_every=64)
+                sd.unet_create(spec.levels, spec.channels, cin, cout)
+                sd.unet_set_params(params)
+                LN.enable(sd, pbd, scale=tscale)
+                sl = sd.solve(rtol=args.rtol, maxits=200000, check_every=4)
+            evals.append((sj["iterations"], sj["solve_ms"], sl["iterations"], sl["solve_ms"],
+                          sl["converged"] and sl["rel_residual"] <= args.rtol))
+        ev = np.array([e[:4] for e in evals], dtype=float)
+        learned_leg = {
+            "workload": "diffusion 48^3 p=2, log-normal kappa (sigma 1.5, length 0.1), smooth "
+                        "source, u=0; U-net levels 3 channels (8,16,48), FP32",
+            "train_pairs": npairs,
+            "train_seeds": args.learned_train_seeds, "pairs_generation_s": round(pairs_s, 1),
+            "epochs": args.learned_epochs, "train_s": round(train_s, 1),
+            "loss_first_last": [losses[0], losses[-1]], "parameters": nparam,
+            "eval_seeds": args.learned_eval_seeds,
+            "jacobi_iterations_median": float(np.median(ev[:, 0])),
+            "learned_iterations_median": float(np.median(ev[:, 2])),
+            "iteration_ratio_median": float(np.median(ev[:, 0] / ev[:, 2])),
+            "jacobi_solve_ms_median": round(float(np.median(ev[:, 1])), 3),
+            "learned_solve_ms_median": round(float(np.median(ev[:, 3])), 3),
+            "all_learned_converged": bool(all(e[4] for e in evals)),
+        }
+        log(f"[bench] learned: {learned_leg}")
+
     # ---- largest single-GPU case: 96^3 p=3 elasticity, 4x4x4 Gauss points -----
     large = None
     if not args.no_large:
@@ -460,6 +526,7 @@ def main():
         "batch": batch,
         "multipatch": multipatch,
         "large": large,
+        "learned": learned_leg,
     }
     s.close()
     print(json.dumps(out), flush=True)

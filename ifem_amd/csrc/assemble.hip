@@ -731,6 +731,63 @@ __device__ __forceinline__ void line_impl(
       }
       // rounds over the bundle elements: in round r the items of element r
       // add their column sums (distinct window entries within a round)
+      // p <= 2 (STAGED): the geometry rows are no longer needed once the sums
+      // are in registers; their LDS takes the sums of GS elements at a time,
+      // which are then reduced per window entry in fixed element order (as
+      // phase C below) and added to the window, group after group.  p = 3:
+      // element rounds (measured faster there).
+      constexpr bool STAGED = P <= 2;
+      if constexpr (STAGED) {
+      constexpr int GS0 = (CH * GP3 * LL::GEO) / (NEN * P1 * NS);
+      constexpr int GS = GS0 < 1 ? 1 : (GS0 > CH ? CH : GS0);
+      T (*stg)[NEN][P1][NS] = reinterpret_cast<T (*)[NEN][P1][NS]>(&geo[0][0]);
+      for (int g0 = 0; g0 < nch; g0 += GS) {
+        const int g1 = g0 + GS < nch ? g0 + GS : nch;
+        __syncthreads(); // geometry reads (or the previous group) done
+        if (has && le >= g0 && le < g1) {
+#pragma unroll
+          for (int a1 = 0; a1 < P1; a1++)
+#pragma unroll
+            for (int s = 0; s < NS; s++) stg[le - g0][b][a1][s] = acc[a1][s];
+        }
+        __syncthreads();
+        for (int t = tid; t < P1 * P1 * W1 * W1; t += kBlock) {
+          const int a1 = t % P1, r = t / P1;
+          const int cb1 = r % P1, r2 = r / P1;
+          const int cjr = r2 % W1, ckr = r2 / W1;
+          const int cjj = jc - P + cjr, ckk = kc - P + ckr;
+          T sm[NS];
+#pragma unroll
+          for (int m = 0; m < NS; m++) sm[m] = 0;
+          bool any = false;
+          for (int l = g0; l < g1; l++) {
+            const int cb2 = cjj - be2[c0 + l], cb3 = ckk - be3[c0 + l];
+            if (cb2 < 0 || cb2 > P || cb3 < 0 || cb3 > P) continue;
+            any = true;
+            const int cb = cb1 + P1 * (cb2 + P1 * cb3);
+#pragma unroll
+            for (int m = 0; m < NS; m++) sm[m] += stg[l - g0][cb][a1][m];
+          }
+          if (!any) continue;
+          const int slot = (e1 + a1) % P1;
+          const int di = cb1 - a1, dj = cjj - jc, dk = ckk - kc;
+          const int nbr = ((dk + P) * W1 + (dj + P)) * W1 + (di + P);
+          if (NC == 1) {
+            win[slot][0][nbr][0] += sm[0];
+          } else if (SACC) {
+            const T tr = sm[0] + sm[4] + sm[8];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+              for (int j = 0; j < 3; j++)
+                win[slot][i][nbr][j] += lam * sm[3 * i + j] + mu * sm[3 * j + i] + (i == j ? mu * tr : T(0));
+          } else {
+#pragma unroll
+            for (int j = 0; j < 3; j++) win[slot][0][nbr][j] += sm[j];
+          }
+        }
+      }
+      } else {
       // Each thread's window entries are distinct, so all of them are read
       // first and written after (one LDS round trip per round, not one per
       // entry).
@@ -780,6 +837,7 @@ __device__ __forceinline__ void line_impl(
             for (int v = 0; v < NV; v++)
               wp[a1][(NV == 9 ? v / 3 : 0) * ISTR + (NV == 9 ? v % 3 : v)] = cur[a1][v] + add[a1][v];
         }
+      }
       }
       } else if constexpr (SEMI) {
       // Larger Gauss rules: loop over the (q2, q3) rows with the q1 row

@@ -422,9 +422,11 @@ struct LineLds {
   // neighbour equation numbers kept in LDS for phase D (not for p = 3,
   // whose LDS budget goes to the element chunk)
   static constexpr bool PFD = P < 3;
-  alignas(16) int eqw[PFD ? NT * 64 : 1];
-  alignas(16) int64_t pbase[NCR];
-  int prow[NCR];
+  // two buffers: row e1's (being filled) and row e1-1's (written out at the
+  // top of layer e1)
+  alignas(16) int eqw[2][PFD ? NT * 64 : 1];
+  alignas(16) int64_t pbase[2][NCR];
+  int prow[2][NCR];
   int nbundle;
 };
 
@@ -446,8 +448,11 @@ __device__ __forceinline__ void line_impl(
     T (*__restrict__ tN2)[NG][P + 1], T (*__restrict__ tD2)[NG][P + 1],
     T (*__restrict__ tN3)[NG][P + 1], T (*__restrict__ tD3)[NG][P + 1],
     int* __restrict__ be2, int* __restrict__ be3,
-    int* __restrict__ eqw,
-    int64_t* __restrict__ pbase, int* __restrict__ prow, int* __restrict__ nbundlep)
+    int (*__restrict__ eqw)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::PFD
+                            ? LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NT * 64 : 1],
+    int64_t (*__restrict__ pbase)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NCR],
+    int (*__restrict__ prow)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NCR],
+    int* __restrict__ nbundlep)
 {
   using LL = LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>;
   constexpr int P1 = LL::P1;
@@ -510,6 +515,76 @@ __device__ __forceinline__ void line_impl(
   const T lam = (T)a.lam, mu = (T)a.mu;
 
   constexpr int NT = LL::NT; // phase D: 64-column steps per row
+  // ---- phase D: write row i of the window out (row e1-1 at the top of layer
+  // e1, while phase A runs; the last layer's rows at its end).  pf: the
+  // equation numbers of the row are in buffer pb (from the layer top);
+  // zero_rhs: clear the row's right-hand side entries after reading them.
+  auto row_out = [&](int i, bool pf, int pb, bool zero_rhs) {
+    const int slot = i % P1;
+    const int64_t node = (int64_t)i + (int64_t)a.n1 * (jc + (int64_t)a.n2 * kc);
+    const int wave = tid >> 6, lane = tid & 63;
+    if (wave < NCR) {
+      const int ic = split * NCR + wave;
+      const int req = pf ? prow[pb][wave] : a.eq[node * NC + ic];
+      const bool pfe = LL::PFD && pf;
+      if (req >= 0) {
+        const int64_t base = pf ? pbase[pb][wave] : a.rowptr[req];
+        const double* __restrict__ gv = a.gval;
+        double corr = 0.0;
+        int64_t pos = 0;
+#pragma unroll 1
+        for (int k = 0; k < NT; k++) {
+          const int t = k * 64 + lane;
+          const int ev = pfe ? eqw[pb][t] : window_eq<P, NC>(a, i, jc, kc, t);
+          const bool valid = ev >= 0;
+          const unsigned long long m = __ballot(valid);
+          if (valid && a.accum && a.pmap) {
+            const T v = win[slot][wave][t / NC][t % NC];
+            const int64_t at = a.pmap[a.pptr[node * NC + ic] + pos + __popcll(m & ((1ull << lane) - 1ull))];
+            if (a.val32) a.val32[at] += (float)v;
+            else a.val64[at] += (double)v;
+          } else if (valid && a.accum) {
+            const T v = win[slot][wave][t / NC][t % NC];
+            int64_t lo = base, hi = a.rowptr[req + 1] - 1;
+            while (lo <= hi) {
+              const int64_t mid = (lo + hi) >> 1;
+              const int cm = a.colind[mid];
+              if (cm == ev) {
+                if (a.val32) a.val32[mid] += (float)v;
+                else a.val64[mid] += (double)v;
+                break;
+              }
+              if (cm < ev) lo = mid + 1;
+              else hi = mid - 1;
+            }
+          } else if (valid) {
+            const int64_t at = base + pos + __popcll(m & ((1ull << lane) - 1ull));
+            const T v = win[slot][wave][t / NC][t % NC];
+            if (a.val32) a.val32[at] = (float)v;
+            else a.val64[at] = (double)v;
+          } else if (gv && ev == -1) { // eliminated column inside the patch
+            const int nbr = t / NC;
+            const int ii = i + nbr % W1 - P, jj = jc + (nbr / W1) % W1 - P, kk = kc + nbr / (W1 * W1) - P;
+            const int64_t gdof = ((int64_t)ii + (int64_t)a.n1 * (jj + (int64_t)a.n2 * kk)) * NC + t % NC;
+            corr += (double)win[slot][wave][t / NC][t % NC] * gv[gdof];
+          }
+          pos += __popcll(m);
+        }
+        if (gv) {
+#pragma unroll
+          for (int off = 32; off >= 1; off >>= 1) corr += __shfl_xor(corr, off, 64);
+        }
+        if ((HAS_RHS || gv) && a.rhs && lane == 0) {
+          if (a.accum) a.rhs[req] += (double)rwin[slot][ic] - corr;
+          else a.rhs[req] = (double)rwin[slot][ic] - corr;
+        }
+      }
+      if (zero_rhs && lane == 0) rwin[slot][ic] = T(0);
+    }
+    if (zero_rhs && tid == 0)
+      for (int c = 0; c < NC; c++)
+        if (c < split * NCR || c >= split * NCR + NCR) rwin[slot][c] = T(0);
+  };
   for (int e1 = 0; e1 < a.nel1; e1++) {
     // ---- direction-1 tables of this layer ----------------------------------
     // with the equation numbers of row e1 and of its neighbour columns for
@@ -518,16 +593,21 @@ __device__ __forceinline__ void line_impl(
       (&tN1[0][0])[t] = (T)a.tabN1[e1 * NG * P1 + t];
       (&tD1[0][0])[t] = (T)a.tabD1[e1 * NG * P1 + t];
     }
+    const int buf = e1 & 1;
     if (LL::PFD)
-      for (int t = tid; t < NT * 64; t += kBlock) eqw[t] = window_eq<P, NC>(a, e1, jc, kc, t);
+      for (int t = tid; t < NT * 64; t += kBlock) eqw[buf][t] = window_eq<P, NC>(a, e1, jc, kc, t);
     if (tid < NCR) {
       const int64_t node = (int64_t)e1 + (int64_t)a.n1 * (jc + (int64_t)a.n2 * kc);
-      prow[tid] = a.eq[node * NC + split * NCR + tid];
+      prow[buf][tid] = a.eq[node * NC + split * NCR + tid];
     }
     __syncthreads();
     // row start of row e1, read while phase A runs
     int64_t pf_base = 0;
-    if (tid < NCR && prow[tid] >= 0) pf_base = a.rowptr[prow[tid]];
+    if (tid < NCR && prow[buf][tid] >= 0) pf_base = a.rowptr[prow[buf][tid]];
+    // row e1-1, complete since the last layer, goes out now: its loads and
+    // stores overlap phase A (its window slot is cleared after phase A, and
+    // its right-hand side entries by the threads that read them)
+    if (e1 > 0) row_out(e1 - 1, true, buf ^ 1, true);
 
     for (int c0 = 0; c0 < nb; c0 += CH) {
       const int nch = (nb - c0) < CH ? (nb - c0) : CH;
@@ -582,8 +662,12 @@ __device__ __forceinline__ void line_impl(
           }
         }
       }
-      if (c0 == 0 && tid < NCR) pbase[tid] = pf_base;
+      if (c0 == 0 && tid < NCR) pbase[buf][tid] = pf_base;
       __syncthreads();
+      if (c0 == 0 && e1 > 0) { // slot of row e1-1, read out before the barrier
+        const int zs = (e1 - 1) % P1;
+        for (int t = tid; t < NCR * WN * NC; t += kBlock) (&win[zs][0][0][0])[t] = T(0);
+      }
 
       // ---- RHS: fixed-order reduction over (element, GP) -------------------
       if (HAS_RHS) {
@@ -1086,74 +1170,16 @@ __device__ __forceinline__ void line_impl(
       __syncthreads();
     }
 
-    // ---- phase D: rows that are complete after layer e1 ---------------------
-    const int ilast = (e1 == a.nel1 - 1) ? e1 + P : e1;
-    for (int i = e1; i <= ilast; i++) {
-      const int slot = i % P1;
-      const int64_t node = (int64_t)i + (int64_t)a.n1 * (jc + (int64_t)a.n2 * kc);
-      const int wave = tid >> 6, lane = tid & 63;
-      if (wave < NCR) {
-        const int ic = split * NCR + wave;
-        const bool pf = i == e1;
-        const int req = pf ? prow[wave] : a.eq[node * NC + ic];
-        const bool pfe = LL::PFD && pf;
-        if (req >= 0) {
-          const int64_t base = pf ? pbase[wave] : a.rowptr[req];
-          const double* __restrict__ gv = a.gval;
-          double corr = 0.0;
-          int64_t pos = 0;
-#pragma unroll 1
-          for (int k = 0; k < NT; k++) {
-            const int t = k * 64 + lane;
-            const int ev = pfe ? eqw[t] : window_eq<P, NC>(a, i, jc, kc, t);
-            const bool valid = ev >= 0;
-            const unsigned long long m = __ballot(valid);
-            if (valid && a.accum && a.pmap) {
-              const T v = win[slot][wave][t / NC][t % NC];
-              const int64_t at = a.pmap[a.pptr[node * NC + ic] + pos + __popcll(m & ((1ull << lane) - 1ull))];
-              if (a.val32) a.val32[at] += (float)v;
-              else a.val64[at] += (double)v;
-            } else if (valid && a.accum) {
-              const T v = win[slot][wave][t / NC][t % NC];
-              int64_t lo = base, hi = a.rowptr[req + 1] - 1;
-              while (lo <= hi) {
-                const int64_t mid = (lo + hi) >> 1;
-                const int cm = a.colind[mid];
-                if (cm == ev) {
-                  if (a.val32) a.val32[mid] += (float)v;
-                  else a.val64[mid] += (double)v;
-                  break;
-                }
-                if (cm < ev) lo = mid + 1;
-                else hi = mid - 1;
-              }
-            } else if (valid) {
-              const int64_t at = base + pos + __popcll(m & ((1ull << lane) - 1ull));
-              const T v = win[slot][wave][t / NC][t % NC];
-              if (a.val32) a.val32[at] = (float)v;
-              else a.val64[at] = (double)v;
-            } else if (gv && ev == -1) { // eliminated column inside the patch
-              const int nbr = t / NC;
-              const int ii = i + nbr % W1 - P, jj = jc + (nbr / W1) % W1 - P, kk = kc + nbr / (W1 * W1) - P;
-              const int64_t gdof = ((int64_t)ii + (int64_t)a.n1 * (jj + (int64_t)a.n2 * kk)) * NC + t % NC;
-              corr += (double)win[slot][wave][t / NC][t % NC] * gv[gdof];
-            }
-            pos += __popcll(m);
-          }
-          if (gv) {
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) corr += __shfl_xor(corr, off, 64);
-          }
-          if ((HAS_RHS || gv) && a.rhs && lane == 0) {
-            if (a.accum) a.rhs[req] += (double)rwin[slot][ic] - corr;
-            else a.rhs[req] = (double)rwin[slot][ic] - corr;
-          }
-        }
+    // ---- phase D (last layer): rows e1 .. e1+p are complete ------------------
+    if (e1 == a.nel1 - 1) {
+      for (int i = e1; i <= e1 + P; i++) {
+        const int slot = i % P1;
+        row_out(i, i == e1, buf, false);
+        __syncthreads();
+        for (int t = tid; t < NCR * WN * NC; t += kBlock) (&win[slot][0][0][0])[t] = T(0);
+        if (tid < NC) rwin[slot][tid] = T(0);
+        __syncthreads();
       }
-      __syncthreads();
-      for (int t = tid; t < NCR * WN * NC; t += kBlock) (&win[slot][0][0][0])[t] = T(0);
-      if (tid < NC) rwin[slot][tid] = T(0);
-      __syncthreads();
     }
   }
 }
@@ -1195,9 +1221,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble_lines(const AsmArgs* __rest
   __shared__ T tN2[P1][NG][P1], tD2[P1][NG][P1];
   __shared__ T tN3[P1][NG][P1], tD3[P1][NG][P1];
   __shared__ int be2[LL::NB], be3[LL::NB];
-  __shared__ int eqw[LL::PFD ? LL::NT * 64 : 1];
-  __shared__ int64_t pbase[LL::NCR];
-  __shared__ int prow[LL::NCR];
+  __shared__ int eqw[2][LL::PFD ? LL::NT * 64 : 1];
+  __shared__ int64_t pbase[2][LL::NCR];
+  __shared__ int prow[2][LL::NCR];
   __shared__ int nbundle;
   line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF, PIPE, ORD, !PIPE>(a, blockIdx.x, blockIdx.y, win, geo, stage,
                                                       rstage, rwin, tN1, tD1, tN2, tD2, tN3, tD3,

@@ -797,50 +797,104 @@ __device__ __forceinline__ void line_impl(
         T Gc[LL::GEO];
 #pragma unroll
         for (int m = 0; m < LL::GEO; m++) Gc[m] = Gbase[m];
-        for (int g23 = 0; g23 < NG * NG; g23++) {
-          const int q2 = g23 % NG, q3 = g23 / NG;
-          const T n3 = tN3[m3][q3][b3], d3 = tD3[m3][q3][b3];
-          const T n2 = tN2[m2][q2][b2], d2 = tD2[m2][q2][b2];
-          const T n23 = n2 * n3, d23 = d2 * n3, n2d3 = n2 * d3;
+        // fully unrolled up to 4x4 rows: the loads of later points can then be
+        // issued well ahead of their use (not for scalar FP64 problems, where
+        // the extra registers cost a wave per SIMD and the time); the two
+        // copies differ only in the unroll directive
+        if constexpr (NG * NG <= 16 && !(NC == 1 && sizeof(T) == 8)) {
 #pragma unroll
-          for (int q1 = 0; q1 < NG; q1++) {
-            const int qn = g23 * NG + q1 + 1 < GP3 ? g23 * NG + q1 + 1 : GP3 - 1;
-            T Gn[LL::GEO];
+          for (int g23 = 0; g23 < NG * NG; g23++) {
+            const int q2 = g23 % NG, q3 = g23 / NG;
+            const T n3 = tN3[m3][q3][b3], d3 = tD3[m3][q3][b3];
+            const T n2 = tN2[m2][q2][b2], d2 = tD2[m2][q2][b2];
+            const T n23 = n2 * n3, d23 = d2 * n3, n2d3 = n2 * d3;
 #pragma unroll
-            for (int m = 0; m < LL::GEO; m++) Gn[m] = Gbase[qn * LL::GEO + m];
-            const T r0 = vd1[q1] * n23, r1 = vn1[q1] * d23, r2 = vn1[q1] * n2d3;
-            const T* G = Gc;
-            if (NC == 1) {
+            for (int q1 = 0; q1 < NG; q1++) {
+              const int qn = g23 * NG + q1 + 1 < GP3 ? g23 * NG + q1 + 1 : GP3 - 1;
+              T Gn[LL::GEO];
 #pragma unroll
-              for (int a1 = 0; a1 < P1; a1++)
-                acc[a1][0] += r0 * G[3 * a1] + r1 * G[3 * a1 + 1] + r2 * G[3 * a1 + 2];
-            } else {
-              const T gb0 = r0 * G[0] + r1 * G[3] + r2 * G[6];
-              const T gb1 = r0 * G[1] + r1 * G[4] + r2 * G[7];
-              const T gb2 = r0 * G[2] + r1 * G[5] + r2 * G[8];
+              for (int m = 0; m < LL::GEO; m++) Gn[m] = Gbase[qn * LL::GEO + m];
+              const T r0 = vd1[q1] * n23, r1 = vn1[q1] * d23, r2 = vn1[q1] * n2d3;
+              const T* G = Gc;
+              if (NC == 1) {
 #pragma unroll
-              for (int a1 = 0; a1 < P1; a1++) {
-                const T wa0 = G[9 + 3 * a1], wa1 = G[10 + 3 * a1], wa2 = G[11 + 3 * a1];
-                if constexpr (SACC) {
-                  acc[a1][0] += wa0 * gb0; acc[a1][1] += wa0 * gb1; acc[a1][2] += wa0 * gb2;
-                  acc[a1][3] += wa1 * gb0; acc[a1][4] += wa1 * gb1; acc[a1][5] += wa1 * gb2;
-                  acc[a1][6] += wa2 * gb0; acc[a1][7] += wa2 * gb1; acc[a1][8] += wa2 * gb2;
-                } else if constexpr (PERM) {
-                  acc[a1][0] += wa0 * gb0; acc[a1][1] += wa0 * gb1; acc[a1][2] += wa0 * gb2;
-                  acc[a1][3] += wa1 * gb0; acc[a1][4] += wa2 * gb0;
-                  acc[a1][5] += wa1 * gb1; acc[a1][6] += wa2 * gb2;
-                } else {
-                  const T wa[3] = {wa0, wa1, wa2}, gb[3] = {gb0, gb1, gb2};
-                  const T dot = wa0 * gb0 + wa1 * gb1 + wa2 * gb2;
+                for (int a1 = 0; a1 < P1; a1++)
+                  acc[a1][0] += r0 * G[3 * a1] + r1 * G[3 * a1 + 1] + r2 * G[3 * a1 + 2];
+              } else {
+                const T gb0 = r0 * G[0] + r1 * G[3] + r2 * G[6];
+                const T gb1 = r0 * G[1] + r1 * G[4] + r2 * G[7];
+                const T gb2 = r0 * G[2] + r1 * G[5] + r2 * G[8];
 #pragma unroll
-                  for (int jj = 0; jj < 3; jj++)
-                    acc[a1][jj] += lam * wa[rc] * gb[jj] + mu * wa[jj] * gb[rc] +
-                                   (jj == rc ? mu * dot : T(0));
+                for (int a1 = 0; a1 < P1; a1++) {
+                  const T wa0 = G[9 + 3 * a1], wa1 = G[10 + 3 * a1], wa2 = G[11 + 3 * a1];
+                  if constexpr (SACC) {
+                    acc[a1][0] += wa0 * gb0; acc[a1][1] += wa0 * gb1; acc[a1][2] += wa0 * gb2;
+                    acc[a1][3] += wa1 * gb0; acc[a1][4] += wa1 * gb1; acc[a1][5] += wa1 * gb2;
+                    acc[a1][6] += wa2 * gb0; acc[a1][7] += wa2 * gb1; acc[a1][8] += wa2 * gb2;
+                  } else if constexpr (PERM) {
+                    acc[a1][0] += wa0 * gb0; acc[a1][1] += wa0 * gb1; acc[a1][2] += wa0 * gb2;
+                    acc[a1][3] += wa1 * gb0; acc[a1][4] += wa2 * gb0;
+                    acc[a1][5] += wa1 * gb1; acc[a1][6] += wa2 * gb2;
+                  } else {
+                    const T wa[3] = {wa0, wa1, wa2}, gb[3] = {gb0, gb1, gb2};
+                    const T dot = wa0 * gb0 + wa1 * gb1 + wa2 * gb2;
+#pragma unroll
+                    for (int jj = 0; jj < 3; jj++)
+                      acc[a1][jj] += lam * wa[rc] * gb[jj] + mu * wa[jj] * gb[rc] +
+                                     (jj == rc ? mu * dot : T(0));
+                  }
                 }
               }
-            }
 #pragma unroll
-            for (int m = 0; m < LL::GEO; m++) Gc[m] = Gn[m];
+              for (int m = 0; m < LL::GEO; m++) Gc[m] = Gn[m];
+            }
+          }
+        } else {
+          for (int g23 = 0; g23 < NG * NG; g23++) {
+            const int q2 = g23 % NG, q3 = g23 / NG;
+            const T n3 = tN3[m3][q3][b3], d3 = tD3[m3][q3][b3];
+            const T n2 = tN2[m2][q2][b2], d2 = tD2[m2][q2][b2];
+            const T n23 = n2 * n3, d23 = d2 * n3, n2d3 = n2 * d3;
+#pragma unroll
+            for (int q1 = 0; q1 < NG; q1++) {
+              const int qn = g23 * NG + q1 + 1 < GP3 ? g23 * NG + q1 + 1 : GP3 - 1;
+              T Gn[LL::GEO];
+#pragma unroll
+              for (int m = 0; m < LL::GEO; m++) Gn[m] = Gbase[qn * LL::GEO + m];
+              const T r0 = vd1[q1] * n23, r1 = vn1[q1] * d23, r2 = vn1[q1] * n2d3;
+              const T* G = Gc;
+              if (NC == 1) {
+#pragma unroll
+                for (int a1 = 0; a1 < P1; a1++)
+                  acc[a1][0] += r0 * G[3 * a1] + r1 * G[3 * a1 + 1] + r2 * G[3 * a1 + 2];
+              } else {
+                const T gb0 = r0 * G[0] + r1 * G[3] + r2 * G[6];
+                const T gb1 = r0 * G[1] + r1 * G[4] + r2 * G[7];
+                const T gb2 = r0 * G[2] + r1 * G[5] + r2 * G[8];
+#pragma unroll
+                for (int a1 = 0; a1 < P1; a1++) {
+                  const T wa0 = G[9 + 3 * a1], wa1 = G[10 + 3 * a1], wa2 = G[11 + 3 * a1];
+                  if constexpr (SACC) {
+                    acc[a1][0] += wa0 * gb0; acc[a1][1] += wa0 * gb1; acc[a1][2] += wa0 * gb2;
+                    acc[a1][3] += wa1 * gb0; acc[a1][4] += wa1 * gb1; acc[a1][5] += wa1 * gb2;
+                    acc[a1][6] += wa2 * gb0; acc[a1][7] += wa2 * gb1; acc[a1][8] += wa2 * gb2;
+                  } else if constexpr (PERM) {
+                    acc[a1][0] += wa0 * gb0; acc[a1][1] += wa0 * gb1; acc[a1][2] += wa0 * gb2;
+                    acc[a1][3] += wa1 * gb0; acc[a1][4] += wa2 * gb0;
+                    acc[a1][5] += wa1 * gb1; acc[a1][6] += wa2 * gb2;
+                  } else {
+                    const T wa[3] = {wa0, wa1, wa2}, gb[3] = {gb0, gb1, gb2};
+                    const T dot = wa0 * gb0 + wa1 * gb1 + wa2 * gb2;
+#pragma unroll
+                    for (int jj = 0; jj < 3; jj++)
+                      acc[a1][jj] += lam * wa[rc] * gb[jj] + mu * wa[jj] * gb[rc] +
+                                     (jj == rc ? mu * dot : T(0));
+                  }
+                }
+              }
+#pragma unroll
+              for (int m = 0; m < LL::GEO; m++) Gc[m] = Gn[m];
+            }
           }
         }
         }

@@ -469,14 +469,7 @@ __device__ __forceinline__ void line_impl(
   // the body force is applied after the sum
   static_assert(!ORD || CH * LL::NEN <= kBlock, "ordered updates need one item per thread");
   constexpr int RS = ORD ? (HAS_RHS ? P1 : 1) : LL::NRHS; // RHS stage row length
-  // PERM (ordered updates, one row component c per block): phase A stores the
-  // columns of J^-1 and the weighted row gradients rotated to the order
-  // (c, c+1, c+2) mod 3, so that phase B accumulates at fixed positions the
-  // seven sums of S = sum w g_a g_b^T that row c of the block needs:
-  // S[c][c..c+2], S[c+1][c], S[c+2][c], S[c+1][c+1], S[c+2][c+2]
-  constexpr bool PERM = ORD && NC == 3 && !SACC && GP3 <= 64; // (5^3 points: register bound)
-  constexpr int NA = PERM ? 7 : NS; // phase-B sums per row function
-  const int rc = split * NCR;       // PERM: the block's row component
+  const int rc = split * NCR; // first row component of the block
 
   const int tid = threadIdx.x;
   const int jc = line % a.n2, kc = line / a.n2;
@@ -634,15 +627,7 @@ __device__ __forceinline__ void line_impl(
         T R[GeoRec<NC>::N];
 #pragma unroll
         for (int m = 0; m < GeoRec<NC>::N; m++) R[m] = (T)rec[m];
-        if constexpr (PERM) {
-#pragma unroll
-          for (int k = 0; k < 3; k++) {
-            const T c0 = R[3 * k], c1 = R[3 * k + 1], c2 = R[3 * k + 2];
-            geo[t][3 * k + 0] = rc == 0 ? c0 : (rc == 1 ? c1 : c2);
-            geo[t][3 * k + 1] = rc == 0 ? c1 : (rc == 1 ? c2 : c0);
-            geo[t][3 * k + 2] = rc == 0 ? c2 : (rc == 1 ? c0 : c1);
-          }
-        } else if (NC == 3) {
+        if (NC == 3) {
 #pragma unroll
           for (int m = 0; m < 9; m++) geo[t][m] = R[m];
         }
@@ -661,18 +646,9 @@ __device__ __forceinline__ void line_impl(
           } else {
             // w g_a = w J^-T grad_ref N_a  (dNdX = dNdu * J^-1)
             const T w = R[9];
-            const T g0 = w * (r0 * R[0] + r1 * R[3] + r2 * R[6]);
-            const T g1 = w * (r0 * R[1] + r1 * R[4] + r2 * R[7]);
-            const T g2 = w * (r0 * R[2] + r1 * R[5] + r2 * R[8]);
-            if constexpr (PERM) {
-              geo[t][9 + 3 * a1 + 0] = rc == 0 ? g0 : (rc == 1 ? g1 : g2);
-              geo[t][9 + 3 * a1 + 1] = rc == 0 ? g1 : (rc == 1 ? g2 : g0);
-              geo[t][9 + 3 * a1 + 2] = rc == 0 ? g2 : (rc == 1 ? g0 : g1);
-            } else {
-              geo[t][9 + 3 * a1 + 0] = g0;
-              geo[t][9 + 3 * a1 + 1] = g1;
-              geo[t][9 + 3 * a1 + 2] = g2;
-            }
+            geo[t][9 + 3 * a1 + 0] = w * (r0 * R[0] + r1 * R[3] + r2 * R[6]);
+            geo[t][9 + 3 * a1 + 1] = w * (r0 * R[1] + r1 * R[4] + r2 * R[7]);
+            geo[t][9 + 3 * a1 + 2] = w * (r0 * R[2] + r1 * R[5] + r2 * R[8]);
           }
           if (HAS_RHS) {
             const T Na = n1 * n2a * n3a;
@@ -720,11 +696,11 @@ __device__ __forceinline__ void line_impl(
       const int le = has ? tid / NEN : 0, b = has ? tid % NEN : 0;
       const int b1 = b % P1, b2 = (b / P1) % P1, b3 = b / (P1 * P1);
       const int e2 = be2[c0 + le], e3 = be3[c0 + le];
-      T acc[P1][NA];
+      T acc[P1][NS];
 #pragma unroll
       for (int a1 = 0; a1 < P1; a1++)
 #pragma unroll
-        for (int s = 0; s < NA; s++) acc[a1][s] = 0;
+        for (int s = 0; s < NS; s++) acc[a1][s] = 0;
       if (has) {
         if constexpr (PIPE) {
         const int m2 = e2 - jc + P, m3 = e3 - kc + P;
@@ -765,11 +741,6 @@ __device__ __forceinline__ void line_impl(
                 acc[a1][0] += wa0 * gb0; acc[a1][1] += wa0 * gb1; acc[a1][2] += wa0 * gb2;
                 acc[a1][3] += wa1 * gb0; acc[a1][4] += wa1 * gb1; acc[a1][5] += wa1 * gb2;
                 acc[a1][6] += wa2 * gb0; acc[a1][7] += wa2 * gb1; acc[a1][8] += wa2 * gb2;
-              } else if constexpr (PERM) {
-                // rotated order: index 0 is the block's component
-                acc[a1][0] += wa0 * gb0; acc[a1][1] += wa0 * gb1; acc[a1][2] += wa0 * gb2;
-                acc[a1][3] += wa1 * gb0; acc[a1][4] += wa2 * gb0;
-                acc[a1][5] += wa1 * gb1; acc[a1][6] += wa2 * gb2;
               } else {
                 const T wa[3] = {wa0, wa1, wa2}, gb[3] = {gb0, gb1, gb2};
                 const T dot = wa0 * gb0 + wa1 * gb1 + wa2 * gb2;
@@ -831,10 +802,6 @@ __device__ __forceinline__ void line_impl(
                     acc[a1][0] += wa0 * gb0; acc[a1][1] += wa0 * gb1; acc[a1][2] += wa0 * gb2;
                     acc[a1][3] += wa1 * gb0; acc[a1][4] += wa1 * gb1; acc[a1][5] += wa1 * gb2;
                     acc[a1][6] += wa2 * gb0; acc[a1][7] += wa2 * gb1; acc[a1][8] += wa2 * gb2;
-                  } else if constexpr (PERM) {
-                    acc[a1][0] += wa0 * gb0; acc[a1][1] += wa0 * gb1; acc[a1][2] += wa0 * gb2;
-                    acc[a1][3] += wa1 * gb0; acc[a1][4] += wa2 * gb0;
-                    acc[a1][5] += wa1 * gb1; acc[a1][6] += wa2 * gb2;
                   } else {
                     const T wa[3] = {wa0, wa1, wa2}, gb[3] = {gb0, gb1, gb2};
                     const T dot = wa0 * gb0 + wa1 * gb1 + wa2 * gb2;
@@ -878,10 +845,6 @@ __device__ __forceinline__ void line_impl(
                     acc[a1][0] += wa0 * gb0; acc[a1][1] += wa0 * gb1; acc[a1][2] += wa0 * gb2;
                     acc[a1][3] += wa1 * gb0; acc[a1][4] += wa1 * gb1; acc[a1][5] += wa1 * gb2;
                     acc[a1][6] += wa2 * gb0; acc[a1][7] += wa2 * gb1; acc[a1][8] += wa2 * gb2;
-                  } else if constexpr (PERM) {
-                    acc[a1][0] += wa0 * gb0; acc[a1][1] += wa0 * gb1; acc[a1][2] += wa0 * gb2;
-                    acc[a1][3] += wa1 * gb0; acc[a1][4] += wa2 * gb0;
-                    acc[a1][5] += wa1 * gb1; acc[a1][6] += wa2 * gb2;
                   } else {
                     const T wa[3] = {wa0, wa1, wa2}, gb[3] = {gb0, gb1, gb2};
                     const T dot = wa0 * gb0 + wa1 * gb1 + wa2 * gb2;
@@ -984,15 +947,6 @@ __device__ __forceinline__ void line_impl(
 #pragma unroll
             for (int j = 0; j < 3; j++)
               add[a1][3 * i + j] = lam * sv[3 * i + j] + mu * sv[3 * j + i] + (i == j ? mu * tr : T(0));
-        } else if constexpr (PERM) {
-          // rotated sums -> K[c][c], K[c][c+1], K[c][c+2], then natural order
-          const T tr = sv[0] + sv[5] + sv[6];
-          const T k0 = (lam + mu) * sv[0] + mu * tr;
-          const T k1 = lam * sv[1] + mu * sv[3];
-          const T k2 = lam * sv[2] + mu * sv[4];
-          add[a1][0] = rc == 0 ? k0 : (rc == 1 ? k2 : k1);
-          add[a1][1] = rc == 0 ? k1 : (rc == 1 ? k0 : k2);
-          add[a1][2] = rc == 0 ? k2 : (rc == 1 ? k1 : k0);
         } else {
 #pragma unroll
           for (int j = 0; j < 3; j++) add[a1][j] = sv[j];
@@ -1727,7 +1681,12 @@ static int asm_half(Context& c, int p, int ng, int nc, bool fp32)
 template <int P, int NG, int NC, typename T, int HALF>
 static void launch_lines_h(Context& c, const AsmArgs& args, bool has_rhs)
 {
-  constexpr int SPLIT = (NC == 3 && P >= 3) ? 3 : 1;
+  // One block per line holds all three row components where the window fits
+  // the LDS (p = 3 with full chunks of 4x4x4 points: 144 KB in FP64, one
+  // workgroup per CU), measured 1.7-1.9x faster than three blocks that each
+  // recompute the geometry rows and the column gradients; half chunks and
+  // 5x5x5 rules keep three blocks (their window and chunk exceed 160 KB).
+  constexpr int SPLIT = (NC == 3 && P >= 3 && (HALF || NG > 4)) ? 3 : 1;
   dim3 grid(args.n2 * args.n3, SPLIT);
   const AsmArgs* d = static_cast<const AsmArgs*>(c.d_asmargs);
   if (has_rhs)

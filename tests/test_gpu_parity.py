@@ -47,9 +47,9 @@ CASES = [
     ("elast6p2", lambda: problems.elasticity_cube(6, 2, seed=3)),
     ("elast4p1", lambda: problems.elasticity_cube(4, 1, seed=4)),
     ("elast3p3", lambda: problems.elasticity_cube(3, 3, seed=5)),
-    # full 16-element bundles on the interior lines (three row-component blocks)
+    # full 16-element bundles on the interior lines (one block per line)
     ("elast5p3", lambda: problems.elasticity_cube(5, 3, seed=6)),
-    # 5x5x5 Gauss points: the unrotated row-component variant
+    # 5x5x5 Gauss points: three row-component blocks per line
     ("elast3p3ng5", lambda: problems.elasticity_cube(3, 3, ng=5, seed=7)),
     ("diffusion8p2", lambda: problems.diffusion_cube(8, 2, sigma=1.0, ell=0.2, seed=9)),
     ("diffusion4p3", lambda: problems.diffusion_cube(4, 3, sigma=1.0, ell=0.2, seed=10)),

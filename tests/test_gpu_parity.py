@@ -79,7 +79,8 @@ def test_autotuned_assembly_keeps_parity():
     """Both element-chunk variants of the line kernel give the oracle's system
     (the autotuner leaves the faster one assembled)."""
     for pb in (problems.elasticity_cube(6, 2, seed=3), problems.poisson_cube(5, 3),
-               problems.diffusion_cube(6, 2, sigma=1.0, ell=0.2, seed=4)):
+               problems.diffusion_cube(6, 2, sigma=1.0, ell=0.2, seed=4),
+               problems.elasticity_cube(4, 3, seed=9)):
         _say(f"[gpu] autotuned assembly {pb.name}")
         with setup_problem(pb) as s:
             for prec in (FP64, FP32):

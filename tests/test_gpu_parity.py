@@ -39,6 +39,13 @@ def _compare_system(s, pb, tol=1e-12):
     return rp, ci, A, b
 
 
+def _with_body(pb, body=(0.2, -0.1, 0.3)):
+    """The same problem with a body force (the right-hand-side variant of the
+    line kernel)."""
+    import dataclasses
+    return dataclasses.replace(pb, integrand=dataclasses.replace(pb.integrand, body=body))
+
+
 CASES = [
     ("poisson16p2", lambda: problems.poisson_cube(16, 2)),
     ("poisson6p1", lambda: problems.poisson_cube(6, 1)),
@@ -51,6 +58,8 @@ CASES = [
     ("elast5p3", lambda: problems.elasticity_cube(5, 3, seed=6)),
     # 5x5x5 Gauss points: three row-component blocks per line
     ("elast3p3ng5", lambda: problems.elasticity_cube(3, 3, ng=5, seed=7)),
+    ("elast4p3body", lambda: _with_body(problems.elasticity_cube(4, 3, seed=11))),
+    ("elast5p2body", lambda: _with_body(problems.elasticity_cube(5, 2, seed=12))),
     ("diffusion8p2", lambda: problems.diffusion_cube(8, 2, sigma=1.0, ell=0.2, seed=9)),
     ("diffusion4p3", lambda: problems.diffusion_cube(4, 3, sigma=1.0, ell=0.2, seed=10)),
 ]

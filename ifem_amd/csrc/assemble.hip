@@ -769,10 +769,9 @@ __device__ __forceinline__ void line_impl(
 #pragma unroll
         for (int m = 0; m < LL::GEO; m++) Gc[m] = Gbase[m];
         // fully unrolled up to 4x4 rows: the loads of later points can then be
-        // issued well ahead of their use (not for scalar FP64 problems, where
-        // the extra registers cost a wave per SIMD and the time); the two
-        // copies differ only in the unroll directive
-        if constexpr (NG * NG <= 16 && !(NC == 1 && sizeof(T) == 8)) {
+        // issued well ahead of their use; the two copies differ only in the
+        // unroll directive
+        if constexpr (NG * NG <= 16) {
 #pragma unroll
           for (int g23 = 0; g23 < NG * NG; g23++) {
             const int q2 = g23 % NG, q3 = g23 / NG;

@@ -595,7 +595,20 @@ __device__ __forceinline__ void line_impl(
       (&tD1[0][0])[t] = (T)a.tabD1[e1 * NG * P1 + t];
     }
     const int buf = e1 & 1;
-    if (LL::PFD)
+    // equation numbers of row e1's neighbour columns for its phase D (at the
+    // top of the next layer): for elasticity without a body force loaded now
+    // and stored after phase A (EQE), so that their latency is not exposed at
+    // the barrier below; elsewhere the two registers would cost a wave per
+    // SIMD, and they are stored here
+    constexpr bool EQE = LL::PFD && NC == 3 && !HAS_RHS;
+    constexpr int NEQV = EQE ? (NT * 64 + kBlock - 1) / kBlock : 1;
+    int eqv[NEQV];
+#pragma unroll
+    for (int k = 0; k < NEQV; k++) {
+      const int t = tid + k * kBlock;
+      eqv[k] = (EQE && t < NT * 64) ? window_eq<P, NC>(a, e1, jc, kc, t) : -2;
+    }
+    if (LL::PFD && !EQE)
       for (int t = tid; t < NT * 64; t += kBlock) eqw[buf][t] = window_eq<P, NC>(a, e1, jc, kc, t);
     if (tid < NCR) {
       const int64_t node = (int64_t)e1 + (int64_t)a.n1 * (jc + (int64_t)a.n2 * kc);
@@ -665,6 +678,14 @@ __device__ __forceinline__ void line_impl(
       }
       if (c0 == 0 && tid < NCR) pbase[buf][tid] = pf_base;
       __syncthreads();
+      // (ordered updates: after phase B, further from the loads)
+      if (EQE && !ORD && c0 == 0) {
+#pragma unroll
+        for (int k = 0; k < NEQV; k++) {
+          const int t = tid + k * kBlock;
+          if (t < NT * 64) eqw[buf][t] = eqv[k];
+        }
+      }
       if (c0 == 0 && e1 > 0) { // slot of row e1-1, read out before the barrier
         const int zs = (e1 - 1) % P1;
         for (int t = tid; t < NCR * WN * NC; t += kBlock) (&win[zs][0][0][0])[t] = T(0);
@@ -868,6 +889,13 @@ __device__ __forceinline__ void line_impl(
       // which are then reduced per window entry in fixed element order (as
       // phase C below) and added to the window, group after group.  p = 3:
       // element rounds (measured faster there).
+      if (EQE && c0 == 0) {
+#pragma unroll
+        for (int k = 0; k < NEQV; k++) {
+          const int t = tid + k * kBlock;
+          if (t < NT * 64) eqw[buf][t] = eqv[k];
+        }
+      }
       constexpr bool STAGED = P <= 2;
       if constexpr (STAGED) {
       constexpr int GS0 = (CH * GP3 * LL::GEO) / (NEN * P1 * NS);

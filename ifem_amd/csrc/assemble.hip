@@ -597,9 +597,9 @@ __device__ __forceinline__ void line_impl(
     const int buf = e1 & 1;
     // equation numbers of row e1's neighbour columns for its phase D (at the
     // top of the next layer): for elasticity without a body force loaded now
-    // and stored after phase A (EQE), so that their latency is not exposed at
-    // the barrier below; elsewhere the two registers would cost a wave per
-    // SIMD, and they are stored here
+    // and stored after phase B (after phase A in the unordered variants; EQE),
+    // so that their latency is not exposed at the barrier below; elsewhere the
+    // two registers would cost a wave per SIMD, and they are stored here
     constexpr bool EQE = LL::PFD && NC == 3 && !HAS_RHS;
     constexpr int NEQV = EQE ? (NT * 64 + kBlock - 1) / kBlock : 1;
     int eqv[NEQV];

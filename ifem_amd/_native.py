@@ -67,7 +67,8 @@ class UNetConfig(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("outer_iterations", C.c_int), ("converged", C.c_int),
-                ("rel_residual", C.c_double), ("solve_ms", C.c_double)]
+                ("rel_residual", C.c_double), ("solve_ms", C.c_double),
+                ("pc_iterations", C.c_int), ("fallback", C.c_int), ("pc_rejected", C.c_int)]
 
 
 _lib = None
@@ -345,7 +346,8 @@ class Session:
             check(rc)
         return {"iterations": st.iterations, "outer": st.outer_iterations,
                 "converged": bool(st.converged), "rel_residual": st.rel_residual,
-                "solve_ms": st.solve_ms, "rc": rc}
+                "solve_ms": st.solve_ms, "rc": rc, "pc_iterations": st.pc_iterations,
+                "fallback": bool(st.fallback), "pc_rejected": st.pc_rejected}
 
     def system(self, values=True):
         rowptr = np.zeros(self.neq + 1, dtype=np.int64)

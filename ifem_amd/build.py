@@ -40,10 +40,24 @@ def _run(cmd, cwd=ROOT):
 
 
 def build_library() -> str:
+    """One object per source, compiled in parallel (no device symbol crosses
+    a file), then linked into the shared library."""
+    from concurrent.futures import ThreadPoolExecutor
     srcs = sorted(glob.glob(os.path.join(PKG, "csrc", "*.hip")))
+    odir = os.path.join(ROOT, "_build", "obj")
+    os.makedirs(odir, exist_ok=True)
+    inc = "-I" + os.path.join(ROOT, "include")
+
+    def compile_one(src):
+        obj = os.path.join(odir, os.path.basename(src)[:-4] + ".o")
+        _run([HIPCC, *HIP_FLAGS, inc, "-c", "-o", obj, src])
+        return obj
+
+    jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(compile_one, srcs))
     tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-I" + os.path.join(ROOT, "include"),
-          "-o", tmp, *srcs])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     return LIB
 

@@ -1929,10 +1929,9 @@ int assemble_volume(Context& c, const ifem_gpu_integrand& itg, const int ng[3], 
     if (fp32 && !x.d_val32)
       IFEM_HIP(hipMalloc(&x.d_val32, sizeof(float) * (x.nnz > 0 ? x.nnz : 1)));
     SysView xv{x.d_rowptr, x.d_colind, x.d_val, x.d_val32, x.d_rhs, x.nnz, x.ne};
+    // a shallow copy: x.ext owns the device arrays (SubPatch frees nothing)
     std::vector<SubPatch> subs(1, x.ext);
     int rc = assemble_sum(c, xv, subs, itg, ng[0], precision);
-    subs[0].d_leq = nullptr; // owned by x.ext
-    subs[0].d_lgval = nullptr;
     if (rc) return rc;
     rc = mpc_condense(c, fp32, false);
     if (rc) return rc;

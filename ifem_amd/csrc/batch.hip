@@ -602,6 +602,9 @@ int batch_solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* s
     st->converged = nconv == np ? 1 : 0;
     st->rel_residual = maxrel;
     st->solve_ms = ms;
+    st->pc_iterations = 0; // Jacobi only
+    st->fallback = 0;
+    st->pc_rejected = 0;
   }
   if (diverged) return fail(IFEM_GPU_ENOCONV, "batch_solve: a patch diverged (residual above dtol*||b||)");
   if (nconv != np)

@@ -109,12 +109,19 @@ int mpc_build_pattern(Context& c)
   const int neq = c.neq;
   std::vector<int64_t> xrp;
   std::vector<int32_t> xci;
+  // maps of an earlier call on this system are replaced: free them first
+  hipFree(x.ext.d_pmap);
+  hipFree(x.ext.d_pptr);
+  x.ext.d_pmap = nullptr;
+  x.ext.d_pptr = nullptr;
+  // subs[0] is a shallow copy: the device arrays it points at stay owned by
+  // x.ext (mpc_free releases them)
   std::vector<SubPatch> subs(1, x.ext);
   int rc = host_union_pattern(c, subs, x.ne, xrp, xci);
   if (!rc) rc = sub_position_maps(c, subs, xrp, xci);
-  if (rc) return rc;
-  x.ext.d_pmap = subs[0].d_pmap; // owned by x.ext
+  x.ext.d_pmap = subs[0].d_pmap;
   x.ext.d_pptr = subs[0].d_pptr;
+  if (rc) return rc;
   x.nnz = xrp[x.ne];
   // contributions per reduced row
   std::vector<std::vector<std::pair<int, double>>> contrib(neq > 0 ? neq : 1);

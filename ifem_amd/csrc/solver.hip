@@ -775,7 +775,7 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
     check_pc = std::min(check, 8); // few, expensive iterations
   }
   const int pc_kind = use_mg ? PC_MULTIGRID : PC_LEARNED;
-  int total_iters = 0, outer = 0;
+  int total_iters = 0, outer = 0, pc_iters = 0;
   bool use_learned = c.learned || use_mg, fell_back = false;
   double rel = bnorm > 0 ? 1.0 : 0.0;
   bool converged = bnorm == 0.0;
@@ -795,6 +795,7 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
                                      prm.maxits - total_iters, check, &it, &rr, &status);
       if (rc) return rc;
       total_iters += it;
+      if (use_learned) pc_iters += it;
       outer++;
       double trr = 0.0;
       rc = true_residual(c, w.x, w.r, &trr); // w.r <- b - A x (replacement)
@@ -837,6 +838,7 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
                                    &status);
       if (rc) return rc;
       total_iters += it;
+      if (use_learned) pc_iters += it;
       outer++;
       k_axpy_promote<<<nb, 256, 0, c.stream>>>(n, w.x, w.x32, rn);
       rc = true_residual(c, w.x, w.r, &rr64);
@@ -862,6 +864,9 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
     st->converged = converged ? 1 : 0;
     st->rel_residual = rel;
     st->solve_ms = ms;
+    st->pc_iterations = pc_iters;
+    st->fallback = fell_back ? 1 : 0;
+    st->pc_rejected = 0;
   }
   c.last_fallback = fell_back;
   if (diverged) return fail(IFEM_GPU_ENOCONV, "solve: diverged (residual above dtol*||b||)");

@@ -515,6 +515,14 @@ static GPUMatrix* gpu_matrix(AlgEqSystem& glInt, const char* where)
   return G;
 }
 
+int ASMs3D::getNoGaussPt(int p, bool neumann) const
+{
+  if (nGauss <= -p) return 0;
+  if (nGauss > 0 && nGauss < 11) return nGauss;
+  if (neumann) return p;
+  return p + nGauss % 10;
+}
+
 bool ASMs3D::integrate(Integrand& integrand, AlgEqSystem& glInt, const TimeDomain&)
 {
   GPUMatrix* G = gpu_matrix(glInt, "ASMs3D::integrate");
@@ -524,8 +532,10 @@ bool ASMs3D::integrate(Integrand& integrand, AlgEqSystem& glInt, const TimeDomai
     std::cerr << " *** ASMs3D::integrate: integrand kind not available on the GPU" << std::endl;
     return false;
   }
-  const int ng = nGauss > 0 ? nGauss : order[0];
-  const int ngv[3] = {ng, ng, ng};
+  // per direction from the spline order, as BasisFunctionCache::
+  // setupQuadrature does (src/ASM/ASMs3D.C:4024-4036); the GPU refuses
+  // counts it does not support
+  const int ngv[3] = {getNoGaussPt(order[0]), getNoGaussPt(order[1]), getNoGaussPt(order[2])};
   if (!report(ifem_gpu_assemble(G->context(), &d, ngv, IFEM_GPU_FP64), "ASMs3D::integrate"))
     return false;
   // the FP32 inner solver needs the FP32 element integrals as well
@@ -540,8 +550,15 @@ bool ASMs3D::integrate(Integrand& integrand, int lIndex, AlgEqSystem& glInt, con
   if (!G) return false;
   double t[3];
   if (!integrand.gpuTraction(lIndex % 10, t)) return true; // no load on this face
-  const int ng = nGauss > 0 ? nGauss : order[0];
-  return report(ifem_gpu_assemble_neumann(G->context(), lIndex % 10, t, ng),
+  // face rule of src/ASM/ASMs3D.C:2593 in the two tangent directions
+  const int face = lIndex % 10;
+  const int d0 = face <= 2 ? 1 : 0, d1 = face <= 4 ? 2 : 1;
+  const int ng = getNoGaussPt(order[d0], true);
+  if (getNoGaussPt(order[d1], true) != ng) {
+    std::cerr << " *** ASMs3D::integrate(face): unequal Gauss counts on the face" << std::endl;
+    return false;
+  }
+  return report(ifem_gpu_assemble_neumann(G->context(), face, t, ng),
                 "ASMs3D::integrate(face)");
 }
 

@@ -305,8 +305,15 @@ public:
   //! prescribed faces takes the last value.
   void prescribeFace(int face, int cmask, double value)
   { prescribed.push_back({face, cmask, value}); }
-  //! Gauss points per direction (SIMoptions nGauss, ASMbase::getNoGaussPt).
+  //! Integration rule code of the patch (ASMbase::setGauss; SIMbase passes
+  //! SIMoptions::nGauss[0], default 4): 1..10 a fixed count, else see
+  //! getNoGaussPt.
   void setGauss(int ng) { nGauss = ng; }
+  //! Gauss points in a direction of spline order p (= degree + 1), as
+  //! ASMbase::getNoGaussPt (src/ASM/ASMbase.C:1788-1798): 0 for
+  //! nGauss <= -p, nGauss for 1..10, p on faces (neumann), else
+  //! p + nGauss%10 (C remainder).
+  int getNoGaussPt(int p, bool neumann = false) const;
 
   //! Volume integration (ASMs3D::integrate, src/ASM/ASMs3D.C:2071-2335).
   bool integrate(Integrand& integrand, AlgEqSystem& glInt, const TimeDomain& time);
@@ -321,7 +328,7 @@ public:
   std::map<int, int> dirichlet;
   struct Prescribed { int face, cmask; double value; };
   std::vector<Prescribed> prescribed;
-  int nGauss = 0; //!< 0: p+1 per direction
+  int nGauss = 0; //!< ASMbase's initial value (src/ASM/ASMbase.C:65): order points
 
 private:
   bool uploaded(GPUMatrix* M) const;

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <sstream>
+#include <string>
 #include <vector>
 
 #include "HDF5Writer.h"
@@ -171,8 +172,24 @@ static int uniaxial(int nel)
   return 0;
 }
 
-int main()
+// ASMs3D::getNoGaussPt for (order, nGauss, neumann) triples given as
+// arguments; prints the counts as one JSON line (no GPU call).
+static int gauss_rule(int argc, char** argv)
 {
+  auto pch = ASMs3D::unitCube(1, 1);
+  std::printf("{\"case\": \"gauss_rule\", \"ng\": [");
+  for (int a = 2; a + 2 < argc; a += 3) {
+    pch->setGauss(std::atoi(argv[a + 1]));
+    std::printf("%s%d", a > 2 ? ", " : "",
+                pch->getNoGaussPt(std::atoi(argv[a]), std::atoi(argv[a + 2]) != 0));
+  }
+  std::printf("]}\n");
+  return 0;
+}
+
+int main(int argc, char** argv)
+{
+  if (argc > 1 && std::string(argv[1]) == "gauss") return gauss_rule(argc, argv);
   int rc = g2_cube();
   if (!rc) rc = poisson(16);
   if (!rc) rc = elasticity(8, "fp64");

@@ -187,6 +187,40 @@ def _items(text: Optional[str]) -> List[int]:
     return [int(x) for x in (text or "").split()]
 
 
+def _atoi(text: str) -> int:
+    """C atoi: optional sign and leading digits, 0 when there are none."""
+    s = text.strip()
+    k = 1 if s[:1] in ("+", "-") else 0
+    while k < len(s) and s[k].isdigit():
+        k += 1
+    digits = s[:k]
+    return int(digits) if digits not in ("", "+", "-") else 0
+
+
+# SIMoptions::nGauss default (src/SIM/SIMoptions.C:58)
+NGAUSS_DEFAULT = 4
+
+
+def parse_ngauss(el: Optional[ET.Element]) -> Tuple[int, int]:
+    """(main, recovery) integration rules of <discretization><nGauss>, as
+    SIMoptions::parseDiscretizationTag reads them (src/SIM/SIMoptions.C:
+    139-150): default="N" means 10+N for N > 0 (order + N points) and N
+    otherwise; a text value sets both rules, a second value the recovery
+    rule; no tag keeps the default 4."""
+    ng = [NGAUSS_DEFAULT, NGAUSS_DEFAULT]
+    if el is None:
+        return ng[0], ng[1]
+    dflt = el.get("default")
+    if dflt is not None:
+        g = _atoi(dflt)
+        ng[0] = ng[1] = 10 + g if g > 0 else g
+    elif el.text is not None:
+        for i, tok in enumerate(el.text.split()[:2]):
+            for j in range(i, 2):
+                ng[j] = _atoi(tok)
+    return ng[0], ng[1]
+
+
 def _function(el: Optional[ET.Element], default: M.Function) -> M.Function:
     """<conductivity>/<source>: a number, or type="sinprod" with an amplitude
     (scale * sin(pi x) sin(pi y) sin(pi z))."""
@@ -197,6 +231,39 @@ def _function(el: Optional[ET.Element], default: M.Function) -> M.Function:
     if typ not in _FN_TYPES:
         raise ValueError(f"unsupported field type {typ!r} (constant, sinprod)")
     return M.Function(_FN_TYPES[typ], val)
+
+
+def read_topologysets(geo: ET.Element) -> Dict[str, List[Tuple[int, int]]]:
+    """Face sets of <topologysets> as {name: [(patch, face), ...]}
+    (1-based patch and IFEM face numbers).  SIMinput::parse
+    (src/SIM/SIMinput.C:229-252) reads the set's dimension from its type:
+    volume, face/surface, edge/curve, vertex/point, nodes, elements, bbox, and
+    a set without a type is a whole volume.  Only face sets are read here;
+    every other dimension, the untyped set included, raises ValueError rather
+    than being taken for faces."""
+    sets: Dict[str, List[Tuple[int, int]]] = {}
+    ts = geo.find("topologysets")
+    if ts is None:
+        return sets
+    for st in ts.findall("set"):
+        name = st.get("name")
+        if name is None:
+            continue
+        typ = (st.get("type") or "").lower()
+        if typ not in ("face", "surface"):
+            what = typ or f"untyped (volume, dimension {st.get('dimension', '3')})"
+            raise ValueError(f"topology set {name!r}: {what} sets are not supported "
+                             "(face sets only)")
+        if (st.get("closure") or "").lower() == "open":
+            raise ValueError(f"topology set {name!r}: open face sets are not supported")
+        items = sets.setdefault(name, [])
+        for it in st.findall("item"):
+            pq = int(it.get("patch", 1))
+            for f in _items(it.text):
+                if not 1 <= f <= 6:
+                    raise ValueError(f"topology set {name!r}: face {f} is not 1..6")
+                items.append((pq, f))
+    return sets
 
 
 def read_xinp(source: str, base_dir: Optional[str] = None) -> Tuple[Problem, Dict[str, str]]:
@@ -228,16 +295,11 @@ def read_xinp(source: str, base_dir: Optional[str] = None) -> Tuple[Problem, Dic
         elif tag == "refine":
             for d, key in enumerate("uvw"):
                 P = uniform_refine(P, d, int(el.get(key, 0)))
-    sets: Dict[str, List[int]] = {}
-    ts = geo.find("topologysets")
-    if ts is not None:
-        for st in ts.findall("set"):
-            faces = []
-            for it in st.findall("item"):
-                if int(it.get("patch", 1)) != 1:
-                    raise ValueError("only patch 1")
-                faces += _items(it.text)
-            sets[st.get("name")] = faces
+    sets = read_topologysets(geo)
+    for name, items in sets.items():
+        if any(pq != 1 for pq, _ in items):
+            raise ValueError(f"topology set {name!r}: only patch 1")
+    sets = {name: [f for _, f in items] for name, items in sets.items()}
     # physics
     if root.find("elasticity") is not None:
         el = root.find("elasticity")
@@ -299,11 +361,17 @@ def read_xinp(source: str, base_dir: Optional[str] = None) -> Tuple[Problem, Dic
     else:
         meqn, neq = M.equation_numbers(msc)
     disc = root.find("discretization")
-    ng_el = disc.find("nGauss") if disc is not None else None
-    ngv = int(ng_el.text) if ng_el is not None else P.p[0] + 1
-    ng = (ngv, ngv, ngv)
+    nGauss, _ = parse_ngauss(disc.find("nGauss") if disc is not None else None)
+    # ASMbase::getNoGaussPt per direction with the spline order p+1; the GPU
+    # kernels take the counts as given and refuse those they do not support
+    ng = tuple(M.gauss_points(P.p[d] + 1, nGauss) for d in range(3))
     if traction is not None:
-        traction = (traction[0], traction[1], ngv)
+        f = traction[0]
+        tdirs = [d for d in range(3) if d != f // 2]
+        ngf = {M.gauss_points(P.p[d] + 1, nGauss, neumann=True) for d in tdirs}
+        if len(ngf) != 1:
+            raise ValueError("the traction face needs equal Gauss counts in its two directions")
+        traction = (traction[0], traction[1], ngf.pop())
     solver: Dict[str, str] = {}
     ls = root.find("linearsolver")
     if ls is not None:

@@ -151,6 +151,24 @@ def face_status(patch: Patch, ncomp: int, fixmask: Sequence[int]) -> np.ndarray:
     return msc.ravel()
 
 
+def gauss_points(order: int, nGauss: int, neumann: bool = False) -> int:
+    """Gauss points per direction, ASMbase::getNoGaussPt
+    (src/ASM/ASMbase.C:1788-1798).  ``order`` is the spline order p+1, as
+    ASMs3D passes it (getOrder, src/ASM/ASMs3D.C:1898-1906, used at
+    src/ASM/ASMs3D.C:4033 for volumes and 2593 for faces).  1..10 is a fixed
+    count; nGauss <= -order gives 0; otherwise faces take ``order`` and
+    volumes ``order + nGauss % 10`` with C's remainder (sign of nGauss)."""
+    nGauss = int(nGauss)
+    if nGauss <= -order:
+        return 0
+    if 0 < nGauss < 11:
+        return nGauss
+    if neumann:
+        return order
+    rem = abs(nGauss) % 10
+    return order + (-rem if nGauss < 0 else rem)
+
+
 def equation_numbers(msc: np.ndarray):
     """SAM::initSystemEquations for status codes 0/1: returns (meqn, neq)."""
     free = msc == 1

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define IFEM_GPU_ABI_VERSION 1
+#define IFEM_GPU_ABI_VERSION 2
 
 enum ifem_gpu_status {
   IFEM_GPU_OK = 0,
@@ -229,6 +229,16 @@ typedef struct {
   int converged;
   double rel_residual;   /* true ||b-Ax||_2/||b||_2 in FP64 */
   double solve_ms;       /* device time of the solve on the ctx stream */
+  /* Preconditioners other than Jacobi (multigrid, learned) run in flexible
+   * CG.  pc_iterations: iterations that applied that preconditioner;
+   * fallback: 1 if flexible CG broke down (<r, z> <= 0) and the solve went
+   * on with Jacobi-PCG from the current iterate (iterations - pc_iterations
+   * of them); pc_rejected: iterations in which the learned correction was
+   * dropped for that one step because it made <r, z> non-positive (the step
+   * then used the rest of the preconditioner). */
+  int pc_iterations;
+  int fallback;
+  int pc_rejected;
 } ifem_gpu_stats;
 
 void ifem_gpu_solver_defaults(ifem_gpu_solver_params* p);

@@ -61,6 +61,8 @@ def lib():
             raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
         L = C.CDLL(path)
         L.orc_gauss.argtypes = [C.c_int, _dp, _dp]
+        L.orc_num_gauss.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.orc_num_gauss.restype = C.c_int
         L.orc_find_span.argtypes = [C.c_int, C.c_int, _dp, C.c_double]
         L.orc_find_span.restype = C.c_int
         L.orc_basis_funs.argtypes = [C.c_int, _dp, C.c_int, C.c_double, _dp, _dp]
@@ -178,6 +180,11 @@ def gauss(n: int):
     if lib().orc_gauss(n, xi, w):
         raise ValueError(n)
     return xi, w
+
+
+def num_gauss(order: int, nGauss: int, neumann: bool = False) -> int:
+    """ASMbase::getNoGaussPt with the spline order (p+1)."""
+    return int(lib().orc_num_gauss(int(order), int(nGauss), 1 if neumann else 0))
 
 
 def basis(p: int, t: np.ndarray, u: float):

@@ -68,11 +68,15 @@ int orc_gauss(int n, double* xi, double* w)
 }
 
 /* ---- a2: ASMbase::getNoGaussPt (src/ASM/ASMbase.C:1788-1798) ------------ */
-int orc_num_gauss(int order, int nGauss)
+/* order = spline order p+1, as ASMs3D passes it (getOrder,
+ * src/ASM/ASMs3D.C:1898-1906; calls at 4033 and 2593).  C's % keeps the sign
+ * of nGauss, as in the reference. */
+int orc_num_gauss(int order, int nGauss, int neumann)
 {
-  if (nGauss >= 1 && nGauss <= 10) return nGauss;
-  int n = order - 1 + nGauss % 10; /* p + nGauss%10 */
-  return n < 1 ? 1 : (n > 10 ? 10 : n);
+  if (nGauss <= -order) return 0;
+  if (nGauss > 0 && nGauss < 11) return nGauss;
+  if (neumann) return order;
+  return order + nGauss % 10;
 }
 
 /* ---- a3: SplineUtils::getGaussParameters (src/Utility/SplineUtils.C:217) */

@@ -50,7 +50,7 @@ typedef struct {
 /* a1: n-point Gauss-Legendre rule on [-1,1] (n <= 10). */
 int orc_gauss(int n, double* xi, double* w);
 /* a2: Gauss point count rule of ASMbase::getNoGaussPt. */
-int orc_num_gauss(int order, int nGauss);
+int orc_num_gauss(int order, int nGauss, int neumann);
 /* a3: Gauss point parameters of span [u0,u1]. */
 double orc_gauss_param(double u0, double u1, double xi);
 /* a4: knot span and B-spline values/first derivatives at u.

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_defaults():
     from ifem_amd import _native
     L = _native.lib()
-    assert L.ifem_gpu_abi_version() == 1
+    assert L.ifem_gpu_abi_version() == 2
     p = _native.SolverParams()
     L.ifem_gpu_solver_defaults(C.byref(p))
     # LinSolParams defaults (src/LinAlg/LinSolParams.C:70-76)

@@ -21,6 +21,19 @@ def test_facade_built():
     assert os.path.exists(EXE), "build with __graft_entry__.build()"
 
 
+def test_facade_gauss_rule_known_answers():
+    """ASMs3D::getNoGaussPt of the facade against the known answers read off
+    src/ASM/ASMbase.C:1788-1798 (the table in test_inputs)."""
+    from test_inputs import GAUSS_KAT
+    args = []
+    for order, nGauss, neumann, _ in GAUSS_KAT:
+        args += [str(order), str(nGauss), "1" if neumann else "0"]
+    r = subprocess.run([EXE, "gauss", *args], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    assert d["ng"] == [k[3] for k in GAUSS_KAT]
+
+
 @pytest.mark.gpu
 def test_facade_driver_matches_oracle(tmp_path):
     print("[gpu] facade driver", flush=True)

@@ -138,7 +138,12 @@ def test_xinp_elasticity_matches_generated_problem():
     pb, solver = inputs.read_xinp(ELASTICITY_XINP)
     ref = problems.elasticity_cube(4, 2, jitter=0.0)
     assert pb.neq == ref.neq and np.array_equal(pb.meqn, ref.meqn)
-    assert pb.traction == ref.traction
+    # no <nGauss>: SIMoptions' default 4 (a fixed count, also on the face)
+    assert pb.ng == (4, 4, 4)
+    assert pb.traction[:2] == ref.traction[:2] and pb.traction[2] == 4
+    pb3, _ = inputs.read_xinp(ELASTICITY_XINP.replace(
+        "</elasticity>", "</elasticity><discretization><nGauss>3</nGauss></discretization>"))
+    assert pb3.traction == ref.traction and pb3.ng == ref.ng
     assert pb.integrand.kind == M.ELASTICITY and pb.integrand.nu == 0.3
     assert solver == {}
     with pytest.raises(ValueError):
@@ -216,3 +221,77 @@ def test_xinp_inhomogeneous_dirichlet_on_gpu():
         assert st["converged"]
         xd = s.expanded_solution(pb.ndof)
     assert np.max(np.abs(xd - _uniaxial_exact(pb))) <= 1e-12
+
+
+# Known answers of ASMbase::getNoGaussPt (src/ASM/ASMbase.C:1788-1798), read
+# off the reference with the spline order as its argument (ASMs3D::getOrder
+# returns svol->order(d), src/ASM/ASMs3D.C:1898-1906): (order, nGauss,
+# neumann) -> points.
+GAUSS_KAT = [
+    (3, 4, False, 4),     # SIMoptions default 4 (SIMoptions.C:58): fixed count
+    (3, 4, True, 4),      # fixed counts hold on faces too
+    (2, 10, False, 10),   # 10 is still a fixed count (nGauss < 11)
+    (3, 0, False, 3),     # ASMbase's own initial nGauss = 0 (ASMbase.C:65): order
+    (3, 0, True, 3),
+    (3, 11, False, 4),    # <nGauss default="1"/> = 11: order + 1
+    (4, 11, False, 5),
+    (3, 13, False, 6),    # order + 13 % 10
+    (3, 13, True, 3),     # faces: the order
+    (2, 20, False, 2),    # 20 % 10 = 0
+    (3, -1, False, 2),    # order + (-1 % 10) with C's remainder = order - 1
+    (3, -1, True, 3),
+    (3, -2, False, 1),
+    (3, -3, False, 0),    # nGauss <= -order: none
+    (3, -3, True, 0),     # checked before the Neumann branch
+    (2, -13, False, 0),
+    (4, -13, False, 0),
+    (4, -3, True, 4),     # above -order and not fixed: faces take the order
+    (4, -3, False, 1),
+]
+
+
+@pytest.mark.parametrize("order,nGauss,neumann,expect", GAUSS_KAT)
+def test_gauss_point_rule_known_answers(order, nGauss, neumann, expect):
+    assert M.gauss_points(order, nGauss, neumann) == expect
+    assert oracle.num_gauss(order, nGauss, neumann) == expect
+
+
+@pytest.mark.parametrize("tag,expect", [
+    (None, (4, 4)),                                    # SIMoptions.C:58
+    ('<nGauss default="1"/>', (11, 11)),               # SIMoptions.C:140-142
+    ('<nGauss default="0"/>', (0, 0)),
+    ('<nGauss default="-1"/>', (-1, -1)),
+    ("<nGauss>5</nGauss>", (5, 5)),                    # SIMoptions.C:143-149
+    ("<nGauss>3 2</nGauss>", (3, 2)),
+    ("<nGauss> 13 </nGauss>", (13, 13)),
+    ("<nGauss>x</nGauss>", (0, 0)),                    # atoi of a non-number
+])
+def test_ngauss_tag_forms(tag, expect):
+    import xml.etree.ElementTree as ET
+    el = ET.fromstring(tag) if tag else None
+    assert inputs.parse_ngauss(el) == expect
+
+
+def test_xinp_gauss_rule_applied_per_direction():
+    """<nGauss default="1"/> on a p = 2 patch integrates with order + 1 = 4
+    points; 13 gives 6, which the GPU kernels refuse when asked, and the
+    traction face always takes the order (3) for nGauss > 10."""
+    x = POISSON_XINP.replace("<nGauss>3</nGauss>", '<nGauss default="1"/>')
+    pb, _ = inputs.read_xinp(x)
+    assert pb.ng == (4, 4, 4)
+    pb, _ = inputs.read_xinp(POISSON_XINP.replace("<nGauss>3</nGauss>", "<nGauss>13</nGauss>"))
+    assert pb.ng == (6, 6, 6)
+    e = ELASTICITY_XINP.replace(
+        "</elasticity>", "</elasticity><discretization><nGauss>13</nGauss></discretization>")
+    pb, _ = inputs.read_xinp(e)
+    assert pb.ng == (6, 6, 6) and pb.traction[2] == 3
+
+
+@pytest.mark.parametrize("settype", ["edge", "vertex", "volume", "nodes", None])
+def test_topology_sets_other_than_faces_are_refused(settype):
+    """An edge, vertex or untyped (whole-volume, SIMinput.C:235-252) set must
+    not be read as a list of faces."""
+    attr = f' type="{settype}"' if settype else ""
+    x = ELASTICITY_XINP.replace('<set name="Left" type="face">', f'<set name="Left"{attr}>')
+    with pytest.raises(ValueError, match="not supported"):
+        inputs.read_xinp(x)

@@ -20,7 +20,7 @@ EXPORTS = [
     "ifem_gpu_init_assembly", "ifem_gpu_assemble", "ifem_gpu_assemble_neumann",
     "ifem_gpu_solver_defaults", "ifem_gpu_solve", "ifem_gpu_system_info", "ifem_gpu_get_system",
     "ifem_gpu_get_values_fp32", "ifem_gpu_get_solution", "ifem_gpu_expand_solution",
-    "ifem_gpu_spmv", "ifem_gpu_spmv_device", "ifem_gpu_set_rhs", "ifem_gpu_device_pointers",
+    "ifem_gpu_spmv", "ifem_gpu_spmv_noalloc", "ifem_gpu_spmv_device", "ifem_gpu_set_rhs", "ifem_gpu_device_pointers",
     "ifem_gpu_last_timings", "ifem_gpu_unet_create", "ifem_gpu_unet_get_params",
     "ifem_gpu_unet_set_params", "ifem_gpu_unet_forward", "ifem_gpu_unet_gradient",
     "ifem_gpu_unet_train_step", "ifem_gpu_set_learned_preconditioner",
@@ -117,6 +117,7 @@ def lib():
     L.ifem_gpu_get_solution.argtypes = [vp, vp]
     L.ifem_gpu_expand_solution.argtypes = [vp, vp]
     L.ifem_gpu_spmv.argtypes = [vp, vp, vp]
+    L.ifem_gpu_spmv_noalloc.argtypes = [vp, vp, vp]
     L.ifem_gpu_set_rhs.argtypes = [vp, vp]
     L.ifem_gpu_spmv_device.argtypes = [vp, ip, vp, vp]
     L.ifem_gpu_device_pointers.argtypes = [vp] + [C.POINTER(vp)] * 5

@@ -4,6 +4,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -20,6 +21,32 @@ int fail(int code, const std::string& msg)
 {
   g_last_error = msg;
   return code;
+}
+
+// Staging memory of ifem_gpu_spmv_noalloc: x in the first kNoAllocMaxEq
+// doubles, y in the second (2 x 4,194,304 doubles = 64 MiB on every GPU the
+// library runs on, held until the process ends).  Host code never passes
+// this name to a kernel or a copy: it uses the address that
+// hipGetSymbolAddress returns for the current device (noalloc_scratch).
+constexpr int kNoAllocMaxEq = 4194304;
+static __device__ __align__(256) __attribute__((used)) double scratch[2 * (size_t)kNoAllocMaxEq];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (which must be current), resolved
+// once per device; nullptr when the lookup fails (the entry point then
+// reports it).
+static double* noalloc_scratch(int device)
+{
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch_ptr[device] = (double*)p;
+  }
+  return g_scratch_ptr[device];
 }
 
 static void free_system(Context& c)
@@ -115,6 +142,8 @@ int ifem_gpu_ctx_create(int device, void* stream, ifem_gpu_ctx** out)
     delete ctx;
     return fail(IFEM_GPU_EDEVICE, "hipEventCreate failed");
   }
+  // set-up of the no-allocation entry point, with the device current
+  ctx->c.noalloc_scratch = noalloc_scratch(device);
   *out = ctx;
   return IFEM_GPU_OK;
 }
@@ -599,6 +628,30 @@ int ifem_gpu_spmv(ifem_gpu_ctx* ctx, const double* x, double* y)
   if (!rc && hipStreamSynchronize(c.stream) != hipSuccess) rc = fail(IFEM_GPU_EDEVICE, "spmv: sync failed");
   hipFree(dx);
   hipFree(dy);
+  return rc;
+}
+
+int ifem_gpu_spmv_noalloc(ifem_gpu_ctx* ctx, const double* x, double* y)
+{
+  CHECK_CTX(ctx);
+  Context& c = ctx->c;
+  if (!c.fp64_ready || !x || !y)
+    return fail(IFEM_GPU_EINVAL, "spmv_noalloc: no FP64 system or null vector");
+  if (c.neq > kNoAllocMaxEq)
+    return fail(IFEM_GPU_EINVAL, "spmv_noalloc: the system has " + std::to_string(c.neq) +
+                                     " equations, the staging buffer holds at most " +
+                                     std::to_string(kNoAllocMaxEq) + " (use ifem_gpu_spmv)");
+  if (!c.noalloc_scratch)
+    return fail(IFEM_GPU_EDEVICE, "spmv_noalloc: hipGetSymbolAddress of the staging buffer failed");
+  double* dx = c.noalloc_scratch;
+  double* dy = c.noalloc_scratch + kNoAllocMaxEq;
+  if (hipMemcpyAsync(dx, x, sizeof(double) * c.neq, hipMemcpyHostToDevice, c.stream) != hipSuccess)
+    return fail(IFEM_GPU_EDEVICE, "spmv_noalloc: copy in failed");
+  int rc = spmv_device(c, dx, dy);
+  if (!rc && hipMemcpyAsync(y, dy, sizeof(double) * c.neq, hipMemcpyDeviceToHost, c.stream) != hipSuccess)
+    rc = fail(IFEM_GPU_EDEVICE, "spmv_noalloc: copy out failed");
+  if (!rc && hipStreamSynchronize(c.stream) != hipSuccess)
+    rc = fail(IFEM_GPU_EDEVICE, "spmv_noalloc: sync failed");
   return rc;
 }
 

@@ -140,6 +140,7 @@ struct Context {
   bool fp64_ready = false, fp32_ready = false;
 
   SolverWork w;
+  double* noalloc_scratch = nullptr; // this device's staging array (ifem_gpu_spmv_noalloc)
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_mid = nullptr;
   double last_assemble_ms = 0.0, last_solve_ms = 0.0;
   double last_geometry_ms = 0.0, last_lines_ms = 0.0; // parts of the last assembly

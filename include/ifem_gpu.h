@@ -262,6 +262,16 @@ int ifem_gpu_expand_solution(ifem_gpu_ctx* ctx, double* xdof);
 /* y = K x with host vectors (SparseMatrix::multiply,
  * src/LinAlg/SparseMatrix.C:578-605). */
 int ifem_gpu_spmv(ifem_gpu_ctx* ctx, const double* x, double* y);
+/* No counterpart in the reference: ifem_gpu_spmv for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same product as ifem_gpu_spmv with
+ * host vectors, staged through a fixed device array of the library (64 MiB
+ * per GPU, held until the process ends) instead of buffers allocated per
+ * call; no hipMalloc/hipFree or allocating library call.  Systems above
+ * 4,194,304 equations are refused (IFEM_GPU_EINVAL).  All calls on one GPU
+ * share that array: calls that overlap on one GPU (other threads or
+ * contexts) race. */
+int ifem_gpu_spmv_noalloc(ifem_gpu_ctx* ctx, const double* x, double* y);
 /* y = K x on device vectors (neq each, resident in HBM): precision
  * IFEM_GPU_FP64 uses the FP64 values with double x, y; IFEM_GPU_FP32 the FP32
  * values with float x, y.  Runs on the ctx stream, does not synchronize. */

@@ -410,6 +410,12 @@ class Session:
         """y = K x on device pointers (no synchronisation)."""
         check(lib().ifem_gpu_spmv_device(self.h, precision, C.c_void_p(x_ptr), C.c_void_p(y_ptr)))
 
+    def rhs(self):
+        """The device right-hand side b (equation order)."""
+        b = np.zeros(max(self.neq, 1))
+        check(lib().ifem_gpu_get_system(self.h, None, None, None, _ptr(b)))
+        return b[:self.neq]
+
     def set_rhs(self, b):
         b = np.ascontiguousarray(b, dtype=np.float64)
         check(lib().ifem_gpu_set_rhs(self.h, _ptr(b)))

@@ -83,6 +83,10 @@ static void free_system(Context& c)
   hipFree(c.d_netin);
   hipFree(c.d_rold);
   hipFree(c.d_rold32);
+  hipFree(c.d_zn);
+  hipFree(c.d_zn32);
+  c.d_zn = nullptr;
+  c.d_zn32 = nullptr;
   hipFree(c.d_aux);
   c.d_netin = nullptr;
   c.d_rold = nullptr;

@@ -12,7 +12,9 @@ constexpr int kThreads = 256;
 constexpr int kGrid = 2048; // fixed grid: partial sums have a fixed layout
 
 enum Scal {
-  S_RZ = 0, S_RZ1, S_RR, S_DONE, S_TOL2, S_ITER, S_MAXIT, S_BB, S_NSCAL
+  S_RZ = 0, S_RZ1, S_RR, S_DONE, S_TOL2, S_ITER, S_MAXIT, S_BB,
+  S_REJ, // iterations whose learned correction was dropped (flexible CG)
+  S_NSCAL
 };
 
 template <typename A>

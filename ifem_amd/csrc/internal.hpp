@@ -169,6 +169,8 @@ struct Context {
   float* d_netout = nullptr;
   double* d_rold = nullptr; // previous residual (flexible CG)
   float* d_rold32 = nullptr;
+  double* d_zn = nullptr;   // learned correction of the current iteration
+  float* d_zn32 = nullptr;
   double learned_scale = 1.0;
   bool last_fallback = false; // the last solve fell back to Jacobi-PCG
 

@@ -363,7 +363,7 @@ int solver_setup(Context& c)
   for (double** p : d) IFEM_HIP(hipMalloc(p, sizeof(double) * n));
   float** f[] = {&w.r32, &w.z32, &w.p32, &w.q32, &w.x32, &w.dinv32};
   for (float** p : f) IFEM_HIP(hipMalloc(p, sizeof(float) * n));
-  IFEM_HIP(hipMalloc(&w.partial, sizeof(double) * 4 * kGrid));
+  IFEM_HIP(hipMalloc(&w.partial, sizeof(double) * 8 * kGrid));
   IFEM_HIP(hipMalloc(&w.scal, sizeof(double) * 16));
   IFEM_HIP(hipMemsetAsync(w.x, 0, sizeof(double) * n, c.stream));
   IFEM_HIP(hipHostMalloc(&w.h_scal, sizeof(double) * 16, hipHostMallocDefault));
@@ -612,11 +612,122 @@ __global__ __launch_bounds__(kThreads) void k_pupdate_f(int n, V* __restrict__ p
     p[i] = z[i] + beta * p[i];
 }
 
+// Learned correction zn = s*|r|_rms*U(r/|r|_rms, aux), mapped back from the
+// lattice; with a Jacobi base (dinv != null) also z0 = D^-1 r, otherwise z0
+// holds the base preconditioner's output already.  Per-block partials (part4,
+// kGrid each): <r,z0>, <r_old,z0>, <r,zn>, <r_old,zn>.
 template <typename V>
-static void apply_learned(Context& c, const V* r, const V* rold, const V* dinv, V* z,
-                          double* part_rr, double* part_rz, double* part_roz)
+__global__ __launch_bounds__(kThreads) void k_from_lattice_l(
+    int n, int nc, const V* __restrict__ r, const V* __restrict__ rold, const V* __restrict__ dinv,
+    V* __restrict__ z0, const float* __restrict__ out, V* __restrict__ zn, double scale, LatMap lm,
+    const double* __restrict__ part_rr, double* __restrict__ part4, const double* __restrict__ scal)
+{
+  if (scal[S_DONE] != 0.0) return;
+  __shared__ double sh[kThreads / 64];
+  const double rr = sum_parts(part_rr, kGrid, sh);
+  const double rms = sqrt(rr / n);
+  const int64_t vol = lm.vol;
+  double rz0 = 0.0, roz0 = 0.0, rzn = 0.0, rozn = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    const int node = i / nc, comp = i - node * nc;
+    V a;
+    if (dinv) {
+      a = dinv[i] * r[i];
+      z0[i] = a;
+    } else {
+      a = z0[i];
+    }
+    const V b = (V)(scale * rms * (double)out[(int64_t)comp * vol + lm.at(node)]);
+    zn[i] = b;
+    rz0 += (double)(r[i] * a);
+    rzn += (double)(r[i] * b);
+    if (rold) {
+      roz0 += (double)(rold[i] * a);
+      rozn += (double)(rold[i] * b);
+    }
+  }
+  const double b1 = block_sum<double>(rz0, sh);
+  const double b2 = block_sum<double>(roz0, sh);
+  const double b3 = block_sum<double>(rzn, sh);
+  const double b4 = block_sum<double>(rozn, sh);
+  if (threadIdx.x == 0) {
+    part4[blockIdx.x] = b1;
+    part4[kGrid + blockIdx.x] = b2;
+    part4[2 * kGrid + blockIdx.x] = b3;
+    part4[3 * kGrid + blockIdx.x] = b4;
+  }
+}
+
+// The learned correction is kept in an iteration when the combined
+// preconditioner stays clearly positive on r: <r, z0 + zn> >= theta <r, z0>.
+// Otherwise that iteration uses z0 alone (counted in scal[S_REJ]); flexible
+// CG then never breaks down because of the network.
+constexpr double kLearnedTheta = 0.1;
+
+// p update of flexible CG with the learned correction (first: p = z at the
+// start, no beta); block 0 records the scalars.
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_pupdate_l(int n, V* __restrict__ p,
+                                                        const V* __restrict__ z0,
+                                                        const V* __restrict__ zn,
+                                                        const double* __restrict__ part4,
+                                                        const double* __restrict__ part_rr,
+                                                        double* __restrict__ scal, int parity,
+                                                        int first)
+{
+  if (scal[S_DONE] != 0.0) return;
+  __shared__ double sh[kThreads / 64];
+  const double rz0 = sum_parts(part4, kGrid, sh);
+  const double roz0 = sum_parts(part4 + kGrid, kGrid, sh);
+  const double rzn = sum_parts(part4 + 2 * kGrid, kGrid, sh);
+  const double rozn = sum_parts(part4 + 3 * kGrid, kGrid, sh);
+  const double rr = sum_parts(part_rr, kGrid, sh);
+  const bool use = __builtin_isfinite(rzn) && __builtin_isfinite(rozn) &&
+                   rz0 + rzn >= kLearnedTheta * rz0;
+  const double rz = use ? rz0 + rzn : rz0;
+  const double roz = use ? roz0 + rozn : roz0;
+  const bool conv = !(rr > scal[S_TOL2]);
+  if (first) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      scal[S_RZ] = rz;
+      scal[S_RR] = rr;
+      scal[S_DONE] = conv ? 1.0 : 0.0;
+      if (!conv && !(rz > 0.0)) scal[S_DONE] = 2.0; // base preconditioner not positive on r
+      if (scal[S_ITER] >= scal[S_MAXIT]) scal[S_DONE] = 3.0;
+      if (!use) scal[S_REJ] += 1.0;
+    }
+    if (conv || !(rz > 0.0)) return;
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
+      p[i] = use ? z0[i] + zn[i] : z0[i];
+    return;
+  }
+  const double rzo = scal[S_RZ + parity];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[S_RZ + 1 - parity] = rz;
+    scal[S_RR] = rr;
+    scal[S_ITER] += 1.0;
+    if (conv) scal[S_DONE] = 1.0;
+    else if (!(rz > 0.0)) scal[S_DONE] = 2.0;
+    else if (scal[S_ITER] >= scal[S_MAXIT]) scal[S_DONE] = 3.0;
+    if (!use) scal[S_REJ] += 1.0;
+  }
+  if (conv) return;
+  const V beta = (V)(rzo != 0.0 ? (rz - roz) / rzo : 0.0);
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
+    p[i] = (use ? z0[i] + zn[i] : z0[i]) + beta * p[i];
+}
+
+// z0 (base: Jacobi inside k_from_lattice_l, or one multigrid V-cycle) and
+// the learned correction zn, with the partials of k_pupdate_l
+template <typename V>
+static int apply_learned(Context& c, bool mg_base, const V* r, const V* rold, const V* dinv, V* z0,
+                         V* zn, double* part_rr, double* part4)
 {
   const int n = c.neq;
+  if (mg_base) {
+    int rc = mg_apply(c, r, z0, c.w.scal);
+    if (rc) return rc;
+  }
   LatMap lm;
   lm.bw = c.box[2];
   lm.bh = c.box[1];
@@ -626,9 +737,11 @@ static void apply_learned(Context& c, const V* r, const V* rold, const V* dinv, 
   k_to_lattice<V><<<kGrid, kThreads, 0, c.stream>>>(n, c.ncomp, r, c.d_netin, c.d_aux, c.naux,
                                                     c.features, lm, part_rr, kGrid, c.w.scal);
   const float* out = c.net->forward(c.d_netin, 1, c.lat[0], c.lat[1], c.lat[2]);
-  k_from_lattice<V><<<kGrid, kThreads, 0, c.stream>>>(n, c.ncomp, r, rold, dinv, out, z,
-                                                      c.learned_scale, lm, part_rr, kGrid, part_rz,
-                                                      part_roz, c.w.scal);
+  k_from_lattice_l<V><<<kGrid, kThreads, 0, c.stream>>>(n, c.ncomp, r, rold,
+                                                        mg_base ? (const V*)nullptr : dinv, z0, out,
+                                                        zn, c.learned_scale, lm, part_rr, part4,
+                                                        c.w.scal);
+  return IFEM_GPU_OK;
 }
 
 // partials <r,z> and <r_old,z> after a preconditioner that does not form them
@@ -656,26 +769,16 @@ __global__ __launch_bounds__(kThreads) void k_pc_dots(int n, const V* __restrict
   }
 }
 
-enum PcKind { PC_LEARNED = 0, PC_MULTIGRID = 1 };
+enum PcKind { PC_LEARNED = 0, PC_MULTIGRID = 1, PC_LEARNED_MG = 2 };
 
-template <typename V>
-static int apply_pc(Context& c, int kind, const V* r, const V* rold, const V* dinv, V* z,
-                    double* part_rr, double* part_rz, double* part_roz)
-{
-  if (kind == PC_LEARNED) {
-    apply_learned<V>(c, r, rold, dinv, z, part_rr, part_rz, part_roz);
-    return IFEM_GPU_OK;
-  }
-  int rc = mg_apply(c, r, z, c.w.scal);
-  if (rc) return rc;
-  k_pc_dots<V><<<kGrid, kThreads, 0, c.stream>>>(c.neq, r, rold, z, part_rz, part_roz, c.w.scal);
-  return IFEM_GPU_OK;
-}
-
+// Flexible CG with the spline multigrid (PC_MULTIGRID) or with a learned
+// correction on a Jacobi (PC_LEARNED) or multigrid (PC_LEARNED_MG) base.
+// zn: the learned correction (learned kinds only).  *rejected: iterations
+// whose learned correction the safeguard dropped.
 template <typename V, typename ACC>
-static int fcg_run(Context& c, int kind, const V* val, V* x, V* r, V* z, V* p, V* q, V* rold,
-                   const V* dinv, double tol2, int maxits, int check_every, int* iters,
-                   double* rr_out, int* status)
+static int fcg_run(Context& c, int kind, const V* val, V* x, V* r, V* z, V* zn, V* p, V* q,
+                   V* rold, const V* dinv, double tol2, int maxits, int check_every, int* iters,
+                   double* rr_out, int* status, int* rejected)
 {
   SolverWork& w = c.w;
   const int n = c.neq;
@@ -684,15 +787,26 @@ static int fcg_run(Context& c, int kind, const V* val, V* x, V* r, V* z, V* p, V
   s[S_MAXIT] = maxits;
   int rc = write_scalars(c, s);
   if (rc) return rc;
+  const bool learned = kind != PC_MULTIGRID;
+  const bool mg_base = kind != PC_LEARNED;
   double* part_pq = w.partial;
   double* part_rz = w.partial + kGrid;
   double* part_rr = w.partial + 2 * kGrid;
   double* part_roz = w.partial + 3 * kGrid;
+  double* part4 = w.partial + 4 * kGrid;
   k_norm_parts<V><<<kGrid, kThreads, 0, c.stream>>>(n, r, part_rr);
-  rc = apply_pc<V>(c, kind, r, (const V*)nullptr, dinv, z, part_rr, part_rz, part_roz);
-  if (rc) return rc;
-  k_start_check_f<<<1, kThreads, 0, c.stream>>>(part_rz, part_rr, kGrid, w.scal);
-  k_copy<V><<<(n + 255) / 256, 256, 0, c.stream>>>(n, z, p);
+  if (learned) {
+    rc = apply_learned<V>(c, mg_base, r, (const V*)nullptr, dinv, z, zn, part_rr, part4);
+    if (rc) return rc;
+    k_pupdate_l<V><<<kGrid, kThreads, 0, c.stream>>>(n, p, z, zn, part4, part_rr, w.scal, 0, 1);
+  } else {
+    rc = mg_apply(c, r, z, w.scal);
+    if (rc) return rc;
+    k_pc_dots<V><<<kGrid, kThreads, 0, c.stream>>>(n, r, (const V*)nullptr, z, part_rz, part_roz,
+                                                   w.scal);
+    k_start_check_f<<<1, kThreads, 0, c.stream>>>(part_rz, part_rr, kGrid, w.scal);
+    k_copy<V><<<(n + 255) / 256, 256, 0, c.stream>>>(n, z, p);
+  }
   IFEM_HIP(hipGetLastError());
   int launched = 0;
   while (true) {
@@ -701,10 +815,18 @@ static int fcg_run(Context& c, int kind, const V* val, V* x, V* r, V* z, V* p, V
       launch_spmv<V, ACC, true>(c, val, p, q);
       k_update_f<V><<<kGrid, kThreads, 0, c.stream>>>(n, x, r, rold, p, q, part_pq, part_rr,
                                                       w.scal, parity);
-      rc = apply_pc<V>(c, kind, r, rold, dinv, z, part_rr, part_rz, part_roz);
-      if (rc) return rc;
-      k_pupdate_f<V><<<kGrid, kThreads, 0, c.stream>>>(n, p, z, part_rz, part_roz, part_rr,
-                                                       w.scal, parity);
+      if (learned) {
+        rc = apply_learned<V>(c, mg_base, r, rold, dinv, z, zn, part_rr, part4);
+        if (rc) return rc;
+        k_pupdate_l<V><<<kGrid, kThreads, 0, c.stream>>>(n, p, z, zn, part4, part_rr, w.scal,
+                                                         parity, 0);
+      } else {
+        rc = mg_apply(c, r, z, w.scal);
+        if (rc) return rc;
+        k_pc_dots<V><<<kGrid, kThreads, 0, c.stream>>>(n, r, rold, z, part_rz, part_roz, w.scal);
+        k_pupdate_f<V><<<kGrid, kThreads, 0, c.stream>>>(n, p, z, part_rz, part_roz, part_rr,
+                                                         w.scal, parity);
+      }
     }
     IFEM_HIP(hipGetLastError());
     launched += check_every;
@@ -715,6 +837,7 @@ static int fcg_run(Context& c, int kind, const V* val, V* x, V* r, V* z, V* p, V
   *iters = (int)w.h_scal[S_ITER];
   *rr_out = w.h_scal[S_RR];
   *status = (int)w.h_scal[S_DONE];
+  *rejected = (int)w.h_scal[S_REJ];
   return IFEM_GPU_OK;
 }
 
@@ -774,8 +897,14 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
     if (!c.d_rold32) IFEM_HIP(hipMalloc(&c.d_rold32, sizeof(float) * (n > 0 ? n : 1)));
     check_pc = std::min(check, 8); // few, expensive iterations
   }
-  const int pc_kind = use_mg ? PC_MULTIGRID : PC_LEARNED;
-  int total_iters = 0, outer = 0, pc_iters = 0;
+  const int pc_kind = c.learned ? (use_mg ? PC_LEARNED_MG : PC_LEARNED) : PC_MULTIGRID;
+  if (c.learned) {
+    if (!c.d_zn) IFEM_HIP(hipMalloc(&c.d_zn, sizeof(double) * (n > 0 ? n : 1)));
+    if (!c.d_zn32) IFEM_HIP(hipMalloc(&c.d_zn32, sizeof(float) * (n > 0 ? n : 1)));
+    if (!c.d_rold) IFEM_HIP(hipMalloc(&c.d_rold, sizeof(double) * (n > 0 ? n : 1)));
+    if (!c.d_rold32) IFEM_HIP(hipMalloc(&c.d_rold32, sizeof(float) * (n > 0 ? n : 1)));
+  }
+  int total_iters = 0, outer = 0, pc_iters = 0, rejected = 0;
   bool use_learned = c.learned || use_mg, fell_back = false;
   double rel = bnorm > 0 ? 1.0 : 0.0;
   bool converged = bnorm == 0.0;
@@ -786,16 +915,18 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
     while (true) {
       int it = 0, status = 0;
       double rr = 0.0;
+      int rej = 0;
       if (use_learned)
-        rc = fcg_run<double, double>(c, pc_kind, c.d_val, w.x, w.r, w.z, w.p, w.q, c.d_rold,
-                                     w.dinv, tol * tol, prm.maxits - total_iters, check_pc, &it,
-                                     &rr, &status);
+        rc = fcg_run<double, double>(c, pc_kind, c.d_val, w.x, w.r, w.z, c.d_zn, w.p, w.q,
+                                     c.d_rold, w.dinv, tol * tol, prm.maxits - total_iters,
+                                     check_pc, &it, &rr, &status, &rej);
       else
         rc = pcg_run<double, double>(c, c.d_val, w.x, w.r, w.z, w.p, w.q, w.dinv, tol * tol,
                                      prm.maxits - total_iters, check, &it, &rr, &status);
       if (rc) return rc;
       total_iters += it;
       if (use_learned) pc_iters += it;
+      rejected += rej;
       outer++;
       double trr = 0.0;
       rc = true_residual(c, w.x, w.r, &trr); // w.r <- b - A x (replacement)
@@ -828,10 +959,11 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
       // residual target relative to the scaled (unit-norm) residual; never
       // ask the inner solve for more than the outer tolerance needs
       double target = std::fmax(inner, 0.5 * tol / rn);
+      int rej = 0;
       if (use_learned)
-        rc = fcg_run<float, float>(c, pc_kind, c.d_val32, w.x32, w.r32, w.z32, w.p32, w.q32,
-                                   c.d_rold32, w.dinv32, target * target, prm.maxits - total_iters,
-                                   check_pc, &it, &rr, &status);
+        rc = fcg_run<float, float>(c, pc_kind, c.d_val32, w.x32, w.r32, w.z32, c.d_zn32, w.p32,
+                                   w.q32, c.d_rold32, w.dinv32, target * target,
+                                   prm.maxits - total_iters, check_pc, &it, &rr, &status, &rej);
       else
         rc = pcg_run<float, float>(c, c.d_val32, w.x32, w.r32, w.z32, w.p32, w.q32, w.dinv32,
                                    target * target, prm.maxits - total_iters, check, &it, &rr,
@@ -839,6 +971,7 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
       if (rc) return rc;
       total_iters += it;
       if (use_learned) pc_iters += it;
+      rejected += rej;
       outer++;
       k_axpy_promote<<<nb, 256, 0, c.stream>>>(n, w.x, w.x32, rn);
       rc = true_residual(c, w.x, w.r, &rr64);
@@ -866,7 +999,7 @@ int solve(Context& c, const ifem_gpu_solver_params& prm, ifem_gpu_stats* st)
     st->solve_ms = ms;
     st->pc_iterations = pc_iters;
     st->fallback = fell_back ? 1 : 0;
-    st->pc_rejected = 0;
+    st->pc_rejected = rejected;
   }
   c.last_fallback = fell_back;
   if (diverged) return fail(IFEM_GPU_ENOCONV, "solve: diverged (residual above dtol*||b||)");

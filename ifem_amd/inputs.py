@@ -15,7 +15,8 @@ the ``.xinp`` subset that describes the hot path's models.
   simple interior knots; such patches are refused (raise before refining, as
   IFEM inputs do).
 * ``read_xinp``: ``<geometry>`` (``<patch>``/``<patchfile>``, ``<refine>``,
-  ``<raiseorder>``, ``<topologysets>`` with face sets),
+  ``<raiseorder>`` per patch or patch range, ``<topology>`` connections with
+  orientation flags, ``<topologysets>`` with face sets),
   ``<boundaryconditions>`` (``<dirichlet>`` on face sets, homogeneous or
   with a constant value,
   constant ``<neumann>`` traction), ``<discretization><nGauss>``,
@@ -179,6 +180,114 @@ def evaluate(P: M.Patch, u: float, v: float, w: float) -> np.ndarray:
     return np.einsum("k,j,i,kjid->d", N3, N2, N1, sub)
 
 
+# ---- multi-patch topology ------------------------------------------------------
+def face_size(P: M.Patch, face: int) -> Tuple[int, int]:
+    """Control points of face 1..6 in its two directions, ASMs3D::getFaceSize
+    (src/ASM/ASMs3D.C:3796-3810)."""
+    n1, n2, n3 = P.n
+    if face in (1, 2):
+        return n2, n3
+    if face in (3, 4):
+        return n1, n3
+    return n1, n2
+
+
+def face_nodes(P: M.Patch, face: int) -> np.ndarray:
+    """0-based local nodes of face 1..6 in the order of
+    ASMs3D::getBoundaryNodes (src/ASM/ASMs3D.C:1770-1818): faces 1/2 with j
+    fastest then k, faces 3/4 i then k, faces 5/6 i then j."""
+    n1, n2, n3 = P.n
+    if face in (1, 2):
+        i = 0 if face == 1 else n1 - 1
+        k, j = np.meshgrid(np.arange(n3), np.arange(n2), indexing="ij")
+        return (i + n1 * (j + n2 * k)).ravel()
+    if face in (3, 4):
+        j = 0 if face == 3 else n2 - 1
+        k, i = np.meshgrid(np.arange(n3), np.arange(n1), indexing="ij")
+        return (i + n1 * (j + n2 * k)).ravel()
+    k = 0 if face == 5 else n3 - 1
+    j, i = np.meshgrid(np.arange(n2), np.arange(n1), indexing="ij")
+    return (i + n1 * (j + n2 * k)).ravel()
+
+
+def connect_patches(patches: List[M.Patch], connections, xtol: float = 1.0e-4) -> List[np.ndarray]:
+    """Global node numbers (MLGN, 1-based) of every patch's local nodes after
+    collapsing the nodes of each connected face pair, as
+    ASMs3D::connectBasis does (src/ASM/ASMs3D.C:721-808): the slave face's
+    node (k, l) is matched to the master face's node (i, j) by the
+    orientation flag 0..7, and the two must coincide within xtol per
+    coordinate (Vec3::equal, src/Utility/Vec3.h:133-141), else ValueError.
+    connections: (master, mface, slave, sface, orient), 1-based patches.
+    Global nodes are numbered by first appearance, patch by patch."""
+    off = np.concatenate([[0], np.cumsum([P.nnod for P in patches])]).astype(np.int64)
+    parent = np.arange(off[-1], dtype=np.int64)
+
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+
+    for master, mface, slave, sface, orient in connections:
+        if master == slave or not (1 <= master <= len(patches) and 1 <= slave <= len(patches)):
+            raise ValueError(f"connection: invalid patch indices {master} {slave}")
+        if not 0 <= orient <= 7:
+            raise ValueError(f"connection: orientation flag {orient} out of range [0,7]")
+        Ps, Pm = patches[slave - 1], patches[master - 1]
+        m1, m2 = face_size(Ps, sface)
+        n1, n2 = face_size(Pm, mface)
+        if orient > 3:
+            n1, n2 = n2, n1
+        if n1 != m1 or n2 != m2:
+            raise ValueError(f"connection P{slave} F{sface} - P{master} F{mface}: non-matching "
+                             f"faces, sizes {n1},{n2} and {m1},{m2}")
+        if orient > 3:
+            n1, n2 = n2, n1
+        snodes = face_nodes(Ps, sface)
+        mnodes = face_nodes(Pm, mface)
+        node = 0
+        for j in range(n2):
+            for i in range(n1):
+                k, l = {0: (i, j), 1: (i, n2 - j - 1), 2: (n1 - i - 1, j),
+                        3: (n1 - i - 1, n2 - j - 1), 4: (j, i), 5: (j, m2 - i - 1),
+                        6: (m1 - j - 1, i), 7: (m1 - j - 1, m2 - i - 1)}[orient]
+                sn = int(snodes[l * m1 + k])
+                mn = int(mnodes[node])
+                node += 1
+                if np.any(np.abs(Pm.coefs[mn] - Ps.coefs[sn]) > xtol):
+                    raise ValueError(f"connection P{slave} F{sface} - P{master} F{mface}: "
+                                     f"non-matching nodes {mn + 1}: {Pm.coefs[mn]} and "
+                                     f"{sn + 1}: {Ps.coefs[sn]}")
+                a, b = find(off[master - 1] + mn), find(off[slave - 1] + sn)
+                if a != b:
+                    parent[max(a, b)] = min(a, b)
+    roots = np.array([find(a) for a in range(off[-1])], dtype=np.int64)
+    _, first, inv = np.unique(roots, return_index=True, return_inverse=True)
+    # number the global nodes in order of their first appearance
+    order = np.argsort(first)
+    rank = np.empty_like(order)
+    rank[order] = np.arange(len(order))
+    gid = rank[inv] + 1
+    return [gid[off[q]:off[q + 1]].astype(np.int32) for q in range(len(patches))]
+
+
+def _patch_list(el: ET.Element, npatch: int) -> List[int]:
+    """Patches of a <refine>/<raiseorder> tag: patch="n", or lowerpatch ..
+    upperpatch (SIMinput::parsePatchList, src/SIM/SIMinput.C:1036-1050);
+    default patch 1."""
+    lo = up = 1
+    if el.get("patch") is not None:
+        lo = up = int(el.get("patch"))
+    if el.get("lowerpatch") is not None:
+        lo = int(el.get("lowerpatch"))
+        up = npatch
+    if el.get("upperpatch") is not None:
+        up = int(el.get("upperpatch"))
+    if lo < 1 or up > npatch:
+        raise ValueError(f"<{el.tag}>: patch range {lo}..{up} outside 1..{npatch}")
+    return list(range(lo, up + 1))
+
+
 # ---- .xinp ------------------------------------------------------------------
 _FN_TYPES = {"constant": M.FN_CONST, "sinprod": M.FN_SINPROD}
 
@@ -266,8 +375,20 @@ def read_topologysets(geo: ET.Element) -> Dict[str, List[Tuple[int, int]]]:
     return sets
 
 
-def read_xinp(source: str, base_dir: Optional[str] = None) -> Tuple[Problem, Dict[str, str]]:
-    """Problem and linear-solver keys of an .xinp file (path or XML text)."""
+def _read_patchfile(base_dir: str, name: str) -> List[M.Patch]:
+    """A <patchfile>: relative to the input's folder, or (for inputs copied
+    out of their tree) the file of that name beside the input."""
+    path = os.path.join(base_dir, name)
+    if not os.path.exists(path):
+        path = os.path.join(base_dir, os.path.basename(name))
+    with open(path) as f:
+        return read_g2(f.read())
+
+
+def read_xinp(source: str, base_dir: Optional[str] = None):
+    """Problem (one patch) or MultiProblem (several patches, <topology>) and
+    the linear-solver keys of an .xinp file (path or XML text)."""
+    from .problems import MultiProblem
     if os.path.exists(source):
         base_dir = base_dir or os.path.dirname(os.path.abspath(source))
         root = ET.parse(source).getroot()
@@ -283,23 +404,46 @@ def read_xinp(source: str, base_dir: Optional[str] = None) -> Tuple[Problem, Dic
         if tag == "patch":
             patches += read_g2(el.text or "")
         elif tag == "patchfile":
-            with open(os.path.join(base_dir, (el.text or "").strip())) as f:
-                patches += read_g2(f.read())
-    if len(patches) != 1:
-        raise ValueError("the GPU path takes exactly one patch per model")
-    P = patches[0]
+            patches += _read_patchfile(base_dir, (el.text or "").strip())
+    if not patches:
+        raise ValueError("no patches in <geometry>")
     for el in geo:
         tag = el.tag.lower()
         if tag == "raiseorder":
-            P = raise_order(P, int(el.get("u", 0)), int(el.get("v", 0)), int(el.get("w", 0)))
+            for q in _patch_list(el, len(patches)):
+                patches[q - 1] = raise_order(patches[q - 1], int(el.get("u", 0)),
+                                             int(el.get("v", 0)), int(el.get("w", 0)))
         elif tag == "refine":
-            for d, key in enumerate("uvw"):
-                P = uniform_refine(P, d, int(el.get(key, 0)))
+            if (el.text or "").strip():
+                raise ValueError("<refine> with explicit knots is not supported")
+            for q in _patch_list(el, len(patches)):
+                for d, key in enumerate("uvw"):
+                    patches[q - 1] = uniform_refine(patches[q - 1], d, int(el.get(key, 0)))
+    connections = []
+    top = geo.find("topology")
+    if top is not None:
+        offset = int(top.get("offset", 0))
+        for c in top.findall("connection"):
+            if int(c.get("dim", 2)) != 2:
+                continue  # edge/vertex connections are ignored in serial (SIM3D.C:67)
+            if c.get("periodic") not in (None, "false", "0"):
+                raise ValueError("periodic connections are not supported")
+            if c.get("basis") not in (None, "0", "1"):
+                raise ValueError("mixed-basis connections are not supported")
+            connections.append((int(c.get("master", 0)) + offset,
+                                int(c.get("midx", c.get("mface", 0))),
+                                int(c.get("slave", 0)) + offset,
+                                int(c.get("sidx", c.get("sface", 0))),
+                                int(c.get("orient", 0))))
+    multi = len(patches) > 1 or bool(connections)
+    mlgn = connect_patches(patches, connections) if multi else \
+        [np.arange(1, patches[0].nnod + 1, dtype=np.int32)]
+    nnod = int(max(int(m.max()) for m in mlgn))
     sets = read_topologysets(geo)
     for name, items in sets.items():
-        if any(pq != 1 for pq, _ in items):
-            raise ValueError(f"topology set {name!r}: only patch 1")
-    sets = {name: [f for _, f in items] for name, items in sets.items()}
+        for pq, _ in items:
+            if not 1 <= pq <= len(patches):
+                raise ValueError(f"topology set {name!r}: patch {pq} out of range")
     # physics
     if root.find("elasticity") is not None:
         el = root.find("elasticity")
@@ -315,14 +459,27 @@ def read_xinp(source: str, base_dir: Optional[str] = None) -> Tuple[Problem, Dic
                         M.Function(M.FN_ZERO, 0.0))
         itg = M.Integrand(M.POISSON, kappa=kappa, source=src)
     nc = itg.ncomp
-    # boundary conditions: homogeneous Dirichlet on faces, constant traction
-    fix = [0] * 6
-    inhom = []  # (face, mask, value) of constant inhomogeneous conditions
-    traction = None
+
+    def set_dofs(items, mask):
+        """Global DOFs (0-based) of the masked components on the set's faces."""
+        out = []
+        for pq, f in items:
+            nodes = mlgn[pq - 1][face_nodes(patches[pq - 1], f)].astype(np.int64) - 1
+            for c in range(nc):
+                if (mask >> c) & 1:
+                    out.append(nodes * nc + c)
+        return np.unique(np.concatenate(out)) if out else np.zeros(0, dtype=np.int64)
+
+    # boundary conditions: Dirichlet on face sets (homogeneous or a constant
+    # value), constant tractions
+    msc = np.ones(nnod * nc, dtype=np.int32)
+    vals = np.full(nnod * nc, np.nan)
+    inhom = False
+    tractions = []
     bc = root.find("boundaryconditions")
     if bc is not None:
         for d in bc.findall("dirichlet"):
-            faces = sets.get(d.get("set"), [])
+            items = sets.get(d.get("set"), [])
             comp = str(d.get("comp", d.get("component", "1" if nc == 1 else "123")))
             mask = 0
             for ch in comp:
@@ -336,26 +493,24 @@ def read_xinp(source: str, base_dir: Optional[str] = None) -> Tuple[Problem, Dic
             if typ in ("anasol", "expression", "field") or typ.startswith("symm"):
                 raise ValueError(f"<dirichlet type='{typ}'> is not supported")
             value = float(text) if text else 0.0
-            for f in faces:
-                if value != 0.0:
-                    inhom.append((f - 1, mask, value))
-                else:
-                    fix[f - 1] |= mask
+            dofs = set_dofs(items, mask)
+            if value != 0.0:
+                vals[dofs] = value  # later conditions win on shared DOFs
+                inhom = True
+            else:
+                msc[dofs] = 0
         for nm in bc.findall("neumann"):
-            faces = sets.get(nm.get("set"), [])
-            if len(faces) != 1 or traction is not None:
-                raise ValueError("one <neumann> face is supported")
+            items = sets.get(nm.get("set"), [])
+            if not items:
+                raise ValueError("<neumann> on an empty set")
             direction = int(nm.get("direction", 3))
             t = [0.0, 0.0, 0.0]
             t[direction - 1] = float((nm.text or "0").strip())
-            traction = (faces[0] - 1, tuple(t), None)
-    msc = M.face_status(P, nc, fix)
+            for pq, f in items:
+                tractions.append((pq - 1, f - 1, tuple(t[:nc] if nc < 3 else t)))
     sam = None
     if inhom:
-        vals = np.full(P.nnod * nc, np.nan)
-        for f, mask, value in inhom:  # later conditions win on shared DOFs
-            on = M.face_status(P, nc, [mask if g == f else 0 for g in range(6)]) == 0
-            vals[on] = value
+        vals[msc == 0] = np.nan  # a homogeneously fixed DOF stays fixed
         sam = M.prescribed_equations(msc, vals)
         meqn, neq = sam.meqn, sam.neq
     else:
@@ -364,20 +519,29 @@ def read_xinp(source: str, base_dir: Optional[str] = None) -> Tuple[Problem, Dic
     nGauss, _ = parse_ngauss(disc.find("nGauss") if disc is not None else None)
     # ASMbase::getNoGaussPt per direction with the spline order p+1; the GPU
     # kernels take the counts as given and refuse those they do not support
-    ng = tuple(M.gauss_points(P.p[d] + 1, nGauss) for d in range(3))
-    if traction is not None:
-        f = traction[0]
+    ngs = {tuple(M.gauss_points(P.p[d] + 1, nGauss) for d in range(3)) for P in patches}
+    if len(ngs) != 1:
+        raise ValueError("the patches need equal Gauss counts (equal degrees)")
+    ng = ngs.pop()
+    tr = []
+    for q, f, t in tractions:
+        P = patches[q]
         tdirs = [d for d in range(3) if d != f // 2]
         ngf = {M.gauss_points(P.p[d] + 1, nGauss, neumann=True) for d in tdirs}
         if len(ngf) != 1:
-            raise ValueError("the traction face needs equal Gauss counts in its two directions")
-        traction = (traction[0], traction[1], ngf.pop())
+            raise ValueError("a traction face needs equal Gauss counts in its two directions")
+        tr.append((q, f, t, ngf.pop()))
     solver: Dict[str, str] = {}
     ls = root.find("linearsolver")
     if ls is not None:
         for ch in ls:
             solver[ch.tag.lower()] = (ch.text or "").strip()
-    return Problem("xinp", P, itg, ng, meqn, neq, traction, sam), solver
+    if not multi:
+        if len(tr) > 1:
+            raise ValueError("one <neumann> face is supported on a single patch")
+        traction = (tr[0][1], tr[0][2], tr[0][3]) if tr else None
+        return Problem("xinp", patches[0], itg, ng, meqn, neq, traction, sam), solver
+    return MultiProblem("xinp", patches, mlgn, itg, ng, meqn, neq, tr, sam), solver
 
 
 def solver_kwargs(solver: Dict[str, str]) -> dict:

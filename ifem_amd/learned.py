@@ -1,15 +1,21 @@
-"""Learned preconditioner: data from the method's own Jacobi-PCG solves,
-training of the U-net on the GPU, and evaluation inside flexible CG.
+"""Learned preconditioner: data from the method's own solves, training of
+the U-net on the GPU, and evaluation inside flexible CG.
 
 The free-DOF box can be placed in a zero-padded lattice whose extents divide
 by 2^(levels-1) (``lattice_dims``, ``to_lattice``, ``from_lattice``).
 
 The network sees the residual on the lattice of free DOFs (normalised to unit
 RMS) plus auxiliary channels describing the problem (for diffusion: the
-log-conductivity at the nodes), and predicts the part of the error that the
-Jacobi step misses:  target = (e - D^-1 r) / |r|_rms  with e = x* - x_k and
-r = b - A x_k taken at PCG iterates x_k.  Inside flexible CG the
-preconditioner is z = D^-1 r + |r|_rms * U(r / |r|_rms, aux).
+log-conductivity at the nodes), and predicts the part of the error that a
+base preconditioner M0 misses: target = (e - M0 r) / |r|_rms with e = x* - x_k
+and r = b - A x_k taken at the base solver's own iterates x_k.  M0 is the
+Jacobi step D^-1 (base "jacobi", iterates of Jacobi-PCG) or one V-cycle of the
+spline multigrid (base "mg", iterates of multigrid flexible CG); the second is
+the multigrid-augmented arrangement of Azulay and Treister (SIAM J. Sci.
+Comput. 2022), where the network learns what the cycle leaves.  Inside
+flexible CG the preconditioner is z = M0 r + s |r|_rms U(r / |r|_rms, aux),
+and an iteration drops the learned term when it would make <r, z> smaller
+than a tenth of <r, M0 r> (counted in the solver's statistics).
 
 All network arithmetic runs in the library's own HIP kernels
 (ifem_gpu_unet_*); torch only holds device buffers here.
@@ -118,40 +124,93 @@ def jacobi_inverse(s: Session) -> np.ndarray:
     return np.where(d != 0, 1.0 / d, 1.0)
 
 
-def pcg_pairs(s: Session, pb, iters: Sequence[int], torch_device="cuda", features=1):
-    """(input, target) tensors from Jacobi-PCG iterates of an assembled
-    system: x* from a tight solve, x_k = the k-th PCG iterate.  features as
-    in ifem_gpu_set_learned_preconditioner."""
+def pairs(s: Session, pb, base: str = "jacobi", rhs=None, rtol: float = 1e-10, every: int = 1,
+          kmax: int = 100000, torch_device="cuda", features=1, target: str = "defect"):
+    """(input, target) tensors from the iterates of one flexible-CG solve of
+    an assembled system (right-hand side replaced by rhs when given), driven
+    from the host with the device SpMV and base preconditioner M0 = D^-1
+    (base "jacobi") or one V-cycle (base "mg"): every `every`-th iterate
+    x_k before convergence to rtol gives the pair (r_k / |r_k|_rms [, with
+    aux], (x* - x_k - M0 r_k) / |r_k|_rms), x* the converged iterate
+    (target "defect"), or (x* - x_k) / |r_k|_rms (target "error").
+    features as in ifem_gpu_set_learned_preconditioner."""
     import torch
     D, H, W, nc = lattice(pb)
     aux = aux_channels(pb)
     naux = aux.shape[0]
-    s.solve(rtol=1e-12, maxits=200000, check_every=64)
-    xs = torch.from_numpy(s.solution()).to(torch_device)
-    dinv = torch.from_numpy(jacobi_inverse(s)).to(torch_device)
-    b = torch.from_numpy(s.system(values=False)[3]).to(torch_device)
+    if rhs is not None:
+        s.set_rhs(rhs)
+    b = s.rhs()
+    if base == "mg":
+        s.mg_setup()
+        prec = s.mg_apply
+    else:
+        dinv = jacobi_inverse(s)
+        prec = lambda r: dinv * r
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = prec(r)
+    p = z.copy()
+    rz = r @ z
+    bn = np.linalg.norm(b)
+    hist = []
+    for k in range(kmax):
+        if np.linalg.norm(r) <= rtol * bn:
+            break
+        if k % every == 0:
+            hist.append((x.copy(), r.copy(), z.copy()))
+        q = s.spmv(p)
+        a = rz / (p @ q)
+        x += a * p
+        ro = r
+        r = r - a * q
+        z = prec(r)
+        rzn = r @ z
+        p = z + ((rzn - ro @ z) / rz) * p
+        rz = rzn
+    xs = x
     ins, tgs = [], []
     auxt = torch.from_numpy(aux).to(torch_device)
-    for k in iters:
-        st = s.solve(rtol=1e-30, maxits=int(k), check_every=min(int(k), 64))
-        xk = torch.from_numpy(s.solution()).to(torch_device)
-        Ax = torch.empty_like(xk)
-        s.spmv_device(xk.data_ptr(), Ax.data_ptr(), FP64)
-        torch.cuda.synchronize()
-        r = b - Ax
-        rms = torch.sqrt(torch.mean(r * r))
-        if not bool(rms > 0):
+    for xk, rk, zk in hist:
+        rms = float(np.sqrt(np.mean(rk * rk)))
+        if not rms > 0:
             continue
-        e = xs - xk
-        rin = (r / rms).float().reshape(D, H, W, nc).permute(3, 0, 1, 2)
-        tgt = ((e - dinv * r) / rms).float().reshape(D, H, W, nc).permute(3, 0, 1, 2)
+        rin = torch.from_numpy((rk / rms).astype(np.float32)).to(torch_device)
+        rin = rin.reshape(D, H, W, nc).permute(3, 0, 1, 2)
+        e = xs - xk if target == "error" else xs - xk - zk
+        tgt = torch.from_numpy((e / rms).astype(np.float32)).to(torch_device)
+        tgt = tgt.reshape(D, H, W, nc).permute(3, 0, 1, 2)
         if naux and features:
             mod = [rin * auxt[j:j + 1] for j in range(naux)]
-            ins.append(torch.cat([rin] + mod, 0))
+            ins.append(torch.cat([rin] + mod, 0).contiguous())
         else:
-            ins.append(torch.cat([rin, auxt], 0) if naux else rin)
+            ins.append((torch.cat([rin, auxt], 0) if naux else rin).contiguous())
         tgs.append(tgt.contiguous())
     return ins, tgs
+
+
+def pcg_pairs(s: Session, pb, every: int = 4, torch_device="cuda", features=1):
+    """pairs() with the Jacobi base."""
+    return pairs(s, pb, "jacobi", None, every=every, torch_device=torch_device, features=features)
+
+
+def random_rhs(pb, seed: int, nmodes: int = 8, kmax: int = 3) -> np.ndarray:
+    """A seeded smooth right-hand side in equation order: per component a sum
+    of nmodes low-frequency products of sines at the free control points."""
+    n1, n2, n3 = pb.patch.n
+    nc = pb.ncomp
+    X = pb.patch.coefs
+    u = M.uniform(seed, nmodes * 4 * nc).reshape(nc, nmodes, 4)
+    vals = np.zeros((len(X), nc))
+    for c in range(nc):
+        for m in range(nmodes):
+            k = 1 + np.floor(u[c, m, :3] * kmax)
+            vals[:, c] += (2 * u[c, m, 3] - 1) * np.prod(np.sin(np.pi * k[None, :] * X), axis=1)
+    v = vals.reshape(-1)
+    out = np.zeros(pb.neq)
+    free = pb.meqn > 0
+    out[pb.meqn[free] - 1] = v[free]
+    return out * (1.0 / max(1.0, float(pb.patch.nel[0]) ** 3))
 
 
 def default_iters(n: int, kmax: int) -> List[int]:

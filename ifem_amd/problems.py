@@ -48,6 +48,35 @@ class Problem:
         return self.patch.nnod * self.ncomp
 
 
+@dataclasses.dataclass
+class MultiProblem:
+    """Several patches sharing the nodes of their connected faces: mlgn[q]
+    maps patch q's local nodes (GoTools order) to global nodes (1-based);
+    meqn holds ncomp equation numbers per global node; tractions are
+    (patch, face 0..5, t, ng)."""
+    name: str
+    patches: list
+    mlgn: list
+    integrand: M.Integrand
+    ng: Tuple[int, int, int]
+    meqn: np.ndarray
+    neq: int
+    tractions: list
+    sam: Optional[M.SamArrays] = None
+
+    @property
+    def ncomp(self) -> int:
+        return self.integrand.ncomp
+
+    @property
+    def nnod(self) -> int:
+        return int(max(int(m.max()) for m in self.mlgn))
+
+    @property
+    def ndof(self) -> int:
+        return self.nnod * self.ncomp
+
+
 def poisson_cube(nel: int = 16, p: int = 2, ng: Optional[int] = None, jitter: float = 0.0,
                  seed: int = 0) -> Problem:
     P = M.cube_patch(p, nel, jitter, seed)

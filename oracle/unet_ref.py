@@ -10,6 +10,12 @@ Same architecture and flat parameter layout as ifem_amd/csrc/unet.hip:
 Convolutions are 3x3x3, zero padding 1; per conv the weights [co][ci][27]
 (tap index k = kz*9 + ky*3 + kx) are followed by the biases [co].
 Loss: 0.5*sum_n w_n*sum((y_n - t_n)^2)/N (w_n = 1 without weights).
+
+With quant (a rounding function, e.g. to bf16) the tensors that the GPU
+kernels store between layers are rounded as they are there: every hidden
+convolution's output, the pooled inputs, the output gradient, every data
+gradient, the summed up-sampling adjoint and the skip + pooling-adjoint sum.
+The arithmetic itself stays FP64.
 """
 import numpy as np
 
@@ -85,8 +91,10 @@ def up_bwd(g):
     return g.reshape(N, C, D // 2, 2, H // 2, 2, W // 2, 2).sum(axis=(3, 5, 7))
 
 
-def forward_backward(params, x, t, levels, channels, cin, cout, weights=None, linear=False):
+def forward_backward(params, x, t, levels, channels, cin, cout, weights=None, linear=False,
+                     quant=None):
     """Returns (y, loss, grad) in FP64; weights: optional per-sample loss weights."""
+    q = quant if quant is not None else (lambda v: v)
     convs, n = layout(levels, channels, cin, cout)
     P = np.asarray(params, dtype=np.float64)
     act = (lambda v: v) if linear else (lambda v: np.maximum(v, 0.0))
@@ -102,13 +110,13 @@ def forward_backward(params, x, t, levels, channels, cin, cout, weights=None, li
     h = x.astype(np.float64)
     enc_in, enc_a, enc_b = [], [], []
     for l in range(levels):
-        inp = h if l == 0 else pool(enc_b[l - 1])
+        inp = h if l == 0 else q(pool(enc_b[l - 1]))
         enc_in.append(inp)
         w, b = wb(ic); ic += 1
-        a = act(conv(inp, w, b))
+        a = q(act(conv(inp, w, b)))
         enc_a.append(a)
         w, b = wb(ic); ic += 1
-        bb = act(conv(a, w, b))
+        bb = q(act(conv(a, w, b)))
         enc_b.append(bb)
     cur = enc_b[-1]
     dec = {}
@@ -116,9 +124,9 @@ def forward_backward(params, x, t, levels, channels, cin, cout, weights=None, li
         u = up(cur)
         cat = np.concatenate([u, enc_b[l]], axis=1)
         w, b = wb(ic); ic += 1
-        a = act(conv(cat, w, b))
+        a = q(act(conv(cat, w, b)))
         w, b = wb(ic); ic += 1
-        bb = act(conv(a, w, b))
+        bb = q(act(conv(a, w, b)))
         dec[l] = (cat, a, bb)
         cur = bb
     w, b = wb(ic)
@@ -134,32 +142,33 @@ def forward_backward(params, x, t, levels, channels, cin, cout, weights=None, li
         grad[wo:wo + co * ci * 27] = gw.ravel()
         grad[bo:bo + co] = gb
 
-    g = d / N
+    g = q(d / N)
     gx, gw, gb = conv_bwd(cur, wb(ic)[0], g)
     put(ic, gw, gb)
     ic -= 1
-    g = gx
+    g = q(gx)
     skip = {}
     for l in range(0, levels - 1):
         cat, a, bb = dec[l]
         g = g * mask(bb)
         gx, gw, gb = conv_bwd(a, wb(ic)[0], g)
         put(ic, gw, gb); ic -= 1
-        g = gx * mask(a)
+        g = q(gx) * mask(a)
         gx, gw, gb = conv_bwd(cat, wb(ic)[0], g)
         put(ic, gw, gb); ic -= 1
+        gx = q(gx)
         cu = channels[l + 1]
         skip[l] = gx[:, cu:]
-        g = up_bwd(gx[:, :cu])
+        g = q(up_bwd(gx[:, :cu]))
     for l in range(levels - 1, -1, -1):
         if l < levels - 1:
-            g = g + skip[l]
+            g = q(g + skip[l])
         g = g * mask(enc_b[l])
         gx, gw, gb = conv_bwd(enc_a[l], wb(2 * l + 1)[0], g)
         put(2 * l + 1, gw, gb)
-        g = gx * mask(enc_a[l])
+        g = q(gx) * mask(enc_a[l])
         gx, gw, gb = conv_bwd(enc_in[l], wb(2 * l)[0], g)
         put(2 * l, gw, gb)
         if l > 0:
-            g = pool_bwd(gx)
+            g = pool_bwd(q(gx))
     return y, loss, grad

@@ -11,6 +11,7 @@ import types
 import numpy as np
 import pytest
 import scipy.sparse as sp
+import scipy.sparse.linalg as spla
 
 import oracle
 from ifem_amd import FP32, FP64, model as M
@@ -110,7 +111,7 @@ def _gpu_setup(mp, itg, ng, meqn, neq, sam=None, precision=FP64):
     ("elast2p", 2, [4, 3], 4, M.ELASTICITY),
 ])
 def test_multipatch_assembly_and_solve_match_oracle(name, p, nels, nyz, kind):
-    """K, b (1e-12), the PCG solution (1e-10) and the norms of the solution
+    """K, b (1e-12), the PCG solution (1e-12) and the norms of the solution
     (1e-12) of a jittered multi-patch model against the oracle."""
     mp = M.x_split_patches(p, nels, nyz, jitter=0.15, seed=len(name))
     nc = 3 if kind == M.ELASTICITY else 1
@@ -125,18 +126,23 @@ def test_multipatch_assembly_and_solve_match_oracle(name, p, nels, nyz, kind):
     ng = [p + 1] * 3
     print(f"[gpu] multi-patch {name} neq={neq}", flush=True)
     rp, ci, A, b = _oracle_system(mp, itg, ng, meqn, neq)
+    # solved to where both lie within ~2e-14 of the direct solution (FP64
+    # CG stagnates just above 1e-14 on the elasticity model)
+    rtol = 1e-13 if kind == M.ELASTICITY else 1e-14
     with _gpu_setup(mp, itg, ng, meqn, neq) as s:
         grp, gci, gA, gb = s.system()
         assert np.array_equal(grp, rp) and np.array_equal(gci, ci)
         assert np.max(np.abs(gA - A)) <= 1e-12 * np.max(np.abs(A))
         assert np.max(np.abs(gb - b)) <= 1e-12 * np.max(np.abs(b))
-        st = s.solve(rtol=1e-13, maxits=20000)
+        st = s.solve(rtol=rtol, maxits=20000)
         assert st["converged"]
         x = s.solution()
         xd = s.expanded_solution(len(meqn))
         n = s.norms(itg, ng)
-    xo, _, _ = oracle.pcg(rp, ci, A, b, rtol=1e-13)
-    assert np.max(np.abs(x - xo)) <= 1e-10 * np.max(np.abs(xo))
+    xo, _, _ = oracle.pcg(rp, ci, A, b, rtol=rtol)
+    assert np.max(np.abs(x - xo)) <= 1e-12 * np.max(np.abs(xo))
+    xdir = spla.spsolve(sp.csr_matrix((A, ci, rp), shape=(neq, neq)).tocsc(), b)
+    assert np.max(np.abs(x - xdir)) <= 1e-12 * np.max(np.abs(xdir))
     sq = np.zeros(6)
     for P, ml in zip(mp.patches, mp.mlgn):
         sq += oracle.norms(P, itg, ng, _local(xd, ml, nc)) ** 2

@@ -1,8 +1,17 @@
 """Learned preconditioner: the U-net's GPU forward and parameter gradients
-against the FP64 numpy restatement (oracle/unet_ref.py), training, and the
-gain of flexible CG with the trained network over plain Jacobi-PCG on held-out
-coefficient fields.  Tolerance for the FP32 kernels vs the FP64 restatement:
-1e-4 relative (max-abs error over max-abs value)."""
+(BF16 operands on matrix cores, FP32 accumulation, bf16 activations) against
+the FP64 restatement oracle/unet_ref.py (itself pinned by torch autograd and
+finite differences in test_unet_oracle.py), training, and flexible CG with the
+learned correction.
+
+Tolerance for the BF16 kernels: the oracle is fed the same bf16-rounded
+weights and inputs and rounds every tensor the kernels store between layers
+to bf16 as well (its quant option), so the ReLU masks agree and what remains
+is the FP32 accumulation order and values that round to neighbouring bf16
+numbers, which spread from layer to layer (most in the deepest, widest
+network, 12/24/48 channels: 5e-3): relative L2 error <= 1e-2 for the output
+and the parameter gradient, <= 2e-3 for the loss.  A wrong tap, channel or
+adjoint gives errors of order one."""
 import numpy as np
 import pytest
 
@@ -15,25 +24,44 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
 
 
-# seeds: with ReLU, a pre-activation within FP32 rounding of zero flips its
-# mask between the FP32 kernels and the FP64 restatement and changes the
-# gradient by O(1) there; the seeds below keep every pre-activation clear of
-# zero (the linear network has no such ties).
+def _rel2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(np.ravel(b)), 1e-30))
+
+
+def _bf16(a):
+    """Round float32 values to bf16 (nearest even), as the kernels store them."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def _weights_bf16(p, levels, channels, cin, cout):
+    """Parameters with the convolution weights rounded to bf16 (the biases
+    stay FP32, as in the kernels)."""
+    q = p.astype(np.float32).copy()
+    convs, _ = unet_ref.layout(levels, list(channels), cin, cout)
+    for ci, co, relu, w, b in convs:
+        q[w:w + co * ci * 27] = _bf16(q[w:w + co * ci * 27])
+    return q
+
+
 @pytest.mark.parametrize("levels,channels,dims,seed,linear", [
     (3, (4, 6, 8), (8, 8, 4), 3, False), (2, (3, 5), (8, 8, 4), 2, False),
     (1, (4,), (8, 8, 4), 1, False), (4, (3, 4, 5, 6), (16, 16, 8), 4, False),
-    (3, (4, 5, 6), (12, 20, 12), 4, False), (3, (4, 5, 6), (12, 20, 12), 3, True)])
+    (3, (4, 5, 6), (12, 20, 12), 4, False), (3, (4, 5, 6), (12, 20, 12), 3, True),
+    (3, (12, 24, 48), (16, 16, 16), 5, False), (2, (20, 36), (8, 24, 40), 6, False)])
 def test_unet_forward_and_gradient_match_oracle(levels, channels, dims, seed, linear):
     import torch
     from ifem_amd import Session
-    print(f"[gpu] unet gradient check levels={levels} dims={dims} linear={linear}", flush=True)
+    print(f"[gpu] unet gradient check levels={levels} channels={channels} dims={dims} "
+          f"linear={linear}", flush=True)
     (D, H, W), N, cin, cout = dims, 2, 2, 1
     rng = np.random.default_rng(seed)
-    x = rng.standard_normal((N, cin, D, H, W)).astype(np.float32)
+    x = _bf16(rng.standard_normal((N, cin, D, H, W)).astype(np.float32))
     t = rng.standard_normal((N, cout, D, H, W)).astype(np.float32)
     with Session() as s:
         n = s.unet_create(levels, channels, cin, cout, seed=7, linear=linear)
-        assert n == unet_ref.layout(levels, channels, cin, cout)[1]
+        assert n == unet_ref.layout(levels, list(channels), cin, cout)[1]
         p = s.unet_params()
         # non-zero biases so that their gradients and ReLU masks are exercised
         p = p + 0.05 * rng.standard_normal(p.shape).astype(np.float32)
@@ -47,11 +75,16 @@ def test_unet_forward_and_gradient_match_oracle(levels, channels, dims, seed, li
         wd = torch.from_numpy(wts).cuda()
         loss, g = s.unet_gradient(xd.data_ptr(), td.data_ptr(), N, D, H, W, w_ptr=wd.data_ptr())
         y = yd.cpu().numpy()
-    yr, lr_, gr = unet_ref.forward_backward(p.astype(np.float64), x, t, levels, list(channels),
-                                            cin, cout, weights=wts, linear=linear)
-    assert _rel(y, yr) <= 1e-4
-    assert abs(loss - lr_) <= 1e-4 * abs(lr_)
-    assert _rel(g, gr) <= 1e-4
+    pq = _weights_bf16(p, levels, channels, cin, cout).astype(np.float64)
+    yr, lr_, gr = unet_ref.forward_backward(pq, x, t, levels, list(channels), cin, cout,
+                                            weights=wts, linear=linear,
+                                            quant=lambda v: _bf16(v).astype(np.float64))
+    ey, eg = _rel2(y, yr), _rel2(g, gr)
+    print(f"  rel. L2 error: output {ey:.2e}, loss {abs(loss - lr_) / abs(lr_):.2e}, "
+          f"gradient {eg:.2e}", flush=True)
+    assert ey <= 1e-2
+    assert abs(loss - lr_) <= 2e-3 * abs(lr_)
+    assert eg <= 1e-2
 
 
 def test_unet_backward_with_output_gradient():
@@ -72,7 +105,9 @@ def test_unet_backward_with_output_gradient():
         t = (y - N * dy).contiguous()
         _, g_ref = s.unet_gradient(x.data_ptr(), t.data_ptr(), N, D, H, W)
         g = s.unet_backward(x.data_ptr(), dy.data_ptr(), N, D, H, W, want_grad=True)
-    assert _rel(g, g_ref) <= 1e-4
+    # both paths round the same output gradient dy (up to FP32 rounding of
+    # (y - t)/N) to bf16
+    assert _rel2(g, g_ref) <= 1e-3
 
 
 def test_unet_training_reduces_loss():
@@ -96,65 +131,24 @@ def test_unet_training_reduces_loss():
     assert l < 0.2 * l0, (l0, l)
 
 
-def test_learned_flexible_cg_measured_against_jacobi():
-    """Diffusion 16^3 p=2 with log-normal conductivity: train on PCG pairs of
-    6 seeds; on 3 held-out seeds flexible CG with the trained network must
-    reach a true relative residual of 1e-8.  The gain over Jacobi-PCG
-    (iteration ratio) is measured and printed, not assumed: at these sizes
-    Jacobi-PCG needs only ~30 iterations and the network does not beat it."""
-    import torch
-    from ifem_amd import Session, learned, problems, setup_problem
-    print("[gpu] learned preconditioner 16^3", flush=True)
-    spec = learned.NetSpec(levels=3, channels=(8, 12, 16), seed=5)
-    mk = lambda seed: problems.diffusion_cube(16, 2, sigma=1.0, ell=0.2, seed=seed)
-    X, T = [], []
-    for seed in range(100, 106):
-        pb = mk(seed)
-        with setup_problem(pb) as s:
-            ins, tgs = learned.pcg_pairs(s, pb, learned.default_iters(10, 60))
-        X += ins
-        T += tgs
-    X = torch.stack(X).contiguous()
-    T = torch.stack(T).contiguous()
-    tscale = learned.target_scale(T)
-    T = (T / tscale).contiguous()
-    trainer = Session()
-    trainer.unet_create(spec.levels, spec.channels, X.shape[1], T.shape[1], seed=spec.seed)
-    losses = learned.train(trainer, X, T, epochs=30, batch=4, lr=5e-4,
-                           weights=learned.sample_weights(T))
-    assert losses[-1] < losses[0]
-    params = trainer.unet_params()
-    trainer.close()
-    gains = []
-    for seed in (900, 901, 902):
-        pb = mk(seed)
-        with setup_problem(pb) as s:
-            st_j = s.solve(rtol=1e-8, maxits=20000)
-            s.unet_create(spec.levels, spec.channels, X.shape[1], T.shape[1])
-            s.unet_set_params(params)
-            learned.enable(s, pb, scale=tscale)
-            st_l = s.solve(rtol=1e-8, maxits=20000, check_every=4)
-        assert st_j["converged"] and st_l["converged"]
-        assert st_l["rel_residual"] <= 1e-8
-        gains.append(st_j["iterations"] / st_l["iterations"])
-        print(f"  seed {seed}: jacobi {st_j['iterations']} learned {st_l['iterations']}", flush=True)
-    print(f"  iteration gain of the learned preconditioner: {gains}", flush=True)
-
-
-def test_learned_preconditioner_falls_back_to_jacobi():
-    """An untrained network with a wild output scale is not a usable
-    preconditioner: flexible CG breaks down and the solve falls back to
-    Jacobi-PCG from the current iterate, still reaching the tolerance."""
-    from ifem_amd import learned, problems, setup_problem
-    print("[gpu] learned preconditioner fallback", flush=True)
-    pb = problems.diffusion_cube(8, 2, sigma=1.0, ell=0.2, seed=3)
+def test_harmful_correction_is_dropped_by_the_safeguard_without_fallback():
+    """A learned correction that points against the residual (an identity
+    network with a large negative scale: zn = -1e6 |r|_rms r^) is not usable:
+    the safeguard drops it at the start of every flexible-CG pass and in every
+    iteration, and the solve runs on its base (Jacobi or multigrid) to the
+    tolerance without falling back to Jacobi-PCG."""
+    from ifem_amd import PC_JACOBI, PC_MG, learned, problems, setup_problem
+    print("[gpu] learned preconditioner safeguard", flush=True)
+    pb = problems.poisson_cube(8, 2)
     with setup_problem(pb) as s:
-        s.unet_create(2, (4, 4), 2, 1, seed=9)
-        p = s.unet_params()
-        s.unet_set_params(-50.0 * np.abs(p))  # strongly negative output
-        learned.enable(s, pb, scale=1e6)
-        st = s.solve(rtol=1e-8, maxits=5000, check_every=2)
-    assert st["converged"] and st["rel_residual"] <= 1e-8
+        s.unet_create(2, (2, 2), 1, 1, linear=True)
+        s.unet_set_params(_identity_unet_params(2, 2, 1))
+        learned.enable(s, pb, scale=-1e6)
+        for pc in (PC_JACOBI, PC_MG):
+            st = s.solve(rtol=1e-8, maxits=5000, check_every=2, pc=pc)
+            assert st["converged"] and st["rel_residual"] <= 1e-8
+            assert not st["fallback"] and st["pc_iterations"] == st["iterations"]
+            assert st["pc_rejected"] == st["iterations"] + st["outer"], st
 
 
 def _identity_unet_params(levels, c, ncomp):
@@ -219,4 +213,6 @@ def test_learned_preconditioner_on_padded_lattice_matches_numpy_pcg():
         rzn = r @ z
         p = z + (rzn / rz) * p
         rz = rzn
-    assert np.max(np.abs(x - xr)) <= 1e-4 * np.max(np.abs(xr))
+    # the network output is r rounded to bf16 (relative 2^-9): a slightly
+    # perturbed M^-1; a wrong box <-> lattice map would differ by O(1)
+    assert np.max(np.abs(x - xr)) <= 1e-2 * np.max(np.abs(xr))

@@ -2346,7 +2346,9 @@ int batch_assemble(Context& c, int precision, double* ms_out)
   const bool fp32 = precision == IFEM_GPU_FP32;
   k_fill<int><<<1, 64, 0, c.stream>>>(1, B.d_queue, 0);
   IFEM_HIP(hipEventRecord(c.ev0, c.stream));
-  if (B.ngp > 0)
+  // geometry pass: always for FP64 (the full assembly); an FP32 assembly
+  // reuses the records of an FP64 assembly of the same batch
+  if (B.ngp > 0 && !(fp32 && B.geo_valid))
     k_gp_geometry_batch<<<grid_for(B.ngp), 256, 0, c.stream>>>(
         static_cast<const AsmArgs*>(fp32 ? B.d_args32 : B.d_args64), B.d_gp0, (int)B.pp.size(),
         B.ngp, fp32 ? 0 : 1);
@@ -2364,8 +2366,12 @@ int batch_assemble(Context& c, int precision, double* ms_out)
   float ms = 0.f;
   IFEM_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
   if (ms_out) *ms_out = ms;
-  if (fp32) B.fp32_ready = true;
-  else B.fp64_ready = true;
+  if (fp32) {
+    B.fp32_ready = true;
+  } else {
+    B.fp64_ready = true;
+    B.geo_valid = true;
+  }
   return IFEM_GPU_OK;
 }
 

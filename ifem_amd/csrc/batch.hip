@@ -294,7 +294,7 @@ void Batch::release()
     graph[g] = nullptr;
     graph_check[g] = 0;
   }
-  ready = fp64_ready = fp32_ready = tuned = false;
+  ready = fp64_ready = fp32_ready = tuned = geo_valid = false;
   neq = nitems = nchunk = 0;
   nnz = ngp = 0;
   h_gp0.clear();

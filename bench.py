@@ -51,6 +51,300 @@ def measured_traffic(kernel):
     return None
 
 
+def _train_net(LN, Session, spec, X, T, epochs, lr=1e-3):
+    """Adam in three stages of halving learning rate; returns (params,
+    target scale, loss first/last, seconds)."""
+    import time as _t
+    tscale = LN.target_scale(T)
+    Tn = (T / tscale).contiguous()
+    w = LN.sample_weights(Tn)
+    tr = Session()
+    tr.unet_create(spec.levels, spec.channels, X.shape[1], T.shape[1], seed=spec.seed)
+    t0 = _t.perf_counter()
+    first, last = None, None
+    for stage in range(3):
+        losses = LN.train(tr, X, Tn, epochs=max(1, epochs // 3), batch=8, lr=lr * 0.5 ** stage,
+                          weights=w, seed=stage)
+        first = losses[0] if first is None else first
+        last = losses[-1]
+    import torch
+    torch.cuda.synchronize()
+    secs = _t.perf_counter() - t0
+    params = tr.unet_params()
+    tr.close()
+    return params, tscale, (first, last), secs
+
+
+def learned_legs(args, stream, s, pb, x64, solve64_ms):
+    """The learned correction inside flexible CG (library kernels for
+    training and inference):
+    * config 4: diffusion 48^3 p=2, log-normal kappa (sigma 1.5, length 0.1):
+      a 3-level U-net trained on (residual, multigrid defect) pairs from the
+      method's own multigrid flexible-CG iterates over seeded coefficient
+      fields, evaluated on held-out seeds against Jacobi-PCG and multigrid
+      without the network;
+    * config 2: the elasticity network trained the same way at 48^3, applied
+      at 64^3 (learned + FP32: FP32 Krylov, FP32 multigrid and the network
+      inside FP64 refinement), timed against plain FP64 Jacobi-PCG."""
+    import numpy as np
+    import torch
+
+    from ifem_amd import FP32, FP64, PC_JACOBI, PC_MG, Session, learned as LN, problems, \
+        setup_problem
+    spec = LN.NetSpec(levels=3, channels=(12, 24, 48), seed=1)
+    out = {}
+
+    def gen(mk, seed0, npairs, dims=None, rhs_per_seed=2):
+        X, T = [], []
+        seed = seed0
+        while len(X) < npairs:
+            pbq = mk(seed)
+            with setup_problem(pbq) as sq:
+                for j in range(rhs_per_seed):
+                    rhs = None if j == 0 else LN.random_rhs(pbq, 7919 * seed + j)
+                    ins, tgs = LN.pairs(sq, pbq, "mg", rhs, every=1, dims=dims)
+                    X += ins
+                    T += tgs
+            seed += 1
+        return (torch.stack(X[:npairs]).contiguous(), torch.stack(T[:npairs]).contiguous(),
+                seed - seed0)
+
+    # ---- config 4 -------------------------------------------------------------
+    mkd = lambda seed: problems.diffusion_cube(48, 2, sigma=1.5, ell=0.1, seed=seed)
+    t0 = time.perf_counter()
+    X, T, nseeds = gen(mkd, 100, args.learned_pairs)
+    gen_s = time.perf_counter() - t0
+    params, tscale, loss, train_s = _train_net(LN, Session, spec, X, T, args.learned_epochs)
+    cin, cout = int(X.shape[1]), int(T.shape[1])
+    del X, T
+    torch.cuda.empty_cache()
+    # output scale chosen on a validation seed (neither a training nor a test seed)
+    scale, cal = LN.calibrate_scale(mkd(99999), params, spec, cin, cout, tscale)
+    rows = []
+    for seed in range(100000, 100000 + args.learned_eval_seeds):
+        pbd = mkd(seed)
+        with setup_problem(pbd, precision=FP32) as sd:
+            j = sd.solve(rtol=args.rtol, maxits=200000, check_every=64)
+            m = sd.solve(rtol=args.rtol, maxits=5000, pc=PC_MG, check_every=8)
+            sd.unet_create(spec.levels, spec.channels, cin, cout)
+            sd.unet_set_params(params)
+            LN.enable(sd, pbd, scale=scale)
+            l64 = sd.solve(rtol=args.rtol, maxits=5000, pc=PC_MG, check_every=8)
+            l32 = sd.solve(rtol=args.rtol, maxits=5000, pc=PC_MG, precision=FP32, check_every=8)
+        rows.append((j["iterations"], j["solve_ms"], m["iterations"], m["solve_ms"],
+                     l64["iterations"], l64["solve_ms"], l32["iterations"], l32["solve_ms"],
+                     l64["pc_rejected"] + l32["pc_rejected"],
+                     int(l64["fallback"]) + int(l32["fallback"]),
+                     bool(j["converged"] and m["converged"] and l64["converged"] and l32["converged"]
+                          and max(l64["rel_residual"], l32["rel_residual"]) <= args.rtol)))
+    R = np.array([r[:10] for r in rows], dtype=float)
+    med = lambda k: float(np.median(R[:, k]))
+    out["diffusion48"] = {
+        "workload": "diffusion 48^3 p=2, log-normal kappa (sigma 1.5, length 0.1), smooth source, "
+                    "u=0; U-net 3 levels (12,24,48 channels), BF16 operands, FP32 accumulation",
+        "parameters": int(params.size), "train_pairs": args.learned_pairs, "train_seeds": nseeds,
+        "pairs_generation_s": round(gen_s, 1), "epochs": args.learned_epochs,
+        "train_s": round(train_s, 1), "loss_first_last": [round(loss[0], 4), round(loss[1], 4)],
+        "output_scale_factor": round(scale / tscale, 3),
+        "validation_iterations_by_factor": {str(k): v for k, v in cal.items()},
+        "eval_seeds": len(rows),
+        "iterations_median": {"jacobi_fp64": med(0), "multigrid_fp64": med(2),
+                              "learned_mg_fp64": med(4), "learned_mg_fp32": med(6)},
+        "time_to_tol_ms_median": {"jacobi_fp64": round(med(1), 3), "multigrid_fp64": round(med(3), 3),
+                                  "learned_mg_fp64": round(med(5), 3),
+                                  "learned_mg_fp32": round(med(7), 3)},
+        "learned_gain_iterations_vs_multigrid": round(float(np.median(R[:, 2] / R[:, 4])), 3),
+        "learned_gain_time_vs_multigrid": round(float(np.median(R[:, 3] / R[:, 5])), 3),
+        "learned_fp32_speedup_vs_jacobi_fp64": round(float(np.median(R[:, 1] / R[:, 7])), 3),
+        "safeguard_rejections_total": int(R[:, 8].sum()), "fallbacks_total": int(R[:, 9].sum()),
+        "all_converged": bool(all(r[10] for r in rows)),
+    }
+    log(f"[bench] learned diffusion48: {out['diffusion48']}")
+
+    if s is None:
+        return out
+    # ---- config 2: elasticity network trained at 48^3, applied at 64^3 -----------
+    mke = lambda seed: problems.elasticity_cube(48, 2, seed=seed)
+    dims48 = LN.lattice_dims(mke(0), spec.levels)
+    t0 = time.perf_counter()
+    X, T, nseeds = gen(mke, 300, args.learned_elast_pairs, dims=dims48)
+    gen_s = time.perf_counter() - t0
+    params, tscale, loss, train_s = _train_net(LN, Session, spec, X, T, args.learned_elast_epochs)
+    cin, cout = int(X.shape[1]), int(T.shape[1])
+    del X, T
+    torch.cuda.empty_cache()
+    scale, cal = LN.calibrate_scale(mke(999), params, spec, cin, cout, tscale, precision=FP32)
+    dims64 = LN.lattice_dims(pb, spec.levels)
+    with torch.cuda.stream(stream):
+        s.assemble(pb.integrand, pb.ng, FP32)  # FP32 K for the inner Krylov loop
+    stream.synchronize()
+    s.unet_create(spec.levels, spec.channels, cin, cout)
+    s.unet_set_params(params)
+    LN.enable(s, pb, scale=scale, dims=dims64)
+    runs = [s.solve(rtol=args.rtol, maxits=5000, pc=PC_MG, precision=FP32, check_every=8)
+            for _ in range(3)]
+    x = s.solution()
+    s.set_learned(False)
+    base = [s.solve(rtol=args.rtol, maxits=5000, pc=PC_MG, precision=FP32, check_every=8)
+            for _ in range(3)]  # the same solve without the network
+    e = x - x64
+    en = float(np.sqrt(abs(e @ s.spmv(e))) / np.sqrt(x64 @ s.spmv(x64)))
+    mn = float(np.max(np.abs(e)) / np.max(np.abs(x64)))
+    ms = float(np.median([r["solve_ms"] for r in runs]))
+    out["elasticity64"] = {
+        "workload": "elasticity 64^3 p=2 (the benchmark patch); network trained on 48^3 "
+                    "elasticity (seeded jitter, random smooth loads), applied on the padded "
+                    f"{dims64[0]}x{dims64[1]}x{dims64[2]} lattice",
+        "train_pairs": args.learned_elast_pairs, "train_seeds": nseeds,
+        "pairs_generation_s": round(gen_s, 1), "epochs": args.learned_elast_epochs,
+        "train_s": round(train_s, 1), "loss_first_last": [round(loss[0], 4), round(loss[1], 4)],
+        "output_scale_factor": round(scale / tscale, 3),
+        "validation_iterations_by_factor": {str(k): v for k, v in cal.items()},
+        "learned_fp32_solve_ms": round(ms, 3), "iterations": runs[-1]["iterations"],
+        "outer": runs[-1]["outer"], "rel_residual": runs[-1]["rel_residual"],
+        "converged": all(r["converged"] for r in runs),
+        "safeguard_rejections": runs[-1]["pc_rejected"], "fallback": runs[-1]["fallback"],
+        "energy_norm_err_vs_fp64": en, "max_nodal_err_vs_fp64": mn,
+        "speedup_vs_fp64_jacobi": round(solve64_ms / ms, 3),
+        "multigrid_fp32_without_network": {
+            "solve_ms": round(float(np.median([r["solve_ms"] for r in base])), 3),
+            "iterations": base[-1]["iterations"]},
+        "learned_gain_time_vs_multigrid": round(float(np.median([r["solve_ms"] for r in base]))
+                                                / ms, 3),
+    }
+    return out
+
+
+def large_leg(args, stream):
+    """Largest single-GPU case: elasticity 96^3 p=3, 4x4x4 Gauss points."""
+    import numpy as np
+    import torch
+    from ifem_amd import FP32, FP64, Session, problems
+    from ifem_amd import PC_MG
+    pbL = problems.elasticity_cube(96, 3)
+    log(f"[bench] large: elasticity 96^3 p=3, ndof={pbL.ndof}")
+    with Session(stream=stream.cuda_stream) as sL:
+        sL.bind(sL.add_patch(pbL.patch), 3, pbL.meqn, pbL.neq)
+        t0 = time.perf_counter()
+        nnzL = sL.init_assembly()
+        symL = (time.perf_counter() - t0) * 1e3
+        faceL, tracL, ngL = pbL.traction
+        runs = []
+        for _ in range(3):
+            with torch.cuda.stream(stream):
+                sL.assemble(pbL.integrand, pbL.ng, FP64)
+                sL.add_traction(faceL, tracL, ngL)
+            stream.synchronize()
+            runs.append((sL.timings()[0], *sL.assembly_parts()))
+        kL = float(np.median([r[0] for r in runs]))
+        gL = float(np.median([r[1] for r in runs]))
+        lL = float(np.median([r[2] for r in runs]))
+        nenL = pbL.patch.nen
+        flopsL = (36 * nenL * nenL + 6 * nenL) * int(np.prod(pbL.ng)) * int(np.prod(pbL.patch.nel))
+        if args.assembly_only:
+            return {"assembly_kernels_ms": round(kL, 3), "line_kernel_ms": round(lL, 3),
+                    "geometry_pass_ms": round(gL, 3), "nnz": nnzL}
+        st64 = sL.solve(rtol=args.rtol, maxits=200000, check_every=64)
+        xL = sL.solution()
+        KxL = sL.spmv(xL)
+
+        def errL(x):
+            e = x - xL
+            return (float(np.sqrt(abs(e @ sL.spmv(e))) / np.sqrt(xL @ KxL)),
+                    float(np.max(np.abs(e)) / np.max(np.abs(xL))))
+        with torch.cuda.stream(stream):
+            sL.assemble(pbL.integrand, pbL.ng, FP32)
+        stream.synchronize()
+        a32 = sL.timings()[0]
+        stm = sL.solve(rtol=args.rtol, maxits=200000, precision=FP32, inner_rtol=1e-4,
+                       check_every=64)
+        em = errL(sL.solution())
+        stg = [sL.solve(rtol=args.rtol, maxits=5000, pc=PC_MG, check_every=8) for _ in range(2)]
+        eg = errL(sL.solution())
+        large = {
+            "workload": "elasticity 96^3 p=3 jittered cube, 4x4x4 Gauss, clamped x=0, "
+                        "traction (0,0,-1) on x=1",
+            "dofs": pbL.ndof, "neq": pbL.neq, "nnz": nnzL, "symbolic_ms": round(symL, 1),
+            "assembly_kernels_ms": round(kL, 3),
+            "assembled_dof_per_s": round(pbL.ndof / (kL * 1e-3), 1),
+            "line_kernel_ms": round(lL, 3), "geometry_pass_ms": round(gL, 3),
+            "line_kernel_tflops": round(flopsL / (lL * 1e-3) / 1e12, 3),
+            "line_kernel_frac_fp64_peak": round(flopsL / (lL * 1e-3) / PEAK_FP64, 4),
+            "fp64": {"solve_ms": round(st64["solve_ms"], 3), "iterations": st64["iterations"],
+                     "rel_residual": st64["rel_residual"], "converged": st64["converged"]},
+            "mixed": {"fp32_assembly_kernels_ms": round(a32, 3),
+                      "solve_ms": round(stm["solve_ms"], 3), "iterations": stm["iterations"],
+                      "outer": stm["outer"], "rel_residual": stm["rel_residual"],
+                      "converged": stm["converged"], "energy_norm_err_vs_fp64": em[0],
+                      "max_nodal_err_vs_fp64": em[1]},
+            "multigrid": {"solve_ms": round(stg[1]["solve_ms"], 3),
+                          "first_solve_ms": round(stg[0]["solve_ms"], 3),
+                          "iterations": stg[1]["iterations"],
+                          "rel_residual": stg[1]["rel_residual"],
+                          "converged": all(r["converged"] for r in stg),
+                          "energy_norm_err_vs_fp64": eg[0], "max_nodal_err_vs_fp64": eg[1]},
+            "speedup_mg_vs_fp64": round(st64["solve_ms"] / stg[1]["solve_ms"], 3),
+        }
+    return large
+
+
+
+def batch_leg(args):
+    """256 uneven diffusion patches: one persistent assembly launch per precision."""
+    import numpy as np
+    from ifem_amd import FP32, FP64, problems
+    from ifem_amd import setup_batch
+    bpbs = problems.diffusion_batch()
+    bdofs = int(sum(q.ndof for q in bpbs))
+    bs = setup_batch(bpbs, precisions=())
+    tune = bs.batch_autotune(FP32)
+    bt = {}
+    for prec in (FP64, FP32):
+        ts = []
+        for _ in range(3):
+            bs.batch_assemble(prec)
+            ts.append(bs.batch_timings()["assemble_ms"])
+        bt[prec] = float(np.median(ts))
+    if args.assembly_only:
+        bs.close()
+        return {"assembly_fp64_ms": round(bt[FP64], 2), "assembly_fp32_ms": round(bt[FP32], 2)}
+    b64 = bs.batch_solve(rtol=args.rtol, maxits=20000, check_every=32)
+    bmx = bs.batch_solve(rtol=args.rtol, maxits=20000, precision=FP32, check_every=32)
+    degs = [q.patch.p[0] for q in bpbs]
+    batch = {
+        "workload": "256 diffusion patches, nel per direction U{8..48}, p in {2,3}, "
+                    "log-normal kappa (sigma 1, length 0.2), smooth source, u=0",
+        "patches": len(bpbs), "p2": degs.count(2), "p3": degs.count(3),
+        "dofs": bdofs, "neq": int(sum(q.neq for q in bpbs)),
+        "symbolic_ms": round(bs.batch_timings()["symbolic_ms"], 2),
+        "assembly_fp64_ms": round(bt[FP64], 2), "assembly_fp32_ms": round(bt[FP32], 2),
+        "assembly_fp32_note": "FP32 values after an FP64 assembly of the same batch: the "
+                              "Gauss-point geometry records of the FP64 pass are reused",
+        "assembled_dof_per_s_fp64": round(bdofs / (bt[FP64] * 1e-3), 1),
+        "fp64_solve_ms": round(b64["solve_ms"], 2), "fp64_max_iterations": b64["iterations"],
+        "mixed_solve_ms": round(bmx["solve_ms"], 2), "mixed_max_iterations": bmx["iterations"],
+        "mixed_outer": bmx["outer"],
+        "all_converged": bool(b64["converged"] and bmx["converged"]),
+        "max_rel_residual": float(max(b64["rel_residual"], bmx["rel_residual"])),
+        "tuning": {f"p{p}": tune[p] for p in (2, 3)},
+    }
+    bs.close()
+    return batch
+
+
+
+def copy_calibration(stream):
+    """A 1 GiB device-to-device copy: the reference kernel whose FETCH_SIZE /
+    WRITE_SIZE counters scale the per-kernel HBM traffic."""
+    import torch
+    src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    with torch.cuda.stream(stream):
+        dst.copy_(src)
+    stream.synchronize()
+    log("[bench] copy calibration: 1 GiB read + 1 GiB written")
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,13 +357,20 @@ def parse():
     ap.add_argument("--no-mg", action="store_true", help="skip the multigrid-preconditioned leg")
     ap.add_argument("--no-multipatch", action="store_true",
                     help="skip the two-patch assembly leg")
-    ap.add_argument("--learned", action="store_true",
-                    help="run the learned-preconditioner leg (48^3 heterogeneous diffusion: "
-                         "train on PCG pairs, compare with Jacobi-PCG on held-out seeds)")
-    ap.add_argument("--learned-train-seeds", type=int, default=20)
-    ap.add_argument("--learned-pairs-per-seed", type=int, default=100)
-    ap.add_argument("--learned-eval-seeds", type=int, default=20)
-    ap.add_argument("--learned-epochs", type=int, default=4)
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-preconditioner leg")
+    ap.add_argument("--learned-pairs", type=int, default=2000,
+                    help="training pairs of the 48^3 diffusion network")
+    ap.add_argument("--learned-epochs", type=int, default=30)
+    ap.add_argument("--learned-eval-seeds", type=int, default=200)
+    ap.add_argument("--learned-elast-pairs", type=int, default=1500,
+                    help="training pairs of the elasticity network (48^3)")
+    ap.add_argument("--learned-elast-epochs", type=int, default=30)
+    ap.add_argument("--only", choices=["large", "batch", "learned"], default=None,
+                    help="run only that leg and print its record (for kernel profiles of one "
+                         "workload)")
+    ap.add_argument("--assembly-only", action="store_true",
+                    help="with --only large/batch: assemble, skip the solves (counter passes)")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the 96^3 p=3 elasticity leg (largest single-GPU case)")
     ap.add_argument("--no-batch", action="store_true",
@@ -99,6 +400,18 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     stream = torch.cuda.Stream()  # default priority
+    if args.only and args.copy_calibration:
+        copy_calibration(stream)
+    if args.only == "large":
+        print(json.dumps({"large": large_leg(args, stream)}), flush=True)
+        return
+    if args.only == "batch":
+        print(json.dumps({"batch": batch_leg(args)}), flush=True)
+        return
+    if args.only == "learned":  # the 48^3 diffusion part (config 4) alone
+        print(json.dumps({"learned": learned_legs(args, stream, None, None, None, None)}),
+              flush=True)
+        return
     pb = problems.elasticity_cube(args.nel, 2)
     ndof, neq = pb.ndof, pb.neq
     nel_tot = int(np.prod(pb.patch.nel))
@@ -169,12 +482,7 @@ def main():
     spmv_bytes = nnz * 8 + (nnz // 9) * 4 + (neq // 3) * 2 * 8 + 2 * neq * 8
 
     if args.copy_calibration:
-        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
-        dst = torch.empty_like(src)
-        with torch.cuda.stream(stream):
-            dst.copy_(src)
-        stream.synchronize()
-        log("[bench] copy calibration: 1 GiB read + 1 GiB written")
+        copy_calibration(stream)
 
     # ---- mixed precision: FP32 inner PCG + FP64 refinement ---------------------
     mixed = None
@@ -263,7 +571,7 @@ def main():
             "workload": "elasticity p=2, two 16x32x32-element patches sharing the x=0.5 face, "
                         "clamped x=0",
             "dofs": int(len(mmeqn)), "neq": int(mneq), "nnz": int(mnnz),
-            "host_pattern_ms": round(msym, 1), "assembly_kernels_ms": round(sp_ms, 3),
+            "pattern_ms": round(msym, 1), "pattern_built_on": "device", "assembly_kernels_ms": round(sp_ms, 3),
             "assembled_dof_per_s": round(len(mmeqn) / (sp_ms * 1e-3), 1),
             "one_patch_32cubed_assembly_kernels_ms": round(float(np.median(ot[1:])), 3),
         }
@@ -271,167 +579,19 @@ def main():
         os_.close()
         log(f"[bench] multipatch: {multipatch}")
 
-    # ---- learned preconditioner: 48^3 p=2 heterogeneous diffusion -------------
+    # ---- learned preconditioner: 48^3 diffusion (config 4) and 64^3 elasticity --
     learned_leg = None
-    if args.learned:
-        from ifem_amd import learned as LN, setup_problem
-        mkd = lambda seed: problems.diffusion_cube(48, 2, sigma=1.5, ell=0.1, seed=seed)
-        spec = LN.NetSpec(levels=3, channels=(8, 16, 48), seed=5)
-        t0 = time.perf_counter()
-        X, T = [], []
-        for seed in range(1000, 1000 + args.learned_train_seeds):
-            pbd = mkd(seed)
-            with setup_problem(pbd) as sd:
-                ins, tgs = LN.pcg_pairs(sd, pbd, LN.default_iters(args.learned_pairs_per_seed, 300))
-            X += ins
-            T += tgs
-        X = torch.stack(X).contiguous()
-        T = torch.stack(T).contiguous()
-        pairs_s = time.perf_counter() - t0
-        npairs, cin, cout = int(X.shape[0]), int(X.shape[1]), int(T.shape[1])
-        tscale = LN.target_scale(T)
-        T = (T / tscale).contiguous()
-        tr = Session()
-        tr.unet_create(spec.levels, spec.channels, X.shape[1], T.shape[1], seed=spec.seed)
-        t0 = time.perf_counter()
-        losses = LN.train(tr, X, T, epochs=args.learned_epochs, batch=8, lr=5e-4,
-                          weights=LN.sample_weights(T))
-        train_s = time.perf_counter() - t0
-        params = tr.unet_params()
-        nparam = int(params.size)
-        tr.close()
-        del X, T
-        evals = []
-        for seed in range(5000, 5000 + args.learned_eval_seeds):
-            pbd = mkd(seed)
-            with setup_problem(pbd) as sd:
-                sj = sd.solve(rtol=args.rtol, maxits=200000, check_every=64)
-                sd.unet_create(spec.levels, spec.channels, cin, cout)
-                sd.unet_set_params(params)
-                LN.enable(sd, pbd, scale=tscale)
-                sl = sd.solve(rtol=args.rtol, maxits=200000, check_every=4)
-            evals.append((sj["iterations"], sj["solve_ms"], sl["iterations"], sl["solve_ms"],
-                          sl["converged"] and sl["rel_residual"] <= args.rtol))
-        ev = np.array([e[:4] for e in evals], dtype=float)
-        learned_leg = {
-            "workload": "diffusion 48^3 p=2, log-normal kappa (sigma 1.5, length 0.1), smooth "
-                        "source, u=0; U-net levels 3 channels (8,16,48), FP32",
-            "train_pairs": npairs,
-            "train_seeds": args.learned_train_seeds, "pairs_generation_s": round(pairs_s, 1),
-            "epochs": args.learned_epochs, "train_s": round(train_s, 1),
-            "loss_first_last": [losses[0], losses[-1]], "parameters": nparam,
-            "eval_seeds": args.learned_eval_seeds,
-            "jacobi_iterations_median": float(np.median(ev[:, 0])),
-            "learned_iterations_median": float(np.median(ev[:, 2])),
-            "iteration_ratio_median": float(np.median(ev[:, 0] / ev[:, 2])),
-            "jacobi_solve_ms_median": round(float(np.median(ev[:, 1])), 3),
-            "learned_solve_ms_median": round(float(np.median(ev[:, 3])), 3),
-            "all_learned_converged": bool(all(e[4] for e in evals)),
-        }
+    if not args.no_learned:
+        learned_leg = learned_legs(args, stream, s, pb, x64, solve64_ms)
         log(f"[bench] learned: {learned_leg}")
 
     # ---- largest single-GPU case: 96^3 p=3 elasticity, 4x4x4 Gauss points -----
-    large = None
-    if not args.no_large:
-        from ifem_amd import PC_MG
-        pbL = problems.elasticity_cube(96, 3)
-        log(f"[bench] large: elasticity 96^3 p=3, ndof={pbL.ndof}")
-        with Session(stream=stream.cuda_stream) as sL:
-            sL.bind(sL.add_patch(pbL.patch), 3, pbL.meqn, pbL.neq)
-            t0 = time.perf_counter()
-            nnzL = sL.init_assembly()
-            symL = (time.perf_counter() - t0) * 1e3
-            faceL, tracL, ngL = pbL.traction
-            runs = []
-            for _ in range(3):
-                with torch.cuda.stream(stream):
-                    sL.assemble(pbL.integrand, pbL.ng, FP64)
-                    sL.add_traction(faceL, tracL, ngL)
-                stream.synchronize()
-                runs.append((sL.timings()[0], *sL.assembly_parts()))
-            kL = float(np.median([r[0] for r in runs]))
-            gL = float(np.median([r[1] for r in runs]))
-            lL = float(np.median([r[2] for r in runs]))
-            nenL = pbL.patch.nen
-            flopsL = (36 * nenL * nenL + 6 * nenL) * int(np.prod(pbL.ng)) * int(np.prod(pbL.patch.nel))
-            st64 = sL.solve(rtol=args.rtol, maxits=200000, check_every=64)
-            xL = sL.solution()
-            KxL = sL.spmv(xL)
-
-            def errL(x):
-                e = x - xL
-                return (float(np.sqrt(abs(e @ sL.spmv(e))) / np.sqrt(xL @ KxL)),
-                        float(np.max(np.abs(e)) / np.max(np.abs(xL))))
-            with torch.cuda.stream(stream):
-                sL.assemble(pbL.integrand, pbL.ng, FP32)
-            stream.synchronize()
-            a32 = sL.timings()[0]
-            stm = sL.solve(rtol=args.rtol, maxits=200000, precision=FP32, inner_rtol=1e-4,
-                           check_every=64)
-            em = errL(sL.solution())
-            stg = [sL.solve(rtol=args.rtol, maxits=5000, pc=PC_MG, check_every=8) for _ in range(2)]
-            eg = errL(sL.solution())
-            large = {
-                "workload": "elasticity 96^3 p=3 jittered cube, 4x4x4 Gauss, clamped x=0, "
-                            "traction (0,0,-1) on x=1",
-                "dofs": pbL.ndof, "neq": pbL.neq, "nnz": nnzL, "symbolic_ms": round(symL, 1),
-                "assembly_kernels_ms": round(kL, 3),
-                "assembled_dof_per_s": round(pbL.ndof / (kL * 1e-3), 1),
-                "line_kernel_ms": round(lL, 3), "geometry_pass_ms": round(gL, 3),
-                "line_kernel_tflops": round(flopsL / (lL * 1e-3) / 1e12, 3),
-                "line_kernel_frac_fp64_peak": round(flopsL / (lL * 1e-3) / PEAK_FP64, 4),
-                "fp64": {"solve_ms": round(st64["solve_ms"], 3), "iterations": st64["iterations"],
-                         "rel_residual": st64["rel_residual"], "converged": st64["converged"]},
-                "mixed": {"fp32_assembly_kernels_ms": round(a32, 3),
-                          "solve_ms": round(stm["solve_ms"], 3), "iterations": stm["iterations"],
-                          "outer": stm["outer"], "rel_residual": stm["rel_residual"],
-                          "converged": stm["converged"], "energy_norm_err_vs_fp64": em[0],
-                          "max_nodal_err_vs_fp64": em[1]},
-                "multigrid": {"solve_ms": round(stg[1]["solve_ms"], 3),
-                              "first_solve_ms": round(stg[0]["solve_ms"], 3),
-                              "iterations": stg[1]["iterations"],
-                              "rel_residual": stg[1]["rel_residual"],
-                              "converged": all(r["converged"] for r in stg),
-                              "energy_norm_err_vs_fp64": eg[0], "max_nodal_err_vs_fp64": eg[1]},
-                "speedup_mg_vs_fp64": round(st64["solve_ms"] / stg[1]["solve_ms"], 3),
-            }
-        log(f"[bench] large: {large}")
+    large = None if args.no_large else large_leg(args, stream)
+    log(f"[bench] large: {large}")
 
     # ---- batched uneven patches: one persistent assembly launch ----------------
-    batch = None
-    if not args.no_batch:
-        from ifem_amd import setup_batch
-        bpbs = problems.diffusion_batch()
-        bdofs = int(sum(q.ndof for q in bpbs))
-        bs = setup_batch(bpbs, precisions=())
-        tune = bs.batch_autotune(FP32)
-        bt = {}
-        for prec in (FP64, FP32):
-            ts = []
-            for _ in range(3):
-                bs.batch_assemble(prec)
-                ts.append(bs.batch_timings()["assemble_ms"])
-            bt[prec] = float(np.median(ts))
-        b64 = bs.batch_solve(rtol=args.rtol, maxits=20000, check_every=32)
-        bmx = bs.batch_solve(rtol=args.rtol, maxits=20000, precision=FP32, check_every=32)
-        degs = [q.patch.p[0] for q in bpbs]
-        batch = {
-            "workload": "256 diffusion patches, nel per direction U{8..48}, p in {2,3}, "
-                        "log-normal kappa (sigma 1, length 0.2), smooth source, u=0",
-            "patches": len(bpbs), "p2": degs.count(2), "p3": degs.count(3),
-            "dofs": bdofs, "neq": int(sum(q.neq for q in bpbs)),
-            "symbolic_ms": round(bs.batch_timings()["symbolic_ms"], 2),
-            "assembly_fp64_ms": round(bt[FP64], 2), "assembly_fp32_ms": round(bt[FP32], 2),
-            "assembled_dof_per_s_fp64": round(bdofs / (bt[FP64] * 1e-3), 1),
-            "fp64_solve_ms": round(b64["solve_ms"], 2), "fp64_max_iterations": b64["iterations"],
-            "mixed_solve_ms": round(bmx["solve_ms"], 2), "mixed_max_iterations": bmx["iterations"],
-            "mixed_outer": bmx["outer"],
-            "all_converged": bool(b64["converged"] and bmx["converged"]),
-            "max_rel_residual": float(max(b64["rel_residual"], bmx["rel_residual"])),
-            "tuning": {f"p{p}": tune[p] for p in (2, 3)},
-        }
-        bs.close()
-        log(f"[bench] batch: {batch}")
+    batch = None if args.no_batch else batch_leg(args)
+    log(f"[bench] batch: {batch}")
 
     # ---- CPU baseline: the oracle on a bounded sample ----------------------------
     cpu = None
@@ -487,7 +647,10 @@ def main():
     achieved = flops / (lines_ms * 1e-3)
     traffic = measured_traffic("ifem::k_assemble_lines<2, 3, 3, double, 1, false, 0>") \
         if args.nel == 64 else None
+    # compute-bound: FP64 FMAs on the vector ALUs (MI355X's FP64 vector and
+    # FP64 matrix peaks are the same 78.6 TFLOP/s)
     roofline = {"kernel": "k_assemble_lines<2,3,3,double,1,false,0>", "bound": "mfma",
+                "engine": "valu_fp64",
                 "achieved": round(achieved / 1e12, 3), "peak": PEAK_FP64 / 1e12,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64, 4), "traffic": traffic,
                 "algorithmic_flops": flops, "algorithmic_bytes": asm_bytes,
@@ -522,6 +685,10 @@ def main():
         "speedup_mg_vs_fp64": (round(solve64_ms / min(v["solve_ms"] for k, v in mg.items()
                                                       if isinstance(v, dict)), 3)
                                if mg else None),
+        "learned_fp32_time_to_tol_ms": (learned_leg["elasticity64"]["learned_fp32_solve_ms"]
+                                        if learned_leg else None),
+        "speedup_learned_fp32_vs_fp64": (learned_leg["elasticity64"]["speedup_vs_fp64_jacobi"]
+                                         if learned_leg else None),
         "roofline": roofline, "roofline_spmv": spmv_roof, "cpu_baseline": cpu,
         "batch": batch,
         "multipatch": multipatch,

@@ -1543,32 +1543,31 @@ int sub_position_maps(Context& c, std::vector<SubPatch>& subs, const std::vector
   return IFEM_GPU_OK;
 }
 
+int device_union_pattern(Context& c, std::vector<SubPatch>& subs, int neq, int64_t** rowptr_out,
+                         int32_t** colind_out, int64_t* nnz_out);
+
+// Several patches sharing nodes: the union pattern and the per-patch
+// position maps are built on the device (pattern.hip).
 static int build_pattern_multi(Context& c)
 {
   const int neq = c.neq;
-  std::vector<int64_t> rowptr;
-  std::vector<int32_t> colind;
-  int rc0 = host_union_pattern(c, c.multi, neq, rowptr, colind);
-  if (!rc0) rc0 = sub_position_maps(c, c.multi, rowptr, colind);
+  int64_t* rowptr = nullptr;
+  int32_t* colind = nullptr;
+  int64_t nnz = 0;
+  int rc0 = device_union_pattern(c, c.multi, neq, &rowptr, &colind, &nnz);
   if (rc0) return rc0;
-  const int64_t nnz = rowptr[neq];
   hipFree(c.d_rowptr);
   hipFree(c.d_colind);
   hipFree(c.d_val);
   hipFree(c.d_val32);
   hipFree(c.d_rhs);
-  c.d_rowptr = nullptr;
-  c.d_colind = nullptr;
+  c.d_rowptr = rowptr;
+  c.d_colind = colind;
   c.d_val = nullptr;
   c.d_val32 = nullptr;
   c.d_rhs = nullptr;
   c.fp64_ready = c.fp32_ready = false;
   c.nnz = nnz;
-  IFEM_HIP(hipMalloc(&c.d_rowptr, sizeof(int64_t) * (neq + 1)));
-  IFEM_HIP(hipMemcpy(c.d_rowptr, rowptr.data(), sizeof(int64_t) * (neq + 1), hipMemcpyHostToDevice));
-  IFEM_HIP(hipMalloc(&c.d_colind, sizeof(int32_t) * colind.size()));
-  IFEM_HIP(hipMemcpy(c.d_colind, colind.data(), sizeof(int32_t) * colind.size(),
-                     hipMemcpyHostToDevice));
   IFEM_HIP(hipMalloc(&c.d_val, sizeof(double) * (nnz > 0 ? nnz : 1)));
   IFEM_HIP(hipMalloc(&c.d_rhs, sizeof(double) * (neq > 0 ? neq : 1)));
   IFEM_HIP(hipMemsetAsync(c.d_rhs, 0, sizeof(double) * (neq > 0 ? neq : 1), c.stream));

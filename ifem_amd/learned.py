@@ -125,14 +125,16 @@ def jacobi_inverse(s: Session) -> np.ndarray:
 
 
 def pairs(s: Session, pb, base: str = "jacobi", rhs=None, rtol: float = 1e-10, every: int = 1,
-          kmax: int = 100000, torch_device="cuda", features=1, target: str = "defect"):
+          kmax: int = 100000, torch_device="cuda", features=1, target: str = "defect",
+          dims=None):
     """(input, target) tensors from the iterates of one flexible-CG solve of
     an assembled system (right-hand side replaced by rhs when given), driven
     from the host with the device SpMV and base preconditioner M0 = D^-1
     (base "jacobi") or one V-cycle (base "mg"): every `every`-th iterate
     x_k before convergence to rtol gives the pair (r_k / |r_k|_rms [, with
     aux], (x* - x_k - M0 r_k) / |r_k|_rms), x* the converged iterate
-    (target "defect"), or (x* - x_k) / |r_k|_rms (target "error").
+    (target "defect"), or (x* - x_k) / |r_k|_rms (target "error"), on the
+    free-DOF box zero-padded to dims (default: the box itself).
     features as in ifem_gpu_set_learned_preconditioner."""
     import torch
     D, H, W, nc = lattice(pb)
@@ -170,16 +172,25 @@ def pairs(s: Session, pb, base: str = "jacobi", rhs=None, rtol: float = 1e-10, e
         rz = rzn
     xs = x
     ins, tgs = [], []
-    auxt = torch.from_numpy(aux).to(torch_device)
+    pdims = tuple(dims) if dims is not None else (D, H, W)
+    auxt = torch.from_numpy(padded_aux(pb, pdims)).to(torch_device)
+
+    def pad(t):
+        if pdims == (D, H, W):
+            return t
+        out = torch.zeros((t.shape[0],) + pdims, dtype=t.dtype, device=t.device)
+        out[:, :D, :H, :W] = t
+        return out
+
     for xk, rk, zk in hist:
         rms = float(np.sqrt(np.mean(rk * rk)))
         if not rms > 0:
             continue
         rin = torch.from_numpy((rk / rms).astype(np.float32)).to(torch_device)
-        rin = rin.reshape(D, H, W, nc).permute(3, 0, 1, 2)
+        rin = pad(rin.reshape(D, H, W, nc).permute(3, 0, 1, 2))
         e = xs - xk if target == "error" else xs - xk - zk
         tgt = torch.from_numpy((e / rms).astype(np.float32)).to(torch_device)
-        tgt = tgt.reshape(D, H, W, nc).permute(3, 0, 1, 2)
+        tgt = pad(tgt.reshape(D, H, W, nc).permute(3, 0, 1, 2))
         if naux and features:
             mod = [rin * auxt[j:j + 1] for j in range(naux)]
             ins.append(torch.cat([rin] + mod, 0).contiguous())
@@ -288,3 +299,24 @@ def padded_aux(pb, dims) -> np.ndarray:
     out = np.zeros((aux.shape[0], D, H, W), dtype=np.float32)
     out[:, :aux.shape[1], :aux.shape[2], :aux.shape[3]] = aux
     return out
+
+
+def calibrate_scale(pb, params, spec, cin, cout, tscale, candidates=(1.0, 0.5, 0.25), pc=None,
+                    rtol=1e-8, precision=None):
+    """Output scale of a trained network, chosen on a validation problem pb
+    (not a test problem): the factor in candidates (times the training
+    target scale) with the fewest flexible-CG iterations to rtol, ties to the
+    larger factor.  Returns (scale, {factor: iterations})."""
+    from ._native import FP64, PC_MG
+    pc = PC_MG if pc is None else pc
+    precision = FP64 if precision is None else precision
+    its = {}
+    with setup_problem(pb, precision=precision) as s:
+        s.unet_create(spec.levels, spec.channels, cin, cout)
+        s.unet_set_params(params)
+        for f in candidates:
+            enable(s, pb, scale=tscale * f, dims=lattice_dims(pb, spec.levels))
+            st = s.solve(rtol=rtol, maxits=5000, pc=pc, precision=precision, check_every=8)
+            its[f] = st["iterations"] if st["converged"] else 10 ** 9
+    best = min(candidates, key=lambda f: (its[f], -f))
+    return tscale * best, its

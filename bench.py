@@ -40,13 +40,16 @@ def measured_traffic(kernel):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (calibrated against a 1 GiB copy), or None."""
     fn = os.path.join(ROOT, "profiles", "hbm_traffic_elasticity64.md")
+    col = None
     try:
         with open(fn) as f:
             for line in f:
                 cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
-                if len(cells) >= 4 and cells[0] == kernel:
-                    return round(float(cells[3]) * 1e9)
-    except OSError:
+                if cells and cells[0] == "kernel":
+                    col = next((i for i, c in enumerate(cells) if c.startswith("traffic GB")), None)
+                elif col is not None and len(cells) > col and cells[0] == kernel:
+                    return round(float(cells[col]) * 1e9)
+    except (OSError, ValueError):
         pass
     return None
 
@@ -73,6 +76,31 @@ def _train_net(LN, Session, spec, X, T, epochs, lr=1e-3):
     params = tr.unet_params()
     tr.close()
     return params, tscale, (first, last), secs
+
+
+def _net_forward(Session, stream, spec, params, cin, cout, dims, reps=20):
+    """Device time of one network application (one sample, the solver's
+    use) on a D x H x W lattice: HIP events on the library's stream around
+    reps forward calls; with its forward work in TFLOP/s."""
+    import torch
+    D, H, W = dims
+    x = torch.randn((1, cin, D, H, W), device="cuda", dtype=torch.float32)
+    y = torch.empty((1, cout, D, H, W), device="cuda", dtype=torch.float32)
+    torch.cuda.synchronize()
+    with Session(stream=stream.cuda_stream) as t:
+        t.unet_create(spec.levels, spec.channels, cin, cout)
+        t.unet_set_params(params)
+        t.unet_forward(x.data_ptr(), y.data_ptr(), 1, D, H, W)  # workspace, warm-up
+        stream.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            t.unet_forward(x.data_ptr(), y.data_ptr(), 1, D, H, W)
+        e1.record(stream)
+        e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return {"lattice": f"{D}x{H}x{W}", "ms": round(ms, 4),
+            "tflops": round(spec.flops(cin, cout, dims) / ms / 1e9, 2)}
 
 
 def learned_legs(args, stream, s, pb, x64, solve64_ms):
@@ -120,6 +148,7 @@ def learned_legs(args, stream, s, pb, x64, solve64_ms):
     torch.cuda.empty_cache()
     # output scale chosen on a validation seed (neither a training nor a test seed)
     scale, cal = LN.calibrate_scale(mkd(99999), params, spec, cin, cout, tscale)
+    fwd48 = _net_forward(Session, stream, spec, params, cin, cout, LN.lattice_dims(mkd(0), 3))
     rows = []
     for seed in range(100000, 100000 + args.learned_eval_seeds):
         pbd = mkd(seed)
@@ -147,6 +176,7 @@ def learned_legs(args, stream, s, pb, x64, solve64_ms):
         "train_s": round(train_s, 1), "loss_first_last": [round(loss[0], 4), round(loss[1], 4)],
         "output_scale_factor": round(scale / tscale, 3),
         "validation_iterations_by_factor": {str(k): v for k, v in cal.items()},
+        "network_forward": fwd48,
         "eval_seeds": len(rows),
         "iterations_median": {"jacobi_fp64": med(0), "multigrid_fp64": med(2),
                               "learned_mg_fp64": med(4), "learned_mg_fp32": med(6)},
@@ -175,6 +205,7 @@ def learned_legs(args, stream, s, pb, x64, solve64_ms):
     torch.cuda.empty_cache()
     scale, cal = LN.calibrate_scale(mke(999), params, spec, cin, cout, tscale, precision=FP32)
     dims64 = LN.lattice_dims(pb, spec.levels)
+    fwd64 = _net_forward(Session, stream, spec, params, cin, cout, dims64)
     with torch.cuda.stream(stream):
         s.assemble(pb.integrand, pb.ng, FP32)  # FP32 K for the inner Krylov loop
     stream.synchronize()
@@ -200,6 +231,7 @@ def learned_legs(args, stream, s, pb, x64, solve64_ms):
         "train_s": round(train_s, 1), "loss_first_last": [round(loss[0], 4), round(loss[1], 4)],
         "output_scale_factor": round(scale / tscale, 3),
         "validation_iterations_by_factor": {str(k): v for k, v in cal.items()},
+        "network_forward": fwd64,
         "learned_fp32_solve_ms": round(ms, 3), "iterations": runs[-1]["iterations"],
         "outer": runs[-1]["outer"], "rel_residual": runs[-1]["rel_residual"],
         "converged": all(r["converged"] for r in runs),

@@ -30,7 +30,7 @@ EXPORTS = [
     "ifem_gpu_batch_timings", "ifem_gpu_autotune_assembly", "ifem_gpu_unet_backward",
     "ifem_gpu_last_assembly_parts", "ifem_gpu_integrate_norms", "ifem_gpu_sam_bind_constraints",
     "ifem_gpu_mg_setup", "ifem_gpu_mg_level", "ifem_gpu_mg_apply", "ifem_gpu_sam_bind_patches",
-    "ifem_gpu_assemble_neumann_patch",
+    "ifem_gpu_assemble_neumann_patch", "ifem_gpu_set_assembly_variant",
 ]
 
 FP64, FP32 = 64, 32
@@ -108,6 +108,7 @@ def lib():
     L.ifem_gpu_assemble_neumann.argtypes = [vp, ip, C.POINTER(dp), ip]
     L.ifem_gpu_autotune_assembly.argtypes = [vp, C.POINTER(Integrand), C.POINTER(ip), ip,
                                              C.POINTER(ip)]
+    L.ifem_gpu_set_assembly_variant.argtypes = [vp, ip, ip, ip, ip, ip]
     L.ifem_gpu_solver_defaults.argtypes = [C.POINTER(SolverParams)]
     L.ifem_gpu_solver_defaults.restype = None
     L.ifem_gpu_solve.argtypes = [vp, C.POINTER(SolverParams), C.POINTER(Stats)]
@@ -308,14 +309,20 @@ class Session:
         check(lib().ifem_gpu_assemble(self.h, C.byref(s), ngv, precision))
 
     def autotune_assembly(self, itg, ng, precision: int = FP64) -> int:
-        """Pick the faster element-chunk variant for this system; returns 1
-        for half chunks, 0 for full.  K and b are left as after assemble()."""
+        """Pick the fastest element-chunk variant for this system; returns 0
+        for full chunks, 1 for half chunks, 2 for full chunks in two
+        Gauss-point slabs.  K and b are left as after assemble()."""
         keep = []
         s = self._integrand(itg, keep)
         ngv = (C.c_int * 3)(*ng)
         h = C.c_int(-1)
         check(lib().ifem_gpu_autotune_assembly(self.h, C.byref(s), ngv, precision, C.byref(h)))
         return h.value
+
+    def set_assembly_variant(self, p, ng, ncomp, precision, variant):
+        """Fix the element-chunk variant for (p, ng, ncomp, precision); -1
+        restores the built-in default."""
+        check(lib().ifem_gpu_set_assembly_variant(self.h, p, ng, ncomp, precision, variant))
 
     def norms(self, itg, ng, exact=None):
         """[||u_h||_L2, ||u_h||_a, ||u||_L2, ||u||_a, ||u-u_h||_L2, ||u-u_h||_a] of the

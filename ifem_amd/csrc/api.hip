@@ -437,6 +437,16 @@ int ifem_gpu_autotune_assembly(ifem_gpu_ctx* ctx, const ifem_gpu_integrand* itg,
   return autotune_assembly(c, *itg, ng, precision, half);
 }
 
+int ifem_gpu_set_assembly_variant(ifem_gpu_ctx* ctx, int p, int ng, int ncomp, int precision,
+                                  int variant)
+{
+  CHECK_CTX(ctx);
+  if (p < 1 || p > 3 || (ng != p + 1 && ng != p + 2) || (ncomp != 1 && ncomp != 3) ||
+      (precision != IFEM_GPU_FP64 && precision != IFEM_GPU_FP32))
+    return fail(IFEM_GPU_EINVAL, "set_assembly_variant: (p, ng, ncomp, precision) out of range");
+  return set_assembly_variant(ctx->c, p, ng, ncomp, precision == IFEM_GPU_FP32, variant);
+}
+
 int ifem_gpu_integrate_norms(ifem_gpu_ctx* ctx, const ifem_gpu_integrand* itg, const int ng[3],
                              const ifem_gpu_function* exact, double norms[6])
 {

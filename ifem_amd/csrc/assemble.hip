@@ -18,6 +18,8 @@
 // of Integrand::evalInt.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <cmath>
 #include <string>
@@ -204,14 +206,18 @@ __device__ inline ConstArgs* const_args(const AsmArgs* p)
 }
 
 // Chunk of layer elements processed together so that CH*NG^3 ~ one block;
-// HALF selects half of that (an autotuning variant: less LDS, more phases).
+// HALF = 1 selects half of that (an autotuning variant: less LDS, more
+// phases); HALF = 2 keeps the full chunk and splits its Gauss points into
+// two slabs of q3 rows (half the geometry LDS: three workgroups per CU).
 template <int P, int NG, int HALF = 0>
 struct Chunk {
   static constexpr int P1 = P + 1;
   static constexpr int NB = P1 * P1;
   static constexpr int GP3 = NG * NG * NG;
   static constexpr int full = (256 / GP3) < 1 ? 1 : ((256 / GP3) > NB ? NB : (256 / GP3));
-  static constexpr int v = HALF ? ((full + 1) / 2) : full;
+  static constexpr int v = HALF == 1 ? ((full + 1) / 2) : full;
+  static constexpr int NSL = HALF == 2 ? 2 : 1;                    // Gauss-point slabs
+  static constexpr int SLQ = NG * NG * ((NG + NSL - 1) / NSL);     // points per slab (max)
 };
 
 // Geometry pre-pass: one thread per (element, Gauss point) computes, once
@@ -400,15 +406,17 @@ struct LineLds {
   static constexpr bool SACC = (NC == 3 && SPLIT == 1); // accumulate S_ij, apply lam/mu later
   static constexpr int NS = NC == 1 ? 1 : (SACC ? 9 : 3);
   static constexpr int CH = Chunk<P, NG, HALF>::v;
+  static constexpr int NSL = Chunk<P, NG, HALF>::NSL;
+  static constexpr int SLQ = Chunk<P, NG, HALF>::SLQ;
   static constexpr int GEO = (NC == 3 ? 9 : 0) + 3 * P1;
   static constexpr int NRHS = HAS_RHS ? P1 * NC : 1;
 
   // every member 16-byte aligned: lets the compiler use 128-bit LDS loads
   alignas(16) T win[P1][NCR][WN][NC];
-  alignas(16) T geo[CH * GP3][GEO];
+  alignas(16) T geo[CH * SLQ][GEO];
   // ORDL (ordered window updates): no stage; RHS stage of w*detJ*N_a only
   alignas(16) T stage[ORDL ? 1 : CH][NEN][P1][NS];
-  alignas(16) T rstage[HAS_RHS ? CH * GP3 : 1][ORDL ? (HAS_RHS ? P1 : 1) : NRHS];
+  alignas(16) T rstage[HAS_RHS ? CH * SLQ : 1][ORDL ? (HAS_RHS ? P1 : 1) : NRHS];
   alignas(16) T rwin[P1][NC];
   alignas(16) T tN1[NG][P1];
   alignas(16) T tD1[NG][P1];
@@ -420,8 +428,9 @@ struct LineLds {
   alignas(16) int be3[NB];
   static constexpr int NT = (WN * NC + 63) / 64;
   // neighbour equation numbers kept in LDS for phase D (not for p = 3,
-  // whose LDS budget goes to the element chunk)
-  static constexpr bool PFD = P < 3;
+  // whose LDS budget goes to the element chunk, nor for the slab variant,
+  // whose budget is three workgroups per CU)
+  static constexpr bool PFD = P < 3 && NSL == 1;
   // two buffers: row e1's (being filled) and row e1-1's (written out at the
   // top of layer e1)
   alignas(16) int eqw[2][PFD ? NT * 64 : 1];
@@ -464,6 +473,9 @@ __device__ __forceinline__ void line_impl(
   constexpr bool SACC = LL::SACC;
   constexpr int NS = LL::NS;
   constexpr int CH = LL::CH;
+  constexpr int NSL = LL::NSL; // Gauss-point slabs per chunk (geometry rows in LDS per slab)
+  constexpr int SLQ = LL::SLQ;
+  static_assert(NSL == 1 || (ORD && PIPE), "Gauss-point slabs need the ordered, pipelined path");
   // ORD: the whole bundle in one chunk, column sums added to the window
   // element by element (no stage); the RHS stage then holds w*detJ*N_a and
   // the body force is applied after the sum
@@ -626,9 +638,11 @@ __device__ __forceinline__ void line_impl(
     for (int c0 = 0; c0 < nb; c0 += CH) {
       const int nch = (nb - c0) < CH ? (nb - c0) : CH;
 
-      // ---- phase A: geometry per (element, Gauss point) -------------------
-      for (int t = tid; t < nch * GP3; t += kBlock) {
-        const int le = t / GP3, q = t % GP3;
+      // ---- phase A: geometry per (element, Gauss point) of the points
+      // [qa, qa + ngs) (all of them unless the chunk runs in slabs) --------
+      auto phase_a = [&](const int qa, const int ngs) {
+      for (int t = tid; t < nch * ngs; t += kBlock) {
+        const int le = t / ngs, q = qa + t % ngs;
         const int e2 = be2[c0 + le], e3 = be3[c0 + le];
         const int m2 = e2 - jc + P, m3 = e3 - kc + P; // table slots
         const int q1 = q % NG, q2 = (q / NG) % NG, q3 = q / (NG * NG);
@@ -676,6 +690,22 @@ __device__ __forceinline__ void line_impl(
           }
         }
       }
+      };
+      // ---- RHS: fixed-order reduction over (element, GP) of a slab ---------
+      auto rhs_reduce = [&](const int ngs) {
+        const int v = tid >> 4, l = tid & 15;
+        T s = T(0);
+        const int vs = (ORD && NC == 3) ? v / NC : v;
+        if (v < P1 * NC)
+          for (int pr = l; pr < nch * ngs; pr += 16) s += rstage[pr * RS + vs];
+#pragma unroll
+        for (int off = 8; off >= 1; off >>= 1) s += __shfl_xor(s, off, 16);
+        if (v < P1 * NC && l == 0) {
+          const int a1 = v / NC, c = v % NC;
+          rwin[(e1 + a1) % P1][c] += (ORD && NC == 3) ? s * (T)a.body[c] : s;
+        }
+      };
+      phase_a(0, NSL == 1 ? GP3 : SLQ);
       if (c0 == 0 && tid < NCR) pbase[buf][tid] = pf_base;
       __syncthreads();
       // (ordered updates: after phase B, further from the loads)
@@ -691,20 +721,7 @@ __device__ __forceinline__ void line_impl(
         for (int t = tid; t < NCR * WN * NC; t += kBlock) (&win[zs][0][0][0])[t] = T(0);
       }
 
-      // ---- RHS: fixed-order reduction over (element, GP) -------------------
-      if (HAS_RHS) {
-        const int v = tid >> 4, l = tid & 15;
-        T s = T(0);
-        const int vs = (ORD && NC == 3) ? v / NC : v;
-        if (v < P1 * NC)
-          for (int pr = l; pr < nch * GP3; pr += 16) s += rstage[pr * RS + vs];
-#pragma unroll
-        for (int off = 8; off >= 1; off >>= 1) s += __shfl_xor(s, off, 16);
-        if (v < P1 * NC && l == 0) {
-          const int a1 = v / NC, c = v % NC;
-          rwin[(e1 + a1) % P1][c] += (ORD && NC == 3) ? s * (T)a.body[c] : s;
-        }
-      }
+      if (HAS_RHS) rhs_reduce(NSL == 1 ? GP3 : SLQ);
 
       // ---- phase B: element-row x column-node products ----------------------
       if constexpr (ORD) {
@@ -722,7 +739,6 @@ __device__ __forceinline__ void line_impl(
       for (int a1 = 0; a1 < P1; a1++)
 #pragma unroll
         for (int s = 0; s < NS; s++) acc[a1][s] = 0;
-      if (has) {
         if constexpr (PIPE) {
         const int m2 = e2 - jc + P, m3 = e3 - kc + P;
         T vn1[NG], vd1[NG], vn2[NG], vd2[NG], vn3[NG], vd3[NG];
@@ -732,21 +748,36 @@ __device__ __forceinline__ void line_impl(
           vn2[q] = tN2[m2][q][b2]; vd2[q] = tD2[m2][q][b2];
           vn3[q] = tN3[m3][q][b3]; vd3[q] = tD3[m3][q][b3];
         }
-        const T* __restrict__ Gbase = geo[le * GP3];
-        T Gc[LL::GEO];
+        // Gauss points in NSL slabs: the geometry rows of slab sl > 0 replace
+        // those of slab sl-1 in LDS (every thread takes part in the barriers)
+        // (a compile-time slab index, so that each slab's Gauss loop unrolls)
+        auto slab_b = [&](auto slc) {
+        constexpr int qa = decltype(slc)::value * SLQ;
+        constexpr int ngs = (GP3 - qa) < SLQ ? (GP3 - qa) : SLQ;
+        if (has) {
+        // one workgroup per CU and SIMD pair: the next point's geometry row
+        // is prefetched into registers; with slabs (three workgroups per CU)
+        // the rows are read from LDS where they are used, the other
+        // workgroups hiding the latency
+        constexpr bool PREF = NSL == 1;
+        const T* __restrict__ Gbase = geo[le * ngs];
+        T Gc[PREF ? LL::GEO : 1];
+        if constexpr (PREF) {
 #pragma unroll
-        for (int m = 0; m < LL::GEO; m++) Gc[m] = Gbase[m];
+          for (int m = 0; m < LL::GEO; m++) Gc[m] = Gbase[m];
+        }
 #pragma unroll
-        for (int q = 0; q < GP3; q++) {
-          T Gn[LL::GEO];
-          if (q + 1 < GP3) {
+        for (int qq = 0; qq < ngs; qq++) {
+          const int q = qa + qq;
+          T Gn[PREF ? LL::GEO : 1];
+          if (PREF && qq + 1 < ngs) {
 #pragma unroll
-            for (int m = 0; m < LL::GEO; m++) Gn[m] = Gbase[(q + 1) * LL::GEO + m];
+            for (int m = 0; m < LL::GEO; m++) Gn[m] = Gbase[(qq + 1) * LL::GEO + m];
           }
           const int q1 = q % NG, q2 = (q / NG) % NG, q3 = q / (NG * NG);
           const T n23 = vn2[q2] * vn3[q3], d23 = vd2[q2] * vn3[q3], n2d3 = vn2[q2] * vd3[q3];
           const T r0 = vd1[q1] * n23, r1 = vn1[q1] * d23, r2 = vn1[q1] * n2d3;
-          const T* G = Gc;
+          const T* G = PREF ? Gc : Gbase + qq * LL::GEO;
           if (NC == 1) {
 #pragma unroll
             for (int a1 = 0; a1 < P1; a1++)
@@ -772,12 +803,25 @@ __device__ __forceinline__ void line_impl(
               }
             }
           }
-          if (q + 1 < GP3) {
+          if (PREF && qq + 1 < ngs) {
 #pragma unroll
             for (int m = 0; m < LL::GEO; m++) Gc[m] = Gn[m];
           }
         }
-              } else {
+        }
+        };
+        slab_b(std::integral_constant<int, 0>{});
+        if constexpr (NSL > 1) {
+          static_assert(NSL == 2, "two slabs");
+          constexpr int qa1 = SLQ, ngs1 = GP3 - SLQ;
+          __syncthreads();
+          phase_a(qa1, ngs1);
+          __syncthreads();
+          if (HAS_RHS) rhs_reduce(ngs1);
+          slab_b(std::integral_constant<int, 1>{});
+        }
+        } else {
+      if (has) {
         const int m2 = e2 - jc + P, m3 = e3 - kc + P;
         T vn1[NG], vd1[NG];
 #pragma unroll
@@ -880,8 +924,8 @@ __device__ __forceinline__ void line_impl(
             }
           }
         }
-        }
       }
+        }
       // rounds over the bundle elements: in round r the items of element r
       // add their column sums (distinct window entries within a round)
       // p <= 2 (STAGED): the geometry rows are no longer needed once the sums
@@ -898,7 +942,7 @@ __device__ __forceinline__ void line_impl(
       }
       constexpr bool STAGED = P <= 2;
       if constexpr (STAGED) {
-      constexpr int GS0 = (CH * GP3 * LL::GEO) / (NEN * P1 * NS);
+      constexpr int GS0 = (CH * SLQ * LL::GEO) / (NEN * P1 * NS);
       constexpr int GS = GS0 < 1 ? 1 : (GS0 > CH ? CH : GS0);
       T (*stg)[NEN][P1][NS] = reinterpret_cast<T (*)[NEN][P1][NS]>(&geo[0][0]);
       for (int g0 = 0; g0 < nch; g0 += GS) {
@@ -1260,26 +1304,11 @@ __device__ __forceinline__ void line_impl(
   }
 }
 
-// Batch variants with full chunks of at most one item per thread take the
-// ordered window updates (no stage).
-template <int P, int NG, int HALF>
-struct BatchOrd {
-  static constexpr bool v = HALF == 0 && Chunk<P, NG, 0>::v * (P + 1) * (P + 1) * (P + 1) <= 256;
-};
-
+// Single patch: the LDS arrays are separate __shared__ variables.  The
+// Gauss-point slab variant (HALF = 2) is built for three workgroups per CU.
 template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF>
-__device__ __forceinline__ void line_body(
-    ConstArgs& a, const int line, const int split,
-    LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF, BatchOrd<P, NG, HALF>::v>& L)
-{
-  line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF, false, BatchOrd<P, NG, HALF>::v, true>(
-      a, line, split, L.win, L.geo, L.stage, &L.rstage[0][0], L.rwin, L.tN1, L.tD1, L.tN2, L.tD2, L.tN3,
-      L.tD3, L.be2, L.be3, L.eqw, L.pbase, L.prow, &L.nbundle);
-}
-
-// Single patch: the LDS arrays are separate __shared__ variables.
-template <int P, int NG, int NC, typename T, int SPLIT, bool HAS_RHS, int HALF>
-__global__ __launch_bounds__(kBlock) void k_assemble_lines(const AsmArgs* __restrict__ args)
+__global__ __launch_bounds__(kBlock, HALF == 2 ? 3 : 1) void k_assemble_lines(
+    const AsmArgs* __restrict__ args)
 {
   ConstArgs& a = *const_args(args);
   using LL = LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>;
@@ -1287,11 +1316,11 @@ __global__ __launch_bounds__(kBlock) void k_assemble_lines(const AsmArgs* __rest
   // register-pipelined phase B where the unrolled Gauss loop stays small;
   // ordered window updates (no stage) where the whole bundle is one chunk
   constexpr bool PIPE = LL::GP3 <= 27;
-  constexpr bool ORD = HALF == 0 && LL::CH * LL::NEN <= kBlock;
+  constexpr bool ORD = (HALF == 0 || HALF == 2) && LL::CH * LL::NEN <= kBlock;
   __shared__ T win[P1][LL::NCR][LL::WN][NC];
-  __shared__ T geo[LL::CH * LL::GP3][LL::GEO];
+  __shared__ T geo[LL::CH * LL::SLQ][LL::GEO];
   __shared__ T stage[ORD ? 1 : LL::CH][LL::NEN][P1][LL::NS];
-  __shared__ T rstage[HAS_RHS ? LL::CH * LL::GP3 * (ORD ? P1 : LL::NRHS) : 1];
+  __shared__ T rstage[HAS_RHS ? LL::CH * LL::SLQ * (ORD ? P1 : LL::NRHS) : 1];
   __shared__ T rwin[P1][NC];
   __shared__ T tN1[NG][P1], tD1[NG][P1];
   __shared__ T tN2[P1][NG][P1], tD2[P1][NG][P1];
@@ -1306,29 +1335,38 @@ __global__ __launch_bounds__(kBlock) void k_assemble_lines(const AsmArgs* __rest
                                                       be2, be3, eqw, pbase, prow, &nbundle);
 }
 
-// Persistent batch assembly over many independent patches (degree p = 1..3,
-// ng = p+1).  Work items are (patch, line) pairs sorted by decreasing cost;
-// blocks take the next item from a queue counter, so load balances over
-// patches of uneven size.  Each line's rows are written by exactly one
-// block, so the result does not depend on which block takes which item.
-// The element-chunk variant (full / half) of each degree is chosen per patch
-// (AsmArgs::variant = 2*(p-1) + half); the LDS is one union of the variants.
-template <typename T, bool HAS_RHS>
-union BatchLds {
-  LineLds<1, 2, 1, T, 1, HAS_RHS, 0, BatchOrd<1, 2, 0>::v> v0;
-  LineLds<1, 2, 1, T, 1, HAS_RHS, 1, BatchOrd<1, 2, 1>::v> v1;
-  LineLds<2, 3, 1, T, 1, HAS_RHS, 0, BatchOrd<2, 3, 0>::v> v2;
-  LineLds<2, 3, 1, T, 1, HAS_RHS, 1, BatchOrd<2, 3, 1>::v> v3;
-  LineLds<3, 4, 1, T, 1, HAS_RHS, 0, BatchOrd<3, 4, 0>::v> v4;
-  LineLds<3, 4, 1, T, 1, HAS_RHS, 1, BatchOrd<3, 4, 1>::v> v5;
-};
-
-template <typename T, bool HAS_RHS>
-__global__ __launch_bounds__(kBlock) void k_assemble_batch(const AsmArgs* __restrict__ args,
+// Persistent batch assembly over many independent patches, one launch per
+// degree p = 1..3 (ng = p+1) present in the batch, each with the single-patch
+// kernel's own phase-B and window-update variants and its own register and
+// LDS budget.  Work items are (patch, line) pairs sorted by decreasing cost
+// within the degree; blocks take the next item from the degree's queue
+// counter, so load balances over patches of uneven size.  Each line's rows
+// are written by exactly one block, so the result does not depend on which
+// block takes which item.  HALF: the element-chunk variant chosen for the
+// degree (AsmArgs::variant = 2*(p-1) + HALF for every patch of it).
+template <int P, typename T, bool HAS_RHS, int HALF>
+__global__ __launch_bounds__(kBlock, 3) void k_assemble_batch(const AsmArgs* __restrict__ args,
                                                            const int2* __restrict__ items,
                                                            int nitems, int* __restrict__ queue)
 {
-  __shared__ BatchLds<T, HAS_RHS> U;
+  constexpr int NG = P + 1;
+  using LL = LineLds<P, NG, 1, T, 1, HAS_RHS, HALF>;
+  constexpr int P1 = P + 1;
+  constexpr bool PIPE = LL::GP3 <= 27;
+  constexpr bool ORD = (HALF == 0 || HALF == 2) && LL::CH * LL::NEN <= kBlock;
+  __shared__ T win[P1][LL::NCR][LL::WN][1];
+  __shared__ T geo[LL::CH * LL::SLQ][LL::GEO];
+  __shared__ T stage[ORD ? 1 : LL::CH][LL::NEN][P1][LL::NS];
+  __shared__ T rstage[HAS_RHS ? LL::CH * LL::SLQ * (ORD ? P1 : LL::NRHS) : 1];
+  __shared__ T rwin[P1][1];
+  __shared__ T tN1[NG][P1], tD1[NG][P1];
+  __shared__ T tN2[P1][NG][P1], tD2[P1][NG][P1];
+  __shared__ T tN3[P1][NG][P1], tD3[P1][NG][P1];
+  __shared__ int be2[LL::NB], be3[LL::NB];
+  __shared__ int eqw[2][LL::PFD ? LL::NT * 64 : 1];
+  __shared__ int64_t pbase[2][LL::NCR];
+  __shared__ int prow[2][LL::NCR];
+  __shared__ int nbundle;
   __shared__ int next;
   while (true) {
     if (threadIdx.x == 0) next = atomicAdd(queue, 1);
@@ -1337,14 +1375,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble_batch(const AsmArgs* __rest
     if (it >= nitems) break;
     const int2 w = items[it];
     ConstArgs& a = const_args(args)[w.x];
-    switch (a.variant) {
-    case 0: line_body<1, 2, 1, T, 1, HAS_RHS, 0>(a, w.y, 0, U.v0); break;
-    case 1: line_body<1, 2, 1, T, 1, HAS_RHS, 1>(a, w.y, 0, U.v1); break;
-    case 2: line_body<2, 3, 1, T, 1, HAS_RHS, 0>(a, w.y, 0, U.v2); break;
-    case 3: line_body<2, 3, 1, T, 1, HAS_RHS, 1>(a, w.y, 0, U.v3); break;
-    case 4: line_body<3, 4, 1, T, 1, HAS_RHS, 0>(a, w.y, 0, U.v4); break;
-    default: line_body<3, 4, 1, T, 1, HAS_RHS, 1>(a, w.y, 0, U.v5); break;
-    }
+    line_impl<P, NG, 1, T, 1, HAS_RHS, HALF, PIPE, ORD, !PIPE>(a, w.y, 0, win, geo, stage, rstage,
+                                                               rwin, tN1, tD1, tN2, tD2, tN3, tD3,
+                                                               be2, be3, eqw, pbase, prow, &nbundle);
     __syncthreads();
   }
 }
@@ -1686,8 +1719,8 @@ static int upload_function(Context& c, const ifem_gpu_function& f, DevFunction& 
 }
 
 // Element-chunk variant of the line kernel for (p, ng, ncomp, precision):
-// the tuned choice if ifem_gpu_autotune_assembly ran, else half chunks for
-// FP64 elasticity (LDS for two blocks per CU) and full chunks otherwise.
+// the tuned (or set) choice if ifem_gpu_autotune_assembly or
+// ifem_gpu_set_assembly_variant ran, else the built-in default below.
 int& asm_half_slot(Context& c, int p, int ng, int nc, bool fp32)
 {
   return c.asm_half[p][ng > p + 1 ? 1 : 0][nc == 3 ? 1 : 0][fp32 ? 1 : 0];
@@ -1697,9 +1730,12 @@ static int asm_half(Context& c, int p, int ng, int nc, bool fp32)
 {
   const int h = asm_half_slot(c, p, ng, nc, fp32);
   if (h >= 0) return h;
-  // full chunks take the ordered-update path (no stage) for p <= 2 with
-  // ng <= 3; elsewhere FP64 elasticity needs half chunks to keep two
-  // workgroups per CU
+  // FP32 elasticity at p = 2, 3x3x3 points: Gauss-point slabs (measured on
+  // 64^3: 3.9 ms against 5.2 ms for full chunks; in FP64 the slabs lose,
+  // 7.3 against 6.4 ms, and in scalar problems too).  Full chunks take the
+  // ordered-update path (no stage) for p <= 2 with ng <= 3; elsewhere FP64
+  // elasticity needs half chunks to keep two workgroups per CU.
+  if (fp32 && nc == 3 && p == 2 && ng == 3) return 2;
   const bool ordered = p <= 2 && ng <= 3;
   return (!fp32 && nc == 3 && !ordered && p < 3) ? 1 : 0;
 }
@@ -1721,10 +1757,22 @@ static void launch_lines_h(Context& c, const AsmArgs& args, bool has_rhs)
     k_assemble_lines<P, NG, NC, T, SPLIT, false, HALF><<<grid, kBlock, 0, c.stream>>>(d);
 }
 
+// The Gauss-point slab variant exists where the pipelined, ordered path
+// runs (p <= 2, at most 3x3x3 points).
+static inline bool slab_variant_ok(int p, int ng) { return p <= 2 && ng * ng * ng <= 27; }
+
 template <int P, int NG, int NC, typename T>
 static int launch_lines(Context& c, const AsmArgs& args, bool has_rhs)
 {
-  if (asm_half(c, P, NG, NC, sizeof(T) == 4))
+  const int h = asm_half(c, P, NG, NC, sizeof(T) == 4);
+  if constexpr (P <= 2 && NG * NG * NG <= 27) {
+    if (h == 2) {
+      launch_lines_h<P, NG, NC, T, 2>(c, args, has_rhs);
+      IFEM_HIP(hipGetLastError());
+      return IFEM_GPU_OK;
+    }
+  }
+  if (h == 1)
     launch_lines_h<P, NG, NC, T, 1>(c, args, has_rhs);
   else
     launch_lines_h<P, NG, NC, T, 0>(c, args, has_rhs);
@@ -1843,9 +1891,18 @@ static int assemble_p(Context& c, const SysView& sv, const PatchRec& Pt, const i
   return IFEM_GPU_OK;
 }
 
-// Times both element-chunk variants of the line kernel on this system and
-// keeps the faster one for (p, ng, ncomp, precision); the system is left
-// assembled with the chosen variant.
+int set_assembly_variant(Context& c, int p, int ng, int nc, bool fp32, int variant)
+{
+  if (variant < -1 || variant > 2 || (variant == 2 && !slab_variant_ok(p, ng)))
+    return fail(IFEM_GPU_EUNSUPPORTED, "set_assembly_variant: no such variant for this degree and rule");
+  asm_half_slot(c, p, ng, nc, fp32) = variant;
+  return IFEM_GPU_OK;
+}
+
+// Times the element-chunk variants of the line kernel on this system (full
+// chunks, half chunks, and for p <= 2 with at most 3x3x3 points full chunks
+// in two Gauss-point slabs) and keeps the fastest for (p, ng, ncomp,
+// precision); the system is left assembled with the chosen variant.
 int autotune_assembly(Context& c, const ifem_gpu_integrand& itg, const int ng[3], int precision,
                       int* half_out)
 {
@@ -1855,7 +1912,8 @@ int autotune_assembly(Context& c, const ifem_gpu_integrand& itg, const int ng[3]
   int& slot = asm_half_slot(c, Pt.p[0], ng[0], c.ncomp, fp32);
   double best = 1e300;
   int best_h = 0;
-  for (int h = 0; h <= 1; h++) {
+  const int hmax = slab_variant_ok(Pt.p[0], ng[0]) ? 2 : 1;
+  for (int h = 0; h <= hmax; h++) {
     slot = h;
     double t = 1e300;
     for (int rep = 0; rep < 3; rep++) {
@@ -2306,26 +2364,28 @@ int batch_symbolic(Context& c)
         items.push_back({per_elem * Pt.nel[0] * c2 * c3, q, j + Pt.n[1] * k});
       }
   }
-  std::stable_sort(items.begin(), items.end(),
-                   [](const Item& x, const Item& y) { return x.cost > y.cost; });
+  // grouped by degree (one launch each, highest first), by decreasing cost
+  // within a degree
+  std::stable_sort(items.begin(), items.end(), [&](const Item& x, const Item& y) {
+    const int px = B.pp[x.patch].p, py = B.pp[y.patch].p;
+    return px != py ? px > py : x.cost > y.cost;
+  });
   std::vector<int2> hi(items.size());
   for (size_t i = 0; i < items.size(); i++) hi[i] = make_int2(items[i].patch, items[i].line);
   B.nitems = (int)hi.size();
+  for (int p = 0; p < 4; p++) B.item0[p] = B.item1[p] = 0;
+  for (size_t i = 0; i < items.size(); i++) {
+    const int p = B.pp[items[i].patch].p;
+    if (B.item1[p] == 0) B.item0[p] = (int)i;
+    B.item1[p] = (int)i + 1;
+  }
   IFEM_HIP(hipMalloc(&B.d_items, sizeof(int2) * (hi.empty() ? 1 : hi.size())));
   if (!hi.empty())
     IFEM_HIP(hipMemcpy(B.d_items, hi.data(), sizeof(int2) * hi.size(), hipMemcpyHostToDevice));
-  IFEM_HIP(hipMalloc(&B.d_queue, sizeof(int)));
-
-  // persistent grid: as many blocks as stay resident
+  IFEM_HIP(hipMalloc(&B.d_queue, sizeof(int) * 4));
   hipDeviceProp_t prop;
   IFEM_HIP(hipGetDeviceProperties(&prop, c.device));
-  int per_cu64 = 0, per_cu32 = 0;
-  IFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu64, reinterpret_cast<const void*>(&k_assemble_batch<double, true>), kBlock, 0));
-  IFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu32, reinterpret_cast<const void*>(&k_assemble_batch<float, false>), kBlock, 0));
-  B.grid64 = std::max(1, std::min(B.nitems, std::max(1, per_cu64) * prop.multiProcessorCount));
-  B.grid32 = std::max(1, std::min(B.nitems, std::max(1, per_cu32) * prop.multiProcessorCount));
+  B.ncu = prop.multiProcessorCount;
   int rc = batch_upload_args(c);
   if (rc) return rc;
   IFEM_HIP(hipEventRecord(c.ev1, c.stream));
@@ -2339,11 +2399,40 @@ int batch_symbolic(Context& c)
 // Numeric assembly of every patch of the batch in one persistent launch:
 // FP64 writes K and b, FP32 writes the FP32 copy of K.  Returns the launch's
 // device time in *ms.
+// One persistent launch over the items of degree P, chunk variant HALF: as
+// many blocks as stay resident.
+template <int P, typename T, bool HAS_RHS, int HALF>
+static int launch_batch_degree(Context& c, Batch& B, const void* args)
+{
+  const int n = B.item1[P] - B.item0[P];
+  if (n <= 0) return IFEM_GPU_OK;
+  int per_cu = 0;
+  IFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void*>(&k_assemble_batch<P, T, HAS_RHS, HALF>), kBlock, 0));
+  const int grid = std::max(1, std::min(n, std::max(1, per_cu) * B.ncu));
+  k_assemble_batch<P, T, HAS_RHS, HALF><<<grid, kBlock, 0, c.stream>>>(
+      static_cast<const AsmArgs*>(args), B.d_items + B.item0[P], n, B.d_queue + P);
+  return IFEM_GPU_OK;
+}
+
+template <typename T, bool HAS_RHS>
+static int launch_batch(Context& c, Batch& B, const void* args)
+{
+  int rc = IFEM_GPU_OK;
+  if (!rc) rc = B.tune[3].half ? launch_batch_degree<3, T, HAS_RHS, 1>(c, B, args)
+                               : launch_batch_degree<3, T, HAS_RHS, 0>(c, B, args);
+  if (!rc) rc = B.tune[2].half ? launch_batch_degree<2, T, HAS_RHS, 1>(c, B, args)
+                               : launch_batch_degree<2, T, HAS_RHS, 0>(c, B, args);
+  if (!rc) rc = B.tune[1].half ? launch_batch_degree<1, T, HAS_RHS, 1>(c, B, args)
+                               : launch_batch_degree<1, T, HAS_RHS, 0>(c, B, args);
+  return rc;
+}
+
 int batch_assemble(Context& c, int precision, double* ms_out)
 {
   Batch& B = *c.batch;
   const bool fp32 = precision == IFEM_GPU_FP32;
-  k_fill<int><<<1, 64, 0, c.stream>>>(1, B.d_queue, 0);
+  k_fill<int><<<1, 64, 0, c.stream>>>(4, B.d_queue, 0);
   IFEM_HIP(hipEventRecord(c.ev0, c.stream));
   // geometry pass: always for FP64 (the full assembly); an FP32 assembly
   // reuses the records of an FP64 assembly of the same batch
@@ -2352,12 +2441,9 @@ int batch_assemble(Context& c, int precision, double* ms_out)
         static_cast<const AsmArgs*>(fp32 ? B.d_args32 : B.d_args64), B.d_gp0, (int)B.pp.size(),
         B.ngp, fp32 ? 0 : 1);
   if (B.nitems > 0) {
-    if (fp32)
-      k_assemble_batch<float, false><<<B.grid32, kBlock, 0, c.stream>>>(
-          static_cast<const AsmArgs*>(B.d_args32), B.d_items, B.nitems, B.d_queue);
-    else
-      k_assemble_batch<double, true><<<B.grid64, kBlock, 0, c.stream>>>(
-          static_cast<const AsmArgs*>(B.d_args64), B.d_items, B.nitems, B.d_queue);
+    const int rc = fp32 ? launch_batch<float, false>(c, B, B.d_args32)
+                        : launch_batch<double, true>(c, B, B.d_args64);
+    if (rc) return rc;
   }
   IFEM_HIP(hipGetLastError());
   IFEM_HIP(hipEventRecord(c.ev1, c.stream));

@@ -53,8 +53,9 @@ struct Batch {
   void* d_args32 = nullptr;
   int2* d_items = nullptr;
   int nitems = 0;
-  int* d_queue = nullptr;
-  int grid64 = 0, grid32 = 0;
+  int item0[4] = {0, 0, 0, 0}, item1[4] = {0, 0, 0, 0}; // items of degree p: [item0[p], item1[p])
+  int* d_queue = nullptr;   // one queue counter per degree
+  int ncu = 0;              // compute units (persistent grid size)
   // geometry records per (element, Gauss point) (patch q's records start at
   // gp0[q]); ngp records in all
   double* d_geo = nullptr;

@@ -197,6 +197,7 @@ void mpc_free(Context& c);
 void mpc_expand_host(const Context& c, const std::vector<double>& x, double* xdof);
 int autotune_assembly(Context& c, const ifem_gpu_integrand& itg, const int ng[3], int precision,
                       int* half);
+int set_assembly_variant(Context& c, int p, int ng, int nc, bool fp32, int variant);
 int integrate_norms(Context& c, const ifem_gpu_integrand& itg, const int ng[3],
                     const ifem_gpu_function* exact, double* out);
 int batch_symbolic(Context& c);

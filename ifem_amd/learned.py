@@ -114,6 +114,20 @@ class NetSpec:
         n += c[0] * cout * 27 + cout
         return n
 
+    def flops(self, cin, cout, dims):
+        """Forward multiply-add work of one sample, 2*Cin*Cout*27 per voxel
+        and 3^3 convolution (the same layers as params(), level l at 1/8^l of
+        the D*H*W voxels)."""
+        vox = [int(np.prod(dims)) // 8 ** l for l in range(self.levels)]
+        c = list(self.channels)
+        f, prev = 0, cin
+        for l in range(self.levels):
+            f += 2 * 27 * vox[l] * (prev * c[l] + c[l] * c[l])
+            prev = c[l]
+        for l in range(self.levels - 2, -1, -1):
+            f += 2 * 27 * vox[l] * ((c[l + 1] + c[l]) * c[l] + c[l] * c[l])
+        return f + 2 * 27 * vox[0] * c[0] * cout
+
 
 def jacobi_inverse(s: Session) -> np.ndarray:
     rp, ci, A, _ = s.system()
@@ -301,8 +315,8 @@ def padded_aux(pb, dims) -> np.ndarray:
     return out
 
 
-def calibrate_scale(pb, params, spec, cin, cout, tscale, candidates=(1.0, 0.5, 0.25), pc=None,
-                    rtol=1e-8, precision=None):
+def calibrate_scale(pb, params, spec, cin, cout, tscale, candidates=(1.0, 0.5, 0.25, 0.125),
+                    pc=None, rtol=1e-8, precision=None):
     """Output scale of a trained network, chosen on a validation problem pb
     (not a test problem): the factor in candidates (times the training
     target scale) with the fewest flexible-CG iterations to rtol, ties to the

@@ -166,13 +166,22 @@ int ifem_gpu_assemble(ifem_gpu_ctx* ctx, const ifem_gpu_integrand* itg,
                       const int ng[3], int precision);
 
 /* No counterpart in the reference (part of this library's autotuner): times
- * the line-assembly kernel of the bound system with full and with half
- * element chunks (tile shape of the (p, nen) kernel: LDS per block against
- * blocks per CU), keeps the faster for this (p, ng, ncomp, precision) in the
- * context and leaves the system assembled as ifem_gpu_assemble does (apply
- * Neumann loads afterwards).  *half (may be NULL) receives the choice. */
+ * the line-assembly kernel of the bound system with each element-chunk
+ * variant (tile shape of the (p, nen) kernel: LDS per block against blocks
+ * per CU): 0 full chunks, 1 half chunks, 2 full chunks with the Gauss points
+ * in two slabs (p <= 2 with at most 3 points per direction: three blocks per
+ * CU), keeps the fastest for this (p, ng, ncomp, precision) in the context
+ * and leaves the system assembled as ifem_gpu_assemble does (apply Neumann
+ * loads afterwards).  *half (may be NULL) receives the choice. */
 int ifem_gpu_autotune_assembly(ifem_gpu_ctx* ctx, const ifem_gpu_integrand* itg,
                                const int ng[3], int precision, int* half);
+
+/* No counterpart in the reference: fix the element-chunk variant (as above;
+ * -1 restores the built-in default) of the line kernel for (p, ng, ncomp,
+ * precision) in this context.  EUNSUPPORTED for a variant that does not
+ * exist for that degree and rule. */
+int ifem_gpu_set_assembly_variant(ifem_gpu_ctx* ctx, int p, int ng, int ncomp, int precision,
+                                  int variant);
 
 /* Neumann traction t (constant vector, ncomp components) on patch face
  * face = 1..6 (IFEM numbering: 1 u=0, 2 u=1, 3 v=0, 4 v=1, 5 w=0, 6 w=1),
@@ -234,8 +243,8 @@ typedef struct {
    * fallback: 1 if flexible CG broke down (<r, z> <= 0) and the solve went
    * on with Jacobi-PCG from the current iterate (iterations - pc_iterations
    * of them); pc_rejected: iterations in which the learned correction was
-   * dropped for that one step because it made <r, z> non-positive (the step
-   * then used the rest of the preconditioner). */
+   * dropped for that one step because <r, M0 r + correction> fell below
+   * 0.1 <r, M0 r> (the step then used M0 r alone). */
   int pc_iterations;
   int fallback;
   int pc_rejected;
@@ -330,8 +339,12 @@ int ifem_gpu_unet_train_step(ifem_gpu_ctx* ctx, const float* x, const float* t, 
 int ifem_gpu_unet_backward(ifem_gpu_ctx* ctx, const float* x, const float* dy, int N, int D, int H,
                            int W, float lr, float* grad);
 
-/* Use the network as preconditioner of the next solves (enable != 0) or go
- * back to Jacobi (enable == 0).  The nodes with free DOFs must form a box of
+/* Add the network's correction to the preconditioner of the next solves
+ * (enable != 0) or remove it (enable == 0): flexible CG with
+ * z = M0 r + scale * |r|_rms * U(r / |r|_rms, aux), M0 = D^-1 (pc Jacobi) or
+ * one spline-multigrid V-cycle (pc = IFEM_GPU_PC_MG); an iteration whose
+ * correction would make <r, z> < 0.1 <r, M0 r> uses M0 r alone (counted in
+ * pc_rejected).  The nodes with free DOFs must form a box of
  * the control-point grid (all ncomp components free inside, none outside);
  * the box (node i fastest) is placed at the origin of a D x H x W lattice
  * with D, H, W >= the box extents in k, j, i (zero padding beyond it).

@@ -89,20 +89,60 @@ def test_assembly_is_bitwise_deterministic():
 
 
 def test_autotuned_assembly_keeps_parity():
-    """Both element-chunk variants of the line kernel give the oracle's system
-    (the autotuner leaves the faster one assembled)."""
+    """The element-chunk variants of the line kernel give the oracle's system
+    (the autotuner leaves the fastest one assembled)."""
     for pb in (problems.elasticity_cube(6, 2, seed=3), problems.poisson_cube(5, 3),
                problems.diffusion_cube(6, 2, sigma=1.0, ell=0.2, seed=4),
                problems.elasticity_cube(4, 3, seed=9)):
         _say(f"[gpu] autotuned assembly {pb.name}")
         with setup_problem(pb) as s:
             for prec in (FP64, FP32):
-                assert s.autotune_assembly(pb.integrand, pb.ng, prec) in (0, 1)
+                assert s.autotune_assembly(pb.integrand, pb.ng, prec) in (0, 1, 2)
             if pb.traction is not None:
                 s.add_traction(*pb.traction)
             _, _, A, _ = _compare_system(s, pb)
             A32 = s.values_fp32().astype(np.float64)
             assert np.max(np.abs(A32 - A)) <= 1e-5 * np.max(np.abs(A))
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("kind", ["elasticity", "elasticity_body", "poisson", "poisson_p1"])
+def test_each_assembly_variant_matches_oracle(kind, variant):
+    """Every element-chunk variant (full, half, Gauss-point slabs) forced in
+    turn: K and b against the oracle (1e-12), bitwise equal on a rerun, FP32
+    values within 1e-5."""
+    pb = {"elasticity": lambda: problems.elasticity_cube(6, 2, seed=13),
+          "elasticity_body": lambda: _with_body(problems.elasticity_cube(5, 2, seed=14)),
+          "poisson": lambda: problems.poisson_cube(6, 2),
+          "poisson_p1": lambda: problems.poisson_cube(7, 1)}[kind]()
+    p, ng, nc = pb.patch.p[0], pb.ng[0], pb.integrand.ncomp
+    _say(f"[gpu] assembly variant {variant} {pb.name}")
+    with setup_problem(pb) as s:
+        for prec in (FP64, FP32):
+            s.set_assembly_variant(p, ng, nc, prec, variant)
+        s.assemble(pb.integrand, pb.ng, FP64)
+        if pb.traction is not None:
+            s.add_traction(*pb.traction)
+        _compare_system(s, pb)
+        _, _, A, b = s.system()
+        s.assemble(pb.integrand, pb.ng, FP64)
+        if pb.traction is not None:
+            s.add_traction(*pb.traction)
+        _, _, A2, b2 = s.system()
+        assert np.array_equal(A, A2) and np.array_equal(b, b2)
+        s.assemble(pb.integrand, pb.ng, FP32)
+        A32 = s.values_fp32().astype(np.float64)
+        assert np.max(np.abs(A32 - A)) <= 1e-5 * np.max(np.abs(A))
+
+
+def test_assembly_variant_refusals():
+    from ifem_amd._native import GpuError
+    pb = problems.elasticity_cube(3, 3, seed=1)
+    with setup_problem(pb) as s:
+        with pytest.raises(GpuError):
+            s.set_assembly_variant(3, 4, 3, FP64, 2)  # no slab variant at 4x4x4 points
+        with pytest.raises(GpuError):
+            s.set_assembly_variant(2, 7, 3, FP64, 0)
 
 
 def test_fp32_assembly_close_to_fp64():

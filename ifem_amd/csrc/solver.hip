@@ -736,7 +736,8 @@ static int apply_learned(Context& c, bool mg_base, const V* r, const V* rold, co
   lm.vol = (int64_t)c.lat[0] * c.lat[1] * c.lat[2];
   k_to_lattice<V><<<kGrid, kThreads, 0, c.stream>>>(n, c.ncomp, r, c.d_netin, c.d_aux, c.naux,
                                                     c.features, lm, part_rr, kGrid, c.w.scal);
-  const float* out = c.net->forward(c.d_netin, 1, c.lat[0], c.lat[1], c.lat[2]);
+  const float* out = c.net->forward(c.d_netin, 1, c.lat[0], c.lat[1], c.lat[2],
+                                   c.w.scal + S_DONE);
   k_from_lattice_l<V><<<kGrid, kThreads, 0, c.stream>>>(n, c.ncomp, r, rold,
                                                         mg_base ? (const V*)nullptr : dinv, z0, out,
                                                         zn, c.learned_scale, lm, part_rr, part4,

@@ -123,6 +123,7 @@ struct ConvArgs {
   uint16_t* out;      // bf16 channels-last, or null
   float* out_f32;     // FP32 NCDHW (cout channels), or null
   int relu;
+  const double* skip = nullptr; // return at once when *skip != 0 (a solver's done flag)
 };
 
 __host__ __device__ inline int conv_bricks(int D, int H, int W)
@@ -133,8 +134,12 @@ __host__ __device__ inline int conv_bricks(int D, int H, int W)
 template <int NT>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a)
 {
-  __shared__ uint4 tile[HVOX * (CCMAX / 8)];
+  if (a.skip && *a.skip != 0.0) return;
+  // halo tile of up to CCMAX input channels, sized by the launch to the
+  // channels this convolution has (fewer channels: more blocks per CU)
+  extern __shared__ uint4 tile[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tb = blockIdx.z * NT; // first 16-channel output tile of this block
   const int row = lane & 15, grp = lane >> 4;
   const int nbx = (a.W + BX - 1) / BX, nby = (a.H + BY - 1) / BY;
   const int b = blockIdx.x;
@@ -169,7 +174,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a)
 #pragma unroll
       for (int t = 0; t < NT; t++)
         bw[t] = __builtin_bit_cast(
-            bf16x8, *(const uint4*)(a.wp + (int64_t)(t * 16 + row) * wrow + (kb_base + kb) * 32 + grp * 8));
+            bf16x8, *(const uint4*)(a.wp + (int64_t)((tb + t) * 16 + row) * wrow + (kb_base + kb) * 32 + grp * 8));
 #pragma unroll
       for (int m = 0; m < 4; m++) {
         const int hv = ((wave + dz) * HYs + (m + dy)) * HXs + (row + dx);
@@ -181,28 +186,48 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a)
     }
     kb_base += nkb;
   }
-  // D fragment: column (output channel) t*16 + row, rows (x) 4*grp + i
+  // D fragment: column (output channel) t*16 + row, rows (x) 4*grp + i.
+  // bf16 outputs go through LDS ([brick voxel][NT*16 channels]) so that each
+  // voxel's channels leave as 16-byte stores; FP32 NCDHW outputs (the last
+  // convolution) are written directly (x contiguous per channel).
   const int z = z0 + wave;
   const int64_t V = (int64_t)a.D * a.H * a.W;
+  constexpr int CST = NT * 16; // channels per staged voxel
+  uint16_t* ot = reinterpret_cast<uint16_t*>(tile);
+  if (a.out) __syncthreads(); // every wave is done with the halo tile
 #pragma unroll
   for (int m = 0; m < 4; m++) {
     const int y = y0 + m;
 #pragma unroll
     for (int t = 0; t < NT; t++) {
-      const int co = t * 16 + row;
+      const int co = (tb + t) * 16 + row;
       const float bias = (a.bias && co < a.cout) ? a.bias[co] : 0.f;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int x = x0 + 4 * grp + i;
-        if (x >= a.W || y >= a.H || z >= a.D) continue;
         float v = acc[m][t][i] + bias;
         if (a.relu && v < 0.f) v = 0.f;
+        if (a.out) ot[((wave * BY + m) * BX + 4 * grp + i) * CST + t * 16 + row] =
+            (uint16_t)(co < a.cout ? f2bf(v) : 0u);
+        if (x >= a.W || y >= a.H || z >= a.D) continue;
         const int64_t vox = ((int64_t)z * a.H + y) * a.W + x;
-        if (a.out && co < a.cout_p8)
-          a.out[((int64_t)n * V + vox) * a.cout_p8 + co] = (uint16_t)(co < a.cout ? f2bf(v) : 0u);
         if (a.out_f32 && co < a.cout) a.out_f32[((int64_t)n * a.cout + co) * V + vox] = v;
       }
     }
+  }
+  if (!a.out) return;
+  __syncthreads();
+  // channels [tb*16, tb*16 + CST) of the brick, clipped to cout_p8, in
+  // groups of 8 (cout_p8 and tb*16 are multiples of 8)
+  const int c0 = tb * 16;
+  const int ncg = (a.cout_p8 - c0 < CST ? a.cout_p8 - c0 : CST) / 8;
+  for (int idx = threadIdx.x; idx < BX * BY * BZ * ncg; idx += 256) {
+    const int g = idx % ncg, lv = idx / ncg;
+    const int x = x0 + lv % BX, y = y0 + (lv / BX) % BY, zz = z0 + lv / (BX * BY);
+    if (x >= a.W || y >= a.H || zz >= a.D) continue;
+    const int64_t vox = ((int64_t)zz * a.H + y) * a.W + x;
+    *(uint4*)(a.out + ((int64_t)n * V + vox) * a.cout_p8 + c0 + 8 * g) =
+        *(const uint4*)(ot + lv * CST + 8 * g);
   }
 }
 
@@ -376,8 +401,9 @@ __global__ void k_fill_u16(uint16_t* __restrict__ p, int64_t n)
 
 // FP32 NCDHW (c channels) -> bf16 channels-last (cp channels, zero padded)
 __global__ void k_pack_in(const float* __restrict__ x, int c, int cp, int64_t V, int64_t nvox,
-                          uint16_t* __restrict__ out)
+                          uint16_t* __restrict__ out, const double* __restrict__ skip)
 {
+  if (skip && *skip != 0.0) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int g = cp >> 3;
   if (i >= nvox * g) return;
@@ -537,13 +563,25 @@ constexpr int kMaxConvIn = 128;     // padded input channels of a convolution
 constexpr int kWgradBlocks = 512;   // weight-gradient workgroups (all channel groups)
 
 template <int NT>
-static void launch_conv(const ConvArgs& a, int N, hipStream_t s)
+static void launch_conv(const ConvArgs& a, int N, hipStream_t s, int ztiles = 1)
 {
-  k_conv<NT><<<dim3(conv_bricks(a.D, a.H, a.W), N), 256, 0, s>>>(a);
+  const int cin_p = a.x.c0p + a.x.c1p;
+  const size_t halo = sizeof(uint4) * HVOX * ((cin_p < CCMAX ? cin_p : CCMAX) / 8);
+  const size_t stage = a.out ? sizeof(uint16_t) * BX * BY * BZ * NT * 16 : 0;
+  k_conv<NT><<<dim3(conv_bricks(a.D, a.H, a.W), N, ztiles), 256, halo > stage ? halo : stage, s>>>(
+      a);
 }
 
+// nt output tiles of 16 channels: one block runs all of them over its brick,
+// unless the grid of bricks is too small to fill the GPU (the coarse levels),
+// where each block takes one tile (blockIdx.z) and the bricks are loaded once
+// per tile instead
 static void conv_dispatch(const ConvArgs& a, int nt, int N, hipStream_t s)
 {
+  if (nt > 1 && (int64_t)conv_bricks(a.D, a.H, a.W) * N < 512) {
+    launch_conv<1>(a, N, s, nt);
+    return;
+  }
   switch (nt) {
   case 1: launch_conv<1>(a, N, s); break;
   case 2: launch_conv<2>(a, N, s); break;
@@ -725,9 +763,10 @@ int UNet::ensure(int N, int D, int H, int W)
 }
 
 void UNet::conv_fwd(const Conv& cv, const ConvIn& in, int N, int d, int h, int w, uint16_t* out,
-                    float* out_f32)
+                    float* out_f32, const double* skip)
 {
   ConvArgs a;
+  a.skip = skip;
   a.x = in;
   a.D = d; a.H = h; a.W = w;
   a.wp = wpack + cv.wp;
@@ -797,29 +836,30 @@ static ConvIn cin_of(const uint16_t* p, int c, int mode = 0, const uint16_t* p1 
 
 // Forward pass (bf16 activations, FP32 accumulation); x FP32 NCDHW; returns
 // the FP32 NCDHW output.
-const float* UNet::forward(const float* x, int N, int D, int H, int W)
+const float* UNet::forward(const float* x, int N, int D, int H, int W, const double* skip)
 {
   const int L = cfg.levels;
   hipStream_t s = stream;
   pack();
   const int64_t V0 = (int64_t)D * H * W;
-  k_pack_in<<<grid1(N * V0 * (cinp / 8)), 256, 0, s>>>(x, cfg.in_channels, cinp, V0, N * V0, xin);
+  k_pack_in<<<grid1(N * V0 * (cinp / 8)), 256, 0, s>>>(x, cfg.in_channels, cinp, V0, N * V0, xin,
+                                                       skip);
   int ic = 0;
   for (int l = 0; l < L; l++) {
     const int d = dims[l][0], h = dims[l][1], w = dims[l][2];
     const ConvIn in = l == 0 ? cin_of(xin, cfg.in_channels) : cin_of(enc_b[l - 1], cfg.channels[l - 1], 2);
-    conv_fwd(convs[ic++], in, N, d, h, w, enc_a[l], nullptr);
-    conv_fwd(convs[ic++], cin_of(enc_a[l], cfg.channels[l]), N, d, h, w, enc_b[l], nullptr);
+    conv_fwd(convs[ic++], in, N, d, h, w, enc_a[l], nullptr, skip);
+    conv_fwd(convs[ic++], cin_of(enc_a[l], cfg.channels[l]), N, d, h, w, enc_b[l], nullptr, skip);
   }
   const uint16_t* cur = enc_b[L - 1];
   for (int l = L - 2; l >= 0; l--) {
     const int d = dims[l][0], h = dims[l][1], w = dims[l][2];
     conv_fwd(convs[ic++], cin_of(cur, cfg.channels[l + 1], 1, enc_b[l], cfg.channels[l]), N, d, h,
-             w, dec_a[l], nullptr);
-    conv_fwd(convs[ic++], cin_of(dec_a[l], cfg.channels[l]), N, d, h, w, dec_b[l], nullptr);
+             w, dec_a[l], nullptr, skip);
+    conv_fwd(convs[ic++], cin_of(dec_a[l], cfg.channels[l]), N, d, h, w, dec_b[l], nullptr, skip);
     cur = dec_b[l];
   }
-  conv_fwd(convs[ic], cin_of(cur, cfg.channels[0]), N, D, H, W, nullptr, outbuf);
+  conv_fwd(convs[ic], cin_of(cur, cfg.channels[0]), N, D, H, W, nullptr, outbuf, skip);
   return outbuf;
 }
 

@@ -66,7 +66,12 @@ struct UNet {
   int ensure(int N, int D, int H, int W);
   // repacks the bf16 MFMA weights from the FP32 parameters when they changed
   int pack();
-  const float* forward(const float* x, int N, int D, int H, int W);
+  // skip (device, may be null): the kernels of this forward pass return at
+  // once when *skip != 0 -- a solver's convergence flag, so that the
+  // applications launched after convergence inside a check window cost only
+  // their launches
+  const float* forward(const float* x, int N, int D, int H, int W,
+                       const double* skip = nullptr);
   // gradient of 0.5*sum_n w_n*sum((y-t)^2)/N (w_n = 1 if wts is null), or
   // with dy != nullptr of sum(dy*y), w.r.t. the parameters into grads, after
   // forward(x); returns the loss when want_loss (synchronises)
@@ -76,7 +81,7 @@ struct UNet {
 
 private:
   void conv_fwd(const Conv& cv, const ConvIn& in, int N, int d, int h, int w, uint16_t* out,
-                float* out_f32);
+                float* out_f32, const double* skip = nullptr);
   void conv_dgrad(const Conv& cv, const uint16_t* g, int N, int d, int h, int w, uint16_t* gin);
   void conv_wgrad(const Conv& cv, const ConvIn& in, const uint16_t* g, int N, int d, int h, int w);
 };

@@ -299,7 +299,7 @@ bool AlgEqSystem::init(LinAlg::MatrixType mtype, const LinSolParams& spar)
   return G && G->ready();
 }
 
-void AlgEqSystem::initialize(bool initLHS)
+void AlgEqSystem::initialize(char initLHS)
 {
   if (initLHS && A) A->init();
   b.init();
@@ -504,9 +504,14 @@ std::unique_ptr<ASMs3D> ASMs3D::unitCube(int p, int nel)
   return std::unique_ptr<ASMs3D>(new ASMs3D(ord, ncf, kn, cf));
 }
 
-static GPUMatrix* gpu_matrix(AlgEqSystem& glInt, const char* where)
+// The GPU matrix behind a global integral: only an AlgEqSystem with an
+// initialised GPU matrix can receive the GPU's integrals (any other
+// GlobalIntegral would need element matrices on the host, which the GPU path
+// never forms).
+static GPUMatrix* gpu_matrix(GlobalIntegral& glInt, const char* where)
 {
-  GPUMatrix* G = dynamic_cast<GPUMatrix*>(glInt.getMatrix());
+  AlgEqSystem* eq = dynamic_cast<AlgEqSystem*>(&glInt);
+  GPUMatrix* G = eq ? dynamic_cast<GPUMatrix*>(eq->getMatrix()) : nullptr;
   if (!G || !G->ready()) {
     std::cerr << " *** " << where << ": the global integral has no initialised GPU matrix"
               << std::endl;
@@ -523,7 +528,7 @@ int ASMs3D::getNoGaussPt(int p, bool neumann) const
   return p + nGauss % 10;
 }
 
-bool ASMs3D::integrate(Integrand& integrand, AlgEqSystem& glInt, const TimeDomain&)
+bool ASMs3D::integrate(Integrand& integrand, GlobalIntegral& glInt, const TimeDomain&)
 {
   GPUMatrix* G = gpu_matrix(glInt, "ASMs3D::integrate");
   if (!G) return false;
@@ -544,7 +549,8 @@ bool ASMs3D::integrate(Integrand& integrand, AlgEqSystem& glInt, const TimeDomai
   return true;
 }
 
-bool ASMs3D::integrate(Integrand& integrand, int lIndex, AlgEqSystem& glInt, const TimeDomain&)
+bool ASMs3D::integrate(Integrand& integrand, int lIndex, GlobalIntegral& glInt,
+                       const TimeDomain&)
 {
   GPUMatrix* G = gpu_matrix(glInt, "ASMs3D::integrate(face)");
   if (!G) return false;

@@ -10,6 +10,8 @@
 //                       src/ASM/Integrand.h:206,344
 //   SAM / SAMpatch    equation numbering, expandSolution, normL2
 //                       src/LinAlg/SAM.h, src/ASM/SAMpatch.C:21-161
+//   GlobalIntegral    system-level integrated quantity (initialize,
+//                       finalize, assemble, threadSafe) src/ASM/GlobalIntegral.h
 //   AlgEqSystem       GlobalIntegral owning the system (init, initialize,
 //                       finalize, getMatrix, getVector) src/ASM/AlgEqSystem.C
 //   SystemMatrix      create(adm, MatrixType, LinSolParams), initAssembly,
@@ -250,18 +252,35 @@ public:
   bool init(const ASMs3D& pch, int ncomp);
 };
 
+//! System-level integrated quantity (src/ASM/GlobalIntegral.h:28-50): the
+//! target ASMbase::integrate assembles into.
+class GlobalIntegral {
+public:
+  virtual ~GlobalIntegral() {}
+  //! Initializes the integrated quantity to zero.
+  virtual void initialize(char) {}
+  //! Finalizes the integrated quantity after element assembly.
+  virtual bool finalize(bool) { return true; }
+  //! Adds a LocalIntegral into the global object.
+  virtual bool assemble(const LocalIntegral*, int) { return true; }
+  //! True if all elements can be assembled in parallel.
+  virtual bool threadSafe() const { return false; }
+};
+
 //! GlobalIntegral owning one system matrix and one right-hand side
-//! (src/ASM/AlgEqSystem.C:33-351).
-class AlgEqSystem {
+//! (src/ASM/AlgEqSystem.C:33-351).  The GPU integrates and scatters in one
+//! pass, so no element matrices reach assemble(): integrate() fills the
+//! matrix of an AlgEqSystem directly.
+class AlgEqSystem : public GlobalIntegral {
 public:
   explicit AlgEqSystem(const SAM& s) : sam(s) {}
   bool init(LinAlg::MatrixType mtype, const LinSolParams& spar);
-  void initialize(bool initLHS = true);
-  bool finalize(bool newLHS = true);
+  void initialize(char initLHS = 1) override;
+  bool finalize(bool newLHS = true) override;
   SystemMatrix* getMatrix() { return A.get(); }
   SystemVector* getVector() { return &b; }
   const SAM& getSAM() const { return sam; }
-  bool threadSafe() const { return true; }
+  bool threadSafe() const override { return true; }
 
 private:
   const SAM& sam;
@@ -316,9 +335,9 @@ public:
   int getNoGaussPt(int p, bool neumann = false) const;
 
   //! Volume integration (ASMs3D::integrate, src/ASM/ASMs3D.C:2071-2335).
-  bool integrate(Integrand& integrand, AlgEqSystem& glInt, const TimeDomain& time);
+  bool integrate(Integrand& integrand, GlobalIntegral& glInt, const TimeDomain& time);
   //! Face integration (src/ASM/ASMs3D.C:2547-2812), lIndex = face 1..6.
-  bool integrate(Integrand& integrand, int lIndex, AlgEqSystem& glInt, const TimeDomain& time);
+  bool integrate(Integrand& integrand, int lIndex, GlobalIntegral& glInt, const TimeDomain& time);
 
   int getNoNodes() const { return n[0] * n[1] * n[2]; }
   int order[3];

@@ -633,10 +633,16 @@ def main():
         avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         threads = max(1, min(16, avail or 1))
         layers = min(args.cpu_layers, pb.patch.nel[2])
-        t0 = time.perf_counter()
+        # warm-up on one layer, then the median of 3 timed runs
         oracle.assemble(pb.patch, pb.integrand, pb.ng, pb.meqn, neq, rp, ci, nthreads=threads,
-                        e3_range=(0, layers))
-        t_asm = time.perf_counter() - t0
+                        e3_range=(0, 1))
+        t_runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            oracle.assemble(pb.patch, pb.integrand, pb.ng, pb.meqn, neq, rp, ci,
+                            nthreads=threads, e3_range=(0, layers))
+            t_runs.append(time.perf_counter() - t0)
+        t_asm = float(np.median(t_runs))
         frac = layers / pb.patch.nel[2]
         cpu_dofs = ndof / (t_asm / frac)
         # one thread on a smaller sample (2 element layers)
@@ -656,15 +662,19 @@ def main():
         except OSError:
             pass
         its = 10
-        t0 = time.perf_counter()
-        oracle.pcg_fixed(rp, ci, A, b, its)
-        t_it = (time.perf_counter() - t0) / its
+        t_its = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            oracle.pcg_fixed(rp, ci, A, b, its)
+            t_its.append((time.perf_counter() - t0) / its)
+        t_it = float(np.median(t_its))
         cpu = {"value": round(cpu_dofs, 1), "unit": "assembled DOF/s", "cores": threads,
                "kind": "port",
                "sample": (f"oracle assembly of element layers w<{layers} of the same patch "
-                          f"({layers * pb.patch.nel[0] * pb.patch.nel[1]} of {nel_tot} elements, "
-                          f"{t_asm:.1f} s) scaled to the whole patch; Jacobi-PCG {its} iterations "
-                          f"on the full FP64 system"),
+                          f"({layers * pb.patch.nel[0] * pb.patch.nel[1]} of {nel_tot} elements; "
+                          f"median of 3 runs after a warm-up: {t_asm:.2f} s, runs "
+                          f"{', '.join(f'{t:.2f}' for t in t_runs)} s) scaled to the whole patch; "
+                          f"Jacobi-PCG {its} iterations on the full FP64 system, median of 3"),
                "pcg_ms_per_iter": round(t_it * 1e3, 2),
                "time_to_tol_est_s": round(t_it * iters64, 1),
                "value_1thread": round(cpu_dofs_1, 1),

@@ -35,9 +35,10 @@ struct BatchTune {
 struct Batch {
   std::vector<BatchPatch> pp;
   bool ready = false, fp64_ready = false, fp32_ready = false;
-  // the Gauss-point geometry records hold the last FP64 assembly's pass
-  // (Jacobians, material and source of every patch): an FP32 assembly of
-  // the same batch reuses them instead of recomputing them
+  // the Gauss-point geometry records hold the last assembly's pass
+  // (Jacobians and material of every patch; the source term only after an
+  // FP64 pass, which always recomputes them): an FP32 assembly of the same
+  // batch reuses them instead of recomputing them
   bool geo_valid = false;
   int neq = 0;
   int64_t nnz = 0;

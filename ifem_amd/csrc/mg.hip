@@ -31,6 +31,9 @@ namespace ifem {
 
 void launch_spmv16(Context& c, const __half* val, float scale, const float* p, float* q,
                    bool skip_done, hipStream_t st);
+void launch_spmv16_node3(Context& c, const uint2* val, float scale, const float* p, float* q,
+                         bool skip_done, hipStream_t st);
+int demote16_node3(Context& c, double inv_scale, uint2* out);
 
 Multigrid::~Multigrid()
 {
@@ -45,7 +48,7 @@ Multigrid::~Multigrid()
     hipFree(L.x); hipFree(L.b); hipFree(L.r); hipFree(L.d); hipFree(L.q);
   }
   hipFree(S0); hipFree(T1); hipFree(T2);
-  hipFree(val16); hipFree(cdofs); hipFree(cinv); hipFree(part); hipFree(lam); hipFree(err);
+  hipFree(val16); hipFree(val16n); hipFree(cdofs); hipFree(cinv); hipFree(part); hipFree(lam); hipFree(err);
   hipFree(dof_of_eq); hipFree(part2); hipFree(lzh);
   if (h_lzh) hipHostFree(h_lzh);
   if (h_coarse) hipHostFree(h_coarse);
@@ -864,7 +867,8 @@ static void level_spmv(Context& c, Multigrid& m, int l, const float* x, float* y
 {
   if (!st) st = c.stream;
   if (l == 0) {
-    launch_spmv16(c, m.val16, (float)m.kscale, x, y, scal != nullptr, st);
+    if (m.val16n) launch_spmv16_node3(c, m.val16n, (float)m.kscale, x, y, scal != nullptr, st);
+    else launch_spmv16(c, m.val16, (float)m.kscale, x, y, scal != nullptr, st);
     return;
   }
   const MgLevel& L = m.lev[l];
@@ -984,8 +988,14 @@ int mg_setup(Context& c, int degree)
   IFEM_HIP(hipMemcpyAsync(m.h_lzh, m.lzh, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   IFEM_HIP(hipStreamSynchronize(c.stream));
   m.kscale = m.h_lzh[0] > 0.0 ? m.h_lzh[0] : 1.0;
-  if (!m.val16) IFEM_HIP(hipMalloc(&m.val16, sizeof(__half) * (c.nnz > 0 ? c.nnz : 1)));
-  k_mg_demote16<<<4096, 256, 0, c.stream>>>(c.nnz, c.d_val, m.val16, 1.0 / m.kscale);
+  if (c.d_bcol) { // node blocks: one 8-byte load per row and neighbour in the smoother
+    if (!m.val16n) IFEM_HIP(hipMalloc(&m.val16n, sizeof(uint2) * (c.nnz > 0 ? c.nnz / 3 : 1)));
+    const int rc = demote16_node3(c, 1.0 / m.kscale, m.val16n);
+    if (rc) return rc;
+  } else {
+    if (!m.val16) IFEM_HIP(hipMalloc(&m.val16, sizeof(__half) * (c.nnz > 0 ? c.nnz : 1)));
+    k_mg_demote16<<<4096, 256, 0, c.stream>>>(c.nnz, c.d_val, m.val16, 1.0 / m.kscale);
+  }
   // fine operator as a stencil, then Galerkin products level by level
   IFEM_HIP(hipMemsetAsync(m.err, 0, sizeof(int), c.stream));
   const int ncc = m.nc * m.nc;

@@ -59,7 +59,8 @@ struct Multigrid {
   int* dof_of_eq = nullptr; // fine equation -> DOF
   double *T1 = nullptr, *T2 = nullptr;
   size_t s0cap = 0, t1cap = 0, t2cap = 0;
-  __half* val16 = nullptr;  // K / kscale in FP16 (fine-level smoother)
+  __half* val16 = nullptr;  // K / kscale in FP16 (fine-level smoother), CSR order
+  uint2* val16n = nullptr;  // the same in node blocks (3-component node-uniform systems)
   double kscale = 1.0;      // largest diagonal entry of K
   int ncf = 0;              // free DOFs of the coarsest level
   int* cdofs = nullptr;

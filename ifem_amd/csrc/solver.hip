@@ -124,6 +124,82 @@ __global__ __launch_bounds__(kThreads) void k_spmv_node3(
   if (threadIdx.x == 0) part_pq[blockIdx.x] = bs;
 }
 
+// FP16 values of a 3-component node-uniform system in node blocks: for node
+// nd and its k-th neighbour node (k < cnt, the order of bcol), 12 halfs at
+// ((rowptr[3nd]/9) + k) * 12: [row r of the node][column component j,
+// padded with a zero to 4], so that each row's three values of a neighbour
+// are one 8-byte load (instead of three 2-byte loads).  Built from the FP64
+// CSR values times inv_scale, one wave per node.
+__global__ void k_demote16_node3(int nnode, const int64_t* __restrict__ rowptr,
+                                 const double* __restrict__ a, uint2* __restrict__ b,
+                                 double inv_scale)
+{
+  const int lane = threadIdx.x & 63;
+  const int nw = (int)(((int64_t)gridDim.x * blockDim.x) >> 6);
+  for (int nd = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6); nd < nnode;
+       nd += nw) {
+    const int64_t r0 = rowptr[3 * nd];
+    const int cnt = (int)((rowptr[3 * nd + 1] - r0) / 3);
+    const int64_t nb = r0 / 9;
+    for (int k = lane; k < cnt; k += 64) {
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        const double* src = a + r0 + (int64_t)r * 3 * cnt + 3 * k;
+        const uint32_t h0 = __half_as_ushort(__float2half((float)(src[0] * inv_scale)));
+        const uint32_t h1 = __half_as_ushort(__float2half((float)(src[1] * inv_scale)));
+        const uint32_t h2 = __half_as_ushort(__float2half((float)(src[2] * inv_scale)));
+        b[(nb + k) * 3 + r] = make_uint2(h0 | (h1 << 16), h2);
+      }
+    }
+  }
+}
+
+__device__ inline float lo_half(uint32_t w) { return __half2float(__ushort_as_half((unsigned short)(w & 0xffffu))); }
+__device__ inline float hi_half(uint32_t w) { return __half2float(__ushort_as_half((unsigned short)(w >> 16))); }
+
+// q = scale * (K16 p) with the node-block FP16 values (k_demote16_node3), FP32
+// sums: the fine-level smoother's SpMV.  Same wave-per-node and XCD mapping
+// as k_spmv_node3.
+template <bool SKIP>
+__global__ __launch_bounds__(kThreads) void k_spmv_node3_h(
+    int nnode, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ bcol,
+    const uint2* __restrict__ val, const float* __restrict__ p, float* __restrict__ q,
+    const double* __restrict__ scal, float scale)
+{
+  if (SKIP && scal[S_DONE] != 0.0) return;
+  constexpr int WPB = kThreads / 64;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nxcd = 8;
+  const int xcd = blockIdx.x % nxcd, bl = blockIdx.x / nxcd, nbl = gridDim.x / nxcd;
+  const int lo = (int)((int64_t)nnode * xcd / nxcd), hi = (int)((int64_t)nnode * (xcd + 1) / nxcd);
+  for (int nd = lo + bl * WPB + wave; nd < hi; nd += nbl * WPB) {
+    const int64_t r0 = rowptr[3 * nd];
+    const int cnt = (int)((rowptr[3 * nd + 1] - r0) / 3);
+    const int64_t nb = r0 / 9;
+    const uint2* vb = val + nb * 3;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    for (int k = lane; k < cnt; k += 64) {
+      const int64_t c = 3 * (int64_t)bcol[nb + k];
+      const float x0 = p[c], x1 = p[c + 1], x2 = p[c + 2];
+      const uint2 w0 = vb[3 * k], w1 = vb[3 * k + 1], w2 = vb[3 * k + 2];
+      s0 += lo_half(w0.x) * x0 + hi_half(w0.x) * x1 + lo_half(w0.y) * x2;
+      s1 += lo_half(w1.x) * x0 + hi_half(w1.x) * x1 + lo_half(w1.y) * x2;
+      s2 += lo_half(w2.x) * x0 + hi_half(w2.x) * x1 + lo_half(w2.y) * x2;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      s0 += __shfl_xor(s0, off, 64);
+      s1 += __shfl_xor(s1, off, 64);
+      s2 += __shfl_xor(s2, off, 64);
+    }
+    if (lane == 0) {
+      q[3 * nd] = s0 * scale;
+      q[3 * nd + 1] = s1 * scale;
+      q[3 * nd + 2] = s2 * scale;
+    }
+  }
+}
+
 // bcol[r0/9 + k] = colind[r0 + 3k] / 3 for each node's first row.
 __global__ void k_build_bcol(int nnode, const int64_t* __restrict__ rowptr,
                              const int32_t* __restrict__ colind, int32_t* __restrict__ bcol)
@@ -324,6 +400,26 @@ void launch_spmv16(Context& c, const __half* val, float scale, const float* p, f
 {
   if (skip_done) launch_spmv<float, float, false, true, __half>(c, val, p, q, scale, st);
   else launch_spmv<float, float, false, false, __half>(c, val, p, q, scale, st);
+}
+
+// The node-block FP16 copy of K / scale (3-component node-uniform systems).
+int demote16_node3(Context& c, double inv_scale, uint2* out)
+{
+  k_demote16_node3<<<4096, 256, 0, c.stream>>>(c.neq / 3, c.d_rowptr, c.d_val, out, inv_scale);
+  IFEM_HIP(hipGetLastError());
+  return IFEM_GPU_OK;
+}
+
+void launch_spmv16_node3(Context& c, const uint2* val, float scale, const float* p, float* q,
+                         bool skip_done, hipStream_t st)
+{
+  if (!st) st = c.stream;
+  if (skip_done)
+    k_spmv_node3_h<true><<<kGrid, kThreads, 0, st>>>(c.neq / 3, c.d_rowptr, c.d_bcol, val, p, q,
+                                                      c.w.scal, scale);
+  else
+    k_spmv_node3_h<false><<<kGrid, kThreads, 0, st>>>(c.neq / 3, c.d_rowptr, c.d_bcol, val, p, q,
+                                                       c.w.scal, scale);
 }
 
 int spmv_device(Context& c, const double* d_x, double* d_y)

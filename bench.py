@@ -106,12 +106,12 @@ def _net_forward(Session, stream, spec, params, cin, cout, dims, reps=20):
 def learned_legs(args, stream, s, pb, x64, solve64_ms):
     """The learned correction inside flexible CG (library kernels for
     training and inference):
-    * config 4: diffusion 48^3 p=2, log-normal kappa (sigma 1.5, length 0.1):
+    * diffusion 48^3 p=2, log-normal kappa (sigma 1.5, length 0.1):
       a 3-level U-net trained on (residual, multigrid defect) pairs from the
       method's own multigrid flexible-CG iterates over seeded coefficient
       fields, evaluated on held-out seeds against Jacobi-PCG and multigrid
       without the network;
-    * config 2: the elasticity network trained the same way at 48^3, applied
+    * elasticity: the network trained the same way at 48^3, applied
       at 64^3 (learned + FP32: FP32 Krylov, FP32 multigrid and the network
       inside FP64 refinement), timed against plain FP64 Jacobi-PCG."""
     import numpy as np
@@ -137,7 +137,7 @@ def learned_legs(args, stream, s, pb, x64, solve64_ms):
         return (torch.stack(X[:npairs]).contiguous(), torch.stack(T[:npairs]).contiguous(),
                 seed - seed0)
 
-    # ---- config 4 -------------------------------------------------------------
+    # ---- diffusion 48^3, sigma 1.5, length 0.1 -----------------------------------
     mkd = lambda seed: problems.diffusion_cube(48, 2, sigma=1.5, ell=0.1, seed=seed)
     t0 = time.perf_counter()
     X, T, nseeds = gen(mkd, 100, args.learned_pairs)
@@ -193,7 +193,7 @@ def learned_legs(args, stream, s, pb, x64, solve64_ms):
 
     if s is None:
         return out
-    # ---- config 2: elasticity network trained at 48^3, applied at 64^3 -----------
+    # ---- elasticity network trained at 48^3, applied at 64^3 --------------------
     mke = lambda seed: problems.elasticity_cube(48, 2, seed=seed)
     dims48 = LN.lattice_dims(mke(0), spec.levels)
     t0 = time.perf_counter()
@@ -440,7 +440,7 @@ def main():
     if args.only == "batch":
         print(json.dumps({"batch": batch_leg(args)}), flush=True)
         return
-    if args.only == "learned":  # the 48^3 diffusion part (config 4) alone
+    if args.only == "learned":  # the 48^3 diffusion part alone
         print(json.dumps({"learned": learned_legs(args, stream, None, None, None, None)}),
               flush=True)
         return
@@ -611,7 +611,7 @@ def main():
         os_.close()
         log(f"[bench] multipatch: {multipatch}")
 
-    # ---- learned preconditioner: 48^3 diffusion (config 4) and 64^3 elasticity --
+    # ---- learned preconditioner: 48^3 diffusion and 64^3 elasticity -------------
     learned_leg = None
     if not args.no_learned:
         learned_leg = learned_legs(args, stream, s, pb, x64, solve64_ms)

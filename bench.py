@@ -215,6 +215,8 @@ def learned_legs(args, stream, s, pb, x64, solve64_ms):
     runs = [s.solve(rtol=args.rtol, maxits=5000, pc=PC_MG, precision=FP32, check_every=8)
             for _ in range(3)]
     x = s.solution()
+    # the same network and FP32 V-cycle inside FP64 Krylov (no refinement restarts)
+    runs64 = [s.solve(rtol=args.rtol, maxits=5000, pc=PC_MG, check_every=8) for _ in range(3)]
     s.set_learned(False)
     base = [s.solve(rtol=args.rtol, maxits=5000, pc=PC_MG, precision=FP32, check_every=8)
             for _ in range(3)]  # the same solve without the network
@@ -241,6 +243,10 @@ def learned_legs(args, stream, s, pb, x64, solve64_ms):
         "multigrid_fp32_without_network": {
             "solve_ms": round(float(np.median([r["solve_ms"] for r in base])), 3),
             "iterations": base[-1]["iterations"]},
+        "learned_fp64_krylov": {
+            "solve_ms": round(float(np.median([r["solve_ms"] for r in runs64])), 3),
+            "iterations": runs64[-1]["iterations"], "rel_residual": runs64[-1]["rel_residual"],
+            "converged": all(r["converged"] for r in runs64)},
         "learned_gain_time_vs_multigrid": round(float(np.median([r["solve_ms"] for r in base]))
                                                 / ms, 3),
     }

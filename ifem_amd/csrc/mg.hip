@@ -220,6 +220,49 @@ __global__ __launch_bounds__(256) void k_mg_csr_scatter(
   }
 }
 
+// The same scatter for 3-component node-uniform systems through the node
+// index (bcol): one wave per node, a lane per neighbour node, writing its
+// whole 3x3 block S[node][off][0..2][0..2] (72 contiguous bytes) from the
+// node's three rows.
+template <int P>
+__global__ __launch_bounds__(256) void k_mg_csr_scatter3(
+    int nnode, Grid3 g, const int* __restrict__ dof_of_eq, const int64_t* __restrict__ rowptr,
+    const int32_t* __restrict__ bcol, const double* __restrict__ val, double* __restrict__ S,
+    int* __restrict__ err)
+{
+  constexpr int W1 = 2 * P + 1, NOFF = W1 * W1 * W1;
+  const int wave = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (wave >= nnode) return;
+  const int node = dof_of_eq[3 * wave] / 3;
+  const int i = node % g.n0, j = (node / g.n0) % g.n1, k = node / (g.n0 * g.n1);
+  const int64_t r0 = rowptr[3 * wave];
+  const int cnt = (int)((rowptr[3 * wave + 1] - r0) / 3);
+  const int64_t nb = r0 / 9;
+  double* out = S + (int64_t)node * NOFF * 9;
+  const float inv0 = 1.0f / (float)g.n0, inv1 = 1.0f / (float)g.n1;
+  for (int q = lane; q < cnt; q += 64) {
+    const int cn = dof_of_eq[3 * bcol[nb + q]] / 3;
+    int rr0, rr1;
+    const int q0 = divmod_small(cn, g.n0, inv0, rr0);
+    const int q1 = divmod_small(q0, g.n1, inv1, rr1);
+    const int e0 = rr0 - i, e1 = rr1 - j, e2 = q1 - k;
+    if (e0 < -P || e0 > P || e1 < -P || e1 > P || e2 < -P || e2 > P) {
+      err[0] = 1;
+      continue;
+    }
+    const int off = ((e2 + P) * W1 + (e1 + P)) * W1 + (e0 + P);
+    double* o = out + off * 9;
+#pragma unroll
+    for (int ci = 0; ci < 3; ci++) {
+      const double* v = val + r0 + (int64_t)ci * 3 * cnt + 3 * q;
+      o[3 * ci] = v[0];
+      o[3 * ci + 1] = v[1];
+      o[3 * ci + 2] = v[2];
+    }
+  }
+}
+
 // One direction of the Galerkin product: Sout = P_d^T Sin P_d, where
 // direction d of the grid goes from gin.n_d fine to ncd coarse points.  All
 // stencils are node-major, S[node][off][ci][cj]; one thread per (node,
@@ -1000,10 +1043,25 @@ int mg_setup(Context& c, int degree)
   IFEM_HIP(hipMemsetAsync(m.err, 0, sizeof(int), c.stream));
   const int ncc = m.nc * m.nc;
   IFEM_HIP(hipMemsetAsync(m.S0, 0, sizeof(double) * m.s0cap, c.stream));
-  MG_DISPATCH(m.p, m.nc,
-              (k_mg_csr_scatter<KP, KN><<<(unsigned)(((int64_t)c.neq * 64 + 255) / 256), 256, 0,
-                                          c.stream>>>(c.neq, grid_of(F), m.dof_of_eq, c.d_rowptr,
-                                                      c.d_colind, c.d_val, m.S0, m.err)));
+  if (c.d_bcol && m.nc == 3) {
+    const int nnode = c.neq / 3;
+    const unsigned gb = (unsigned)(((int64_t)nnode * 64 + 255) / 256);
+    if (m.p == 1)
+      k_mg_csr_scatter3<1><<<gb, 256, 0, c.stream>>>(nnode, grid_of(F), m.dof_of_eq, c.d_rowptr,
+                                                     c.d_bcol, c.d_val, m.S0, m.err);
+    else if (m.p == 2)
+      k_mg_csr_scatter3<2><<<gb, 256, 0, c.stream>>>(nnode, grid_of(F), m.dof_of_eq, c.d_rowptr,
+                                                     c.d_bcol, c.d_val, m.S0, m.err);
+    else
+      k_mg_csr_scatter3<3><<<gb, 256, 0, c.stream>>>(nnode, grid_of(F), m.dof_of_eq, c.d_rowptr,
+                                                     c.d_bcol, c.d_val, m.S0, m.err);
+  } else {
+    MG_DISPATCH(m.p, m.nc,
+                (k_mg_csr_scatter<KP, KN><<<(unsigned)(((int64_t)c.neq * 64 + 255) / 256), 256,
+                                            0, c.stream>>>(c.neq, grid_of(F), m.dof_of_eq,
+                                                           c.d_rowptr, c.d_colind, c.d_val, m.S0,
+                                                           m.err)));
+  }
   IFEM_HIP(hipGetLastError());
   for (size_t l = 1; l < m.lev.size(); l++) {
     MgLevel& Fl = m.lev[l - 1];

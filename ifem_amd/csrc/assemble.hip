@@ -450,6 +450,11 @@ struct LineLds {
   static constexpr int SLQ = Chunk<P, NG, HALF>::SLQ;
   static constexpr int GEO = (NC == 3 ? 9 : 0) + 3 * P1;
   static constexpr int NRHS = HAS_RHS ? P1 * NC : 1;
+  // scalar problems with more than 27 Gauss points per element (the ordered,
+  // not pipelined path): phase B by sum factorisation, see line_impl; SFT =
+  // size of its first partial sums
+  static constexpr bool SF = NC == 1 && GP3 > 27 && HALF != 1 && CH * NEN <= 256 && P >= 2;
+  static constexpr int SFT = SF ? CH * 9 * NG * NG * P1 : 1;
 
   // every member 16-byte aligned: lets the compiler use 128-bit LDS loads
   alignas(16) T win[P1][NCR][WN][NC];
@@ -501,7 +506,7 @@ __device__ __forceinline__ void line_impl(
                             ? LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NT * 64 : 1],
     int64_t (*__restrict__ pbase)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NCR],
     int (*__restrict__ prow)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NCR],
-    int* __restrict__ nbundlep)
+    int* __restrict__ nbundlep, T* __restrict__ sft)
 {
   using LL = LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>;
   constexpr int P1 = LL::P1;
@@ -701,11 +706,23 @@ __device__ __forceinline__ void line_impl(
         const int a2r = jc - e2, a3r = kc - e3; // row node's local (a2, a3)
         const T n2a = tN2[m2][q2][a2r], d2a = tD2[m2][q2][a2r];
         const T n3a = tN3[m3][q3][a3r], d3a = tD3[m3][q3][a3r];
+        if constexpr (LL::SF) {
+          // W_kl = M_kl * A2_k(q2) * A3_k(q3) for the line's row functions in
+          // directions 2 and 3 (A_k: derivative in direction k, else value)
+          const T M[9] = {R[0], R[1], R[2], R[1], R[3], R[4], R[2], R[4], R[5]};
+          const T a2[3] = {n2a, d2a, n2a}, a3[3] = {n3a, n3a, d3a};
+#pragma unroll
+          for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int l = 0; l < 3; l++) geo[t][3 * k + l] = M[3 * k + l] * a2[k] * a3[k];
+        }
 #pragma unroll
         for (int a1 = 0; a1 < P1; a1++) {
           const T n1 = tN1[q1][a1], d1 = tD1[q1][a1];
           const T r0 = d1 * n2a * n3a, r1 = n1 * d2a * n3a, r2 = n1 * n2a * d3a;
-          if (NC == 1) {
+          if (NC == 1 && LL::SF) {
+            // written once below the a1 loop
+          } else if (NC == 1) {
             // h_a = M grad_ref N_a: S_ab = grad_ref N_b . h_a
             geo[t][3 * a1 + 0] = R[0] * r0 + R[1] * r1 + R[2] * r2;
             geo[t][3 * a1 + 1] = R[1] * r0 + R[3] * r1 + R[4] * r2;
@@ -859,6 +876,67 @@ __device__ __forceinline__ void line_impl(
           __syncthreads();
           if (HAS_RHS) rhs_reduce(ngs1);
           slab_b(std::integral_constant<int, 1>{});
+        }
+        } else if constexpr (LL::SF) {
+        // Sum factorisation (scalar problems): for the line's row functions
+        // a = (a1, a2, a3) with a2, a3 fixed by the line,
+        //   S_ab = sum_kl sum_q W_kl(q) A1_k(a1,q1) B1_l(b1,q1) B2_l(b2,q2) B3_l(b3,q3)
+        // contracted over q3 (T1), then q2 (T2, in the geometry rows' LDS),
+        // then q1 with the direction-1 functions, each step in fixed order:
+        // 9*NG*NG*P1 + 9*NG*P1*P1 + 9*NG*P1*NEN multiply-adds per element
+        // and line instead of 3*GP3*P1*NEN.
+        T* __restrict__ t2 = &geo[0][0];
+        const int n1t = nch * 9 * NG * NG * P1;
+        for (int t = tid; t < n1t; t += kBlock) {
+          const int bb3 = t % P1;
+          int r = t / P1;
+          const int q2 = r % NG;
+          r /= NG;
+          const int q1 = r % NG;
+          r /= NG;
+          const int kl = r % 9, lq = r / 9;
+          const int l = kl % 3;
+          const int mm3 = be3[c0 + lq] - kc + P;
+          const T* __restrict__ w = &geo[lq * GP3 + q1 + NG * q2][kl];
+          T sum = 0;
+#pragma unroll
+          for (int q3 = 0; q3 < NG; q3++)
+            sum += w[NG * NG * q3 * LL::GEO] * (l == 2 ? tD3[mm3][q3][bb3] : tN3[mm3][q3][bb3]);
+          sft[t] = sum; // [le][kl][q1][q2][b3]
+        }
+        __syncthreads(); // T1 complete, the W rows read
+        const int n2t = nch * 9 * NG * P1 * P1;
+        for (int t = tid; t < n2t; t += kBlock) {
+          const int bb3 = t % P1;
+          int r = t / P1;
+          const int bb2 = r % P1;
+          r /= P1;
+          const int q1 = r % NG;
+          r /= NG;
+          const int kl = r % 9, lq = r / 9;
+          const int l = kl % 3;
+          const int mm2 = be2[c0 + lq] - jc + P;
+          const T* __restrict__ u = sft + (((lq * 9 + kl) * NG + q1) * NG) * P1 + bb3;
+          T sum = 0;
+#pragma unroll
+          for (int q2 = 0; q2 < NG; q2++)
+            sum += u[q2 * P1] * (l == 1 ? tD2[mm2][q2][bb2] : tN2[mm2][q2][bb2]);
+          t2[t] = sum; // [le][kl][q1][b2][b3]
+        }
+        __syncthreads();
+        if (has) {
+          const T* __restrict__ v = t2 + (int64_t)le * 9 * NG * P1 * P1 + b2 * P1 + b3;
+#pragma unroll
+          for (int kl = 0; kl < 9; kl++) {
+            const int k = kl / 3, l = kl % 3;
+#pragma unroll
+            for (int q1 = 0; q1 < NG; q1++) {
+              const T c = v[(kl * NG + q1) * P1 * P1] * (l == 0 ? tD1[q1][b1] : tN1[q1][b1]);
+#pragma unroll
+              for (int a1 = 0; a1 < P1; a1++)
+                acc[a1][0] += c * (k == 0 ? tD1[q1][a1] : tN1[q1][a1]);
+            }
+          }
         }
         } else {
       if (has) {
@@ -1370,9 +1448,10 @@ __global__ __launch_bounds__(kBlock, HALF == 2 ? 3 : 1) void k_assemble_lines(
   __shared__ int64_t pbase[2][LL::NCR];
   __shared__ int prow[2][LL::NCR];
   __shared__ int nbundle;
+  __shared__ T sft[LL::SFT];
   line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF, PIPE, ORD, !PIPE>(a, blockIdx.x, blockIdx.y, win, geo, stage,
                                                       rstage, rwin, tN1, tD1, tN2, tD2, tN3, tD3,
-                                                      be2, be3, eqw, pbase, prow, &nbundle);
+                                                      be2, be3, eqw, pbase, prow, &nbundle, sft);
 }
 
 // Persistent batch assembly over many independent patches, one launch per
@@ -1407,6 +1486,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_assemble_batch(const AsmArgs* __r
   __shared__ int64_t pbase[2][LL::NCR];
   __shared__ int prow[2][LL::NCR];
   __shared__ int nbundle;
+  __shared__ T sft[LL::SFT];
   __shared__ int next;
   while (true) {
     if (threadIdx.x == 0) next = atomicAdd(queue, 1);
@@ -1417,7 +1497,8 @@ __global__ __launch_bounds__(kBlock, 3) void k_assemble_batch(const AsmArgs* __r
     ConstArgs& a = const_args(args)[w.x];
     line_impl<P, NG, 1, T, 1, HAS_RHS, HALF, PIPE, ORD, !PIPE>(a, w.y, 0, win, geo, stage, rstage,
                                                                rwin, tN1, tD1, tN2, tD2, tN3, tD3,
-                                                               be2, be3, eqw, pbase, prow, &nbundle);
+                                                               be2, be3, eqw, pbase, prow, &nbundle,
+                                                               sft);
     __syncthreads();
   }
 }

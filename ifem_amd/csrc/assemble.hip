@@ -454,7 +454,10 @@ struct LineLds {
   // not pipelined path): phase B by sum factorisation, see line_impl; SFT =
   // size of its first partial sums
   static constexpr bool SF = NC == 1 && GP3 > 27 && HALF != 1 && CH * NEN <= 256 && P >= 2;
-  static constexpr int SFT = SF ? CH * 9 * NG * NG * P1 : 1;
+  // FP64: T1 one derivative direction k (three of the nine kl) at a time,
+  // so that the workgroup keeps three blocks per CU; FP32: all at once
+  static constexpr int SFK = sizeof(T) == 8 ? 3 : 1;
+  static constexpr int SFT = SF ? CH * (9 / SFK) * NG * NG * P1 : 1;
 
   // every member 16-byte aligned: lets the compiler use 128-bit LDS loads
   alignas(16) T win[P1][NCR][WN][NC];
@@ -711,10 +714,13 @@ __device__ __forceinline__ void line_impl(
           // directions 2 and 3 (A_k: derivative in direction k, else value)
           const T M[9] = {R[0], R[1], R[2], R[1], R[3], R[4], R[2], R[4], R[5]};
           const T a2[3] = {n2a, d2a, n2a}, a3[3] = {n3a, n3a, d3a};
+          // stored [element][kl][point]: T2 later overwrites each (element,
+          // kl) block in place
+          T* __restrict__ wf = &geo[0][0] + (int64_t)le * 9 * GP3 + (q - qa);
 #pragma unroll
           for (int k = 0; k < 3; k++)
 #pragma unroll
-            for (int l = 0; l < 3; l++) geo[t][3 * k + l] = M[3 * k + l] * a2[k] * a3[k];
+            for (int l = 0; l < 3; l++) wf[(3 * k + l) * GP3] = M[3 * k + l] * a2[k] * a3[k];
         }
 #pragma unroll
         for (int a1 = 0; a1 < P1; a1++) {
@@ -885,53 +891,57 @@ __device__ __forceinline__ void line_impl(
         // then q1 with the direction-1 functions, each step in fixed order:
         // 9*NG*NG*P1 + 9*NG*P1*P1 + 9*NG*P1*NEN multiply-adds per element
         // and line instead of 3*GP3*P1*NEN.
-        T* __restrict__ t2 = &geo[0][0];
-        const int n1t = nch * 9 * NG * NG * P1;
-        for (int t = tid; t < n1t; t += kBlock) {
-          const int bb3 = t % P1;
-          int r = t / P1;
-          const int q2 = r % NG;
-          r /= NG;
-          const int q1 = r % NG;
-          r /= NG;
-          const int kl = r % 9, lq = r / 9;
-          const int l = kl % 3;
-          const int mm3 = be3[c0 + lq] - kc + P;
-          const T* __restrict__ w = &geo[lq * GP3 + q1 + NG * q2][kl];
-          T sum = 0;
+        T* __restrict__ gf = &geo[0][0]; // W, then T2, [element][kl][...] with stride GP3
+        constexpr int KPG = 3 / LL::SFK;   // derivative directions k per T1 pass
+        for (int kg = 0; kg < LL::SFK; kg++) {
+          if (kg > 0) __syncthreads(); // the previous group's T1 read
+          const int n1t = nch * 3 * KPG * NG * NG * P1;
+          for (int t = tid; t < n1t; t += kBlock) {
+            const int bb3 = t % P1;
+            int r = t / P1;
+            const int q2 = r % NG;
+            r /= NG;
+            const int q1 = r % NG;
+            r /= NG;
+            const int klg = r % (3 * KPG), lq = r / (3 * KPG);
+            const int kl = kg * 3 * KPG + klg, l = kl % 3;
+            const int mm3 = be3[c0 + lq] - kc + P;
+            const T* __restrict__ w = gf + (int64_t)(lq * 9 + kl) * GP3 + q1 + NG * q2;
+            T sum = 0;
 #pragma unroll
-          for (int q3 = 0; q3 < NG; q3++)
-            sum += w[NG * NG * q3 * LL::GEO] * (l == 2 ? tD3[mm3][q3][bb3] : tN3[mm3][q3][bb3]);
-          sft[t] = sum; // [le][kl][q1][q2][b3]
-        }
-        __syncthreads(); // T1 complete, the W rows read
-        const int n2t = nch * 9 * NG * P1 * P1;
-        for (int t = tid; t < n2t; t += kBlock) {
-          const int bb3 = t % P1;
-          int r = t / P1;
-          const int bb2 = r % P1;
-          r /= P1;
-          const int q1 = r % NG;
-          r /= NG;
-          const int kl = r % 9, lq = r / 9;
-          const int l = kl % 3;
-          const int mm2 = be2[c0 + lq] - jc + P;
-          const T* __restrict__ u = sft + (((lq * 9 + kl) * NG + q1) * NG) * P1 + bb3;
-          T sum = 0;
+            for (int q3 = 0; q3 < NG; q3++)
+              sum += w[NG * NG * q3] * (l == 2 ? tD3[mm3][q3][bb3] : tN3[mm3][q3][bb3]);
+            sft[t] = sum; // [le][kl in group][q1][q2][b3]
+          }
+          __syncthreads(); // T1 of the group complete, its W rows read
+          const int n2t = nch * 3 * KPG * NG * P1 * P1;
+          for (int t = tid; t < n2t; t += kBlock) {
+            const int bb3 = t % P1;
+            int r = t / P1;
+            const int bb2 = r % P1;
+            r /= P1;
+            const int q1 = r % NG;
+            r /= NG;
+            const int klg = r % (3 * KPG), lq = r / (3 * KPG);
+            const int kl = kg * 3 * KPG + klg, l = kl % 3;
+            const int mm2 = be2[c0 + lq] - jc + P;
+            const T* __restrict__ u = sft + (((lq * 3 * KPG + klg) * NG + q1) * NG) * P1 + bb3;
+            T sum = 0;
 #pragma unroll
-          for (int q2 = 0; q2 < NG; q2++)
-            sum += u[q2 * P1] * (l == 1 ? tD2[mm2][q2][bb2] : tN2[mm2][q2][bb2]);
-          t2[t] = sum; // [le][kl][q1][b2][b3]
+            for (int q2 = 0; q2 < NG; q2++)
+              sum += u[q2 * P1] * (l == 1 ? tD2[mm2][q2][bb2] : tN2[mm2][q2][bb2]);
+            gf[(int64_t)(lq * 9 + kl) * GP3 + (q1 * P1 + bb2) * P1 + bb3] = sum; // T2
+          }
         }
         __syncthreads();
         if (has) {
-          const T* __restrict__ v = t2 + (int64_t)le * 9 * NG * P1 * P1 + b2 * P1 + b3;
+          const T* __restrict__ v = gf + (int64_t)le * 9 * GP3 + b2 * P1 + b3;
 #pragma unroll
           for (int kl = 0; kl < 9; kl++) {
             const int k = kl / 3, l = kl % 3;
 #pragma unroll
             for (int q1 = 0; q1 < NG; q1++) {
-              const T c = v[(kl * NG + q1) * P1 * P1] * (l == 0 ? tD1[q1][b1] : tN1[q1][b1]);
+              const T c = v[kl * GP3 + q1 * P1 * P1] * (l == 0 ? tD1[q1][b1] : tN1[q1][b1]);
 #pragma unroll
               for (int a1 = 0; a1 < P1; a1++)
                 acc[a1][0] += c * (k == 0 ? tD1[q1][a1] : tN1[q1][a1]);

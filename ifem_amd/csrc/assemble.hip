@@ -128,44 +128,47 @@ __global__ void k_basis_tables(const double* __restrict__ t, int nel, int ng, Ga
 }
 
 // ---------------------------------------------------------------------------
-// cos(x) for the Fourier fields: Cody-Waite argument reduction modulo pi/2
+// cos(x) for the Fourier fields: Cody-Waite argument reduction modulo pi
 // (W. J. Cody and W. Waite, Software Manual for the Elementary Functions,
-// 1980: pi/2 split into three doubles, each product subtracted by a fused
-// multiply-add) and the Taylor series of sin and cos on [-pi/4, pi/4] to
-// degree 17 and 16 (truncation below 1e-17), the quadrant selecting +-sin or
-// +-cos.  Within 1 ulp of the C library's cos for |x| <= 400 (2e7 samples,
-// also next to multiples of pi/2); |x| > 1e5 goes to the library routine.
-// The 128-mode log-normal conductivity spends most of the geometry pass in
-// this function (fewer FP64 operations than the library's general path).
+// 1980: pi split into three doubles, each product subtracted by a fused
+// multiply-add; the first difference is exact), cos x = (-1)^n cos r with
+// |r| <= pi/2, and the Taylor series of cos to degree 20 (truncation below
+// 2e-17 at |r| = pi/2).  Absolute error below 1.3 ulp(1) and within 2 ulp of
+// the C library's cos where |cos x| > 1/2, for |x| <= 1e9 (2e7 samples per
+// range, also next to multiples of pi/2 and pi); larger |x| goes to the
+// library routine, kept out of line so that its registers do not count
+// against the callers.  The 128-mode log-normal conductivity spends most of
+// the geometry pass in this function.
+__device__ __noinline__ double cos_library(double x)
+{
+  return cos(x);
+}
+
+__device__ __noinline__ double sin_library(double x)
+{
+  return sin(x);
+}
+
 __device__ inline double cos_cw(double x)
 {
-  if (fabs(x) > 1e5) return cos(x);
-  const double n = rint(x * 0x1.45f306dc9c883p-1); // x * 2/pi
-  double r = fma(-n, 0x1.921fb54442d18p+0, x);     // pi/2, three parts
-  r = fma(-n, 0x1.1a62633145c07p-54, r);
-  r = fma(-n, -0x1.f1976b7ed8fbcp-110, r);
+  if (!(fabs(x) <= 0x1p30)) return cos_library(x);
+  const double n = rint(x * 0x1.45f306dc9c883p-2); // x / pi
+  double r = fma(-n, 0x1.921fb54442d18p+1, x);     // pi, three parts
+  r = fma(-n, 0x1.1a62633145c07p-53, r);
+  r = fma(-n, -0x1.f1976b7ed8fbcp-109, r);
   const double s = r * r;
-  double ps = 1.0 / 355687428096000.0; // 1/17!, then 1/15!, ... with alternating signs
-  ps = fma(ps, s, -1.0 / 1307674368000.0);
-  ps = fma(ps, s, 1.0 / 6227020800.0);
-  ps = fma(ps, s, -1.0 / 39916800.0);
-  ps = fma(ps, s, 1.0 / 362880.0);
-  ps = fma(ps, s, -1.0 / 5040.0);
-  ps = fma(ps, s, 1.0 / 120.0);
-  ps = fma(ps, s, -1.0 / 6.0);
-  const double sn = fma(r * s, ps, r);
-  double pc = 1.0 / 20922789888000.0; // 1/16!, then 1/14!, ...
-  pc = fma(pc, s, -1.0 / 87178291200.0);
-  pc = fma(pc, s, 1.0 / 479001600.0);
-  pc = fma(pc, s, -1.0 / 3628800.0);
-  pc = fma(pc, s, 1.0 / 40320.0);
-  pc = fma(pc, s, -1.0 / 720.0);
-  pc = fma(pc, s, 1.0 / 24.0);
-  pc = fma(pc, s, -0.5);
-  const double cs = fma(s, pc, 1.0);
-  const int q = (int)n & 3; // cos(r + q pi/2) = cos r, -sin r, -cos r, sin r
-  const double v = (q & 1) ? sn : cs;
-  return ((q + 1) & 2) ? -v : v;
+  double c = 1.0 / 2432902008176640000.0; // 1/20!, then -1/18!, 1/16!, ...
+  c = fma(c, s, -1.0 / 6402373705728000.0);
+  c = fma(c, s, 1.0 / 20922789888000.0);
+  c = fma(c, s, -1.0 / 87178291200.0);
+  c = fma(c, s, 1.0 / 479001600.0);
+  c = fma(c, s, -1.0 / 3628800.0);
+  c = fma(c, s, 1.0 / 40320.0);
+  c = fma(c, s, -1.0 / 720.0);
+  c = fma(c, s, 1.0 / 24.0);
+  c = fma(c, s, -0.5);
+  c = fma(c, s, 1.0);
+  return ((int)n & 1) ? -c : c;
 }
 
 // Scalar fields evaluated at Gauss points (same definitions as the oracle).
@@ -175,14 +178,22 @@ __device__ inline double eval_field(const DevFunction& f, double x, double y, do
   case IFEM_GPU_FN_CONST:
     return f.scale;
   case IFEM_GPU_FN_SINPROD:
-    return f.scale * sin(M_PI * x) * sin(M_PI * y) * sin(M_PI * z);
+    return f.scale * sin_library(M_PI * x) * sin_library(M_PI * y) * sin_library(M_PI * z);
   case IFEM_GPU_FN_FOURIER:
   case IFEM_GPU_FN_EXP_FOURIER: {
+    // four modes at a time (independent cosines), summed in mode order
     double s = 0.0;
-    for (int m = 0; m < f.nmodes; m++) {
-      const double* md = f.modes + 5 * m;
-      s += md[4] * cos_cw(md[0] * x + md[1] * y + md[2] * z + md[3]);
+    const double* __restrict__ md = f.modes;
+    int m = 0;
+    for (; m + 4 <= f.nmodes; m += 4, md += 20) {
+      double c[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        c[u] = cos_cw(md[5 * u] * x + md[5 * u + 1] * y + md[5 * u + 2] * z + md[5 * u + 3]);
+#pragma unroll
+      for (int u = 0; u < 4; u++) s += md[5 * u + 4] * c[u];
     }
+    for (; m < f.nmodes; m++, md += 5) s += md[4] * cos_cw(md[0] * x + md[1] * y + md[2] * z + md[3]);
     const double g = f.mean + f.scale * s;
     return f.type == IFEM_GPU_FN_EXP_FOURIER ? exp(g) : g;
   }
@@ -364,8 +375,12 @@ __device__ inline void gp_geometry_at(ConstArgs& a, int64_t id, bool want_rhs)
   out[7] = 0.0;
 }
 
+// Built for four waves per SIMD (at most 128 registers): the Fourier-mode
+// loop is latency-bound, and more waves hide more of it (measured: 4 waves
+// beat 2, 3, 5, 6 and 8 on the 256-patch diffusion batch; beyond 4 the
+// geometry part spills)
 template <int P>
-__global__ __launch_bounds__(256) void k_gp_geometry(const AsmArgs* __restrict__ args,
+__global__ __launch_bounds__(256, 4) void k_gp_geometry(const AsmArgs* __restrict__ args,
                                                      int64_t total, int want_rhs)
 {
   ConstArgs& a = *const_args(args);
@@ -375,7 +390,7 @@ __global__ __launch_bounds__(256) void k_gp_geometry(const AsmArgs* __restrict__
 }
 
 // Batch: gp0[q] = first Gauss-point record of patch q.
-__global__ __launch_bounds__(256) void k_gp_geometry_batch(const AsmArgs* __restrict__ args,
+__global__ __launch_bounds__(256, 4) void k_gp_geometry_batch(const AsmArgs* __restrict__ args,
                                     const int64_t* __restrict__ gp0, int np, int64_t total,
                                     int want_rhs)
 {

@@ -904,7 +904,7 @@ __device__ __forceinline__ void line_impl(
         //   S_ab = sum_kl sum_q W_kl(q) A1_k(a1,q1) B1_l(b1,q1) B2_l(b2,q2) B3_l(b3,q3)
         // contracted over q3 (T1), then q2 (T2, in the geometry rows' LDS),
         // then q1 with the direction-1 functions, each step in fixed order:
-        // 9*NG*NG*P1 + 9*NG*P1*P1 + 9*NG*P1*NEN multiply-adds per element
+        // 9*NG*NG*P1 + 9*NG*P1*P1 + 3*NG*(P1+3)*NEN operations per element
         // and line instead of 3*GP3*P1*NEN.
         T* __restrict__ gf = &geo[0][0]; // W, then T2, [element][kl][...] with stride GP3
         constexpr int KPG = 3 / LL::SFK;   // derivative directions k per T1 pass
@@ -950,16 +950,24 @@ __device__ __forceinline__ void line_impl(
         }
         __syncthreads();
         if (has) {
+          // per (k, q1): U = T2_k0 B1'(b1) + (T2_k1 + T2_k2) B1(b1) (direction 1
+          // carries the derivative for l = 0 only), then U A1_k(a1) per row
           const T* __restrict__ v = gf + (int64_t)le * 9 * GP3 + b2 * P1 + b3;
+          T bn[NG], bd[NG];
 #pragma unroll
-          for (int kl = 0; kl < 9; kl++) {
-            const int k = kl / 3, l = kl % 3;
+          for (int q1 = 0; q1 < NG; q1++) {
+            bn[q1] = tN1[q1][b1];
+            bd[q1] = tD1[q1][b1];
+          }
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
 #pragma unroll
             for (int q1 = 0; q1 < NG; q1++) {
-              const T c = v[kl * GP3 + q1 * P1 * P1] * (l == 0 ? tD1[q1][b1] : tN1[q1][b1]);
+              const T* __restrict__ vk = v + 3 * k * GP3 + q1 * P1 * P1;
+              const T u = vk[0] * bd[q1] + (vk[GP3] + vk[2 * GP3]) * bn[q1];
 #pragma unroll
               for (int a1 = 0; a1 < P1; a1++)
-                acc[a1][0] += c * (k == 0 ? tD1[q1][a1] : tN1[q1][a1]);
+                acc[a1][0] += u * (k == 0 ? tD1[q1][a1] : tN1[q1][a1]);
             }
           }
         }

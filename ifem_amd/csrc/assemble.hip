@@ -296,7 +296,7 @@ __device__ inline DevFunction load_function(const __attribute__((address_space(4
   return d;
 }
 
-template <int P>
+template <int P, bool VEC = false>
 __device__ inline void gp_geometry_at(ConstArgs& a, int64_t id, bool want_rhs)
 {
   constexpr int P1 = P + 1;
@@ -352,7 +352,7 @@ __device__ inline void gp_geometry_at(ConstArgs& a, int64_t id, bool want_rhs)
     const double dV = 0.125 * (a.span1[e1] * a.span2[e2] * a.span3[e3]);
     wdet = det * (dV * a.gw[q1] * a.gw[q2] * a.gw[q3]);
   }
-  if (a.ncomp == 3) {
+  if (VEC || a.ncomp == 3) {
     double* out = a.gp_geo + id * GeoRec<3>::N;
 #pragma unroll
     for (int m = 0; m < 9; m++) out[m] = Ji[m];
@@ -375,10 +375,12 @@ __device__ inline void gp_geometry_at(ConstArgs& a, int64_t id, bool want_rhs)
   out[7] = 0.0;
 }
 
-// Built for four waves per SIMD (at most 128 registers): the Fourier-mode
-// loop is latency-bound, and more waves hide more of it (measured: 4 waves
-// beat 2, 3, 5, 6 and 8 on the 256-patch diffusion batch; beyond 4 the
-// geometry part spills)
+// Scalar problems: built for four waves per SIMD (at most 128 registers):
+// the Fourier-mode loop is latency-bound, and more waves hide more of it
+// (measured: 4 waves beat 2, 3, 5, 6 and 8 on the 256-patch diffusion batch;
+// beyond 4 the geometry part spills).  Vector problems (no fields): their
+// own instance without the field code, 96 registers at p = 2 (64^3
+// elasticity: 0.38 ms against 0.48 ms with the shared instance).
 template <int P>
 __global__ __launch_bounds__(256, 4) void k_gp_geometry(const AsmArgs* __restrict__ args,
                                                      int64_t total, int want_rhs)
@@ -387,6 +389,16 @@ __global__ __launch_bounds__(256, 4) void k_gp_geometry(const AsmArgs* __restric
   for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < total;
        id += (int64_t)gridDim.x * blockDim.x)
     gp_geometry_at<P>(a, id, want_rhs != 0);
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_gp_geometry_vec(const AsmArgs* __restrict__ args,
+                                                         int64_t total)
+{
+  ConstArgs& a = *const_args(args);
+  for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < total;
+       id += (int64_t)gridDim.x * blockDim.x)
+    gp_geometry_at<P, true>(a, id, false);
 }
 
 // Batch: gp0[q] = first Gauss-point record of patch q.
@@ -2028,7 +2040,10 @@ static int assemble_p(Context& c, const SysView& sv, const PatchRec& Pt, const i
   if (!c.d_asmargs) IFEM_HIP(hipMalloc(&c.d_asmargs, sizeof(AsmArgs)));
   IFEM_HIP(hipMemcpyAsync(c.d_asmargs, &a, sizeof(AsmArgs), hipMemcpyHostToDevice, c.stream));
   IFEM_HIP(hipEventRecord(c.ev0, c.stream));
-  if (ngp > 0)
+  if (ngp > 0 && c.ncomp == 3)
+    k_gp_geometry_vec<P><<<grid_for(ngp), 256, 0, c.stream>>>(
+        static_cast<const AsmArgs*>(c.d_asmargs), ngp);
+  else if (ngp > 0)
     k_gp_geometry<P><<<grid_for(ngp), 256, 0, c.stream>>>(static_cast<const AsmArgs*>(c.d_asmargs),
                                                          ngp, has_rhs ? 1 : 0);
   IFEM_HIP(hipEventRecord(c.ev_mid, c.stream));

@@ -162,7 +162,13 @@ int ifem_gpu_ctx_destroy(ifem_gpu_ctx* ctx)
     for (int d = 0; d < 3; d++) hipFree(p.d_knots[d]);
     hipFree(p.d_coefs);
   }
-  for (void* p : ctx->c.scratch_tables) hipFree(p);
+  for (auto& t : ctx->c.basis_cache)
+    for (int d = 0; d < 3; d++) {
+      hipFree(t.tabN[d]);
+      hipFree(t.tabD[d]);
+      hipFree(t.span[d]);
+    }
+  for (int k = 0; k < 3; k++) hipFree(ctx->c.d_modes[k]);
   ctx->c.net.reset();
   ctx->c.batch.reset();
   hipFree(ctx->c.d_asmargs);

@@ -1833,36 +1833,48 @@ int build_pattern(Context& c)
 
 // ---------------------------------------------------------------------------
 // Host dispatch of the numeric assembly.
-struct Tables {
-  double *tabN[3] = {nullptr, nullptr, nullptr}, *tabD[3] = {nullptr, nullptr, nullptr};
-  double* span[3] = {nullptr, nullptr, nullptr};
-  ~Tables()
-  {
-    for (int d = 0; d < 3; d++) {
-      hipFree(tabN[d]);
-      hipFree(tabD[d]);
-      hipFree(span[d]);
-    }
-  }
-};
+// Basis tables of patch Pt for ng Gauss points per direction (owned by the
+// context's cache; built by k_basis_tables on first use).
+using Tables = Context::BasisTables;
 
 template <int P>
 static int make_tables(Context& c, const PatchRec& Pt, int ng, Tables& tb, GaussArgs& ga)
 {
+  for (const Tables& t : c.basis_cache)
+    if (t.knots0 == Pt.d_knots[0] && t.ng == ng) {
+      tb = t;
+      return IFEM_GPU_OK;
+    }
+  Tables t;
+  t.knots0 = Pt.d_knots[0];
+  t.ng = ng;
   for (int d = 0; d < 3; d++) {
     const size_t cnt = (size_t)Pt.nel[d] * ng * (P + 1);
-    IFEM_HIP(hipMalloc(&tb.tabN[d], sizeof(double) * cnt));
-    IFEM_HIP(hipMalloc(&tb.tabD[d], sizeof(double) * cnt));
-    IFEM_HIP(hipMalloc(&tb.span[d], sizeof(double) * Pt.nel[d]));
+    if (hipMalloc(&t.tabN[d], sizeof(double) * cnt) != hipSuccess ||
+        hipMalloc(&t.tabD[d], sizeof(double) * cnt) != hipSuccess ||
+        hipMalloc(&t.span[d], sizeof(double) * Pt.nel[d]) != hipSuccess) {
+      for (int e = 0; e <= d; e++) {
+        hipFree(t.tabN[e]);
+        hipFree(t.tabD[e]);
+        hipFree(t.span[e]);
+      }
+      return fail(IFEM_GPU_EDEVICE, "assemble: out of device memory for the basis tables");
+    }
+  }
+  for (int d = 0; d < 3; d++) {
     const int tot = Pt.nel[d] * ng;
     k_basis_tables<P><<<(tot + 127) / 128, 128, 0, c.stream>>>(Pt.d_knots[d], Pt.nel[d], ng, ga,
-                                                              tb.tabN[d], tb.tabD[d], tb.span[d]);
+                                                              t.tabN[d], t.tabD[d], t.span[d]);
   }
+  c.basis_cache.push_back(t);
+  tb = t;
   IFEM_HIP(hipGetLastError());
   return IFEM_GPU_OK;
 }
 
-static int upload_function(Context& c, const ifem_gpu_function& f, DevFunction& df)
+// slot: 0 conductivity, 1 source, 2 exact solution (one device buffer each,
+// reused by later calls on the same stream)
+static int upload_function(Context& c, const ifem_gpu_function& f, DevFunction& df, int slot)
 {
   df.type = f.type;
   df.nmodes = f.nmodes;
@@ -1873,10 +1885,16 @@ static int upload_function(Context& c, const ifem_gpu_function& f, DevFunction& 
     return fail(IFEM_GPU_EINVAL, "assemble: unknown field type");
   if ((f.type == IFEM_GPU_FN_FOURIER || f.type == IFEM_GPU_FN_EXP_FOURIER) && f.nmodes > 0) {
     if (!f.modes) return fail(IFEM_GPU_EINVAL, "assemble: null mode table");
-    double* d = nullptr;
-    IFEM_HIP(hipMalloc(&d, sizeof(double) * 5 * f.nmodes));
-    IFEM_HIP(hipMemcpyAsync(d, f.modes, sizeof(double) * 5 * f.nmodes, hipMemcpyHostToDevice, c.stream));
-    c.scratch_tables.push_back(d);
+    const size_t need = (size_t)5 * f.nmodes;
+    if (c.modes_cap[slot] < need) {
+      hipFree(c.d_modes[slot]);
+      c.d_modes[slot] = nullptr;
+      c.modes_cap[slot] = 0;
+      IFEM_HIP(hipMalloc(&c.d_modes[slot], sizeof(double) * need));
+      c.modes_cap[slot] = need;
+    }
+    double* d = c.d_modes[slot];
+    IFEM_HIP(hipMemcpyAsync(d, f.modes, sizeof(double) * need, hipMemcpyHostToDevice, c.stream));
     df.modes = d;
   } else {
     df.nmodes = 0;
@@ -2005,10 +2023,8 @@ static int assemble_p(Context& c, const SysView& sv, const PatchRec& Pt, const i
     a.rhs = sv.rhs;
     a.gval = gval;
   }
-  for (void* p : c.scratch_tables) hipFree(p);
-  c.scratch_tables.clear();
-  rc = upload_function(c, itg.kappa, a.kappa);
-  if (!rc) rc = upload_function(c, itg.source, a.source);
+  rc = upload_function(c, itg.kappa, a.kappa, 0);
+  if (!rc) rc = upload_function(c, itg.source, a.source, 1);
   if (rc) return rc;
   for (int i = 0; i < 3; i++) a.body[i] = itg.body[i];
   if (itg.kind == IFEM_GPU_ELASTICITY) {
@@ -2855,9 +2871,7 @@ static int norms_patch(Context& c, const PatchRec& Pt, const int* eqmap, const d
   a.gval = gval;
   a.ng = g;
   a.ncomp = c.ncomp;
-  for (void* q : c.scratch_tables) hipFree(q);
-  c.scratch_tables.clear();
-  rc = upload_function(c, itg.kappa, a.kappa);
+  rc = upload_function(c, itg.kappa, a.kappa, 0);
   if (rc) return rc;
   a.kappa_c = const_value(itg.kappa, 1.0);
   if (itg.kind == IFEM_GPU_ELASTICITY) {
@@ -2865,7 +2879,7 @@ static int norms_patch(Context& c, const PatchRec& Pt, const int* eqmap, const d
     a.mu = itg.E / (2.0 * (1.0 + itg.nu));
   }
   if (exact) {
-    rc = upload_function(c, *exact, na.exact);
+    rc = upload_function(c, *exact, na.exact, 2);
     if (rc) return rc;
     na.has_exact = 1;
   }

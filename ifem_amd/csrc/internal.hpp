@@ -146,7 +146,20 @@ struct Context {
   double last_geometry_ms = 0.0, last_lines_ms = 0.0; // parts of the last assembly
 
   // owned device buffers for integrand mode tables
-  std::vector<void*> scratch_tables;
+  // basis tables (values, derivatives, element spans per direction) of a
+  // patch for one Gauss count: built on first use, kept until the context
+  // ends (patches never change once created)
+  struct BasisTables {
+    const double* knots0 = nullptr; // the patch's device knot vector 1 (identifies it)
+    int ng = 0;
+    double *tabN[3] = {nullptr, nullptr, nullptr}, *tabD[3] = {nullptr, nullptr, nullptr};
+    double* span[3] = {nullptr, nullptr, nullptr};
+  };
+  std::vector<BasisTables> basis_cache;
+  // device copies of the Fourier mode tables of the fields of the current
+  // call (conductivity, source, exact solution), grown as needed
+  double* d_modes[3] = {nullptr, nullptr, nullptr};
+  size_t modes_cap[3] = {0, 0, 0};
   void* d_asmargs = nullptr; // assembly kernel arguments (device copy)
   double* d_gpgeo = nullptr; // geometry records per (element, Gauss point)
   size_t gpgeo_cap = 0;      // doubles

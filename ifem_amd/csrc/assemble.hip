@@ -240,10 +240,14 @@ struct AsmArgs {
 
 // Geometry record of one Gauss point.  Vector problems: J^-1 (row-major, 9)
 // and w*detJ.  Scalar problems: M = w*detJ*kappa(X) * J^-1 J^-T (symmetric:
-// m00 m01 m02 m11 m12 m22) and w*detJ*f(X), padded to 8.
+// m00 m01 m02 m11 m12 m22) and w*detJ*f(X), padded to 8.  Stored per element
+// as N/2 rows of ng^3 pairs (values 2k, 2k+1 of point q of element e in pair
+// (e*N/2 + k)*ng^3 + q), so that the threads of consecutive Gauss points
+// write and read consecutive 16-byte pairs.
 template <int NC>
 struct GeoRec {
   static constexpr int N = NC == 1 ? 8 : 10;
+  static constexpr int NP = N / 2;
 };
 
 // The line kernels read their AsmArgs from a device buffer through the
@@ -353,10 +357,10 @@ __device__ inline void gp_geometry_at(ConstArgs& a, int64_t id, bool want_rhs)
     wdet = det * (dV * a.gw[q1] * a.gw[q2] * a.gw[q3]);
   }
   if (VEC || a.ncomp == 3) {
-    double* out = a.gp_geo + id * GeoRec<3>::N;
+    double2* out = reinterpret_cast<double2*>(a.gp_geo) + e * gp3 * GeoRec<3>::NP + q;
 #pragma unroll
-    for (int m = 0; m < 9; m++) out[m] = Ji[m];
-    out[9] = wdet;
+    for (int k = 0; k < 4; k++) out[k * gp3] = make_double2(Ji[2 * k], Ji[2 * k + 1]);
+    out[4 * gp3] = make_double2(Ji[8], wdet);
     return;
   }
   const DevFunction kf = load_function(a.kappa), sf = load_function(a.source);
@@ -364,15 +368,14 @@ __device__ inline void gp_geometry_at(ConstArgs& a, int64_t id, bool want_rhs)
   const double src =
       want_rhs ? (field_varies(sf) ? eval_field(sf, X[0], X[1], X[2]) : a.source_c) : 0.0;
   const double sk = wdet * kap;
-  double* out = a.gp_geo + id * GeoRec<1>::N;
-  out[0] = sk * (Ji[0] * Ji[0] + Ji[1] * Ji[1] + Ji[2] * Ji[2]);
-  out[1] = sk * (Ji[0] * Ji[3] + Ji[1] * Ji[4] + Ji[2] * Ji[5]);
-  out[2] = sk * (Ji[0] * Ji[6] + Ji[1] * Ji[7] + Ji[2] * Ji[8]);
-  out[3] = sk * (Ji[3] * Ji[3] + Ji[4] * Ji[4] + Ji[5] * Ji[5]);
-  out[4] = sk * (Ji[3] * Ji[6] + Ji[4] * Ji[7] + Ji[5] * Ji[8]);
-  out[5] = sk * (Ji[6] * Ji[6] + Ji[7] * Ji[7] + Ji[8] * Ji[8]);
-  out[6] = wdet * src;
-  out[7] = 0.0;
+  double2* out = reinterpret_cast<double2*>(a.gp_geo) + e * gp3 * GeoRec<1>::NP + q;
+  out[0] = make_double2(sk * (Ji[0] * Ji[0] + Ji[1] * Ji[1] + Ji[2] * Ji[2]),
+                        sk * (Ji[0] * Ji[3] + Ji[1] * Ji[4] + Ji[2] * Ji[5]));
+  out[gp3] = make_double2(sk * (Ji[0] * Ji[6] + Ji[1] * Ji[7] + Ji[2] * Ji[8]),
+                          sk * (Ji[3] * Ji[3] + Ji[4] * Ji[4] + Ji[5] * Ji[5]));
+  out[2 * gp3] = make_double2(sk * (Ji[3] * Ji[6] + Ji[4] * Ji[7] + Ji[5] * Ji[8]),
+                              sk * (Ji[6] * Ji[6] + Ji[7] * Ji[7] + Ji[8] * Ji[8]));
+  out[3 * gp3] = make_double2(wdet * src, 0.0);
 }
 
 // Scalar problems: built for four waves per SIMD (at most 128 registers):
@@ -724,11 +727,16 @@ __device__ __forceinline__ void line_impl(
         // geometry record of this (element, Gauss point), computed once per
         // element by k_gp_geometry: J^-1 and w*detJ (vector), or
         // M = w*detJ*kappa*J^-1 J^-T and w*detJ*f (scalar)
-        const int64_t gpi = ((int64_t)e1 + (int64_t)a.nel1 * (e2 + (int64_t)a.nel2 * e3)) * GP3 + q;
-        const double* __restrict__ rec = a.gp_geo + gpi * GeoRec<NC>::N;
+        const int64_t ge = (int64_t)e1 + (int64_t)a.nel1 * (e2 + (int64_t)a.nel2 * e3);
+        const double2* __restrict__ rec =
+            reinterpret_cast<const double2*>(a.gp_geo) + ge * (GP3 * GeoRec<NC>::NP) + q;
         T R[GeoRec<NC>::N];
 #pragma unroll
-        for (int m = 0; m < GeoRec<NC>::N; m++) R[m] = (T)rec[m];
+        for (int k = 0; k < GeoRec<NC>::NP; k++) {
+          const double2 v = rec[k * GP3];
+          R[2 * k] = (T)v.x;
+          R[2 * k + 1] = (T)v.y;
+        }
         if (NC == 3) {
 #pragma unroll
           for (int m = 0; m < 9; m++) geo[t][m] = R[m];

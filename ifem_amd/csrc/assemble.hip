@@ -149,9 +149,9 @@ __device__ __noinline__ double sin_library(double x)
   return sin(x);
 }
 
-__device__ inline double cos_cw(double x)
+// |x| <= 2^30 (no check)
+__device__ inline double cos_reduced(double x)
 {
-  if (!(fabs(x) <= 0x1p30)) return cos_library(x);
   const double n = rint(x * 0x1.45f306dc9c883p-2); // x / pi
   double r = fma(-n, 0x1.921fb54442d18p+1, x);     // pi, three parts
   r = fma(-n, 0x1.1a62633145c07p-53, r);
@@ -171,6 +171,37 @@ __device__ inline double cos_cw(double x)
   return ((int)n & 1) ? -c : c;
 }
 
+__device__ inline double cos_cw(double x)
+{
+  return fabs(x) <= 0x1p30 ? cos_reduced(x) : cos_library(x);
+}
+
+// sum_m a_m cos(k_m.x + phi_m), four modes at a time (independent
+// cosines), summed in mode order.  FAST: every argument is known to be
+// within 2^30 (no library fallback, so no call in the loop).
+template <bool FAST>
+__device__ inline double fourier_sum(const DevFunction& f, double x, double y, double z)
+{
+  double s = 0.0;
+  const double* __restrict__ md = f.modes;
+  int m = 0;
+  for (; m + 4 <= f.nmodes; m += 4, md += 20) {
+    double c[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const double arg = md[5 * u] * x + md[5 * u + 1] * y + md[5 * u + 2] * z + md[5 * u + 3];
+      c[u] = FAST ? cos_reduced(arg) : cos_cw(arg);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) s += md[5 * u + 4] * c[u];
+  }
+  for (; m < f.nmodes; m++, md += 5) {
+    const double arg = md[0] * x + md[1] * y + md[2] * z + md[3];
+    s += md[4] * (FAST ? cos_reduced(arg) : cos_cw(arg));
+  }
+  return s;
+}
+
 // Scalar fields evaluated at Gauss points (same definitions as the oracle).
 __device__ inline double eval_field(const DevFunction& f, double x, double y, double z)
 {
@@ -181,19 +212,10 @@ __device__ inline double eval_field(const DevFunction& f, double x, double y, do
     return f.scale * sin_library(M_PI * x) * sin_library(M_PI * y) * sin_library(M_PI * z);
   case IFEM_GPU_FN_FOURIER:
   case IFEM_GPU_FN_EXP_FOURIER: {
-    // four modes at a time (independent cosines), summed in mode order
-    double s = 0.0;
-    const double* __restrict__ md = f.modes;
-    int m = 0;
-    for (; m + 4 <= f.nmodes; m += 4, md += 20) {
-      double c[4];
-#pragma unroll
-      for (int u = 0; u < 4; u++)
-        c[u] = cos_cw(md[5 * u] * x + md[5 * u + 1] * y + md[5 * u + 2] * z + md[5 * u + 3]);
-#pragma unroll
-      for (int u = 0; u < 4; u++) s += md[5 * u + 4] * c[u];
-    }
-    for (; m < f.nmodes; m++, md += 5) s += md[4] * cos_cw(md[0] * x + md[1] * y + md[2] * z + md[3]);
+    // the bound (with a margin for rounding) decides once per point
+    const double xm = fmax(fabs(x), fmax(fabs(y), fabs(z)));
+    const double s = f.kbound * xm + f.pbound <= 0x1p29 ? fourier_sum<true>(f, x, y, z)
+                                                        : fourier_sum<false>(f, x, y, z);
     const double g = f.mean + f.scale * s;
     return f.type == IFEM_GPU_FN_EXP_FOURIER ? exp(g) : g;
   }
@@ -240,14 +262,18 @@ struct AsmArgs {
 
 // Geometry record of one Gauss point.  Vector problems: J^-1 (row-major, 9)
 // and w*detJ.  Scalar problems: M = w*detJ*kappa(X) * J^-1 J^-T (symmetric:
-// m00 m01 m02 m11 m12 m22) and w*detJ*f(X), padded to 8.  Stored per element
-// as N/2 rows of ng^3 pairs (values 2k, 2k+1 of point q of element e in pair
-// (e*N/2 + k)*ng^3 + q), so that the threads of consecutive Gauss points
-// write and read consecutive 16-byte pairs.
+// m00 m01 m02 m11 m12 m22) and w*detJ*f(X), padded to 8.  Vector records are
+// stored per element as 5 rows of ng^3 pairs (values 2k, 2k+1 of point q of
+// element e in pair (e*5 + k)*ng^3 + q), so that the threads of consecutive
+// Gauss points write and read consecutive 16-byte pairs; scalar records (64
+// bytes) one after the other (point id at 4*id pairs), which keeps the
+// geometry pass's partial-line writes down where its records outgrow the
+// caches (the 256-patch batch: 15 GB written against 59 GB in rows).
 template <int NC>
 struct GeoRec {
   static constexpr int N = NC == 1 ? 8 : 10;
   static constexpr int NP = N / 2;
+  static constexpr bool ROWS = NC == 3;
 };
 
 // The line kernels read their AsmArgs from a device buffer through the
@@ -297,6 +323,8 @@ __device__ inline DevFunction load_function(const __attribute__((address_space(4
   d.scale = f.scale;
   d.mean = f.mean;
   d.modes = f.modes;
+  d.kbound = f.kbound;
+  d.pbound = f.pbound;
   return d;
 }
 
@@ -368,14 +396,14 @@ __device__ inline void gp_geometry_at(ConstArgs& a, int64_t id, bool want_rhs)
   const double src =
       want_rhs ? (field_varies(sf) ? eval_field(sf, X[0], X[1], X[2]) : a.source_c) : 0.0;
   const double sk = wdet * kap;
-  double2* out = reinterpret_cast<double2*>(a.gp_geo) + e * gp3 * GeoRec<1>::NP + q;
+  double2* out = reinterpret_cast<double2*>(a.gp_geo) + id * GeoRec<1>::NP;
   out[0] = make_double2(sk * (Ji[0] * Ji[0] + Ji[1] * Ji[1] + Ji[2] * Ji[2]),
                         sk * (Ji[0] * Ji[3] + Ji[1] * Ji[4] + Ji[2] * Ji[5]));
-  out[gp3] = make_double2(sk * (Ji[0] * Ji[6] + Ji[1] * Ji[7] + Ji[2] * Ji[8]),
-                          sk * (Ji[3] * Ji[3] + Ji[4] * Ji[4] + Ji[5] * Ji[5]));
-  out[2 * gp3] = make_double2(sk * (Ji[3] * Ji[6] + Ji[4] * Ji[7] + Ji[5] * Ji[8]),
-                              sk * (Ji[6] * Ji[6] + Ji[7] * Ji[7] + Ji[8] * Ji[8]));
-  out[3 * gp3] = make_double2(wdet * src, 0.0);
+  out[1] = make_double2(sk * (Ji[0] * Ji[6] + Ji[1] * Ji[7] + Ji[2] * Ji[8]),
+                        sk * (Ji[3] * Ji[3] + Ji[4] * Ji[4] + Ji[5] * Ji[5]));
+  out[2] = make_double2(sk * (Ji[3] * Ji[6] + Ji[4] * Ji[7] + Ji[5] * Ji[8]),
+                        sk * (Ji[6] * Ji[6] + Ji[7] * Ji[7] + Ji[8] * Ji[8]));
+  out[3] = make_double2(wdet * src, 0.0);
 }
 
 // Scalar problems: built for four waves per SIMD (at most 128 registers):
@@ -724,12 +752,14 @@ __device__ __forceinline__ void line_impl(
         // element by k_gp_geometry: J^-1 and w*detJ (vector), or
         // M = w*detJ*kappa*J^-1 J^-T and w*detJ*f (scalar)
         const int64_t ge = (int64_t)e1 + (int64_t)a.nel1 * (e2 + (int64_t)a.nel2 * e3);
+        constexpr int PS = GeoRec<NC>::ROWS ? GP3 : 1; // pair stride of a record
         const double2* __restrict__ rec =
-            reinterpret_cast<const double2*>(a.gp_geo) + ge * (GP3 * GeoRec<NC>::NP) + q;
+            reinterpret_cast<const double2*>(a.gp_geo) + ge * (GP3 * GeoRec<NC>::NP) +
+            q * (GeoRec<NC>::ROWS ? 1 : GeoRec<NC>::NP);
         T R[GeoRec<NC>::N];
 #pragma unroll
         for (int k = 0; k < GeoRec<NC>::NP; k++) {
-          const double2 v = rec[k * GP3];
+          const double2 v = rec[k * PS];
           R[2 * k] = (T)v.x;
           R[2 * k + 1] = (T)v.y;
         }
@@ -1892,6 +1922,7 @@ static int upload_function(Context& c, const ifem_gpu_function& f, DevFunction& 
     double* d = c.d_modes[slot];
     IFEM_HIP(hipMemcpyAsync(d, f.modes, sizeof(double) * need, hipMemcpyHostToDevice, c.stream));
     df.modes = d;
+    mode_bounds(f.modes, f.nmodes, df.kbound, df.pbound);
   } else {
     df.nmodes = 0;
   }
@@ -2404,6 +2435,8 @@ static DevFunction batch_function(const ifem_gpu_function& f, const double* d_mo
   df.scale = f.scale;
   df.mean = f.mean;
   df.modes = d_modes;
+  if (d_modes && f.modes) mode_bounds(f.modes, f.nmodes, df.kbound, df.pbound);
+  else df.kbound = df.pbound = 1e300; // unknown: the checked loop
   return df;
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -37,7 +38,22 @@ struct DevFunction {
   double scale = 0.0;
   double mean = 0.0;
   const double* modes = nullptr; // device, nmodes*5
+  // bounds over the modes of |k_x| + |k_y| + |k_z| and of |phase| (host
+  // computed): k.x + phase stays below kbound*max|x_i| + pbound
+  double kbound = 0.0, pbound = 0.0;
 };
+
+// kbound and pbound of a mode table (nmodes rows of kx ky kz phase amp).
+inline void mode_bounds(const double* modes, int nmodes, double& kbound, double& pbound)
+{
+  kbound = pbound = 0.0;
+  for (int m = 0; m < nmodes; m++) {
+    const double* md = modes + 5 * m;
+    const double k1 = std::fabs(md[0]) + std::fabs(md[1]) + std::fabs(md[2]);
+    kbound = k1 > kbound ? k1 : kbound;
+    pbound = std::fabs(md[3]) > pbound ? std::fabs(md[3]) : pbound;
+  }
+}
 
 struct PatchRec {
   int p[3] = {0, 0, 0};

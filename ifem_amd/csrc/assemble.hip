@@ -410,11 +410,11 @@ __device__ inline void gp_geometry_at(ConstArgs& a, int64_t id, bool want_rhs)
 // the Fourier-mode loop is latency-bound, and more waves hide more of it
 // (measured: 4 waves beat 2, 3, 5, 6 and 8 on the 256-patch diffusion batch;
 // beyond 4 the geometry part spills).  Vector problems (no fields): their
-// own instance without the field code, 96 registers at p = 2 (64^3
+// own instance without the field code, built for five waves per SIMD (64^3
 // elasticity: 0.38 ms against 0.48 ms with the shared instance).
 template <int P>
-__global__ __launch_bounds__(256, 4) void k_gp_geometry(const AsmArgs* __restrict__ args,
-                                                     int64_t total, int want_rhs)
+__global__ __launch_bounds__(256, 4) __attribute__((flatten)) void k_gp_geometry(
+    const AsmArgs* __restrict__ args, int64_t total, int want_rhs)
 {
   ConstArgs& a = *const_args(args);
   for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < total;
@@ -423,8 +423,8 @@ __global__ __launch_bounds__(256, 4) void k_gp_geometry(const AsmArgs* __restric
 }
 
 template <int P>
-__global__ __launch_bounds__(256) void k_gp_geometry_vec(const AsmArgs* __restrict__ args,
-                                                         int64_t total)
+__global__ __launch_bounds__(256, 5) void k_gp_geometry_vec(const AsmArgs* __restrict__ args,
+                                                            int64_t total)
 {
   ConstArgs& a = *const_args(args);
   for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < total;
@@ -432,26 +432,25 @@ __global__ __launch_bounds__(256) void k_gp_geometry_vec(const AsmArgs* __restri
     gp_geometry_at<P, true>(a, id, false);
 }
 
-// Batch: gp0[q] = first Gauss-point record of patch q.
-__global__ __launch_bounds__(256, 4) void k_gp_geometry_batch(const AsmArgs* __restrict__ args,
-                                    const int64_t* __restrict__ gp0, int np, int64_t total,
-                                    int want_rhs)
+// Batch, one launch per degree P: the patches of that degree are
+// plist[0..np), patch plist[i] holding points [gq0[i], gq0[i+1]) of the
+// launch's total.  The point function is inlined (flatten: no call, so no
+// register saves to scratch per point; the library fallbacks stay calls).
+template <int P>
+__global__ __launch_bounds__(256, 4) __attribute__((flatten)) void k_gp_geometry_batch(
+    const AsmArgs* __restrict__ args, const int* __restrict__ plist,
+    const int64_t* __restrict__ gq0, int np, int64_t total, int want_rhs)
 {
   for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < total;
        id += (int64_t)gridDim.x * blockDim.x) {
     int lo = 0, hi = np - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (gp0[mid] <= id) lo = mid;
+      if (gq0[mid] <= id) lo = mid;
       else hi = mid - 1;
     }
-    ConstArgs& a = const_args(args)[lo];
-    const int64_t local = id - gp0[lo];
-    switch (a.variant >> 1) {
-    case 0: gp_geometry_at<1>(a, local, want_rhs != 0); break;
-    case 1: gp_geometry_at<2>(a, local, want_rhs != 0); break;
-    default: gp_geometry_at<3>(a, local, want_rhs != 0); break;
-    }
+    ConstArgs& a = const_args(args)[plist[lo]];
+    gp_geometry_at<P>(a, id - gq0[lo], want_rhs != 0);
   }
 }
 
@@ -2559,8 +2558,28 @@ int batch_symbolic(Context& c)
   }
   B.ngp = ngp;
   IFEM_HIP(hipMalloc(&B.d_geo, sizeof(double) * GeoRec<1>::N * (ngp > 0 ? ngp : 1)));
-  IFEM_HIP(hipMalloc(&B.d_gp0, sizeof(int64_t) * (np > 0 ? np : 1)));
-  IFEM_HIP(hipMemcpy(B.d_gp0, B.h_gp0.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice));
+  // per degree: its patches and the first point of each among that degree's
+  // points (one geometry launch per degree)
+  std::vector<int> glist;
+  std::vector<int64_t> gq0;
+  for (int p = 1; p <= 3; p++) {
+    B.gdeg_off[p] = (int)glist.size();
+    int64_t t = 0;
+    for (int q = 0; q < np; q++) {
+      const BatchPatch& bp = B.pp[q];
+      if (bp.p != p) continue;
+      const PatchRec& Pt = c.patches[bp.patch];
+      glist.push_back(q);
+      gq0.push_back(t);
+      t += (int64_t)Pt.nel[0] * Pt.nel[1] * Pt.nel[2] * bp.ng * bp.ng * bp.ng;
+    }
+    B.gdeg_n[p] = (int)glist.size() - B.gdeg_off[p];
+    B.gdeg_total[p] = t;
+  }
+  IFEM_HIP(hipMalloc(&B.d_glist, sizeof(int) * (np > 0 ? np : 1)));
+  IFEM_HIP(hipMalloc(&B.d_gq0, sizeof(int64_t) * (np > 0 ? np : 1)));
+  IFEM_HIP(hipMemcpy(B.d_glist, glist.data(), sizeof(int) * glist.size(), hipMemcpyHostToDevice));
+  IFEM_HIP(hipMemcpy(B.d_gq0, gq0.data(), sizeof(int64_t) * gq0.size(), hipMemcpyHostToDevice));
 
   // work list: one item per node line (j, k) of each patch
   struct Item { double cost; int patch, line; };
@@ -2651,10 +2670,21 @@ int batch_assemble(Context& c, int precision, double* ms_out)
   // geometry pass: always for FP64 (the full assembly); an FP32 assembly
   // reuses the records of an earlier assembly of the same batch (the
   // material part of the records is the same in both precisions)
-  if (B.ngp > 0 && !(fp32 && B.geo_valid))
-    k_gp_geometry_batch<<<grid_for(B.ngp), 256, 0, c.stream>>>(
-        static_cast<const AsmArgs*>(fp32 ? B.d_args32 : B.d_args64), B.d_gp0, (int)B.pp.size(),
-        B.ngp, fp32 ? 0 : 1);
+  if (B.ngp > 0 && !(fp32 && B.geo_valid)) {
+    const AsmArgs* ga = static_cast<const AsmArgs*>(fp32 ? B.d_args32 : B.d_args64);
+    for (int p = 3; p >= 1; p--) {
+      const int64_t tot = B.gdeg_total[p];
+      if (tot <= 0) continue;
+      const int* pl = B.d_glist + B.gdeg_off[p];
+      const int64_t* q0 = B.d_gq0 + B.gdeg_off[p];
+      if (p == 3)
+        k_gp_geometry_batch<3><<<grid_for(tot), 256, 0, c.stream>>>(ga, pl, q0, B.gdeg_n[p], tot, fp32 ? 0 : 1);
+      else if (p == 2)
+        k_gp_geometry_batch<2><<<grid_for(tot), 256, 0, c.stream>>>(ga, pl, q0, B.gdeg_n[p], tot, fp32 ? 0 : 1);
+      else
+        k_gp_geometry_batch<1><<<grid_for(tot), 256, 0, c.stream>>>(ga, pl, q0, B.gdeg_n[p], tot, fp32 ? 0 : 1);
+    }
+  }
   if (B.nitems > 0) {
     const int rc = fp32 ? launch_batch<float, false>(c, B, B.d_args32)
                         : launch_batch<double, true>(c, B, B.d_args64);

@@ -282,7 +282,7 @@ void Batch::release()
                 (void**)&x, (void**)&r, (void**)&z, (void**)&p, (void**)&q, (void**)&dinv,
                 (void**)&r32, (void**)&z32, (void**)&p32, (void**)&q32, (void**)&x32,
                 (void**)&dinv32, (void**)&part, (void**)&pscal, (void**)&pnorm, (void**)&d_geo,
-                (void**)&d_gp0};
+                (void**)&d_glist, (void**)&d_gq0};
   for (void** v : d) {
     hipFree(*v);
     *v = nullptr;
@@ -298,6 +298,10 @@ void Batch::release()
   neq = nitems = nchunk = 0;
   nnz = ngp = 0;
   h_gp0.clear();
+  for (int p = 0; p < 4; p++) {
+    gdeg_off[p] = gdeg_n[p] = 0;
+    gdeg_total[p] = 0;
+  }
 }
 
 static int default_lanes(int p)

@@ -58,10 +58,15 @@ struct Batch {
   int* d_queue = nullptr;   // one queue counter per degree
   int ncu = 0;              // compute units (persistent grid size)
   // geometry records per (element, Gauss point) (patch q's records start at
-  // gp0[q]); ngp records in all
+  // h_gp0[q]); ngp records in all.  Per degree p: its patches
+  // d_glist[gdeg_off[p] ..+gdeg_n[p]), the first point of each among that
+  // degree's gdeg_total[p] points in d_gq0 (same offsets)
   double* d_geo = nullptr;
-  int64_t* d_gp0 = nullptr;
   std::vector<int64_t> h_gp0;
+  int* d_glist = nullptr;
+  int64_t* d_gq0 = nullptr;
+  int gdeg_off[4] = {0, 0, 0, 0}, gdeg_n[4] = {0, 0, 0, 0};
+  int64_t gdeg_total[4] = {0, 0, 0, 0};
   int64_t ngp = 0;
   BatchTune tune[4];
   bool tuned = false;

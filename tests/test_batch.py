@@ -145,8 +145,9 @@ def test_batch_errors():
 def test_batch_full_size_properties():
     """The 256-patch benchmark batch (p in {2,3}, 8..48 elements per direction,
     about 5.6M equations, 1.1e9 nonzeros): closed-form nonzero counts per patch,
-    one small block against the oracle, FP64 and mixed solves to 1e-8 on every
-    patch's true residual."""
+    the smallest block and a mid-size p = 3 block against the oracle, every
+    block's FP32 values against its FP64 values, FP64 and mixed solves to 1e-8
+    on every patch's true residual."""
     pbs = problems.diffusion_batch()
     _say("[gpu] full-size batch 256 patches")
 
@@ -167,6 +168,19 @@ def test_batch_full_size_properties():
         _, _, gA, _, gb = s.batch_system(k)
         assert np.max(np.abs(gA - A)) <= 1e-12 * np.max(np.abs(A))
         assert np.max(np.abs(gb - b)) <= 1e-12 * np.max(np.abs(b))
+        # a mid-size p = 3 block against the oracle (full bundles, many work
+        # items per workgroup), and every block's FP32 values against its FP64
+        # ones (the two precisions run different kernels)
+        mid = [(pb.neq, q) for q, pb in enumerate(pbs)
+               if pb.patch.p[0] == 3 and min(np.array(pb.patch.n) - 3) >= 12]
+        k = min(mid)[1]
+        rp, ci, A, b = _oracle(pbs[k])
+        _, _, gA, gA32, gb = s.batch_system(k, values32=True)
+        assert np.max(np.abs(gA - A)) <= 1e-12 * np.max(np.abs(A)), k
+        assert np.max(np.abs(gb - b)) <= 1e-12 * np.max(np.abs(b)), k
+        for q in range(len(pbs)):
+            _, _, gA, gA32, _ = s.batch_system(q, values32=True)
+            assert np.max(np.abs(gA32 - gA)) <= 1e-5 * np.max(np.abs(gA)), q
         st = s.batch_solve(rtol=1e-8, maxits=20000)
         assert st["converged"] and np.all(st["patch_rel_residual"] <= 1e-8)
         _say(f"  fp64: max iterations {st['iterations']}, {st['solve_ms']:.1f} ms")

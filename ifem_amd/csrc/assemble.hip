@@ -266,9 +266,7 @@ struct AsmArgs {
 // stored per element as 5 rows of ng^3 pairs (values 2k, 2k+1 of point q of
 // element e in pair (e*5 + k)*ng^3 + q), so that the threads of consecutive
 // Gauss points write and read consecutive 16-byte pairs; scalar records (64
-// bytes) one after the other (point id at 4*id pairs), which keeps the
-// geometry pass's partial-line writes down where its records outgrow the
-// caches (the 256-patch batch: 15 GB written against 59 GB in rows).
+// bytes) one after the other (point id at 4*id pairs).
 template <int NC>
 struct GeoRec {
   static constexpr int N = NC == 1 ? 8 : 10;
@@ -411,7 +409,7 @@ __device__ inline void gp_geometry_at(ConstArgs& a, int64_t id, bool want_rhs)
 // (measured: 4 waves beat 2, 3, 5, 6 and 8 on the 256-patch diffusion batch;
 // beyond 4 the geometry part spills).  Vector problems (no fields): their
 // own instance without the field code, built for five waves per SIMD (64^3
-// elasticity: 0.38 ms against 0.48 ms with the shared instance).
+// elasticity: 0.34 ms against 0.48 ms with the shared instance).
 template <int P>
 __global__ __launch_bounds__(256, 4) __attribute__((flatten)) void k_gp_geometry(
     const AsmArgs* __restrict__ args, int64_t total, int want_rhs)

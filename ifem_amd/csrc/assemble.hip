@@ -472,6 +472,19 @@ static int grid_for(int64_t total)
 
 constexpr int kBlock = 256;
 
+// Block barrier after LDS writes that other waves read right after it: the
+// wave first waits for its own LDS operations to complete (s_waitcnt
+// lgkmcnt(0)).  The compiler emits no such wait before the barrier of a loop
+// whose body ends in LDS stores (the element rounds of the window update),
+// and without it a round's reads could see a window entry before the
+// previous round's store to it had landed (observed as lost contributions on
+// entries shared by consecutive bundle elements, in workgroups sharing a CU).
+__device__ __forceinline__ void lds_barrier()
+{
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __syncthreads();
+}
+
 // LDS of one line workgroup.  T: arithmetic type of the element integrals
 // (double: FP64 path, float: FP32 path).  NC: components (1 Poisson,
 // 3 elasticity).  SPLIT: number of blocks per line sharing the row
@@ -1145,7 +1158,7 @@ __device__ __forceinline__ void line_impl(
       T (*stg)[NEN][P1][NS] = reinterpret_cast<T (*)[NEN][P1][NS]>(&geo[0][0]);
       for (int g0 = 0; g0 < nch; g0 += GS) {
         const int g1 = g0 + GS < nch ? g0 + GS : nch;
-        __syncthreads(); // geometry reads (or the previous group) done
+        lds_barrier(); // geometry reads (or the previous group) done
         if (has && le >= g0 && le < g1) {
 #pragma unroll
           for (int a1 = 0; a1 < P1; a1++)
@@ -1225,7 +1238,7 @@ __device__ __forceinline__ void line_impl(
       // 3x3 block -> [i][nbr][j]
       constexpr int ISTR = LL::WN * NC;
       for (int r = 0; r < nch; r++) {
-        __syncthreads();
+        lds_barrier();
         if (has && le == r) {
           T cur[P1][NV];
 #pragma unroll

@@ -519,13 +519,9 @@ struct LineLds {
   static constexpr int GEO = (NC == 3 ? 9 : 0) + 3 * P1;
   static constexpr int NRHS = HAS_RHS ? P1 * NC : 1;
   // scalar problems with more than 27 Gauss points per element (the ordered,
-  // not pipelined path): phase B by sum factorisation, see line_impl; SFT =
-  // size of its first partial sums
-  static constexpr bool SF = NC == 1 && GP3 > 27 && HALF != 1 && CH * NEN <= 256 && P >= 2;
-  // FP64: T1 one derivative direction k (three of the nine kl) at a time,
-  // so that the workgroup keeps three blocks per CU; FP32: all at once
-  static constexpr int SFK = sizeof(T) == 8 ? 3 : 1;
-  static constexpr int SFT = SF ? CH * (9 / SFK) * NG * NG * P1 : 1;
+  // not pipelined path): phase B by sum factorisation, see line_impl
+  static constexpr bool SF =
+      NC == 1 && GP3 > 27 && HALF != 1 && CH * NEN <= 256 && P >= 2 && P1 <= NG;
 
   // every member 16-byte aligned: lets the compiler use 128-bit LDS loads
   alignas(16) T win[P1][NCR][WN][NC];
@@ -577,7 +573,7 @@ __device__ __forceinline__ void line_impl(
                             ? LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NT * 64 : 1],
     int64_t (*__restrict__ pbase)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NCR],
     int (*__restrict__ prow)[LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>::NCR],
-    int* __restrict__ nbundlep, T* __restrict__ sft)
+    int* __restrict__ nbundlep)
 {
   using LL = LineLds<P, NG, NC, T, SPLIT, HAS_RHS, HALF>;
   constexpr int P1 = LL::P1;
@@ -789,9 +785,9 @@ __device__ __forceinline__ void line_impl(
           // directions 2 and 3 (A_k: derivative in direction k, else value)
           const T M[9] = {R[0], R[1], R[2], R[1], R[3], R[4], R[2], R[4], R[5]};
           const T a2[3] = {n2a, d2a, n2a}, a3[3] = {n3a, n3a, d3a};
-          // stored [element][kl][point]: T2 later overwrites each (element,
-          // kl) block in place
-          T* __restrict__ wf = &geo[0][0] + (int64_t)le * 9 * GP3 + (q - qa);
+          // stored [element][kl][q1][q2][q3]: the (q2, q3) slab of each q1
+          // is contiguous, and its T2 later overwrites it in place
+          T* __restrict__ wf = &geo[0][0] + (int64_t)le * 9 * GP3 + (q1 * NG + q2) * NG + q3;
 #pragma unroll
           for (int k = 0; k < 3; k++)
 #pragma unroll
@@ -966,47 +962,42 @@ __device__ __forceinline__ void line_impl(
         // then q1 with the direction-1 functions, each step in fixed order:
         // 9*NG*NG*P1 + 9*NG*P1*P1 + 3*NG*(P1+3)*NEN operations per element
         // and line instead of 3*GP3*P1*NEN.
-        T* __restrict__ gf = &geo[0][0]; // W, then T2, [element][kl][...] with stride GP3
-        constexpr int KPG = 3 / LL::SFK;   // derivative directions k per T1 pass
-        for (int kg = 0; kg < LL::SFK; kg++) {
-          if (kg > 0) __syncthreads(); // the previous group's T1 read
-          const int n1t = nch * 3 * KPG * NG * NG * P1;
-          for (int t = tid; t < n1t; t += kBlock) {
-            const int bb3 = t % P1;
-            int r = t / P1;
-            const int q2 = r % NG;
-            r /= NG;
-            const int q1 = r % NG;
-            r /= NG;
-            const int klg = r % (3 * KPG), lq = r / (3 * KPG);
-            const int kl = kg * 3 * KPG + klg, l = kl % 3;
-            const int mm3 = be3[c0 + lq] - kc + P;
-            const T* __restrict__ w = gf + (int64_t)(lq * 9 + kl) * GP3 + q1 + NG * q2;
-            T sum = 0;
+        // one thread per (element, kl, q1): its (q2, q3) slab of W into
+        // registers, T1(q2, b3) over q3, T2(b2, b3) over q2, written back
+        // over the slab (no other thread touches it, so no barrier between
+        // the read and the write)
+        T* __restrict__ gf = &geo[0][0]; // W, then T2, [element][kl][q1][...] with stride GP3
+        for (int t = tid; t < nch * 9 * NG; t += kBlock) {
+          const int q1 = t % NG, kl = (t / NG) % 9, lq = t / (9 * NG);
+          const int l = kl % 3;
+          const int mm2 = be2[c0 + lq] - jc + P, mm3 = be3[c0 + lq] - kc + P;
+          T* __restrict__ w = gf + (int64_t)(lq * 9 + kl) * GP3 + q1 * NG * NG;
+          T wv[NG][NG];
 #pragma unroll
-            for (int q3 = 0; q3 < NG; q3++)
-              sum += w[NG * NG * q3] * (l == 2 ? tD3[mm3][q3][bb3] : tN3[mm3][q3][bb3]);
-            sft[t] = sum; // [le][kl in group][q1][q2][b3]
-          }
-          __syncthreads(); // T1 of the group complete, its W rows read
-          const int n2t = nch * 3 * KPG * NG * P1 * P1;
-          for (int t = tid; t < n2t; t += kBlock) {
-            const int bb3 = t % P1;
-            int r = t / P1;
-            const int bb2 = r % P1;
-            r /= P1;
-            const int q1 = r % NG;
-            r /= NG;
-            const int klg = r % (3 * KPG), lq = r / (3 * KPG);
-            const int kl = kg * 3 * KPG + klg, l = kl % 3;
-            const int mm2 = be2[c0 + lq] - jc + P;
-            const T* __restrict__ u = sft + (((lq * 3 * KPG + klg) * NG + q1) * NG) * P1 + bb3;
-            T sum = 0;
+          for (int q2 = 0; q2 < NG; q2++)
 #pragma unroll
-            for (int q2 = 0; q2 < NG; q2++)
-              sum += u[q2 * P1] * (l == 1 ? tD2[mm2][q2][bb2] : tN2[mm2][q2][bb2]);
-            gf[(int64_t)(lq * 9 + kl) * GP3 + (q1 * P1 + bb2) * P1 + bb3] = sum; // T2
-          }
+            for (int q3 = 0; q3 < NG; q3++) wv[q2][q3] = w[q2 * NG + q3];
+          T t1[NG][P1];
+#pragma unroll
+          for (int q2 = 0; q2 < NG; q2++)
+#pragma unroll
+            for (int bb3 = 0; bb3 < P1; bb3++) {
+              T sum = 0;
+#pragma unroll
+              for (int q3 = 0; q3 < NG; q3++)
+                sum += wv[q2][q3] * (l == 2 ? tD3[mm3][q3][bb3] : tN3[mm3][q3][bb3]);
+              t1[q2][bb3] = sum;
+            }
+#pragma unroll
+          for (int bb2 = 0; bb2 < P1; bb2++)
+#pragma unroll
+            for (int bb3 = 0; bb3 < P1; bb3++) {
+              T sum = 0;
+#pragma unroll
+              for (int q2 = 0; q2 < NG; q2++)
+                sum += t1[q2][bb3] * (l == 1 ? tD2[mm2][q2][bb2] : tN2[mm2][q2][bb2]);
+              w[bb2 * P1 + bb3] = sum; // T2(q1, b2, b3)
+            }
         }
         __syncthreads();
         if (has) {
@@ -1023,7 +1014,7 @@ __device__ __forceinline__ void line_impl(
           for (int k = 0; k < 3; k++) {
 #pragma unroll
             for (int q1 = 0; q1 < NG; q1++) {
-              const T* __restrict__ vk = v + 3 * k * GP3 + q1 * P1 * P1;
+              const T* __restrict__ vk = v + 3 * k * GP3 + q1 * NG * NG;
               const T u = vk[0] * bd[q1] + (vk[GP3] + vk[2 * GP3]) * bn[q1];
 #pragma unroll
               for (int a1 = 0; a1 < P1; a1++)
@@ -1541,10 +1532,9 @@ __global__ __launch_bounds__(kBlock, HALF == 2 ? 3 : 1) void k_assemble_lines(
   __shared__ int64_t pbase[2][LL::NCR];
   __shared__ int prow[2][LL::NCR];
   __shared__ int nbundle;
-  __shared__ T sft[LL::SFT];
   line_impl<P, NG, NC, T, SPLIT, HAS_RHS, HALF, PIPE, ORD, !PIPE>(a, blockIdx.x, blockIdx.y, win, geo, stage,
                                                       rstage, rwin, tN1, tD1, tN2, tD2, tN3, tD3,
-                                                      be2, be3, eqw, pbase, prow, &nbundle, sft);
+                                                      be2, be3, eqw, pbase, prow, &nbundle);
 }
 
 // Persistent batch assembly over many independent patches, one launch per
@@ -1579,7 +1569,6 @@ __global__ __launch_bounds__(kBlock, 3) void k_assemble_batch(const AsmArgs* __r
   __shared__ int64_t pbase[2][LL::NCR];
   __shared__ int prow[2][LL::NCR];
   __shared__ int nbundle;
-  __shared__ T sft[LL::SFT];
   __shared__ int next;
   while (true) {
     if (threadIdx.x == 0) next = atomicAdd(queue, 1);
@@ -1590,8 +1579,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_assemble_batch(const AsmArgs* __r
     ConstArgs& a = const_args(args)[w.x];
     line_impl<P, NG, 1, T, 1, HAS_RHS, HALF, PIPE, ORD, !PIPE>(a, w.y, 0, win, geo, stage, rstage,
                                                                rwin, tN1, tD1, tN2, tD2, tN3, tD3,
-                                                               be2, be3, eqw, pbase, prow, &nbundle,
-                                                               sft);
+                                                               be2, be3, eqw, pbase, prow, &nbundle);
     __syncthreads();
   }
 }

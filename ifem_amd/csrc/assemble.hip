@@ -149,25 +149,32 @@ __device__ __noinline__ double sin_library(double x)
   return sin(x);
 }
 
-// |x| <= 2^30 (no check)
-__device__ inline double cos_reduced(double x)
+// Reduction modulo pi: x = n pi + r, |r| <= pi/2; returns n, r^2 in s.
+__device__ inline double reduce_pi(double x, double& s)
 {
   const double n = rint(x * 0x1.45f306dc9c883p-2); // x / pi
   double r = fma(-n, 0x1.921fb54442d18p+1, x);     // pi, three parts
   r = fma(-n, 0x1.1a62633145c07p-53, r);
   r = fma(-n, -0x1.f1976b7ed8fbcp-109, r);
-  const double s = r * r;
-  double c = 1.0 / 2432902008176640000.0; // 1/20!, then -1/18!, 1/16!, ...
-  c = fma(c, s, -1.0 / 6402373705728000.0);
-  c = fma(c, s, 1.0 / 20922789888000.0);
-  c = fma(c, s, -1.0 / 87178291200.0);
-  c = fma(c, s, 1.0 / 479001600.0);
-  c = fma(c, s, -1.0 / 3628800.0);
-  c = fma(c, s, 1.0 / 40320.0);
-  c = fma(c, s, -1.0 / 720.0);
-  c = fma(c, s, 1.0 / 24.0);
-  c = fma(c, s, -0.5);
-  c = fma(c, s, 1.0);
+  s = r * r;
+  return n;
+}
+
+// Taylor coefficients of cos in r^2, highest first: 1/20!, -1/18!, ..., 1.
+__device__ constexpr double kCosTaylor[11] = {
+    1.0 / 2432902008176640000.0, -1.0 / 6402373705728000.0, 1.0 / 20922789888000.0,
+    -1.0 / 87178291200.0,        1.0 / 479001600.0,         -1.0 / 3628800.0,
+    1.0 / 40320.0,               -1.0 / 720.0,              1.0 / 24.0,
+    -0.5,                        1.0};
+
+// |x| <= 2^30 (no check)
+__device__ inline double cos_reduced(double x)
+{
+  double s;
+  const double n = reduce_pi(x, s);
+  double c = kCosTaylor[0];
+#pragma unroll
+  for (int k = 1; k < 11; k++) c = fma(c, s, kCosTaylor[k]);
   return ((int)n & 1) ? -c : c;
 }
 
@@ -176,30 +183,52 @@ __device__ inline double cos_cw(double x)
   return fabs(x) <= 0x1p30 ? cos_reduced(x) : cos_library(x);
 }
 
-// sum_m a_m cos(k_m.x + phi_m), four modes at a time (independent
-// cosines), summed in mode order.  FAST: every argument is known to be
-// within 2^30 (no library fallback, so no call in the loop).
+// Mode tables are read through the constant address space: where the table
+// pointer is the same for the whole wave the compiler loads the modes with
+// scalar loads (no vector registers, no per-lane memory instructions).
+typedef const __attribute__((address_space(4))) double* ConstModes;
+
+// sum_m a_m cos(k_m.x + phi_m), four modes at a time, summed in mode order.
+// FAST: every argument is known to be within 2^30 (no library fallback, so
+// no call in the loop); the four cosines run the operations of cos_reduced
+// with their polynomial steps interleaved (four independent chains).
 template <bool FAST>
 __device__ inline double fourier_sum(const DevFunction& f, double x, double y, double z)
 {
-  double s = 0.0;
-  const double* __restrict__ md = f.modes;
+  double acc = 0.0;
+  ConstModes md = (ConstModes)f.modes;
   int m = 0;
   for (; m + 4 <= f.nmodes; m += 4, md += 20) {
     double c[4];
+    if (FAST) {
+      double n[4], s[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const double arg = md[5 * u] * x + md[5 * u + 1] * y + md[5 * u + 2] * z + md[5 * u + 3];
-      c[u] = FAST ? cos_reduced(arg) : cos_cw(arg);
+      for (int u = 0; u < 4; u++) {
+        const double arg = md[5 * u] * x + md[5 * u + 1] * y + md[5 * u + 2] * z + md[5 * u + 3];
+        n[u] = reduce_pi(arg, s[u]);
+        c[u] = kCosTaylor[0];
+      }
+#pragma unroll
+      for (int k = 1; k < 11; k++)
+#pragma unroll
+        for (int u = 0; u < 4; u++) c[u] = fma(c[u], s[u], kCosTaylor[k]);
+#pragma unroll
+      for (int u = 0; u < 4; u++) c[u] = ((int)n[u] & 1) ? -c[u] : c[u];
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const double arg = md[5 * u] * x + md[5 * u + 1] * y + md[5 * u + 2] * z + md[5 * u + 3];
+        c[u] = cos_cw(arg);
+      }
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++) s += md[5 * u + 4] * c[u];
+    for (int u = 0; u < 4; u++) acc += md[5 * u + 4] * c[u];
   }
   for (; m < f.nmodes; m++, md += 5) {
     const double arg = md[0] * x + md[1] * y + md[2] * z + md[3];
-    s += md[4] * (FAST ? cos_reduced(arg) : cos_cw(arg));
+    acc += md[4] * (FAST ? cos_reduced(arg) : cos_cw(arg));
   }
-  return s;
+  return acc;
 }
 
 // Scalar fields evaluated at Gauss points (same definitions as the oracle).
@@ -447,8 +476,22 @@ __global__ __launch_bounds__(256, 4) __attribute__((flatten)) void k_gp_geometry
       if (gq0[mid] <= id) lo = mid;
       else hi = mid - 1;
     }
-    ConstArgs& a = const_args(args)[plist[lo]];
-    gp_geometry_at<P>(a, id - gq0[lo], want_rhs != 0);
+    // A wave whose points all lie in one patch (all but the waves that
+    // straddle a patch boundary) takes the patch index as a wave-uniform
+    // value, so that the patch's arguments and mode tables come through
+    // scalar loads.
+    const int l0 = __builtin_amdgcn_readfirstlane(lo);
+    if (__ballot(lo != l0) == 0) {
+      const int pl = __builtin_amdgcn_readfirstlane(plist[l0]);
+      const int64_t g0 = gq0[l0];
+      const int64_t q0 = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(
+                                        (int)(uint32_t)((uint64_t)g0 >> 32))
+                                    << 32) |
+                                   (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)g0));
+      gp_geometry_at<P>(const_args(args)[pl], id - q0, want_rhs != 0);
+    } else {
+      gp_geometry_at<P>(const_args(args)[plist[lo]], id - gq0[lo], want_rhs != 0);
+    }
   }
 }
 

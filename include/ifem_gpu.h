@@ -104,7 +104,7 @@ int ifem_gpu_sam_bind_constraints(ifem_gpu_ctx* ctx, int patch_id, int ncomp, in
  * node, as in ifem_gpu_sam_bind; nceq > 0 adds prescribed values as in
  * ifem_gpu_sam_bind_constraints (constraints with masters:
  * IFEM_GPU_EUNSUPPORTED).  ifem_gpu_init_assembly then builds the
- * union pattern on the host, and ifem_gpu_assemble adds the patches in
+ * union pattern on the device, and ifem_gpu_assemble adds the patches in
  * order into the shared rows (the same integrand on every patch).  The
  * multigrid and learned preconditioners need a single patch. */
 int ifem_gpu_sam_bind_patches(ifem_gpu_ctx* ctx, int npatch, const int* patch_ids,

@@ -470,26 +470,37 @@ __global__ __launch_bounds__(256, 4) __attribute__((flatten)) void k_gp_geometry
 {
   for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < total;
        id += (int64_t)gridDim.x * blockDim.x) {
-    int lo = 0, hi = np - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (gq0[mid] <= id) lo = mid;
-      else hi = mid - 1;
+    // The wave's first point (its lowest index) picks a patch by a
+    // wave-uniform search; a wave whose points all lie in that patch (all
+    // but the waves that straddle a patch boundary) runs with the patch index
+    // as a wave-uniform value, so that the patch's arguments and mode tables
+    // come through scalar loads.
+    const int64_t idw = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(
+                                       (int)(uint32_t)((uint64_t)id >> 32))
+                                   << 32) |
+                                  (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)id));
+    int lw = 0, hw = np - 1;
+    while (lw < hw) {
+      const int mid = (lw + hw + 1) >> 1;
+      if (gq0[mid] <= idw) lw = mid;
+      else hw = mid - 1;
     }
-    // A wave whose points all lie in one patch (all but the waves that
-    // straddle a patch boundary) takes the patch index as a wave-uniform
-    // value, so that the patch's arguments and mode tables come through
-    // scalar loads.
-    const int l0 = __builtin_amdgcn_readfirstlane(lo);
-    if (__ballot(lo != l0) == 0) {
-      const int pl = __builtin_amdgcn_readfirstlane(plist[l0]);
-      const int64_t g0 = gq0[l0];
+    const int64_t wend = lw + 1 < np ? gq0[lw + 1] : total;
+    if (__ballot(id >= wend) == 0) {
+      const int pl = __builtin_amdgcn_readfirstlane(plist[lw]);
+      const int64_t g0 = gq0[lw];
       const int64_t q0 = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(
                                         (int)(uint32_t)((uint64_t)g0 >> 32))
                                     << 32) |
                                    (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)g0));
       gp_geometry_at<P>(const_args(args)[pl], id - q0, want_rhs != 0);
     } else {
+      int lo = 0, hi = np - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (gq0[mid] <= id) lo = mid;
+        else hi = mid - 1;
+      }
       gp_geometry_at<P>(const_args(args)[plist[lo]], id - gq0[lo], want_rhs != 0);
     }
   }

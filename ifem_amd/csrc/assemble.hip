@@ -459,6 +459,14 @@ __global__ __launch_bounds__(256, 5) void k_gp_geometry_vec(const AsmArgs* __res
     gp_geometry_at<P, true>(a, id, false);
 }
 
+// The value of the wave's first active lane (64-bit).
+__device__ __forceinline__ int64_t readfirstlane64(int64_t v)
+{
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // Batch, one launch per degree P: the patches of that degree are
 // plist[0..np), patch plist[i] holding points [gq0[i], gq0[i+1]) of the
 // launch's total.  The point function is inlined (flatten: no call, so no
@@ -475,10 +483,7 @@ __global__ __launch_bounds__(256, 4) __attribute__((flatten)) void k_gp_geometry
     // but the waves that straddle a patch boundary) runs with the patch index
     // as a wave-uniform value, so that the patch's arguments and mode tables
     // come through scalar loads.
-    const int64_t idw = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(
-                                       (int)(uint32_t)((uint64_t)id >> 32))
-                                   << 32) |
-                                  (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)id));
+    const int64_t idw = readfirstlane64(id);
     int lw = 0, hw = np - 1;
     while (lw < hw) {
       const int mid = (lw + hw + 1) >> 1;
@@ -488,12 +493,7 @@ __global__ __launch_bounds__(256, 4) __attribute__((flatten)) void k_gp_geometry
     const int64_t wend = lw + 1 < np ? gq0[lw + 1] : total;
     if (__ballot(id >= wend) == 0) {
       const int pl = __builtin_amdgcn_readfirstlane(plist[lw]);
-      const int64_t g0 = gq0[lw];
-      const int64_t q0 = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(
-                                        (int)(uint32_t)((uint64_t)g0 >> 32))
-                                    << 32) |
-                                   (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)g0));
-      gp_geometry_at<P>(const_args(args)[pl], id - q0, want_rhs != 0);
+      gp_geometry_at<P>(const_args(args)[pl], id - readfirstlane64(gq0[lw]), want_rhs != 0);
     } else {
       int lo = 0, hi = np - 1;
       while (lo < hi) {
